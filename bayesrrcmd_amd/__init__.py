@@ -1,0 +1,271 @@
+"""MI355X-native BayesR marker sweep (BayesRRcmd `ppbayes`), Python face.
+
+Everything here calls the C ABI of libbrr_hip.so (include/brr_hip.h); the
+arrays are numpy. The device path has no CPU fallback: constructing a
+HipSweep on a machine without a HIP device raises BrrHipError.
+
+Reference correspondence (BayesRRcmd src/):
+  HipSweep.sweep        AnalysisGraph::exec -> BayesRBase::processColumn
+  HipSweep.run_chain    BayesRBase::runGibbs iterations, device-resident
+  build_dense           DenseMarkerBuilder (as 2-bit codes + 4-entry table)
+  build_ragged          RaggedSparseMarkerBuilder
+  bayesr_init           BayesRBase::init
+  shuffle               std::random_shuffle of the marker order
+  marker_uz             BayesRBase::prepareForAnylsis
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _abi
+from ._abi import LAYOUT_DENSE_2BIT, LAYOUT_RAGGED
+
+__all__ = [
+    "BrrHipError", "HipSweep", "RaggedArena", "build_dense", "build_ragged", "synth_bed",
+    "bayesr_init", "shuffle", "marker_uz", "LAYOUT_DENSE_2BIT", "LAYOUT_RAGGED", "lib",
+]
+
+
+class BrrHipError(RuntimeError):
+    pass
+
+
+def lib():
+    return _abi.load()
+
+
+def _p(a: np.ndarray | None, ctype):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def _threads() -> int:
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def _check(rc: int, what: str, ctx=None):
+    if rc != 0:
+        msg = ""
+        if ctx is not None:
+            m = lib().brr_hip_last_error(ctx)
+            msg = m.decode() if m else ""
+        raise BrrHipError(f"{what} failed with status {rc}: {msg}")
+
+
+# --------------------------------------------------------------- builders --
+
+def synth_bed(seed: int, n: int, m: int, maf_lo: float, maf_hi: float, miss_rate: float = 0.0,
+              threads: int | None = None):
+    """Synthetic SNP-major PLINK payload (m x ceil(n/4) bytes) and the MAF per SNP."""
+    stride = (n + 3) // 4
+    bed = np.empty((m, stride), dtype=np.uint8)
+    maf = np.empty(m, dtype=np.float64)
+    rc = lib().brr_synth_bed(seed, n, m, maf_lo, maf_hi, miss_rate, _p(bed, C.c_uint8), stride,
+                             _p(maf, C.c_double), threads or _threads())
+    if rc != 0:
+        raise BrrHipError(f"brr_synth_bed failed with status {rc}")
+    return bed, maf
+
+
+def build_dense(bed: np.ndarray, n: int, threads: int | None = None) -> np.ndarray:
+    """DenseMarkerBuilder tables: (m, 4) doubles for dosage 0, 1, 2, missing."""
+    bed = np.ascontiguousarray(bed, dtype=np.uint8)
+    m, stride = bed.shape
+    lut = np.empty((m, 4), dtype=np.float64)
+    _check(lib().brr_dense_luts(_p(bed, C.c_uint8), stride, n, m, _p(lut, C.c_double),
+                                threads or _threads()), "brr_dense_luts")
+    return lut
+
+
+@dataclass
+class RaggedArena:
+    ones: np.ndarray
+    ones_off: np.ndarray
+    twos: np.ndarray
+    twos_off: np.ndarray
+    miss: np.ndarray
+    miss_off: np.ndarray
+    stats: np.ndarray      # (m, 4): mean, sd, sqrdZ, Zsum
+
+    def nbytes(self) -> int:
+        return int(4 * (self.ones.size + self.twos.size + self.miss.size))
+
+
+def build_ragged(bed: np.ndarray, n: int, threads: int | None = None) -> RaggedArena:
+    """RaggedSparseMarkerBuilder over every column of a .bed payload."""
+    bed = np.ascontiguousarray(bed, dtype=np.uint8)
+    m, stride = bed.shape
+    t = threads or _threads()
+    offs = [np.zeros(m + 1, dtype=np.uint64) for _ in range(3)]
+    _check(lib().brr_ragged_count(_p(bed, C.c_uint8), stride, n, m, *(_p(o, C.c_uint64) for o in offs), t),
+           "brr_ragged_count")
+    lists = [np.empty(max(1, int(o[-1])), dtype=np.int32) for o in offs]
+    stats = np.empty((m, 4), dtype=np.float64)
+    _check(lib().brr_ragged_fill(_p(bed, C.c_uint8), stride, n, m, *(_p(o, C.c_uint64) for o in offs),
+                                 *(_p(l, C.c_int32) for l in lists), _p(stats, C.c_double), t),
+           "brr_ragged_fill (a column has no genotype 1 or 2)")
+    lists = [l[: int(o[-1])] for l, o in zip(lists, offs)]
+    return RaggedArena(lists[0], offs[0], lists[1], offs[1], lists[2], offs[2], stats)
+
+
+# ---------------------------------------------------------- host driver --
+
+def bayesr_init(y: np.ndarray, n_groups: int, n_components: int, seed: int, chain: int = 0):
+    """BayesRBase::init: returns (eps, scalars dict)."""
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    n = y.size
+    eps = np.empty(n, dtype=np.float64)
+    sigmaG = np.empty(n_groups, dtype=np.float64)
+    pi = np.empty(n_groups * n_components, dtype=np.float64)
+    s = _abi.Scalars(0.0, 0.0, 0.0, _p(sigmaG, C.c_double), _p(pi, C.c_double), None, None)
+    _check(lib().brr_bayesr_init(_p(y, C.c_double), n, n_groups, n_components, seed, chain,
+                                 _p(eps, C.c_double), C.byref(s)), "brr_bayesr_init")
+    return eps, dict(mu=s.mu, sigmaE=s.sigmaE, epsilon_sum=s.epsilon_sum, sigmaG=sigmaG,
+                     pi=pi.reshape(n_groups, n_components))
+
+
+def shuffle(seed: int, iteration: int, chain: int, order: np.ndarray, mode: int = 0) -> np.ndarray:
+    """In-place std::random_shuffle of order (uint32) over the stream (mode 0) or rand() (mode 1)."""
+    assert order.dtype == np.uint32 and order.flags.c_contiguous
+    _check(lib().brr_shuffle(seed, iteration, chain, order.size, _p(order, C.c_uint32), mode), "brr_shuffle")
+    return order
+
+
+def marker_uz(seed: int, iteration: int, chain: int, m: int):
+    u = np.empty(m, dtype=np.float64)
+    z = np.empty(m, dtype=np.float64)
+    _check(lib().brr_marker_uz(seed, iteration, chain, m, _p(u, C.c_double), _p(z, C.c_double)),
+           "brr_marker_uz")
+    return u, z
+
+
+# ---------------------------------------------------------- device sweep --
+
+class HipSweep:
+    """Device context: marker arena + chain state on one MI355X (HipSweepGraph)."""
+
+    def __init__(self, n: int, m: int, n_components: int, n_groups: int = 1,
+                 layout: int = LAYOUT_RAGGED, workgroups: int = 0, block_markers: int = 0,
+                 device: int = 0):
+        self.n, self.m, self.K, self.G = n, m, n_components, n_groups
+        self.layout = layout
+        cfg = _abi.Config(n, m, n_components, n_groups, layout, workgroups, block_markers, device)
+        ctx = _abi.CTX()
+        rc = lib().brr_hip_create(C.byref(cfg), C.byref(ctx))
+        if rc != 0:
+            raise BrrHipError(f"brr_hip_create failed with status {rc} (N={n}, M={m}, K={n_components}, "
+                              f"G={n_groups}, layout={layout}, workgroups={workgroups})")
+        self.ctx = ctx
+        self._keep = []
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            lib().brr_hip_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _ck(self, rc, what):
+        _check(rc, what, self.ctx)
+
+    # uploads
+    def upload_dense(self, bed: np.ndarray, lut: np.ndarray):
+        bed = np.ascontiguousarray(bed, dtype=np.uint8)
+        lut = np.ascontiguousarray(lut, dtype=np.float64)
+        self._ck(lib().brr_hip_upload_dense(self.ctx, _p(bed, C.c_uint8), bed.shape[1], _p(lut, C.c_double)),
+                 "brr_hip_upload_dense")
+
+    def upload_ragged(self, a: RaggedArena):
+        self._ck(lib().brr_hip_upload_ragged(
+            self.ctx, _p(a.ones, C.c_int32), _p(a.ones_off, C.c_uint64), _p(a.twos, C.c_int32),
+            _p(a.twos_off, C.c_uint64), _p(a.miss, C.c_int32), _p(a.miss_off, C.c_uint64),
+            _p(np.ascontiguousarray(a.stats), C.c_double)), "brr_hip_upload_ragged")
+
+    def set_groups(self, group: np.ndarray):
+        g = np.ascontiguousarray(group, dtype=np.int32)
+        self._ck(lib().brr_hip_set_groups(self.ctx, _p(g, C.c_int32)), "brr_hip_set_groups")
+
+    def set_model(self, cva: np.ndarray, v0E=1e-4, s02E=1e-4, v0G=1e-4, s02G=1e-4):
+        cva = np.ascontiguousarray(cva, dtype=np.float64).reshape(self.G, self.K - 1)
+        self._cva = cva
+        mdl = _abi.Model(_p(cva, C.c_double), v0E, s02E, v0G, s02G)
+        self._ck(lib().brr_hip_set_model(self.ctx, C.byref(mdl)), "brr_hip_set_model")
+
+    # state
+    def set_state(self, eps=None, beta=None, comp=None):
+        e = None if eps is None else np.ascontiguousarray(eps, dtype=np.float64)
+        b = None if beta is None else np.ascontiguousarray(beta, dtype=np.float64)
+        c = None if comp is None else np.ascontiguousarray(comp, dtype=np.int32)
+        self._ck(lib().brr_hip_set_state(self.ctx, _p(e, C.c_double), _p(b, C.c_double), _p(c, C.c_int32)),
+                 "brr_hip_set_state")
+
+    def set_scalars(self, mu, sigmaE, epsilon_sum, sigmaG, pi):
+        sg = np.ascontiguousarray(sigmaG, dtype=np.float64).reshape(self.G)
+        pp = np.ascontiguousarray(pi, dtype=np.float64).reshape(self.G * self.K)
+        s = _abi.Scalars(mu, sigmaE, epsilon_sum, _p(sg, C.c_double), _p(pp, C.c_double), None, None)
+        self._ck(lib().brr_hip_set_scalars(self.ctx, C.byref(s)), "brr_hip_set_scalars")
+
+    def get_state(self):
+        eps = np.empty(self.n, dtype=np.float64)
+        beta = np.empty(self.m, dtype=np.float64)
+        comp = np.empty(self.m, dtype=np.int32)
+        acum = np.empty(self.m, dtype=np.float64)
+        self._ck(lib().brr_hip_get_state(self.ctx, _p(eps, C.c_double), _p(beta, C.c_double),
+                                         _p(comp, C.c_int32), _p(acum, C.c_double)), "brr_hip_get_state")
+        return dict(eps=eps, beta=beta, comp=comp, acum=acum)
+
+    def get_scalars(self):
+        sg = np.empty(self.G, dtype=np.float64)
+        pi = np.empty(self.G * self.K, dtype=np.float64)
+        v = np.empty(self.G * self.K, dtype=np.float64)
+        bs = np.empty(self.G, dtype=np.float64)
+        s = _abi.Scalars(0.0, 0.0, 0.0, _p(sg, C.c_double), _p(pi, C.c_double), _p(v, C.c_double),
+                         _p(bs, C.c_double))
+        self._ck(lib().brr_hip_get_scalars(self.ctx, C.byref(s)), "brr_hip_get_scalars")
+        return dict(mu=s.mu, sigmaE=s.sigmaE, epsilon_sum=s.epsilon_sum, sigmaG=sg,
+                    pi=pi.reshape(self.G, self.K), v=v.reshape(self.G, self.K), betasqn=bs)
+
+    # compute
+    def sweep(self, order: np.ndarray, u: np.ndarray, z: np.ndarray):
+        o = np.ascontiguousarray(order, dtype=np.uint32)
+        uu = np.ascontiguousarray(u, dtype=np.float64)
+        zz = np.ascontiguousarray(z, dtype=np.float64)
+        self._ck(lib().brr_hip_sweep(self.ctx, _p(o, C.c_uint32), _p(uu, C.c_double), _p(zz, C.c_double)),
+                 "brr_hip_sweep")
+
+    def run_chain(self, seed: int, chain: int, first_iteration: int, orders: np.ndarray) -> np.ndarray:
+        o = np.ascontiguousarray(orders, dtype=np.uint32)
+        iters = o.size // self.m
+        assert iters * self.m == o.size
+        hist = np.empty((iters, 4 + self.G), dtype=np.float64)
+        self._ck(lib().brr_hip_run_chain(self.ctx, seed, chain, first_iteration, iters, _p(o, C.c_uint32),
+                                         _p(hist, C.c_double)), "brr_hip_run_chain")
+        return hist
+
+    def compute_num(self, marker: int, beta_old: float) -> float:
+        out = C.c_double()
+        self._ck(lib().brr_hip_compute_num(self.ctx, marker, beta_old, C.byref(out)), "brr_hip_compute_num")
+        return out.value
+
+    def epsilon_change(self, marker: int, beta_old: float, beta: float):
+        self._ck(lib().brr_hip_epsilon_change(self.ctx, marker, beta_old, beta), "brr_hip_epsilon_change")
+
+    def stats(self) -> dict:
+        s = _abi.Stats()
+        self._ck(lib().brr_hip_last_stats(self.ctx, C.byref(s)), "brr_hip_last_stats")
+        return {f: getattr(s, f) for f, _ in _abi.Stats._fields_ if f != "reserved"}

@@ -1,0 +1,680 @@
+// api.hip -- the C ABI of libbrr_hip.so (declared in include/brr_hip.h).
+//
+// Owns all device memory of a context, uploads marker arenas, builds the
+// per-workgroup ragged segment table on the device and launches the
+// persistent sweep kernel cooperatively (all workgroups co-resident).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "brr_hip.h"
+#include "sweep_kernel.hpp"
+
+using namespace brr;
+
+struct brr_hip_ctx {
+    brr_hip_config cfg{};
+    int device = 0;
+    std::string err;
+    uint32_t P = 0, B = 0, S = 0, R = 0;
+    int layout = 0;
+    bool have_markers = false;
+    bool have_model = false;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    brr_hip_model model{};
+    std::vector<double> cva;
+    brr_hip_stats stats{};
+    // dense arena
+    uint32_t* d_dense = nullptr;
+    uint64_t col_words = 0, col_bytes = 0;
+    double* d_lut = nullptr;
+    // ragged arena
+    int32_t* d_ones = nullptr;
+    int32_t* d_twos = nullptr;
+    int32_t* d_miss = nullptr;
+    uint64_t* d_ones_off = nullptr;
+    uint64_t* d_twos_off = nullptr;
+    uint64_t* d_miss_off = nullptr;
+    uint32_t* d_seg = nullptr;
+    double* d_stats = nullptr;
+    // model / state
+    int32_t* d_group = nullptr;
+    double* d_cva = nullptr;
+    double* d_eps = nullptr;
+    double* d_beta = nullptr;
+    int32_t* d_comp = nullptr;
+    double* d_acum = nullptr;
+    double* d_scal = nullptr;
+    double* d_hist = nullptr;
+    size_t hist_cap = 0;
+    uint32_t* d_order = nullptr;
+    size_t order_cap = 0;
+    double* d_u = nullptr;
+    double* d_z = nullptr;
+    uint64_t* d_xchg1 = nullptr;
+    uint64_t* d_xchg2 = nullptr;
+    uint32_t* d_ctl = nullptr;
+    unsigned long long* d_counters = nullptr;
+    double* d_tmp = nullptr;
+};
+
+namespace {
+
+int fail(brr_hip_ctx* c, int code, const std::string& msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                   \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail((ctx), BRR_HIP_EDEVICE,                                              \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                  \
+    } while (0)
+
+template <typename T>
+int dalloc(brr_hip_ctx* c, T** p, size_t count)
+{
+    if (*p) { (void)hipFree(*p); *p = nullptr; }
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)p, sizeof(T) * count);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(c, BRR_HIP_ENOMEM, "hipMalloc of " + std::to_string(sizeof(T) * count) +
+                                           " bytes failed: " + hipGetErrorString(e));
+    }
+    return BRR_HIP_OK;
+}
+
+template <typename T>
+void dfree(T*& p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+// seg[(j*3 + list)*(P+1) + w] = lower_bound(list_j, w*S) - list_j start
+__global__ void build_seg_kernel(const int32_t* ones, const uint64_t* ones_off,
+                                 const int32_t* twos, const uint64_t* twos_off,
+                                 const int32_t* miss, const uint64_t* miss_off,
+                                 uint32_t M, uint32_t P, uint32_t S, uint32_t* seg)
+{
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t total = (uint64_t)M * 3u * (P + 1u);
+    if (t >= total) return;
+    const uint32_t w = (uint32_t)(t % (P + 1u));
+    const uint64_t jl = t / (P + 1u);
+    const uint32_t list = (uint32_t)(jl % 3u);
+    const uint32_t j = (uint32_t)(jl / 3u);
+    const int32_t* base;
+    uint64_t n;
+    if (list == 0) { base = ones + ones_off[j]; n = ones_off[j + 1] - ones_off[j]; }
+    else if (list == 1) { base = twos + twos_off[j]; n = twos_off[j + 1] - twos_off[j]; }
+    else { base = miss + miss_off[j]; n = miss_off[j + 1] - miss_off[j]; }
+    const int64_t key = (int64_t)w * (int64_t)S;
+    uint64_t a = 0, b = n;
+    while (a < b) {
+        const uint64_t mid = (a + b) >> 1;
+        if ((int64_t)base[mid] < key) a = mid + 1; else b = mid;
+    }
+    seg[t] = (uint32_t)a;
+}
+
+// BayesRKernel::computeNum for one marker on the device epsilon (one block).
+__global__ void compute_num_kernel(SweepParams p, uint32_t j, double beta_old, double* out)
+{
+    __shared__ double red[kWaves];
+    const int tid = threadIdx.x;
+    double s = 0.0;
+    if (p.layout == kLayoutDense) {
+        const double* lt = p.lut + 4 * (size_t)j;
+        const double wt[4] = {lt[2], lt[3], lt[1], lt[0]};
+        const uint8_t* col = (const uint8_t*)(p.dense + (size_t)j * p.col_words);
+        for (uint32_t i = tid; i < p.N; i += kThreads) {
+            const uint32_t v = (col[i >> 2] >> ((i & 3u) * 2u)) & 3u;
+            s = fma(wt[v], p.eps[i], s);
+        }
+    } else {
+        for (uint64_t q = p.ones_off[j] + tid; q < p.ones_off[j + 1]; q += kThreads) s += p.eps[p.ones[q]];
+        for (uint64_t q = p.twos_off[j] + tid; q < p.twos_off[j + 1]; q += kThreads) s += 2.0 * p.eps[p.twos[q]];
+    }
+    s = wave_sum(s);
+    if ((tid & 63) == 0) red[tid >> 6] = s;
+    __syncthreads();
+    if (tid == 0) {
+        const double dot = ((red[0] + red[1]) + red[2]) + red[3];
+        const double dN = (double)p.N;
+        if (p.layout == kLayoutDense) {
+            *out = dot + beta_old * (double)(p.N - 1u);
+        } else {
+            const double mean = p.stats[4 * (size_t)j], sd = p.stats[4 * (size_t)j + 1];
+            const double epsSum = p.scal[2];
+            *out = beta_old * (dN - 1.0) - mean * epsSum / sd + dot / sd;
+        }
+    }
+}
+
+// calculateEpsilonChange + eps += delta for one marker (grid over N).
+__global__ void epsilon_change_kernel(SweepParams p, uint32_t j, double beta_old, double beta)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const double dBeta = beta_old - beta;
+    if (p.layout == kLayoutDense) {
+        if (i >= p.N) return;
+        const double* lt = p.lut + 4 * (size_t)j;
+        const double wt[4] = {lt[2], lt[3], lt[1], lt[0]};
+        const uint8_t* col = (const uint8_t*)(p.dense + (size_t)j * p.col_words);
+        const uint32_t v = (col[i >> 2] >> ((i & 3u) * 2u)) & 3u;
+        p.eps[i] = p.eps[i] + dBeta * wt[v];
+        return;
+    }
+    // ragged: classify i by binary search in the three lists
+    if (i >= p.N) return;
+    auto in_list = [](const int32_t* base, uint64_t n, int32_t key) {
+        uint64_t a = 0, b = n;
+        while (a < b) {
+            const uint64_t mid = (a + b) >> 1;
+            if (base[mid] < key) a = mid + 1; else b = mid;
+        }
+        return a < n && base[a] == key;
+    };
+    const double mean = p.stats[4 * (size_t)j], sd = p.stats[4 * (size_t)j + 1];
+    const double meanAdjustment = dBeta * mean / sd;
+    const double oneAdjustment = dBeta / sd;
+    double delta = -meanAdjustment;
+    if (in_list(p.ones + p.ones_off[j], p.ones_off[j + 1] - p.ones_off[j], (int32_t)i)) delta += oneAdjustment;
+    else if (in_list(p.twos + p.twos_off[j], p.twos_off[j + 1] - p.twos_off[j], (int32_t)i)) delta += 2 * oneAdjustment;
+    else if (in_list(p.miss + p.miss_off[j], p.miss_off[j + 1] - p.miss_off[j], (int32_t)i)) delta += meanAdjustment;
+    p.eps[i] = p.eps[i] + delta;
+    if (i == 0) {
+        const double Zsum = p.stats[4 * (size_t)j + 3];
+        p.scal[2] += dBeta * Zsum / sd - dBeta * mean * (double)p.N / sd;
+    }
+}
+
+SweepParams base_params(brr_hip_ctx* c)
+{
+    SweepParams p{};
+    p.N = c->cfg.num_inds;
+    p.M = c->cfg.num_snps;
+    p.K = c->cfg.num_components;
+    p.G = c->cfg.num_groups;
+    p.P = c->P;
+    p.B = c->B;
+    p.S = c->S;
+    p.layout = c->layout;
+    p.dense = c->d_dense;
+    p.col_words = c->col_words;
+    p.col_bytes = c->col_bytes;
+    p.lut = c->d_lut;
+    p.ones = c->d_ones;
+    p.twos = c->d_twos;
+    p.miss = c->d_miss;
+    p.ones_off = c->d_ones_off;
+    p.twos_off = c->d_twos_off;
+    p.miss_off = c->d_miss_off;
+    p.seg = c->d_seg;
+    p.stats = c->d_stats;
+    p.group = c->d_group;
+    p.cva = c->d_cva;
+    p.v0E = c->model.v0E;
+    p.s02E = c->model.s02E;
+    p.v0G = c->model.v0G;
+    p.s02G = c->model.s02G;
+    p.eps = c->d_eps;
+    p.beta = c->d_beta;
+    p.comp = c->d_comp;
+    p.acum = c->d_acum;
+    p.scal = c->d_scal;
+    p.xchg1 = c->d_xchg1;
+    p.xchg2 = c->d_xchg2;
+    p.ctl = c->d_ctl;
+    p.counters = c->d_counters;
+    return p;
+}
+
+const void* kernel_for(uint32_t R)
+{
+    switch (R) {
+    case 1: return (const void*)bayesr_sweep_kernel<1>;
+    case 2: return (const void*)bayesr_sweep_kernel<2>;
+    case 4: return (const void*)bayesr_sweep_kernel<4>;
+    default: return nullptr;
+    }
+}
+
+int reset_exchange(brr_hip_ctx* c)
+{
+    HIP_TRY(c, hipMemsetAsync(c->d_xchg1, 0, sizeof(uint64_t) * 2 * kMaxB * c->P * 2, c->stream));
+    HIP_TRY(c, hipMemsetAsync(c->d_xchg2, 0, sizeof(uint64_t) * 2 * kMaxB * 2, c->stream));
+    HIP_TRY(c, hipMemsetAsync(c->d_ctl, 0, sizeof(uint32_t) * 8, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return BRR_HIP_OK;
+}
+
+int launch(brr_hip_ctx* c, SweepParams& p)
+{
+    const void* fn = kernel_for(c->R);
+    if (!fn) return fail(c, BRR_HIP_EINVAL, "unsupported slice size");
+    HIP_TRY(c, hipMemsetAsync(c->d_ctl + kCtlAbort, 0, sizeof(uint32_t) * 2, c->stream));
+    HIP_TRY(c, hipMemsetAsync(c->d_counters, 0, sizeof(unsigned long long) * 4, c->stream));
+    void* args[] = {&p};
+    HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    HIP_TRY(c, hipLaunchCooperativeKernel(fn, dim3(c->P), dim3(kThreads), args, 0, c->stream));
+    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    HIP_TRY(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    uint32_t ctl[4] = {0, 0, 0, 0};
+    unsigned long long counters[4] = {0, 0, 0, 0};
+    HIP_TRY(c, hipMemcpy(ctl, c->d_ctl, sizeof(ctl), hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(counters, c->d_counters, sizeof(counters), hipMemcpyDeviceToHost));
+    c->stats.kernel_ms = ms;
+    c->stats.rounds = counters[0];
+    c->stats.changes = counters[1];
+    c->stats.workgroups = c->P;
+    c->stats.block_markers = c->B;
+    c->stats.slice = c->S;
+    if (ctl[kCtlAbort]) {
+        reset_exchange(c);
+        return fail(c, BRR_HIP_ETIMEOUT, "device exchange wait exceeded its bound (site " +
+                                             std::to_string(ctl[kCtlAbortWhere]) + ")");
+    }
+    return BRR_HIP_OK;
+}
+
+int check_ready(brr_hip_ctx* c)
+{
+    if (!c) return BRR_HIP_EINVAL;
+    if (!c->have_markers) return fail(c, BRR_HIP_ESTATE, "markers not uploaded");
+    if (!c->have_model) return fail(c, BRR_HIP_ESTATE, "model not set (brr_hip_set_model)");
+    return BRR_HIP_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int brr_hip_abi_version(void) { return BRR_HIP_ABI_VERSION; }
+
+int brr_hip_device_count(int* count)
+{
+    if (!count) return BRR_HIP_EINVAL;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+    return BRR_HIP_OK;
+}
+
+int brr_hip_create(const brr_hip_config* cfg, brr_hip_ctx** out)
+{
+    if (!cfg || !out) return BRR_HIP_EINVAL;
+    *out = nullptr;
+    if (cfg->num_inds < 2 || cfg->num_snps < 1) return BRR_HIP_EINVAL;
+    if (cfg->num_components < 2 || cfg->num_components > (uint32_t)kMaxK) return BRR_HIP_EINVAL;
+    if (cfg->num_groups < 1 || cfg->num_groups > (uint32_t)kMaxG) return BRR_HIP_EINVAL;
+    if (cfg->layout != BRR_HIP_LAYOUT_DENSE_2BIT && cfg->layout != BRR_HIP_LAYOUT_RAGGED) return BRR_HIP_EINVAL;
+    if (cfg->block_markers > (uint32_t)kMaxB) return BRR_HIP_EINVAL;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return BRR_HIP_EDEVICE;
+    if ((int)cfg->device >= ndev) return BRR_HIP_EINVAL;
+
+    brr_hip_ctx* c = new brr_hip_ctx();
+    c->cfg = *cfg;
+    c->device = (int)cfg->device;
+    c->layout = (int)cfg->layout;
+    if (hipSetDevice(c->device) != hipSuccess) { delete c; return BRR_HIP_EDEVICE; }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, c->device) != hipSuccess) { delete c; return BRR_HIP_EDEVICE; }
+    int coop = 0;
+    (void)hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, c->device);
+    if (!coop) { delete c; return BRR_HIP_EDEVICE; }
+
+    // Slice partition: S multiple of 16 individuals, P = ceil(N / S) <= CUs.
+    const uint32_t N = cfg->num_inds;
+    const uint32_t cus = (uint32_t)prop.multiProcessorCount;
+    uint32_t Preq = cfg->workgroups ? cfg->workgroups : cus;
+    Preq = std::min(Preq, cus);
+    uint32_t S = (N + Preq - 1) / Preq;
+    S = (S + kSliceAlign - 1) / kSliceAlign * kSliceAlign;
+    uint32_t R = (S + kRun - 1) / kRun;
+    if (R > (uint32_t)kMaxRuns) { delete c; return BRR_HIP_EINVAL; }
+    if (R == 3) R = 4;
+    c->S = S;
+    c->R = R;
+    c->P = (N + S - 1) / S;
+    c->B = cfg->block_markers ? cfg->block_markers : 32u;
+
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel_for(R), kThreads, 0) != hipSuccess ||
+        (uint64_t)occ * cus < c->P) {
+        delete c;
+        return BRR_HIP_EDEVICE;
+    }
+
+    const uint32_t M = cfg->num_snps, G = cfg->num_groups, K = cfg->num_components;
+    const ScalarLayout SL{G, K};
+    int rc = BRR_HIP_OK;
+    if ((rc = dalloc(c, &c->d_eps, N)) || (rc = dalloc(c, &c->d_beta, M)) ||
+        (rc = dalloc(c, &c->d_comp, M)) || (rc = dalloc(c, &c->d_acum, M)) ||
+        (rc = dalloc(c, &c->d_scal, SL.size())) || (rc = dalloc(c, &c->d_cva, G * (K - 1))) ||
+        (rc = dalloc(c, &c->d_xchg1, (size_t)2 * kMaxB * c->P * 2)) ||
+        (rc = dalloc(c, &c->d_xchg2, (size_t)2 * kMaxB * 2)) || (rc = dalloc(c, &c->d_ctl, 8)) ||
+        (rc = dalloc(c, &c->d_counters, 4)) || (rc = dalloc(c, &c->d_tmp, 4)) ||
+        (rc = dalloc(c, &c->d_u, M)) || (rc = dalloc(c, &c->d_z, M))) {
+        brr_hip_destroy(c);
+        return rc;
+    }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        brr_hip_destroy(c);
+        return BRR_HIP_EDEVICE;
+    }
+    (void)hipMemset(c->d_eps, 0, sizeof(double) * N);
+    (void)hipMemset(c->d_beta, 0, sizeof(double) * M);
+    (void)hipMemset(c->d_comp, 0, sizeof(int32_t) * M);
+    (void)hipMemset(c->d_acum, 0, sizeof(double) * M);
+    (void)hipMemset(c->d_scal, 0, sizeof(double) * SL.size());
+    if (reset_exchange(c) != BRR_HIP_OK) { brr_hip_destroy(c); return BRR_HIP_EDEVICE; }
+    *out = c;
+    return BRR_HIP_OK;
+}
+
+void brr_hip_destroy(brr_hip_ctx* c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    dfree(c->d_dense); dfree(c->d_lut);
+    dfree(c->d_ones); dfree(c->d_twos); dfree(c->d_miss);
+    dfree(c->d_ones_off); dfree(c->d_twos_off); dfree(c->d_miss_off);
+    dfree(c->d_seg); dfree(c->d_stats);
+    dfree(c->d_group); dfree(c->d_cva);
+    dfree(c->d_eps); dfree(c->d_beta); dfree(c->d_comp); dfree(c->d_acum);
+    dfree(c->d_scal); dfree(c->d_hist); dfree(c->d_order); dfree(c->d_u); dfree(c->d_z);
+    dfree(c->d_xchg1); dfree(c->d_xchg2); dfree(c->d_ctl); dfree(c->d_counters); dfree(c->d_tmp);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* brr_hip_last_error(const brr_hip_ctx* c)
+{
+    return c ? c->err.c_str() : "null context";
+}
+
+int brr_hip_upload_dense(brr_hip_ctx* c, const uint8_t* bed, uint64_t column_stride, const double* lut)
+{
+    if (!c || !bed || !lut) return fail(c, BRR_HIP_EINVAL, "null argument");
+    if (c->layout != BRR_HIP_LAYOUT_DENSE_2BIT) return fail(c, BRR_HIP_EINVAL, "context layout is not dense");
+    const uint32_t N = c->cfg.num_inds, M = c->cfg.num_snps;
+    const uint64_t col_bytes = ((uint64_t)N + 3) / 4;
+    if (column_stride < col_bytes) return fail(c, BRR_HIP_EINVAL, "column_stride < ceil(N/4)");
+    HIP_TRY(c, hipSetDevice(c->device));
+    c->col_bytes = col_bytes;
+    c->col_words = (col_bytes + 15) / 16 * 4;   // 16-byte aligned columns
+    int rc = dalloc(c, &c->d_dense, (size_t)c->col_words * M);
+    if (rc) return rc;
+    if ((rc = dalloc(c, &c->d_lut, (size_t)4 * M))) return rc;
+    HIP_TRY(c, hipMemset(c->d_dense, 0, sizeof(uint32_t) * c->col_words * M));
+    HIP_TRY(c, hipMemcpy2D(c->d_dense, c->col_words * 4, bed, column_stride, col_bytes, M,
+                           hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->d_lut, lut, sizeof(double) * 4 * M, hipMemcpyHostToDevice));
+    c->have_markers = true;
+    return BRR_HIP_OK;
+}
+
+int brr_hip_upload_ragged(brr_hip_ctx* c, const int32_t* ones, const uint64_t* ones_off,
+                          const int32_t* twos, const uint64_t* twos_off, const int32_t* miss,
+                          const uint64_t* miss_off, const double* stats)
+{
+    if (!c || !ones_off || !twos_off || !miss_off || !stats) return fail(c, BRR_HIP_EINVAL, "null argument");
+    if (c->layout != BRR_HIP_LAYOUT_RAGGED) return fail(c, BRR_HIP_EINVAL, "context layout is not ragged");
+    const uint32_t N = c->cfg.num_inds, M = c->cfg.num_snps;
+    const uint64_t n1 = ones_off[M], n2 = twos_off[M], nm = miss_off[M];
+    if ((n1 && !ones) || (n2 && !twos) || (nm && !miss)) return fail(c, BRR_HIP_EINVAL, "null list");
+    // validate: offsets non-decreasing, indices ascending and in range
+    const int32_t* lists[3] = {ones, twos, miss};
+    const uint64_t* offs[3] = {ones_off, twos_off, miss_off};
+    for (int l = 0; l < 3; ++l) {
+        if (offs[l][0] != 0) return fail(c, BRR_HIP_EINVAL, "offsets must start at 0");
+        for (uint32_t j = 0; j < M; ++j) {
+            if (offs[l][j + 1] < offs[l][j]) return fail(c, BRR_HIP_EINVAL, "offsets not monotone");
+            int64_t prev = -1;
+            for (uint64_t q = offs[l][j]; q < offs[l][j + 1]; ++q) {
+                const int64_t x = lists[l][q];
+                if (x <= prev || x >= (int64_t)N)
+                    return fail(c, BRR_HIP_EINVAL, "ragged index out of range or not ascending in column " +
+                                                       std::to_string(j));
+                prev = x;
+            }
+        }
+    }
+    HIP_TRY(c, hipSetDevice(c->device));
+    int rc;
+    if ((rc = dalloc(c, &c->d_ones, n1)) || (rc = dalloc(c, &c->d_twos, n2)) ||
+        (rc = dalloc(c, &c->d_miss, nm)) || (rc = dalloc(c, &c->d_ones_off, M + 1)) ||
+        (rc = dalloc(c, &c->d_twos_off, M + 1)) || (rc = dalloc(c, &c->d_miss_off, M + 1)) ||
+        (rc = dalloc(c, &c->d_stats, (size_t)4 * M)) ||
+        (rc = dalloc(c, &c->d_seg, (size_t)M * 3 * (c->P + 1))))
+        return rc;
+    if (n1) HIP_TRY(c, hipMemcpy(c->d_ones, ones, sizeof(int32_t) * n1, hipMemcpyHostToDevice));
+    if (n2) HIP_TRY(c, hipMemcpy(c->d_twos, twos, sizeof(int32_t) * n2, hipMemcpyHostToDevice));
+    if (nm) HIP_TRY(c, hipMemcpy(c->d_miss, miss, sizeof(int32_t) * nm, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->d_ones_off, ones_off, sizeof(uint64_t) * (M + 1), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->d_twos_off, twos_off, sizeof(uint64_t) * (M + 1), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->d_miss_off, miss_off, sizeof(uint64_t) * (M + 1), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->d_stats, stats, sizeof(double) * 4 * M, hipMemcpyHostToDevice));
+    const uint64_t total = (uint64_t)M * 3 * (c->P + 1);
+    const uint32_t blocks = (uint32_t)((total + 255) / 256);
+    build_seg_kernel<<<blocks, 256, 0, c->stream>>>(c->d_ones, c->d_ones_off, c->d_twos, c->d_twos_off,
+                                                   c->d_miss, c->d_miss_off, M, c->P, c->S, c->d_seg);
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->have_markers = true;
+    return BRR_HIP_OK;
+}
+
+int brr_hip_set_groups(brr_hip_ctx* c, const int32_t* group)
+{
+    if (!c || !group) return fail(c, BRR_HIP_EINVAL, "null argument");
+    const uint32_t M = c->cfg.num_snps;
+    for (uint32_t j = 0; j < M; ++j)
+        if (group[j] < 0 || group[j] >= (int32_t)c->cfg.num_groups)
+            return fail(c, BRR_HIP_EINVAL, "group label out of range");
+    HIP_TRY(c, hipSetDevice(c->device));
+    int rc = dalloc(c, &c->d_group, M);
+    if (rc) return rc;
+    HIP_TRY(c, hipMemcpy(c->d_group, group, sizeof(int32_t) * M, hipMemcpyHostToDevice));
+    return BRR_HIP_OK;
+}
+
+int brr_hip_set_model(brr_hip_ctx* c, const brr_hip_model* m)
+{
+    if (!c || !m || !m->cva) return fail(c, BRR_HIP_EINVAL, "null argument");
+    const uint32_t G = c->cfg.num_groups, K = c->cfg.num_components;
+    for (uint32_t t = 0; t < G * (K - 1); ++t)
+        if (!(m->cva[t] > 0.0)) return fail(c, BRR_HIP_EINVAL, "variance components must be > 0");
+    HIP_TRY(c, hipSetDevice(c->device));
+    c->model = *m;
+    c->cva.assign(m->cva, m->cva + G * (K - 1));
+    c->model.cva = c->cva.data();
+    HIP_TRY(c, hipMemcpy(c->d_cva, m->cva, sizeof(double) * G * (K - 1), hipMemcpyHostToDevice));
+    c->have_model = true;
+    return BRR_HIP_OK;
+}
+
+int brr_hip_set_state(brr_hip_ctx* c, const double* eps, const double* beta, const int32_t* comp)
+{
+    if (!c) return BRR_HIP_EINVAL;
+    HIP_TRY(c, hipSetDevice(c->device));
+    const uint32_t N = c->cfg.num_inds, M = c->cfg.num_snps;
+    if (eps) HIP_TRY(c, hipMemcpy(c->d_eps, eps, sizeof(double) * N, hipMemcpyHostToDevice));
+    if (beta) HIP_TRY(c, hipMemcpy(c->d_beta, beta, sizeof(double) * M, hipMemcpyHostToDevice));
+    if (comp) HIP_TRY(c, hipMemcpy(c->d_comp, comp, sizeof(int32_t) * M, hipMemcpyHostToDevice));
+    return BRR_HIP_OK;
+}
+
+int brr_hip_set_scalars(brr_hip_ctx* c, const brr_hip_scalars* s)
+{
+    if (!c || !s || !s->sigmaG || !s->pi) return fail(c, BRR_HIP_EINVAL, "null argument");
+    HIP_TRY(c, hipSetDevice(c->device));
+    const uint32_t G = c->cfg.num_groups, K = c->cfg.num_components;
+    const ScalarLayout SL{G, K};
+    std::vector<double> h(SL.size(), 0.0);
+    h[SL.mu()] = s->mu;
+    h[SL.sigmaE()] = s->sigmaE;
+    h[SL.epsSum()] = s->epsilon_sum;
+    for (uint32_t g = 0; g < G; ++g) h[SL.sigmaG() + g] = s->sigmaG[g];
+    for (uint32_t t = 0; t < G * K; ++t) h[SL.pi() + t] = s->pi[t];
+    HIP_TRY(c, hipMemcpy(c->d_scal, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+    return BRR_HIP_OK;
+}
+
+int brr_hip_get_state(brr_hip_ctx* c, double* eps, double* beta, int32_t* comp, double* acum)
+{
+    if (!c) return BRR_HIP_EINVAL;
+    HIP_TRY(c, hipSetDevice(c->device));
+    const uint32_t N = c->cfg.num_inds, M = c->cfg.num_snps;
+    if (eps) HIP_TRY(c, hipMemcpy(eps, c->d_eps, sizeof(double) * N, hipMemcpyDeviceToHost));
+    if (beta) HIP_TRY(c, hipMemcpy(beta, c->d_beta, sizeof(double) * M, hipMemcpyDeviceToHost));
+    if (comp) HIP_TRY(c, hipMemcpy(comp, c->d_comp, sizeof(int32_t) * M, hipMemcpyDeviceToHost));
+    if (acum) HIP_TRY(c, hipMemcpy(acum, c->d_acum, sizeof(double) * M, hipMemcpyDeviceToHost));
+    return BRR_HIP_OK;
+}
+
+int brr_hip_get_scalars(brr_hip_ctx* c, brr_hip_scalars* s)
+{
+    if (!c || !s) return BRR_HIP_EINVAL;
+    HIP_TRY(c, hipSetDevice(c->device));
+    const uint32_t G = c->cfg.num_groups, K = c->cfg.num_components;
+    const ScalarLayout SL{G, K};
+    std::vector<double> h(SL.size());
+    HIP_TRY(c, hipMemcpy(h.data(), c->d_scal, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+    s->mu = h[SL.mu()];
+    s->sigmaE = h[SL.sigmaE()];
+    s->epsilon_sum = h[SL.epsSum()];
+    if (s->sigmaG) for (uint32_t g = 0; g < G; ++g) s->sigmaG[g] = h[SL.sigmaG() + g];
+    if (s->pi) for (uint32_t t = 0; t < G * K; ++t) s->pi[t] = h[SL.pi() + t];
+    if (s->v) for (uint32_t t = 0; t < G * K; ++t) s->v[t] = h[SL.v() + t];
+    if (s->betasqn) for (uint32_t g = 0; g < G; ++g) s->betasqn[g] = h[SL.betasqn() + g];
+    return BRR_HIP_OK;
+}
+
+int brr_hip_sweep(brr_hip_ctx* c, const uint32_t* order, const double* u, const double* z)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!order || !u || !z) return fail(c, BRR_HIP_EINVAL, "null argument");
+    const uint32_t M = c->cfg.num_snps;
+    std::vector<uint8_t> seen(M, 0);
+    for (uint32_t i = 0; i < M; ++i) {
+        if (order[i] >= M || seen[order[i]]) return fail(c, BRR_HIP_EINVAL, "order is not a permutation of 0..M-1");
+        seen[order[i]] = 1;
+    }
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (c->order_cap < M) {
+        if ((rc = dalloc(c, &c->d_order, M))) return rc;
+        c->order_cap = M;
+    }
+    HIP_TRY(c, hipMemcpy(c->d_order, order, sizeof(uint32_t) * M, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->d_u, u, sizeof(double) * M, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->d_z, z, sizeof(double) * M, hipMemcpyHostToDevice));
+    SweepParams p = base_params(c);
+    p.mode = kModeSweep;
+    p.iterations = 1;
+    p.order = c->d_order;
+    p.u = c->d_u;
+    p.z = c->d_z;
+    rc = launch(c, p);
+    c->stats.markers = M;
+    return rc;
+}
+
+int brr_hip_run_chain(brr_hip_ctx* c, uint64_t seed, uint32_t chain, uint32_t first_iteration,
+                      uint32_t iterations, const uint32_t* orders, double* hist)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!orders || iterations == 0) return fail(c, BRR_HIP_EINVAL, "null orders or zero iterations");
+    const uint32_t M = c->cfg.num_snps, G = c->cfg.num_groups;
+    const size_t total = (size_t)iterations * M;
+    std::vector<uint8_t> seen(M);
+    for (uint32_t it = 0; it < iterations; ++it) {
+        std::fill(seen.begin(), seen.end(), 0);
+        const uint32_t* o = orders + (size_t)it * M;
+        for (uint32_t i = 0; i < M; ++i) {
+            if (o[i] >= M || seen[o[i]]) return fail(c, BRR_HIP_EINVAL, "order is not a permutation of 0..M-1");
+            seen[o[i]] = 1;
+        }
+    }
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (c->order_cap < total) {
+        if ((rc = dalloc(c, &c->d_order, total))) return rc;
+        c->order_cap = total;
+    }
+    const size_t hsz = (size_t)iterations * (4 + G);
+    if (c->hist_cap < hsz) {
+        if ((rc = dalloc(c, &c->d_hist, hsz))) return rc;
+        c->hist_cap = hsz;
+    }
+    HIP_TRY(c, hipMemcpy(c->d_order, orders, sizeof(uint32_t) * total, hipMemcpyHostToDevice));
+    SweepParams p = base_params(c);
+    p.mode = kModeChain;
+    p.iterations = iterations;
+    p.iter_base = first_iteration;
+    p.chain = chain;
+    p.seed = seed;
+    p.order = c->d_order;
+    p.hist = c->d_hist;
+    rc = launch(c, p);
+    c->stats.markers = total;
+    if (rc) return rc;
+    if (hist) HIP_TRY(c, hipMemcpy(hist, c->d_hist, sizeof(double) * hsz, hipMemcpyDeviceToHost));
+    return BRR_HIP_OK;
+}
+
+int brr_hip_compute_num(brr_hip_ctx* c, uint32_t marker, double beta_old, double* num)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!num || marker >= c->cfg.num_snps) return fail(c, BRR_HIP_EINVAL, "bad marker or null output");
+    HIP_TRY(c, hipSetDevice(c->device));
+    SweepParams p = base_params(c);
+    compute_num_kernel<<<1, kThreads, 0, c->stream>>>(p, marker, beta_old, c->d_tmp);
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(num, c->d_tmp, sizeof(double), hipMemcpyDeviceToHost));
+    return BRR_HIP_OK;
+}
+
+int brr_hip_epsilon_change(brr_hip_ctx* c, uint32_t marker, double beta_old, double beta)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (marker >= c->cfg.num_snps) return fail(c, BRR_HIP_EINVAL, "bad marker");
+    HIP_TRY(c, hipSetDevice(c->device));
+    SweepParams p = base_params(c);
+    const uint32_t blocks = (c->cfg.num_inds + kThreads - 1) / kThreads;
+    epsilon_change_kernel<<<blocks, kThreads, 0, c->stream>>>(p, marker, beta_old, beta);
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return BRR_HIP_OK;
+}
+
+int brr_hip_last_stats(const brr_hip_ctx* c, brr_hip_stats* s)
+{
+    if (!c || !s) return BRR_HIP_EINVAL;
+    *s = c->stats;
+    return BRR_HIP_OK;
+}
+
+} // extern "C"

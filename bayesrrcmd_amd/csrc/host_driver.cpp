@@ -1,0 +1,81 @@
+// host_driver.cpp -- the host half of BayesRBase::runGibbs around the device sweep.
+//
+// The reference keeps these on the host thread that calls exec():
+//   init                 BayesRBase.cpp:39-99
+//   random numbers       prepareForAnylsis, BayesRBase.cpp:101-110 (per marker id)
+//   marker order         std::random_shuffle, BayesRBase.cpp:198
+// They are restated here over the counter-based stream of brr_rng.h so a host
+// driver and the device-resident chain consume the same numbers.
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+
+#include "brr_hip.h"
+#include "brr_rng.h"
+
+extern "C" {
+
+int brr_bayesr_init(const double* y, uint32_t N, uint32_t G, uint32_t K, uint64_t seed, uint32_t chain,
+                    double* eps, brr_hip_scalars* s)
+{
+    if (!y || !eps || !s || !s->sigmaG || !s->pi || N < 2 || G < 1 || K < 2) return BRR_HIP_EINVAL;
+    // m_priorPi: 0.5 for the zero component, 0.5/K for the others
+    const double priorPi = 0.5 / K;
+    for (uint32_t g = 0; g < G; ++g) {
+        for (uint32_t k = 0; k < K; ++k) s->pi[g * K + k] = priorPi;
+        s->pi[g * K] = 0.5;
+        // m_sigmaG ~ U(0,1), the first draws of the chain
+        s->sigmaG[g] = brr::uniform01(seed, brr::kInitIteration, g, chain);
+        if (s->v)
+            for (uint32_t k = 0; k < K; ++k) s->v[g * K + k] = 0.0;
+        if (s->betasqn) s->betasqn[g] = 0.0;
+    }
+    s->mu = 0.0;
+    // m_y = (y - mean) / sqrt(|y - mean|^2 / (N - 1)); eps = m_y - mu
+    double ysum = 0.0;
+    for (uint32_t i = 0; i < N; ++i) ysum += y[i];
+    const double ymean = ysum / (double)N;
+    for (uint32_t i = 0; i < N; ++i) eps[i] = y[i] - ymean;
+    double sq = 0.0;
+    for (uint32_t i = 0; i < N; ++i) sq += eps[i] * eps[i];
+    const double scale = std::sqrt(sq / ((double)(N - 1u)));
+    for (uint32_t i = 0; i < N; ++i) eps[i] /= scale;
+    for (uint32_t i = 0; i < N; ++i) eps[i] = eps[i] - s->mu;
+    double e2 = 0.0, es = 0.0;
+    for (uint32_t i = 0; i < N; ++i) e2 += eps[i] * eps[i];
+    for (uint32_t i = 0; i < N; ++i) es += eps[i];
+    s->sigmaE = e2 / (double)N * 0.5;
+    s->epsilon_sum = es;
+    return BRR_HIP_OK;
+}
+
+int brr_shuffle(uint64_t seed, uint32_t iteration, uint32_t chain, uint32_t M, uint32_t* order, int mode)
+{
+    if (!order) return BRR_HIP_EINVAL;
+    // libstdc++ random_shuffle: for i in [1, M): swap(order[i], order[r % (i + 1)])
+    for (uint32_t i = 1; i < M; ++i) {
+        uint64_t r;
+        if (mode == 1) {
+            r = (uint64_t)std::rand();
+        } else {
+            const brr::U4 x = brr::draw_block(seed, i, iteration, brr::kStreamShuffle, chain);
+            r = ((uint64_t)x.x[0] << 32) | x.x[1];
+        }
+        const uint32_t j = (uint32_t)(r % ((uint64_t)i + 1));
+        if (j != i) {
+            const uint32_t t = order[i];
+            order[i] = order[j];
+            order[j] = t;
+        }
+    }
+    return BRR_HIP_OK;
+}
+
+int brr_marker_uz(uint64_t seed, uint32_t iteration, uint32_t chain, uint32_t M, double* u, double* z)
+{
+    if (!u || !z) return BRR_HIP_EINVAL;
+    for (uint32_t j = 0; j < M; ++j) brr::marker_uz(seed, iteration, j, chain, &u[j], &z[j]);
+    return BRR_HIP_OK;
+}
+
+} // extern "C"

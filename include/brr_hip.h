@@ -1,0 +1,208 @@
+/*
+ * brr_hip.h -- C ABI of the MI355X-native BayesR marker sweep (libbrr_hip.so).
+ *
+ * This is the drop-in boundary a host C++ build of BayesRRcmd links against.
+ * Plain C types only: pointers are host memory owned by the caller and are
+ * only touched during the (synchronous) call. Every entry point returns a
+ * status code (0 = ok) and never throws; brr_hip_last_error() holds the text
+ * of the last failure of a context. One context is not re-entrant but may be
+ * used from any single thread at a time (BayesRBase::processColumn is called
+ * from one TBB serial node at a time, limitsequencegraph.cpp:46-55).
+ *
+ * Reference interfaces replaced (paths relative to the BayesRRcmd tree):
+ *   - AnalysisGraph::exec(Analysis*, N, M, order)  src/analysisgraph.hpp:18-21
+ *     as implemented by Sequential (src/sequential.cpp:10-27) and
+ *     LimitSequenceGraph (src/limitsequencegraph.cpp:79-106), each driving
+ *     BayesRBase::processColumn (src/BayesRBase.cpp:287-392)  -> brr_hip_sweep
+ *   - BayesRKernel::computeNum / calculateEpsilonChange
+ *     (src/bayesrkernel.h:11-15; dense src/densebayesrkernel.cpp:10-20,
+ *     ragged src/sparsebayesrkernel.cpp:10-34 + src/raggedbayesrkernel.cpp:10-35)
+ *                                      -> brr_hip_compute_num / brr_hip_epsilon_change
+ *   - Marker storage of DenseMarker (src/densemarker.h:7-20, the column as
+ *     a 4-entry table over the PLINK 2-bit codes) and RaggedSparseMarker
+ *     (src/raggedsparsemarker.h:7-23)  -> brr_hip_upload_dense / _ragged
+ *   - The per-iteration loop of BayesRBase::runGibbs (src/BayesRBase.cpp:186-253)
+ *     with updateMu (src/DenseBayesRRmz.cpp:36-42, src/SparseBayesRRG.cpp:106-113)
+ *     and the sigmaE / sigmaG / pi draws (src/BayesRBase.cpp:237-247), held on
+ *     the device for a whole chain  -> brr_hip_run_chain
+ */
+#ifndef BRR_HIP_H
+#define BRR_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BRR_HIP_ABI_VERSION 1
+
+enum brr_hip_status {
+    BRR_HIP_OK = 0,
+    BRR_HIP_EINVAL = 1,   /* bad argument or size */
+    BRR_HIP_ENOMEM = 2,   /* device allocation failed */
+    BRR_HIP_EDEVICE = 3,  /* HIP runtime error or no device */
+    BRR_HIP_ESTATE = 4,   /* call out of order (e.g. sweep before upload) */
+    BRR_HIP_ETIMEOUT = 5  /* a device-side wait exceeded its bound */
+};
+
+/* Marker layouts (PreprocessDataType, src/common.h:27-32). */
+enum brr_hip_layout {
+    BRR_HIP_LAYOUT_DENSE_2BIT = 1, /* PreprocessDataType::Dense, columns kept as PLINK 2-bit */
+    BRR_HIP_LAYOUT_RAGGED = 2      /* PreprocessDataType::SparseRagged */
+};
+
+typedef struct brr_hip_ctx brr_hip_ctx;
+
+typedef struct {
+    uint32_t num_inds;       /* N, Data::numInds */
+    uint32_t num_snps;       /* M, Data::numSnps */
+    uint32_t num_components; /* K = Options::S.cols() + 1 (zero component included), 2..8 */
+    uint32_t num_groups;     /* G = Data::numGroups, 1..32 */
+    uint32_t layout;         /* brr_hip_layout */
+    uint32_t workgroups;     /* persistent workgroups (0 = choose) */
+    uint32_t block_markers;  /* markers decided per speculation block, 1..64 (0 = choose) */
+    uint32_t device;         /* HIP device ordinal */
+} brr_hip_config;
+
+/* Model constants: Options::S (G x (K-1), row-major) and the priors
+ * v0E, s02E, v0G, s02G (src/options.hpp; BayesRBase.cpp:31-34). */
+typedef struct {
+    const double* cva;
+    double v0E, s02E, v0G, s02G;
+} brr_hip_model;
+
+/* Chain scalars held by BayesRBase (src/BayesRBase.hpp:63-86). In set calls the
+ * v / betasqn arrays are ignored (a sweep clears them, BayesRBase.cpp:200-202). */
+typedef struct {
+    double mu;           /* m_mu */
+    double sigmaE;       /* m_sigmaE */
+    double epsilon_sum;  /* m_epsilonSum (ragged layout only) */
+    double* sigmaG;      /* m_sigmaG, G */
+    double* pi;          /* m_pi, G x K */
+    double* v;           /* m_v, G x K (out) */
+    double* betasqn;     /* m_betasqnG, G (out) */
+} brr_hip_scalars;
+
+/* Counters of the last sweep / chain launch. */
+typedef struct {
+    uint64_t markers;      /* marker-updates performed */
+    uint64_t rounds;       /* device exchange rounds */
+    uint64_t changes;      /* markers whose beta changed (epsilon updated) */
+    double kernel_ms;      /* device time of the launch, HIP events */
+    uint32_t workgroups;   /* persistent workgroups used */
+    uint32_t block_markers;
+    uint32_t slice;        /* individuals per workgroup */
+    uint32_t reserved;
+} brr_hip_stats;
+
+int brr_hip_abi_version(void);
+int brr_hip_device_count(int* count);
+
+/* Create a context and allocate the per-individual / per-marker device state. */
+int brr_hip_create(const brr_hip_config* cfg, brr_hip_ctx** out);
+void brr_hip_destroy(brr_hip_ctx* ctx);
+const char* brr_hip_last_error(const brr_hip_ctx* ctx);
+
+/* Dense columns: M SNP-major PLINK columns of ceil(N/4) bytes each, column j
+ * starting at bed + j * column_stride (the .bed payload after the 3 magic
+ * bytes has column_stride = ceil(N/4)). lut: M x 4 doubles, the standardised
+ * column value for dosage 0, 1, 2 and for a missing call, as DenseMarkerBuilder
+ * computes them (src/densemarkerbuilder.cpp:43-60). */
+int brr_hip_upload_dense(brr_hip_ctx* ctx, const uint8_t* bed, uint64_t column_stride,
+                         const double* lut);
+
+/* Ragged columns: CSR lists of individuals with genotype 1, 2 and missing,
+ * ascending within a column; *_off have M+1 entries. stats: M x 4 doubles
+ * {mean, sd, sqrdZ, Zsum} (src/sparsemarker.h:8-11). */
+int brr_hip_upload_ragged(brr_hip_ctx* ctx,
+                          const int32_t* ones, const uint64_t* ones_off,
+                          const int32_t* twos, const uint64_t* twos_off,
+                          const int32_t* miss, const uint64_t* miss_off,
+                          const double* stats);
+
+/* Group label per marker (Data::G, src/data.cpp:424-453); default all 0. */
+int brr_hip_set_groups(brr_hip_ctx* ctx, const int32_t* group);
+int brr_hip_set_model(brr_hip_ctx* ctx, const brr_hip_model* model);
+
+/* State upload / download. Any pointer may be NULL to skip that array. */
+int brr_hip_set_state(brr_hip_ctx* ctx, const double* epsilon, const double* beta,
+                      const int32_t* components);
+int brr_hip_set_scalars(brr_hip_ctx* ctx, const brr_hip_scalars* s);
+int brr_hip_get_state(brr_hip_ctx* ctx, double* epsilon, double* beta,
+                      int32_t* components, double* acum);
+int brr_hip_get_scalars(brr_hip_ctx* ctx, brr_hip_scalars* s);
+
+/* One AnalysisGraph::exec: processColumn over order[0..M) in sequence.
+ * u, z: the per-marker uniform and standard normal of this iteration, indexed
+ * by marker id (BayesRBase.cpp:343-344). Uses and updates the device state. */
+int brr_hip_sweep(brr_hip_ctx* ctx, const uint32_t* order, const double* u, const double* z);
+
+/* A device-resident chain: `iterations` full runGibbs iterations (updateMu,
+ * sweep, sigmaE / sigmaG / pi draws) in one persistent launch, drawing u, z
+ * and the scalars from the counter-based stream (seed, chain). orders holds
+ * the marker order of each iteration (iterations x M). hist (may be NULL)
+ * receives iterations x (4 + G) doubles {mu, sigmaE, m0, |eps|^2, sigmaG[G]}. */
+int brr_hip_run_chain(brr_hip_ctx* ctx, uint64_t seed, uint32_t chain,
+                      uint32_t first_iteration, uint32_t iterations,
+                      const uint32_t* orders, double* hist);
+
+/* Per-marker kernel calls on the device epsilon (correctness path: one launch
+ * per call). compute_num: BayesRKernel::computeNum with the context's
+ * epsilon_sum. epsilon_change: applies calculateEpsilonChange(beta_old, beta)
+ * to the device epsilon (and epsilon_sum for ragged). */
+int brr_hip_compute_num(brr_hip_ctx* ctx, uint32_t marker, double beta_old, double* num);
+int brr_hip_epsilon_change(brr_hip_ctx* ctx, uint32_t marker, double beta_old, double beta);
+
+int brr_hip_last_stats(const brr_hip_ctx* ctx, brr_hip_stats* stats);
+
+/* ---- host-side marker building (MarkerBuilder semantics, no device) ---- */
+
+/* PLINK .bed magic check (src/preprocessgraph.cpp:195-200): 0 if ok. */
+int brr_bed_check_magic(const uint8_t* header3);
+
+/* DenseMarkerBuilder (src/densemarkerbuilder.cpp:23-60) over a SNP-major
+ * column: lut[4] for dosage 0,1,2,missing. */
+int brr_dense_lut(const uint8_t* column, uint32_t num_inds, double* lut);
+
+/* RaggedSparseMarkerBuilder (src/raggedsparsemarkerbuilder.cpp:25-55): counts
+ * of genotype 1 / 2 / missing for each of M columns (pass 1). */
+int brr_ragged_count(const uint8_t* bed, uint64_t column_stride, uint32_t num_inds,
+                     uint32_t num_snps, uint64_t* ones_off, uint64_t* twos_off,
+                     uint64_t* miss_off, uint32_t threads);
+/* Pass 2: fill the lists (offsets from pass 1, prefix-summed by the call) and
+ * stats (M x 4). Returns BRR_HIP_EINVAL if a column is invalid
+ * (src/raggedsparsemarker.cpp:68-77). */
+int brr_ragged_fill(const uint8_t* bed, uint64_t column_stride, uint32_t num_inds,
+                    uint32_t num_snps, const uint64_t* ones_off, const uint64_t* twos_off,
+                    const uint64_t* miss_off, int32_t* ones, int32_t* twos, int32_t* miss,
+                    double* stats, uint32_t threads);
+/* All M dense tables at once (threads host workers). */
+int brr_dense_luts(const uint8_t* bed, uint64_t column_stride, uint32_t num_inds,
+                   uint32_t num_snps, double* lut, uint32_t threads);
+
+/* ---- host half of runGibbs over the counter-based stream ---- */
+
+/* BayesRBase::init (src/BayesRBase.cpp:39-99): standardise y (N doubles, the
+ * phenotype as the reference stores it, i.e. rounded through float), eps = y,
+ * sigmaE = |eps|^2 / N / 2, epsilon_sum = sum(eps), pi = prior, sigmaG ~ U(0,1).
+ * s->sigmaG (G) and s->pi (G x K) must be provided. */
+int brr_bayesr_init(const double* y, uint32_t num_inds, uint32_t num_groups,
+                    uint32_t num_components, uint64_t seed, uint32_t chain,
+                    double* epsilon, brr_hip_scalars* s);
+
+/* std::random_shuffle of order (src/BayesRBase.cpp:198; libstdc++ rule
+ * j = r % (i + 1), i = 1..M-1) with r from the counter-based stream (mode 0)
+ * or from the C library rand() (mode 1, the reference's stock source). */
+int brr_shuffle(uint64_t seed, uint32_t iteration, uint32_t chain, uint32_t num_snps,
+                uint32_t* order, int mode);
+
+/* BayesRBase::prepareForAnylsis (src/BayesRBase.cpp:101-110): u ~ U[0,1) and
+ * z ~ N(0,1) for every marker id of one iteration. */
+int brr_marker_uz(uint64_t seed, uint32_t iteration, uint32_t chain, uint32_t num_snps,
+                  double* u, double* z);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,201 @@
+/*
+ * brr_oracle.h -- CPU ORACLE (test infrastructure only).
+ *
+ * A plain-C, FP64, dependency-free restatement of the reference BayesRRcmd
+ * per-marker Gibbs sweep. It exists to CHECK the HIP product path: only the
+ * test suite, the smoke check and the CPU-baseline leg of the benchmark may
+ * load it. Product code never links or calls it.
+ *
+ * Every function cites the reference file:line it restates. Randomness is
+ * injected: the reference's Boost mt19937 / glibc rand() streams are replaced
+ * by a counter-based Philox4x32-10 stream whose exact definition is given in
+ * oracle/rng.c (the product implements the same stream independently).
+ *
+ * Pinning status of each part is listed in oracle/README.md.
+ */
+#ifndef BRR_ORACLE_H
+#define BRR_ORACLE_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- RNG ---- */
+
+/* Philox4x32-10 (Salmon et al., SC'11). out[4] <- bijection(ctr[4], key[2]). */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* Stream tags (ctr word 2, low 8 bits). */
+enum { ORC_STREAM_MARKER = 1, ORC_STREAM_SHUFFLE = 2, ORC_STREAM_SCALAR = 3 };
+
+/* Deterministic elementary functions built from + - * / only (bit-identical on
+ * any IEEE-754 binary64 platform without contraction). */
+double orc_det_log(double x);
+double orc_det_exp(double x);
+/* Inverse standard normal CDF, Wichura AS241 (PPND16). */
+double orc_inv_norm_cdf(double p);
+
+/* Per-marker (u, z) pair for (seed, iteration, marker id, chain). */
+void orc_marker_uz(uint64_t seed, uint32_t iteration, uint32_t marker, uint32_t chain,
+                   double* u, double* z);
+void orc_fill_marker_uz(uint64_t seed, uint32_t iteration, uint32_t chain, uint32_t M,
+                        double* u, double* z);
+
+/* In-place Fisher-Yates in the rule of libstdc++ std::random_shuffle
+ * (bits/stl_algo.h: j = r % (i + 1) for i = 1..M-1) with r drawn from the
+ * Philox shuffle stream. */
+void orc_shuffle_philox(uint64_t seed, uint32_t iteration, uint32_t chain, uint32_t M,
+                        uint32_t* order);
+/* Same rule with r = rand() from the C library (the reference's stock source;
+ * the caller seeds it with srand). */
+void orc_shuffle_libc(uint32_t M, uint32_t* order);
+
+/* Scalar draws (distributions_boost.cpp:22-65 parameterisations). draw_id
+ * selects an independent substream. */
+double orc_uniform01(uint64_t seed, uint32_t iteration, uint32_t draw_id, uint32_t chain);
+double orc_norm_rng(uint64_t seed, uint32_t iteration, uint32_t draw_id, uint32_t chain,
+                    double mean, double var);
+double orc_rgamma(uint64_t seed, uint32_t iteration, uint32_t draw_id, uint32_t chain,
+                  double shape, double scale);
+double orc_inv_scaled_chisq_rng(uint64_t seed, uint32_t iteration, uint32_t draw_id,
+                                uint32_t chain, double dof, double scale);
+/* Dirichlet over K components; draw ids first_id .. first_id+K-1. */
+void orc_dirichlet_rng(uint64_t seed, uint32_t iteration, uint32_t first_id, uint32_t chain,
+                       int K, const double* alpha, double* out);
+
+/* ------------------------------------------------------ markers (a1-a4) -- */
+
+enum { ORC_DENSE = 1, ORC_RAGGED = 2 };
+
+/* a1: decode one SNP-major PLINK .bed column (ceil(N/4) bytes) into
+ * (allele1, allele2) pairs, preprocessgraph.cpp:34-57. a1[i], a2[i] in {0,1}. */
+void orc_decode_bed_column(const uint8_t* col, uint32_t N, uint8_t* a1, uint8_t* a2);
+/* Returns 0 if the three magic bytes are 0x6c 0x1b 0x01 (preprocessgraph.cpp:195-200). */
+int orc_check_bed_magic(const uint8_t* header3);
+
+/* a3: DenseMarkerBuilder (densemarkerbuilder.cpp:23-60): Cx[N]. */
+void orc_dense_build(const uint8_t* col, uint32_t N, double* Cx);
+/* Dense column as a 4-entry table indexed by dosage 0,1,2 and 3 = missing; the
+ * column value for individual i is lut[dosage_i]. Values equal orc_dense_build
+ * element-for-element. */
+void orc_dense_lut(const uint8_t* col, uint32_t N, double lut[4]);
+
+/* a2: RaggedSparseMarkerBuilder + SparseMarker::updateStatistics
+ * (raggedsparsemarkerbuilder.cpp:25-55, sparsemarker.cpp:8-19). Fills the
+ * three ascending index lists (caller-sized to N) and returns counts.
+ * Returns 1 if valid (raggedsparsemarker.cpp:68-77), 0 otherwise. */
+int orc_ragged_build(const uint8_t* col, uint32_t N,
+                     int32_t* ones, uint64_t* n_ones,
+                     int32_t* twos, uint64_t* n_twos,
+                     int32_t* miss, uint64_t* n_miss,
+                     double stats[4] /* mean, sd, sqrdZ, Zsum */);
+
+/* a4: serialised RaggedSparseMarker (raggedsparsemarker.cpp:43-66,
+ * sparsemarker.cpp:43-58): 4 x f64 then, for ones/twos/missing, a u64 size
+ * followed by int32[size]. Returns the byte count written (buf may be NULL to
+ * query the size). */
+size_t orc_ragged_serialise(const double stats[4],
+                            const int32_t* ones, uint64_t n_ones,
+                            const int32_t* twos, uint64_t n_twos,
+                            const int32_t* miss, uint64_t n_miss, uint8_t* buf);
+
+/* ---------------------------------------------------------- BayesR (a5-a10) */
+
+typedef struct {
+    uint32_t N, M;
+    int layout;                 /* ORC_DENSE or ORC_RAGGED */
+    /* dense: packed SNP-major columns (no magic), M x ceil(N/4) bytes, and
+     * per-marker 4-entry tables (orc_dense_lut) */
+    const uint8_t* bed;
+    const double* lut;          /* M x 4 */
+    /* ragged: CSR-style lists; offsets have M+1 entries */
+    const int32_t* ones; const uint64_t* ones_off;
+    const int32_t* twos; const uint64_t* twos_off;
+    const int32_t* miss; const uint64_t* miss_off;
+    const double* stats;        /* M x 4 {mean, sd, sqrdZ, Zsum} */
+} orc_data;
+
+typedef struct {
+    int K;                      /* components including the zero one */
+    int G;                      /* groups */
+    const double* S;            /* G x (K-1) */
+    const int32_t* group;       /* M labels; NULL = all group 0 */
+    double v0E, s02E, v0G, s02G;
+} orc_model;
+
+typedef struct {
+    /* chain state */
+    double* eps;                /* N */
+    double* beta;               /* M */
+    int32_t* comp;              /* M */
+    double* acum;               /* M */
+    double mu, sigmaE, epsSum;
+    double* sigmaG;             /* G */
+    double* pi;                 /* G x K */
+    double* v;                  /* G x K */
+    double* betasqn;            /* G */
+} orc_state;
+
+/* Optional per-step trace of a sweep (indexed by position in the order). */
+typedef struct {
+    double* num; double* acum; int32_t* comp; double* beta;
+} orc_trace;
+
+/* a5: BayesRKernel::computeNum. Dense: densebayesrkernel.cpp:10-15.
+ * Ragged: sparsebayesrkernel.cpp:24-27 with raggedbayesrkernel.cpp:32-35. */
+double orc_compute_num(const orc_data* d, uint32_t j, const double* eps,
+                       double beta_old, double epsSum);
+
+/* a7: residual update, applied only when not skipped. Dense:
+ * densebayesrkernel.cpp:17-20; ragged raggedbayesrkernel.cpp:10-30 plus the
+ * epsilonSum update sparsebayesrkernel.cpp:29-34 / SparseBayesRRG.cpp:84-92. */
+void orc_apply_epsilon_change(const orc_data* d, uint32_t j, double* eps,
+                              double* epsSum, double beta_old, double beta_new);
+
+/* a6 + a5 + a7: BayesRBase::processColumn (BayesRBase.cpp:287-392). */
+void orc_process_column(const orc_data* d, const orc_model* m, orc_state* s,
+                        uint32_t j, double u, double z,
+                        double* num_out, double* acum_out);
+
+/* a10: Sequential::exec (sequential.cpp:23-26): processColumn over order.
+ * u, z are indexed by marker id (BayesRBase.cpp:343-344). Clears v/betasqn
+ * first as runGibbs does (BayesRBase.cpp:200-202). */
+void orc_sweep(const orc_data* d, const orc_model* m, orc_state* s,
+               const uint32_t* order, const double* u, const double* z,
+               orc_trace* trace);
+
+/* a8: BayesRBase::init (BayesRBase.cpp:39-99). y is the phenotype as stored
+ * by the reference (already rounded through float, data.cpp:357-359). */
+void orc_init(const orc_data* d, const orc_model* m, orc_state* s,
+              const double* y, uint64_t seed, uint32_t chain);
+
+/* a8: updateMu. Dense: DenseBayesRRmz.cpp:36-42; ragged SparseBayesRRG.cpp:106-113. */
+void orc_update_mu(const orc_data* d, orc_state* s, uint64_t seed, uint32_t iteration,
+                   uint32_t chain);
+
+/* a8: the post-sweep draws of sigmaE, sigmaG, pi (BayesRBase.cpp:237-247). */
+void orc_post_sweep(const orc_data* d, const orc_model* m, orc_state* s,
+                    uint64_t seed, uint32_t iteration, uint32_t chain);
+
+/* a8-a10: runGibbs (BayesRBase.cpp:130-285) for `iterations` iterations with
+ * the Philox streams. order (M) carries the marker order across iterations
+ * (std::iota on entry if order_init != 0). shuffle_mode 0 = Philox, 1 = libc
+ * rand(). Per-iteration scalars go to hist[it * (4 + G)] =
+ * {mu, sigmaE, m0_total, epsSqNorm, sigmaG[0..G-1]} when hist != NULL. */
+void orc_run_gibbs(const orc_data* d, const orc_model* m, orc_state* s,
+                   const double* y, uint64_t seed, uint32_t chain,
+                   uint32_t iterations, uint32_t* order, int shuffle_mode,
+                   double* hist);
+
+/* Scalar-draw ids of the Philox scalar stream. */
+enum { ORC_DRAW_MU = 0, ORC_DRAW_SIGMAE = 1, ORC_DRAW_SIGMAG0 = 2 };
+#define ORC_DRAW_PI0(G) (2u + (uint32_t)(G))
+#define ORC_INIT_ITERATION 0xFFFFFFFFu
+
+#ifdef __cplusplus
+}
+#endif
+#endif

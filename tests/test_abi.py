@@ -1,0 +1,85 @@
+"""The C-ABI library loads and exports every symbol that include/*.h declares.
+
+Runs without a GPU: no compute entry point is called here except the host-only
+builders; brr_hip_create must fail cleanly (not crash) when no device exists.
+"""
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+
+import bayesrrcmd_amd as brr
+from bayesrrcmd_amd import _abi
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def declared_functions():
+    names = set()
+    for h in sorted((ROOT / "include").glob("*.h")):
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[\w\s\*]+?\b(brr_\w+)\s*\(", text, flags=re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_every_declared_symbol_is_exported():
+    lib = brr.lib()
+    names = declared_functions()
+    assert len(names) >= 25
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+    # and every declared function has a ctypes signature in the Python face
+    assert names <= set(_abi.SIGNATURES), sorted(names - set(_abi.SIGNATURES))
+
+
+def test_abi_version():
+    assert brr.lib().brr_hip_abi_version() == 1
+
+
+def test_create_rejects_bad_config():
+    lib = brr.lib()
+    ctx = _abi.CTX()
+    bad = [
+        _abi.Config(0, 10, 4, 1, 2, 0, 0, 0),     # N < 2
+        _abi.Config(10, 0, 4, 1, 2, 0, 0, 0),     # M = 0
+        _abi.Config(10, 10, 1, 1, 2, 0, 0, 0),    # K < 2
+        _abi.Config(10, 10, 9, 1, 2, 0, 0, 0),    # K > 8
+        _abi.Config(10, 10, 4, 0, 2, 0, 0, 0),    # G = 0
+        _abi.Config(10, 10, 4, 1, 7, 0, 0, 0),    # unknown layout
+        _abi.Config(10, 10, 4, 1, 2, 0, 65, 0),   # block > 64
+    ]
+    for cfg in bad:
+        assert lib.brr_hip_create(C.byref(cfg), C.byref(ctx)) == _abi.EINVAL
+
+
+def test_create_without_device_fails_cleanly():
+    lib = brr.lib()
+    n = C.c_int(-1)
+    assert lib.brr_hip_device_count(C.byref(n)) == 0
+    if n.value == 0:
+        ctx = _abi.CTX()
+        cfg = _abi.Config(100, 10, 4, 1, 2, 0, 0, 0)
+        assert lib.brr_hip_create(C.byref(cfg), C.byref(ctx)) == _abi.EDEVICE
+        assert not ctx.value
+
+
+def test_null_context_calls_are_rejected():
+    lib = brr.lib()
+    assert lib.brr_hip_sweep(None, None, None, None) == _abi.EINVAL
+    assert lib.brr_hip_get_state(None, None, None, None, None) == _abi.EINVAL
+    assert lib.brr_hip_last_error(None) == b"null context"
+    lib.brr_hip_destroy(None)
+
+
+def test_host_init_matches_reference_formulas():
+    rng = np.random.default_rng(0)
+    y = rng.normal(3.0, 2.0, 1000).astype(np.float32).astype(np.float64)
+    eps, s = brr.bayesr_init(y, 2, 4, seed=1)
+    ys = (y - y.mean())
+    ys /= np.sqrt(ys @ ys / (y.size - 1))
+    assert np.allclose(eps, ys, atol=1e-14)
+    assert abs(s["sigmaE"] - (eps @ eps) / y.size * 0.5) < 1e-14
+    assert np.allclose(s["pi"], [[0.5, 0.125, 0.125, 0.125]] * 2)
+    assert np.all((s["sigmaG"] >= 0) & (s["sigmaG"] < 1))
