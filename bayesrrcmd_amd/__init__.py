@@ -265,7 +265,13 @@ class HipSweep:
     def epsilon_change(self, marker: int, beta_old: float, beta: float):
         self._ck(lib().brr_hip_epsilon_change(self.ctx, marker, beta_old, beta), "brr_hip_epsilon_change")
 
+    PHASES = ("block_load", "dots", "exchange", "decisions", "commit_update", "iteration_scalars")
+
     def stats(self) -> dict:
         s = _abi.Stats()
         self._ck(lib().brr_hip_last_stats(self.ctx, C.byref(s)), "brr_hip_last_stats")
-        return {f: getattr(s, f) for f, _ in _abi.Stats._fields_ if f != "reserved"}
+        out = {f: getattr(s, f) for f, _ in _abi.Stats._fields_ if f != "reserved"}
+        ph = np.zeros(6)
+        self._ck(lib().brr_hip_last_phase_ms(self.ctx, _p(ph, C.c_double), 6), "brr_hip_last_phase_ms")
+        out["phase_ms"] = dict(zip(self.PHASES, ph.tolist()))
+        return out

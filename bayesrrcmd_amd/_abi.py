@@ -67,6 +67,7 @@ SIGNATURES = {
     "brr_hip_compute_num": (C.c_int, [CTX, C.c_uint32, C.c_double, P_f64]),
     "brr_hip_epsilon_change": (C.c_int, [CTX, C.c_uint32, C.c_double, C.c_double]),
     "brr_hip_last_stats": (C.c_int, [CTX, C.POINTER(Stats)]),
+    "brr_hip_last_phase_ms": (C.c_int, [CTX, P_f64, C.c_uint32]),
     "brr_bed_check_magic": (C.c_int, [P_u8]),
     "brr_dense_lut": (C.c_int, [P_u8, C.c_uint32, P_f64]),
     "brr_dense_luts": (C.c_int, [P_u8, C.c_uint64, C.c_uint32, C.c_uint32, P_f64, C.c_uint32]),

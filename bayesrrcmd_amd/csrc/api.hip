@@ -29,6 +29,7 @@ struct brr_hip_ctx {
     brr_hip_model model{};
     std::vector<double> cva;
     brr_hip_stats stats{};
+    double phase_ms[kPhases] = {0, 0, 0, 0, 0, 0};
     // dense arena
     uint32_t* d_dense = nullptr;
     uint64_t col_words = 0, col_bytes = 0;
@@ -240,12 +241,13 @@ SweepParams base_params(brr_hip_ctx* c)
     return p;
 }
 
-const void* kernel_for(uint32_t R)
+const void* kernel_for(uint32_t R, int layout)
 {
+    const bool d = layout == kLayoutDense;
     switch (R) {
-    case 1: return (const void*)bayesr_sweep_kernel<1>;
-    case 2: return (const void*)bayesr_sweep_kernel<2>;
-    case 4: return (const void*)bayesr_sweep_kernel<4>;
+    case 1: return d ? (const void*)bayesr_sweep_kernel<1, true> : (const void*)bayesr_sweep_kernel<1, false>;
+    case 2: return d ? (const void*)bayesr_sweep_kernel<2, true> : (const void*)bayesr_sweep_kernel<2, false>;
+    case 4: return d ? (const void*)bayesr_sweep_kernel<4, true> : (const void*)bayesr_sweep_kernel<4, false>;
     default: return nullptr;
     }
 }
@@ -261,10 +263,10 @@ int reset_exchange(brr_hip_ctx* c)
 
 int launch(brr_hip_ctx* c, SweepParams& p)
 {
-    const void* fn = kernel_for(c->R);
+    const void* fn = kernel_for(c->R, c->layout);
     if (!fn) return fail(c, BRR_HIP_EINVAL, "unsupported slice size");
     HIP_TRY(c, hipMemsetAsync(c->d_ctl + kCtlAbort, 0, sizeof(uint32_t) * 2, c->stream));
-    HIP_TRY(c, hipMemsetAsync(c->d_counters, 0, sizeof(unsigned long long) * 4, c->stream));
+    HIP_TRY(c, hipMemsetAsync(c->d_counters, 0, sizeof(unsigned long long) * 8, c->stream));
     void* args[] = {&p};
     HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
     HIP_TRY(c, hipLaunchCooperativeKernel(fn, dim3(c->P), dim3(kThreads), args, 0, c->stream));
@@ -273,12 +275,13 @@ int launch(brr_hip_ctx* c, SweepParams& p)
     float ms = 0.f;
     HIP_TRY(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     uint32_t ctl[4] = {0, 0, 0, 0};
-    unsigned long long counters[4] = {0, 0, 0, 0};
+    unsigned long long counters[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     HIP_TRY(c, hipMemcpy(ctl, c->d_ctl, sizeof(ctl), hipMemcpyDeviceToHost));
     HIP_TRY(c, hipMemcpy(counters, c->d_counters, sizeof(counters), hipMemcpyDeviceToHost));
     c->stats.kernel_ms = ms;
     c->stats.rounds = counters[0];
     c->stats.changes = counters[1];
+    for (int q = 0; q < kPhases; ++q) c->phase_ms[q] = (double)counters[2 + q] * 1e-5;
     c->stats.workgroups = c->P;
     c->stats.block_markers = c->B;
     c->stats.slice = c->S;
@@ -353,7 +356,7 @@ int brr_hip_create(const brr_hip_config* cfg, brr_hip_ctx** out)
     c->B = cfg->block_markers ? cfg->block_markers : 32u;
 
     int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel_for(R), kThreads, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel_for(R, (int)cfg->layout), kThreads, 0) != hipSuccess ||
         (uint64_t)occ * cus < c->P) {
         delete c;
         return BRR_HIP_EDEVICE;
@@ -367,7 +370,7 @@ int brr_hip_create(const brr_hip_config* cfg, brr_hip_ctx** out)
         (rc = dalloc(c, &c->d_scal, SL.size())) || (rc = dalloc(c, &c->d_cva, G * (K - 1))) ||
         (rc = dalloc(c, &c->d_xchg1, (size_t)2 * kMaxB * c->P * 2)) ||
         (rc = dalloc(c, &c->d_xchg2, (size_t)2 * kMaxB * 2)) || (rc = dalloc(c, &c->d_ctl, 8)) ||
-        (rc = dalloc(c, &c->d_counters, 4)) || (rc = dalloc(c, &c->d_tmp, 4)) ||
+        (rc = dalloc(c, &c->d_counters, 8)) || (rc = dalloc(c, &c->d_tmp, 4)) ||
         (rc = dalloc(c, &c->d_u, M)) || (rc = dalloc(c, &c->d_z, M))) {
         brr_hip_destroy(c);
         return rc;
@@ -667,6 +670,54 @@ int brr_hip_epsilon_change(brr_hip_ctx* c, uint32_t marker, double beta_old, dou
     epsilon_change_kernel<<<blocks, kThreads, 0, c->stream>>>(p, marker, beta_old, beta);
     HIP_TRY(c, hipGetLastError());
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return BRR_HIP_OK;
+}
+
+// Internal check of the block loaders: one sweep with the code maps of every
+// block dumped as each workgroup first uses them ([block][P][64][R*32] words).
+int brr_hip_debug_block_codes(brr_hip_ctx* c, const uint32_t* order, const double* u, const double* z,
+                              uint32_t* out, uint64_t out_words, double* meta)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    const uint32_t M = c->cfg.num_snps;
+    const uint64_t nblocks = (M + c->B - 1) / c->B;
+    const uint64_t need = nblocks * c->P * kMaxB * c->R * 32;
+    if (!out || out_words < need) return fail(c, BRR_HIP_EINVAL, "debug buffer too small");
+    uint32_t* d = nullptr;
+    if ((rc = dalloc(c, &d, need))) return rc;
+    HIP_TRY(c, hipMemset(d, 0, need * 4));
+    if (c->order_cap < M) {
+        if ((rc = dalloc(c, &c->d_order, M))) { dfree(d); return rc; }
+        c->order_cap = M;
+    }
+    HIP_TRY(c, hipMemcpy(c->d_order, order, sizeof(uint32_t) * M, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->d_u, u, sizeof(double) * M, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->d_z, z, sizeof(double) * M, hipMemcpyHostToDevice));
+    SweepParams p = base_params(c);
+    p.mode = kModeSweep;
+    p.iterations = 1;
+    p.order = c->d_order;
+    p.u = c->d_u;
+    p.z = c->d_z;
+    p.dbg_codes = d;
+    double* dm = nullptr;
+    const uint64_t mneed = nblocks * kMaxB * 8;
+    if ((rc = dalloc(c, &dm, mneed))) { dfree(d); return rc; }
+    HIP_TRY(c, hipMemset(dm, 0, mneed * 8));
+    p.dbg_meta = dm;
+    rc = launch(c, p);
+    if (rc == 0) HIP_TRY(c, hipMemcpy(out, d, need * 4, hipMemcpyDeviceToHost));
+    if (rc == 0 && meta) HIP_TRY(c, hipMemcpy(meta, dm, mneed * 8, hipMemcpyDeviceToHost));
+    dfree(d);
+    dfree(dm);
+    return rc;
+}
+
+int brr_hip_last_phase_ms(const brr_hip_ctx* c, double* out, uint32_t n)
+{
+    if (!c || !out) return BRR_HIP_EINVAL;
+    for (uint32_t q = 0; q < n && q < (uint32_t)kPhases; ++q) out[q] = c->phase_ms[q];
     return BRR_HIP_OK;
 }
 

@@ -21,6 +21,12 @@
 // the changed marker's update is applied to each slice and the remaining
 // markers are re-dotted in the next round. One exchange round per block plus
 // one per beta change replaces one grid barrier per marker.
+//
+// The genotype codes and per-marker constants of block b+1 are loaded into
+// registers while block b waits on its exchanges and are written to LDS when
+// block b ends, so HBM latency stays off the critical path (ragged lists need a
+// second pipeline stage: their per-slice segment bounds are fetched one block
+// earlier still).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -39,12 +45,24 @@ constexpr int kMaxRuns = 4;    // slice <= 512 * kMaxRuns individuals
 constexpr uint32_t kRun = 512; // individuals per lane-run pass: 64 lanes x 8
 constexpr uint32_t kSliceAlign = 16;
 constexpr uint64_t kSpinLimit = 1ull << 22;
+constexpr int kIdxCap = 32;    // prefetched ragged indices per thread per block
 
 enum : int { kModeSweep = 0, kModeChain = 1 };
 enum : int { kLayoutDense = 1, kLayoutRagged = 2 };
+constexpr int kPhases = 6;  // block load, dots, exchange, decisions, commit/update, iteration scalars
 
 // ctl words (persist across launches)
 enum : int { kCtlRound = 0, kCtlAbort = 1, kCtlAbortWhere = 2 };
+
+// Phase timer of workgroup 0, thread 0 (s_memrealtime: 100 MHz constant clock).
+#define BRR_TICK(slot)                                                           \
+    do {                                                                         \
+        if (tid == 0) {                                                          \
+            const uint64_t t_now_ = __builtin_amdgcn_s_memrealtime();           \
+            tk[(slot)] += t_now_ - t_last;                                       \
+            t_last = t_now_;                                                     \
+        }                                                                        \
+    } while (0)
 
 struct SweepParams {
     uint32_t N, M, K, G;
@@ -80,7 +98,9 @@ struct SweepParams {
     uint64_t* xchg1;              // [2][kMaxB][P][2] tagged partials
     uint64_t* xchg2;              // [2][kMaxB][2] tagged totals
     uint32_t* ctl;                // persistent control words
-    unsigned long long* counters; // [0] rounds, [1] changes, [2] markers
+    unsigned long long* counters; // [0] rounds, [1] changes, [2..7] phase ticks (100 MHz)
+    uint32_t* dbg_codes;          // optional: [block][P][kMaxB][R*32] code maps as first used
+    double* dbg_meta;             // optional: [block][kMaxB][8] workgroup 0's first-round inputs
 };
 
 // Scalar block (doubles): mu, sigmaE, epsSum, sigmaG[G], pi[G*K], v[G*K], betasqn[G]
@@ -100,18 +120,20 @@ template <int R>
 struct Shared {
     uint32_t codes[kMaxB][R][32];       // 2-bit PLINK codes, lane-run layout
     double dot[kMaxB];
+    // current block's per-marker constants
     double beta_old[kMaxB];
     double wt[kMaxB][4];                // column value per 2-bit code
     double mstat[kMaxB][3];             // ragged: mean, sd, Zsum
     double uu[kMaxB];
     double zz[kMaxB];
-    double beta_new[kMaxB];
-    double acumv[kMaxB];
     uint32_t mk[kMaxB];
     int32_t gr[kMaxB];
-    int32_t compv[kMaxB];
+    uint32_t mk_next[kMaxB];            // marker ids of the next block
+    uint32_t mk_next2[kMaxB];           // marker ids of the block after it
+    // ragged: segments of the next block (bounds fetched a block ahead)
+    uint64_t seg_start[3 * kMaxB];
+    uint32_t seg_pref[3 * kMaxB + 1];   // exclusive prefix; [3 nb] = total
     double delta[4];
-    double red[kWaves];
     double mu, sigmaE, epsSum;
     double sigmaG[kMaxG];
     double pi[kMaxG * kMaxK];
@@ -121,7 +143,6 @@ struct Shared {
     double gc[kMaxG * kMaxK];           // log pi_k - 0.5 log(logLScale S_k + 1)
     double gden[kMaxG * kMaxK];         // denom_k (index k >= 1)
     double gbsd[kMaxG * kMaxK];         // sqrt(sigmaE / denom_k)
-    double dnext;                       // scalar exchange result
     int change;
     int abort;
 };
@@ -144,9 +165,37 @@ __device__ __forceinline__ double wave_sum(double x)
     return x;
 }
 
+// Sums 16 per-lane values over the 64 lanes of a wave at once: halving
+// exchanges (xor 32, 16, 8, 4) leave lanes 4i..4i+3 with value i's partial
+// over their quarter, then a butterfly over xor 2, 1 completes it. 17 shuffles
+// instead of 16 x 6; the addition tree is fixed, so the result is the same in
+// every workgroup.
+__device__ __forceinline__ double transpose_reduce16(double (&v)[16], int lane)
+{
+#pragma unroll
+    for (int half = 8, bit = 32; half >= 1; half >>= 1, bit >>= 1) {
+        const bool up = (lane & bit) != 0;
+#pragma unroll
+        for (int i = 0; i < half; ++i) {
+            const double send = up ? v[i] : v[i + half];
+            const double recv = __shfl_xor(send, bit, 64);
+            v[i] = (up ? v[i + half] : v[i]) + recv;
+        }
+    }
+    double x = v[0];
+    x += __shfl_xor(x, 2, 64);
+    x += __shfl_xor(x, 1, 64);
+    return x;   // total of value ((lane >> 2) & 15)
+}
+
+// Value of a 2-bit code: two flat selects on 64-bit halves (no control flow).
 __device__ __forceinline__ double wt_of(uint32_t v, double w0, double w1, double w2, double w3)
 {
-    return (v & 2u) ? ((v & 1u) ? w3 : w2) : ((v & 1u) ? w1 : w0);
+    const bool b0 = (v & 1u) != 0;
+    const bool b1 = (v & 2u) != 0;
+    const uint64_t lo = b0 ? double_as_bits(w1) : double_as_bits(w0);
+    const uint64_t hi = b0 ? double_as_bits(w3) : double_as_bits(w2);
+    return bits_as_double(b1 ? hi : lo);
 }
 
 __device__ __forceinline__ void publish_pair(uint64_t* s, uint32_t rnd, double val)
@@ -197,7 +246,7 @@ __device__ __forceinline__ uint64_t* slot2(const SweepParams& p, uint32_t rnd, i
 // round rnd. Slot k is reduced by workgroup k % P. Afterwards L.dot[k] holds
 // the same total in every workgroup. Returns false (in all threads) on abort.
 template <int R>
-__device__ bool exchange(const SweepParams& p, Shared<R>& L, uint32_t rnd, int pos, int nb,
+__device__ __forceinline__ bool exchange(const SweepParams& p, Shared<R>& L, uint32_t rnd, int pos, int nb,
                          uint32_t w, int wave, int lane)
 {
     int cnt = 0;
@@ -234,7 +283,7 @@ __device__ bool exchange(const SweepParams& p, Shared<R>& L, uint32_t rnd, int p
 // A scalar all-reduce through slot 0. Every wave holds the whole slice, so the
 // slice partial is wave 0's sum alone.
 template <int R>
-__device__ bool allreduce_scalar(const SweepParams& p, Shared<R>& L, uint32_t& rnd, double part,
+__device__ __forceinline__ bool allreduce_scalar(const SweepParams& p, Shared<R>& L, uint32_t& rnd, double part,
                                  uint32_t w, int wave, int lane, double* total)
 {
     part = wave_sum(part);
@@ -246,7 +295,7 @@ __device__ bool allreduce_scalar(const SweepParams& p, Shared<R>& L, uint32_t& r
 }
 
 template <int R>
-__device__ void group_constants(const SweepParams& p, Shared<R>& L)
+__device__ __forceinline__ void group_constants(const SweepParams& p, Shared<R>& L)
 {
     // Per-iteration constants of processColumn (BayesRBase.cpp:309-333); they
     // depend only on sigmaE, sigmaG[g], pi[g], S[g].
@@ -272,21 +321,245 @@ __device__ void group_constants(const SweepParams& p, Shared<R>& L)
     }
 }
 
-__device__ __forceinline__ bool guard700(const double* logL, int K, int r)
+// any_{i=1..K-1} |logL[i] - logL[r]| > 700 (BayesRBase.cpp:335, 360); loops run
+// to kMaxK with a predicate so the arrays stay in registers
+__device__ __forceinline__ bool guard700(const double (&logL)[kMaxK], int K, double ref)
 {
     bool g = false;
-    for (int i = 1; i < K; ++i) g = g || (fabs(logL[i] - logL[r]) > 700.0);
+#pragma unroll
+    for (int i = 1; i < kMaxK; ++i)
+        if (i < K) g = g || (fabs(logL[i] - ref) > 700.0);
     return g;
 }
 
-__device__ __forceinline__ double inv_sum_exp(const double* logL, int K, int r)
+// 1 / sum_k exp(logL[k] - ref) in index order (BayesRBase.cpp:338, 363)
+__device__ __forceinline__ double inv_sum_exp(const double (&logL)[kMaxK], int K, double ref)
 {
     double s = 0.0;
-    for (int k = 0; k < K; ++k) s += exp(logL[k] - logL[r]);
+#pragma unroll
+    for (int k = 0; k < kMaxK; ++k)
+        if (k < K) s += exp(logL[k] - ref);
     return 1.0 / s;
 }
 
+// ------------------------------------------------------------ block loading --
+
+// Per-marker constants of the next block, held in registers by thread t < nb.
+struct MetaNext {
+    uint32_t j;
+    int32_t g;
+    uint64_t beta_bits;
+    double w0, w1, w2, w3;   // dense table by 2-bit code / ragged mean, sd, Zsum in w0..w2
+    double u, z;
+};
+
+__device__ __forceinline__ void meta_issue(const SweepParams& p, uint32_t j, uint32_t iteration,
+                                           MetaNext& m)
+{
+    m.j = j;
+    m.g = p.group ? p.group[j] : 0;
+    m.beta_bits = ld_agent((const uint64_t*)(p.beta + j));
+    if (p.layout == kLayoutDense) {
+        const double* lt = p.lut + 4 * (size_t)j;
+        // 2-bit value v -> dosage: 0->2, 1->missing, 2->1, 3->0
+        m.w0 = lt[2];
+        m.w1 = lt[3];
+        m.w2 = lt[1];
+        m.w3 = lt[0];
+    } else {
+        const double* st = p.stats + 4 * (size_t)j;
+        m.w0 = st[0];
+        m.w1 = st[1];
+        m.w2 = st[3];
+        m.w3 = 0.0;
+    }
+    if (p.mode == kModeChain) {
+        marker_uz(p.seed, iteration, j, p.chain, &m.u, &m.z);
+    } else {
+        m.u = p.u[j];
+        m.z = p.z[j];
+    }
+}
+
 template <int R>
+__device__ __forceinline__ void meta_store(const SweepParams& p, Shared<R>& L, int t, const MetaNext& m)
+{
+    L.mk[t] = m.j;
+    L.gr[t] = m.g;
+    L.beta_old[t] = bits_as_double(m.beta_bits);
+    L.uu[t] = m.u;
+    L.zz[t] = m.z;
+    if (p.layout == kLayoutDense) {
+        L.wt[t][0] = m.w0;
+        L.wt[t][1] = m.w1;
+        L.wt[t][2] = m.w2;
+        L.wt[t][3] = m.w3;
+    } else {
+        L.mstat[t][0] = m.w0;
+        L.mstat[t][1] = m.w1;
+        L.mstat[t][2] = m.w2;
+        L.wt[t][0] = 2.0;   // genotype 2
+        L.wt[t][1] = 0.0;   // missing (not in the ones/twos lists)
+        L.wt[t][2] = 1.0;   // genotype 1
+        L.wt[t][3] = 0.0;   // genotype 0
+    }
+}
+
+// Dense codes of the next block: word t of the flattened [nb][R][32] array.
+template <int R>
+__device__ __forceinline__ void dense_issue(const SweepParams& p, const Shared<R>& L, int nb, uint32_t lo,
+                                            int tid, uint32_t (&words)[8 * R])
+{
+    const uint32_t per = (uint32_t)R * 32u;
+    const uint32_t total = (uint32_t)nb * per;
+#pragma unroll
+    for (int u8 = 0; u8 < 8 * R; ++u8) {
+        const uint32_t t = (uint32_t)u8 * kThreads + (uint32_t)tid;
+        words[u8] = 0u;
+        if (t < total) {
+            const uint32_t k = t / per, rem = t % per;
+            const uint32_t j = L.mk_next[k];
+            const uint64_t byte_off = (uint64_t)(lo >> 2) + 4ull * rem;
+            if (byte_off < p.col_bytes) words[u8] = p.dense[(size_t)j * p.col_words + (lo >> 4) + rem];
+        }
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void dense_store(Shared<R>& L, int nb, int tid, const uint32_t (&words)[8 * R])
+{
+    const uint32_t total = (uint32_t)nb * (uint32_t)R * 32u;
+#pragma unroll
+    for (int u8 = 0; u8 < 8 * R; ++u8) {
+        const uint32_t t = (uint32_t)u8 * kThreads + (uint32_t)tid;
+        if (t < total) (&L.codes[0][0][0])[t] = words[u8];
+    }
+}
+
+// Ragged stage 1: segment bounds of (marker, list) pairs of the next block.
+__device__ __forceinline__ void seg_issue(const SweepParams& p, uint32_t j, int list, uint32_t w,
+                                          uint64_t& start, uint32_t& len)
+{
+    const uint32_t* sg = p.seg + ((size_t)j * 3u + (uint32_t)list) * (p.P + 1u);
+    const uint32_t s0 = sg[w], s1 = sg[w + 1];
+    uint64_t off;
+    if (list == 0) off = p.ones_off[j];
+    else if (list == 1) off = p.twos_off[j];
+    else off = p.miss_off[j];
+    start = off + s0;
+    len = s1 - s0;
+}
+
+// Stores the stage-1 results and forms the exclusive prefix (all threads call).
+template <int R>
+__device__ __forceinline__ void seg_store_prefix(Shared<R>& L, int nb, int tid, int wave, int lane,
+                                                 uint64_t start, uint32_t len)
+{
+    if (tid < 3 * nb) {
+        L.seg_start[tid] = start;
+        L.seg_pref[tid] = len;     // lengths for now
+    }
+    __syncthreads();
+    if (wave == 0) {
+        uint32_t a = 0, b = 0, c = 0;
+        if (lane < nb) { a = L.seg_pref[3 * lane]; b = L.seg_pref[3 * lane + 1]; c = L.seg_pref[3 * lane + 2]; }
+        const uint32_t x = a + b + c;
+        uint32_t incl = x;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += y;
+        }
+        const uint32_t excl = incl - x;
+        if (lane < nb) {
+            L.seg_pref[3 * lane] = excl;
+            L.seg_pref[3 * lane + 1] = excl + a;
+            L.seg_pref[3 * lane + 2] = excl + a + b;
+        }
+        if (lane == 63) L.seg_pref[3 * nb] = incl;
+    }
+    __syncthreads();
+}
+
+template <int R>
+__device__ __forceinline__ int seg_item_of(const Shared<R>& L, int items, uint32_t q)
+{
+    int a = 0, b = items - 1;           // last item whose prefix <= q
+    while (a < b) {
+        const int mid = (a + b + 1) >> 1;
+        if (L.seg_pref[mid] <= q) a = mid; else b = mid - 1;
+    }
+    return a;
+}
+
+// One ragged list entry (explicit branches keep the kernel-argument struct in
+// scalar registers; a select between its fields becomes a dynamic offset into it)
+__device__ __forceinline__ int32_t list_entry(const SweepParams& p, int list, uint64_t at)
+{
+    if (list == 0) return p.ones[at];
+    if (list == 1) return p.twos[at];
+    return p.miss[at];
+}
+
+// Ragged stage 2: list entries q = u8 * 256 + tid (u8 < kIdxCap) of the next block.
+template <int R>
+__device__ __forceinline__ void idx_issue(const SweepParams& p, const Shared<R>& L, int nb, int tid,
+                                          int32_t (&idx)[kIdxCap], uint32_t (&itm)[kIdxCap / 4])
+{
+    const uint32_t total = L.seg_pref[3 * nb];
+    const int items = 3 * nb;
+#pragma unroll
+    for (int u4 = 0; u4 < kIdxCap / 4; ++u4) itm[u4] = 0xFFFFFFFFu;
+#pragma unroll
+    for (int u8 = 0; u8 < kIdxCap; ++u8) {
+        const uint32_t q = (uint32_t)u8 * kThreads + (uint32_t)tid;
+        idx[u8] = 0;
+        if (q < total) {
+            const int a = seg_item_of(L, items, q);
+            idx[u8] = list_entry(p, a % 3, L.seg_start[a] + (q - L.seg_pref[a]));
+            itm[u8 >> 2] = (itm[u8 >> 2] & ~(0xFFu << (8 * (u8 & 3)))) | ((uint32_t)a << (8 * (u8 & 3)));
+        }
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void code_clear(Shared<R>& L, int k, uint32_t loc, int list)
+{
+    // ones: code 3 -> 2 (clear bit 0); twos: -> 0 (clear both); missing: -> 1 (clear bit 1)
+    const uint32_t clr = (list == 0) ? 1u : (list == 1) ? 3u : 2u;
+    const uint32_t r = loc >> 9, ln = (loc >> 3) & 63u, e = loc & 7u;
+    const uint32_t bit = 16u * (ln & 1u) + 2u * e;
+    atomicAnd(&L.codes[k][r][ln >> 1], ~(clr << bit));
+}
+
+// Builds the code maps of the block whose segments are in L.seg_*: the
+// prefetched entries first, then any beyond the per-thread cap.
+template <int R>
+__device__ __forceinline__ void ragged_store(const SweepParams& p, Shared<R>& L, int nb, uint32_t lo,
+                                             int tid, const int32_t (&idx)[kIdxCap],
+                                             const uint32_t (&itm)[kIdxCap / 4])
+{
+    const uint32_t per = (uint32_t)R * 32u;
+    for (uint32_t t = tid; t < (uint32_t)nb * per; t += kThreads) (&L.codes[0][0][0])[t] = 0xFFFFFFFFu;
+    __syncthreads();
+    const uint32_t total = L.seg_pref[3 * nb];
+#pragma unroll
+    for (int u8 = 0; u8 < kIdxCap; ++u8) {
+        const uint32_t a = (itm[u8 >> 2] >> (8 * (u8 & 3))) & 0xFFu;
+        if (a == 0xFFu) continue;
+        code_clear(L, (int)(a / 3), (uint32_t)idx[u8] - lo, (int)(a % 3));
+    }
+    const int items = 3 * nb;
+    for (uint32_t q = (uint32_t)kIdxCap * kThreads + (uint32_t)tid; q < total; q += kThreads) {
+        const int a = seg_item_of(L, items, q);
+        const int list = a % 3;
+        code_clear(L, a / 3, (uint32_t)list_entry(p, list, L.seg_start[a] + (q - L.seg_pref[a])) - lo, list);
+    }
+}
+
+// -------------------------------------------------------------------- kernel --
+
+template <int R, bool kDense>
 __global__ void __launch_bounds__(kThreads)
 bayesr_sweep_kernel(SweepParams p)
 {
@@ -301,6 +574,7 @@ bayesr_sweep_kernel(SweepParams p)
     const ScalarLayout SL{p.G, p.K};
     const double dN = (double)p.N;
     const double NM1 = (double)(p.N - 1u);
+    constexpr bool dense = kDense;
 
     // eps slice in registers: individual lo + 512 r + 8 lane + e
     double eps[R][8];
@@ -329,9 +603,12 @@ bayesr_sweep_kernel(SweepParams p)
     for (uint32_t t = tid; t < p.G * (p.K - 1); t += kThreads) L.cva[t] = p.cva[t];
     uint32_t rnd = __hip_atomic_load(p.ctl + kCtlRound, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     unsigned long long n_rounds = 0, n_changes = 0;
+    uint64_t tk[kPhases] = {0, 0, 0, 0, 0, 0};
+    uint64_t t_last = __builtin_amdgcn_s_memrealtime();
     __syncthreads();
 
     const uint32_t iters = (p.mode == kModeChain) ? p.iterations : 1u;
+    const uint32_t nblocks = (p.M + p.B - 1) / p.B;
     for (uint32_t it = 0; it < iters; ++it) {
         const uint32_t iteration = p.iter_base + it;
         const uint32_t* order = p.order + (size_t)((p.mode == kModeChain) ? it : 0u) * p.M;
@@ -347,7 +624,7 @@ bayesr_sweep_kernel(SweepParams p)
                     if (idx < hi) eps[r][e] = eps[r][e] + old_mu;
                 }
             double mean_for_mu;
-            if (p.layout == kLayoutDense) {
+            if (dense) {
                 double part = 0.0;
 #pragma unroll
                 for (int r = 0; r < R; ++r)
@@ -375,7 +652,7 @@ bayesr_sweep_kernel(SweepParams p)
             __syncthreads();
             if (tid == 0) {
                 L.mu = new_mu;
-                if (p.layout != kLayoutDense) L.epsSum -= new_mu * dN;
+                if (!dense) L.epsSum -= new_mu * dN;
             }
             for (uint32_t t = tid; t < p.G; t += kThreads) L.betasqn[t] = 0.0;
             for (uint32_t t = tid; t < p.G * p.K; t += kThreads) L.v[t] = 0.0;
@@ -384,115 +661,129 @@ bayesr_sweep_kernel(SweepParams p)
         group_constants<R>(p, L);
         __syncthreads();
 
-        // ---- the sweep (AnalysisGraph::exec over the order)
-        for (uint32_t b0 = 0; b0 < p.M; b0 += p.B) {
-            const int nb = (int)min(p.B, p.M - b0);
-
-            // block load: per-marker scalars
-            if (tid < nb) {
-                const uint32_t j = order[b0 + tid];
-                L.mk[tid] = j;
-                L.gr[tid] = p.group ? p.group[j] : 0;
-                L.beta_old[tid] = bits_as_double(ld_agent((const uint64_t*)(p.beta + j)));
-                if (p.mode == kModeChain) {
-                    double uu, zz;
-                    marker_uz(p.seed, iteration, j, p.chain, &uu, &zz);
-                    L.uu[tid] = uu;
-                    L.zz[tid] = zz;
-                } else {
-                    L.uu[tid] = p.u[j];
-                    L.zz[tid] = p.z[j];
-                }
-                if (p.layout == kLayoutDense) {
-                    const double* lt = p.lut + 4 * (size_t)j;
-                    // 2-bit value v -> dosage: 0->2, 1->missing, 2->1, 3->0
-                    L.wt[tid][0] = lt[2];
-                    L.wt[tid][1] = lt[3];
-                    L.wt[tid][2] = lt[1];
-                    L.wt[tid][3] = lt[0];
-                } else {
-                    const double* st = p.stats + 4 * (size_t)j;
-                    L.mstat[tid][0] = st[0];
-                    L.mstat[tid][1] = st[1];
-                    L.mstat[tid][2] = st[3];
-                    L.wt[tid][0] = 2.0;   // genotype 2
-                    L.wt[tid][1] = 0.0;   // missing (not in the ones/twos lists)
-                    L.wt[tid][2] = 1.0;   // genotype 1
-                    L.wt[tid][3] = 0.0;   // genotype 0
+        // ---- block 0 of this iteration, loaded in place; the ids (and for
+        // ragged the segments) of block 1 staged for the prefetch pipeline
+        {
+            const int nb0 = (int)min(p.B, p.M);
+            if (tid < nb0) L.mk_next[tid] = order[tid];
+            __syncthreads();
+            MetaNext m{};
+            if (tid < nb0) meta_issue(p, L.mk_next[tid], iteration, m);
+            if constexpr (kDense) {
+                uint32_t words[8 * R];
+                dense_issue<R>(p, L, nb0, lo, tid, words);
+                dense_store<R>(L, nb0, tid, words);
+            } else {
+                uint64_t st = 0;
+                uint32_t ln = 0;
+                if (tid < 3 * nb0) seg_issue(p, L.mk_next[tid / 3], tid % 3, w, st, ln);
+                seg_store_prefix<R>(L, nb0, tid, wave, lane, st, ln);
+                int32_t idx[kIdxCap];
+                uint32_t itm[kIdxCap / 4];
+                idx_issue<R>(p, L, nb0, tid, idx, itm);
+                ragged_store<R>(p, L, nb0, lo, tid, idx, itm);
+            }
+            if (tid < nb0) meta_store<R>(p, L, tid, m);
+            __syncthreads();
+            if (nblocks > 1) {
+                const int nb1 = (int)min(p.B, p.M - p.B);
+                if (tid < nb1) L.mk_next[tid] = order[p.B + tid];
+                __syncthreads();
+                if (!dense) {
+                    uint64_t st = 0;
+                    uint32_t ln = 0;
+                    if (tid < 3 * nb1) seg_issue(p, L.mk_next[tid / 3], tid % 3, w, st, ln);
+                    seg_store_prefix<R>(L, nb1, tid, wave, lane, st, ln);
                 }
             }
-            // codes
-            if (p.layout == kLayoutDense) {
-                const uint32_t per = (uint32_t)R * 32u;
-                for (uint32_t t = tid; t < (uint32_t)nb * per; t += kThreads) {
-                    const uint32_t k = t / per, rem = t % per;
-                    const uint32_t j = order[b0 + k];
-                    const uint64_t byte_off = (uint64_t)(lo >> 2) + 4ull * rem;
-                    uint32_t word = 0u;
-                    if (byte_off < p.col_bytes)
-                        word = p.dense[(size_t)j * p.col_words + (lo >> 4) + rem];
-                    (&L.codes[0][0][0])[k * per + rem] = word;
-                }
-                __syncthreads();
-            } else {
+        }
+        BRR_TICK(5);
+
+        // ---- the sweep (AnalysisGraph::exec over the order)
+        for (uint32_t blk = 0; blk < nblocks; ++blk) {
+            const uint32_t b0 = blk * p.B;
+            const int nb = (int)min(p.B, p.M - b0);
+            const bool has1 = blk + 1 < nblocks;
+            const int nb1 = has1 ? (int)min(p.B, p.M - b0 - p.B) : 0;
+            const bool has2 = blk + 2 < nblocks;
+            const int nb2 = has2 ? (int)min(p.B, p.M - b0 - 2 * p.B) : 0;
+
+            // prefetch issue: everything of block blk+1 whose address is known
+            // (ids in L.mk_next, ragged segments in L.seg_*), and the ids of blk+2
+            MetaNext mnext{};
+            uint32_t jnext2 = 0;
+            uint64_t st2 = 0;
+            uint32_t ln2 = 0;
+            uint32_t words[kDense ? 8 * R : 1];
+            int32_t idx[kDense ? 1 : kIdxCap];
+            uint32_t itm[kDense ? 1 : kIdxCap / 4];
+            if (has1) {
+                if (tid < nb1) meta_issue(p, L.mk_next[tid], iteration, mnext);
+                if constexpr (kDense) dense_issue<R>(p, L, nb1, lo, tid, words);
+                else idx_issue<R>(p, L, nb1, tid, idx, itm);
+            }
+            if (has2 && tid < nb2) jnext2 = order[b0 + 2 * p.B + tid];
+            BRR_TICK(0);
+
+            if (p.dbg_codes) {
                 const uint32_t per = (uint32_t)R * 32u;
                 for (uint32_t t = tid; t < (uint32_t)nb * per; t += kThreads)
-                    (&L.codes[0][0][0])[t] = 0xFFFFFFFFu;   // all genotype 0 (code 3)
-                __syncthreads();
-                for (int k = 0; k < nb; ++k) {
-                    const uint32_t j = order[b0 + k];
-#pragma unroll
-                    for (int list = 0; list < 3; ++list) {
-                        const uint32_t* sg = p.seg + ((size_t)j * 3u + (uint32_t)list) * (p.P + 1u);
-                        const uint32_t s0 = sg[w], s1 = sg[w + 1];
-                        const int32_t* base = (list == 0) ? p.ones + p.ones_off[j]
-                                            : (list == 1) ? p.twos + p.twos_off[j]
-                                                          : p.miss + p.miss_off[j];
-                        // ones: code 3 -> 2 (clear bit 0); twos: -> 0 (clear both); missing: -> 1 (clear bit 1)
-                        const uint32_t clr = (list == 0) ? 1u : (list == 1) ? 3u : 2u;
-                        for (uint32_t q = s0 + (uint32_t)tid; q < s1; q += kThreads) {
-                            const uint32_t loc = (uint32_t)base[q] - lo;
-                            const uint32_t r = loc >> 9, ln = (loc >> 3) & 63u, e = loc & 7u;
-                            const uint32_t bit = 16u * (ln & 1u) + 2u * e;
-                            atomicAnd(&L.codes[k][r][ln >> 1], ~(clr << bit));
-                        }
-                    }
-                }
-                __syncthreads();
+                    p.dbg_codes[((size_t)blk * p.P + w) * kMaxB * per + t] = (&L.codes[0][0][0])[t];
             }
-
             int pos = 0;
+            bool first_round = true;
             while (pos < nb) {
-                // ---- partial dots of slots [pos, nb) over this slice
+                // ---- partial dots of slots [pos, nb) over this slice: wave q
+                // handles slots pos + q + 4 i, reduced together
                 ++rnd;
                 ++n_rounds;
-                for (int k = pos + wave; k < nb; k += kWaves) {
-                    const double w0 = L.wt[k][0], w1 = L.wt[k][1], w2 = L.wt[k][2], w3 = L.wt[k][3];
-                    double s = 0.0;
+                {
+                    double part[16];
 #pragma unroll
-                    for (int r = 0; r < R; ++r) {
-                        const uint32_t bits = L.codes[k][r][lane >> 1] >> (16 * (lane & 1));
+                    for (int i = 0; i < 16; ++i) {
+                        const int k = pos + wave + kWaves * i;
+                        double s = 0.0;
+                        if (k < nb) {
+                            const double w0 = L.wt[k][0], w1 = L.wt[k][1], w2 = L.wt[k][2], w3 = L.wt[k][3];
 #pragma unroll
-                        for (int e = 0; e < 8; ++e)
-                            s = fma(wt_of((bits >> (2 * e)) & 3u, w0, w1, w2, w3), eps[r][e], s);
+                            for (int r = 0; r < R; ++r) {
+                                const uint32_t bits = L.codes[k][r][lane >> 1] >> (16 * (lane & 1));
+#pragma unroll
+                                for (int e = 0; e < 8; ++e)
+                                    s = fma(wt_of((bits >> (2 * e)) & 3u, w0, w1, w2, w3), eps[r][e], s);
+                            }
+                        }
+                        part[i] = s;
                     }
-                    s = wave_sum(s);
-                    if (lane == 0) publish_pair(slot1(p, rnd, k, w), rnd, s);
+                    const double tot = transpose_reduce16(part, lane);
+                    const int i = (lane >> 2) & 15;
+                    const int k = pos + wave + kWaves * i;
+                    if ((lane & 3) == 0 && k < nb) publish_pair(slot1(p, rnd, k, w), rnd, tot);
                 }
+                BRR_TICK(1);
                 if (!exchange<R>(p, L, rnd, pos, nb, w, wave, lane)) goto done;
+                BRR_TICK(2);
+                if (first_round && has2 && tid < nb2) L.mk_next2[tid] = jnext2;
+                if (first_round && p.dbg_meta && w == 0 && tid < nb) {
+                    double* d = p.dbg_meta + ((size_t)blk * kMaxB + tid) * 8;
+                    d[0] = L.dot[tid]; d[1] = L.beta_old[tid]; d[2] = L.uu[tid]; d[3] = L.zz[tid];
+                    d[4] = L.mstat[tid][0]; d[5] = L.mstat[tid][1]; d[6] = L.mstat[tid][2]; d[7] = L.epsSum;
+                }
 
-                // ---- decisions (wave 0, lane = slot), BayesRBase.cpp:300-366
+                // ---- decisions and commit (wave 0, lane = slot), BayesRBase.cpp:300-378
                 if (wave == 0) {
                     const int k = pos + lane;
                     bool changed = false;
+                    double bnew = 0.0, acum0 = 0.0;
+                    int sel = -1, g = 0;
                     if (k < nb) {
-                        const int g = L.gr[k];
+                        g = L.gr[k];
                         const double* gc = L.gc + g * K;
                         const double* gden = L.gden + g * K;
                         const double* gbsd = L.gbsd + g * K;
                         const double beta_old = L.beta_old[k];
                         double num;
-                        if (p.layout == kLayoutDense) {
+                        if (dense) {
                             num = L.dot[k] + beta_old * NM1;
                         } else {
                             const double mean = L.mstat[k][0], sd = L.mstat[k][1];
@@ -501,53 +792,64 @@ bayesr_sweep_kernel(SweepParams p)
                         double logL[kMaxK], muk[kMaxK];
                         logL[0] = gc[0];
                         muk[0] = 0.0;
-                        for (int c = 1; c < K; ++c) {
-                            muk[c] = num / gden[c];
-                            logL[c] = gc[c] + 0.5 * (muk[c] * num) / L.sigmaE;
-                        }
-                        double acum = guard700(logL, K, 0) ? 0.0 : inv_sum_exp(logL, K, 0);
-                        const double acum0 = acum;
-                        const double uu = L.uu[k];
-                        int sel = -1;
-                        double bnew = beta_old;
-                        for (int c = 0; c < K; ++c) {
-                            if (uu <= acum) {
-                                sel = c;
-                                bnew = (c == 0) ? 0.0 : L.zz[k] * gbsd[c] + muk[c];
-                                break;
-                            } else if (c + 1 < K) {
-                                if (!guard700(logL, K, c + 1)) acum += inv_sum_exp(logL, K, c + 1);
+#pragma unroll
+                        for (int c = 1; c < kMaxK; ++c) {
+                            muk[c] = 0.0;
+                            logL[c] = 0.0;
+                            if (c < K) {
+                                muk[c] = num / gden[c];
+                                logL[c] = gc[c] + 0.5 * (muk[c] * num) / L.sigmaE;
                             }
                         }
-                        L.beta_new[k] = bnew;
-                        L.acumv[k] = acum0;
-                        L.compv[k] = sel;
+                        double acum = guard700(logL, K, logL[0]) ? 0.0 : inv_sum_exp(logL, K, logL[0]);
+                        acum0 = acum;
+                        const double uu = L.uu[k];
+                        bnew = beta_old;
+                        bool done_sel = false;
+#pragma unroll
+                        for (int c = 0; c < kMaxK; ++c) {
+                            if (c < K && !done_sel) {
+                                if (uu <= acum) {
+                                    sel = c;
+                                    bnew = (c == 0) ? 0.0 : L.zz[k] * gbsd[c] + muk[c];
+                                    done_sel = true;
+                                } else if (c + 1 < K) {
+                                    const double ref = logL[c + 1 < kMaxK ? c + 1 : 0];
+                                    if (!guard700(logL, K, ref)) acum += inv_sum_exp(logL, K, ref);
+                                }
+                            }
+                        }
                         changed = (bnew != beta_old);
                     }
-                    const unsigned long long m = __ballot(changed);
-                    if (lane == 0) L.change = m ? pos + (int)__ffsll((long long)m) - 1 : nb;
-                }
-                __syncthreads();
-                const int c = L.change;
-                const int last = (c < nb) ? c : nb - 1;
-
-                // ---- commit accepted slots [pos, last] in order
-                if (tid == 0) {
-                    for (int k = pos; k <= last; ++k) {
-                        const int sel = L.compv[k];
-                        if (sel < 0) continue;
-                        const int g = L.gr[k];
-                        L.v[g * K + sel] += 1.0;
-                        if (sel > 0) L.betasqn[g] += L.beta_new[k] * L.beta_new[k];
+                    const unsigned long long cm = __ballot(changed);
+                    const int c = cm ? pos + (int)__ffsll((long long)cm) - 1 : nb;
+                    const int last = (c < nb) ? c : nb - 1;
+                    const bool accepted = k < nb && k <= last;
+                    // v counts (integer-valued doubles: exact in any order)
+                    if (accepted && sel >= 0) atomicAdd(&L.v[g * K + sel], 1.0);
+                    // betasqnG += beta^2 in marker order (BayesRBase.cpp:353)
+                    unsigned long long sq = __ballot(accepted && sel > 0);
+                    while (sq) {
+                        const int b = (int)__ffsll((long long)sq) - 1;
+                        const double bb = __shfl(bnew, b, 64);
+                        const int gb = __shfl(g, b, 64);
+                        if (lane == 0) L.betasqn[gb] += bb * bb;
+                        sq &= sq - 1;
                     }
-                    if (c < nb) {
-                        const double dBeta = L.beta_old[c] - L.beta_new[c];
-                        if (p.layout == kLayoutDense) {
+                    if (w == 0 && accepted) {
+                        const uint32_t j = L.mk[k];
+                        st_agent((uint64_t*)(p.beta + j), double_as_bits(bnew));
+                        if (sel >= 0) p.comp[j] = sel;
+                        p.acum[j] = acum0;
+                    }
+                    if (k < nb && k == c) {
+                        const double dBeta = L.beta_old[k] - bnew;
+                        if (dense) {
 #pragma unroll
-                            for (int q = 0; q < 4; ++q) L.delta[q] = dBeta * L.wt[c][q];
+                            for (int q = 0; q < 4; ++q) L.delta[q] = dBeta * L.wt[k][q];
                         } else {
                             // raggedbayesrkernel.cpp:10-30 and sparsebayesrkernel.cpp:29-34
-                            const double mean = L.mstat[c][0], sd = L.mstat[c][1], Zsum = L.mstat[c][2];
+                            const double mean = L.mstat[k][0], sd = L.mstat[k][1], Zsum = L.mstat[k][2];
                             const double meanAdjustment = dBeta * mean / sd;
                             const double oneAdjustment = dBeta / sd;
                             L.delta[3] = -meanAdjustment;
@@ -557,14 +859,18 @@ bayesr_sweep_kernel(SweepParams p)
                             L.epsSum += dBeta * Zsum / sd - dBeta * mean * dN / sd;
                         }
                     }
-                }
-                if (w == 0 && tid >= pos && tid <= last) {
-                    const uint32_t j = L.mk[tid];
-                    st_agent((uint64_t*)(p.beta + j), double_as_bits(L.beta_new[tid]));
-                    if (L.compv[tid] >= 0) p.comp[j] = L.compv[tid];
-                    p.acum[j] = L.acumv[tid];
+                    if (lane == 0) L.change = c;
                 }
                 __syncthreads();
+                BRR_TICK(3);
+                if (first_round) {
+                    first_round = false;
+                    // ragged stage 1 of block blk+2: segment bounds to registers,
+                    // stored once block blk+1's segments have been consumed
+                    if (!dense && has2 && tid < 3 * nb2) seg_issue(p, L.mk_next2[tid / 3], tid % 3, w, st2, ln2);
+                }
+                const int c = L.change;
+                const int last = (c < nb) ? c : nb - 1;
                 if (c < nb) {
                     ++n_changes;
                     const double d0 = L.delta[0], d1 = L.delta[1], d2 = L.delta[2], d3 = L.delta[3];
@@ -573,17 +879,32 @@ bayesr_sweep_kernel(SweepParams p)
                         const uint32_t bits = L.codes[c][r][lane >> 1] >> (16 * (lane & 1));
 #pragma unroll
                         for (int e = 0; e < 8; ++e) {
-                            const uint32_t idx = lo + (uint32_t)r * kRun + 8u * (uint32_t)lane + (uint32_t)e;
-                            if (idx < hi) eps[r][e] = eps[r][e] + wt_of((bits >> (2 * e)) & 3u, d0, d1, d2, d3);
+                            const uint32_t idx2 = lo + (uint32_t)r * kRun + 8u * (uint32_t)lane + (uint32_t)e;
+                            if (idx2 < hi) eps[r][e] = eps[r][e] + wt_of((bits >> (2 * e)) & 3u, d0, d1, d2, d3);
                         }
                     }
                 }
                 pos = last + 1;
+                BRR_TICK(4);
             }
             // workgroup 0's beta stores must be complete before later rounds
             // let other workgroups read them
             if (w == 0) __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
+
+            // ---- commit the prefetch: block blk+1 becomes current
+            if (has1) {
+                if constexpr (kDense) dense_store<R>(L, nb1, tid, words);
+                else ragged_store<R>(p, L, nb1, lo, tid, idx, itm);
+                if (tid < nb1) meta_store<R>(p, L, tid, mnext);
+                __syncthreads();
+                if (has2) {
+                    if (tid < nb2) L.mk_next[tid] = jnext2;
+                    if (!dense) seg_store_prefix<R>(L, nb2, tid, wave, lane, st2, ln2);
+                    else __syncthreads();
+                }
+            }
+            BRR_TICK(0);
         }
 
         if (p.mode == kModeChain) {
@@ -626,6 +947,7 @@ bayesr_sweep_kernel(SweepParams p)
                 }
             }
             __syncthreads();
+            BRR_TICK(5);
         }
     }
 
@@ -649,6 +971,7 @@ done:
                 __hip_atomic_store(p.ctl + kCtlRound, rnd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 p.counters[0] = n_rounds;
                 p.counters[1] = n_changes;
+                for (int q = 0; q < kPhases; ++q) p.counters[2 + q] = tk[q];
             }
             for (uint32_t t = tid; t < p.G; t += kThreads) {
                 p.scal[SL.sigmaG() + t] = L.sigmaG[t];

@@ -1,0 +1,262 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X BayesR marker sweep on one GPU.
+
+Workload (default, `--config c2`): plain BayesR ppbayes at scale -- N = 100,000
+individuals, M = 100,000 SNPs on the ragged layout (ascending index lists of
+genotype 1 and 2), MAF ~ U(0.005, 0.5), 0.5% causal SNPs, h2 = 0.4, synthetic
+data from a fixed seed. A step is one Gibbs iteration of the chain (updateMu,
+the sweep over all M markers, the sigmaE / sigmaG / pi draws), run by the
+persistent device kernel with genotypes and chain state resident in HBM.
+`--config c1` is the small dense case (N = 5,000, M = 10,000 .bed 2-bit).
+
+Prints one JSON line: marker-updates/s (whole job), the roofline of the sweep
+kernel measured with HIP events on its stream, and the CPU reference sweep
+(the FP64 oracle built with the reference's Release flags) timed on a bounded
+sample of the same workload on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "Gibbs marker-updates/sec at N=100k (batched vs plain GPU) + % HBM roofline"
+HBM_PEAK_GBS = 8000.0
+S_DEFAULT = np.array([0.01, 0.001, 0.0001])
+
+CONFIGS = {
+    "c2": dict(workload="plain BayesR ppbayes, ragged layout, N=100000 M=100000, single chain",
+               n=100_000, m=100_000, layout="ragged", maf=(0.005, 0.5), causal=0.005, h2=0.4,
+               chain=200),
+    "c2-dense": dict(workload="plain BayesR ppbayes, dense .bed 2-bit layout, N=100000 M=100000, single chain",
+                     n=100_000, m=100_000, layout="dense", maf=(0.005, 0.5), causal=0.005, h2=0.4,
+                     chain=200),
+    "c1": dict(workload="plain BayesR ppbayes, dense .bed 2-bit, N=5000 M=10000, single chain",
+               n=5_000, m=10_000, layout="dense", maf=(0.01, 0.5), causal=0.01, h2=0.5, chain=10),
+}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_data(cfg, seed, threads):
+    import bayesrrcmd_amd as brr
+    n, m = cfg["n"], cfg["m"]
+    t0 = time.time()
+    bed, maf = brr.synth_bed(seed, n, m, cfg["maf"][0], cfg["maf"][1], 0.0, threads=threads)
+    log(f"genotypes {m} x {n} generated in {time.time() - t0:.1f}s")
+    rng = np.random.default_rng(seed)
+    mc = max(1, int(round(cfg["causal"] * m)))
+    causal = np.sort(rng.choice(m, mc, replace=False))
+    b = rng.normal(0.0, np.sqrt(cfg["h2"] / mc), mc)
+    lut_c = brr.build_dense(bed[causal], n, threads=threads)
+    bits = np.unpackbits(bed[causal][..., None], axis=-1, bitorder="little").reshape(mc, -1)
+    code = (bits[:, 0::2] | (bits[:, 1::2] << 1))[:, :n]
+    dos = np.array([2, 3, 1, 0])[code]
+    g = b @ np.take_along_axis(lut_c, dos, axis=1)
+    y = (g + rng.normal(0.0, np.sqrt(1.0 - cfg["h2"]), n)).astype(np.float32).astype(np.float64)
+    t0 = time.time()
+    if cfg["layout"] == "ragged":
+        arena = brr.build_ragged(bed, n, threads=threads)
+        log(f"ragged lists built in {time.time() - t0:.1f}s ({arena.nbytes() / 1e9:.2f} GB)")
+        return dict(bed=bed, y=y, ragged=arena, lut=None)
+    lut = brr.build_dense(bed, n, threads=threads)
+    log(f"dense tables built in {time.time() - t0:.1f}s")
+    return dict(bed=bed, y=y, ragged=None, lut=lut)
+
+
+def algorithmic_bytes(cfg, data, markers_per_step, changes):
+    """Per step: sum_j B_j with B_j = G_j + 4N (1 + delta_j) (FP64 eps counted
+    as 8N), G_j = 4 (n1 + n2 + nmiss) + 32 for ragged, ceil(N/4) for dense."""
+    n, m = cfg["n"], cfg["m"]
+    if cfg["layout"] == "ragged":
+        r = data["ragged"]
+        g_total = 4.0 * (r.ones.size + r.twos.size + r.miss.size) + 32.0 * m
+    else:
+        g_total = float(m * ((n + 3) // 4))
+    eps_bytes = 8.0 * n
+    return g_total * (markers_per_step / m) + eps_bytes * (markers_per_step + changes)
+
+
+def cpu_baseline(cfg, data, seed, budget_s):
+    """Reference-style sweep on the host: the FP64 oracle built with -O3
+    -ffast-math -funroll-loops, one thread, over the first markers of the same
+    workload for a chain of a few iterations."""
+    import ctypes as C
+    sys.path.insert(0, str(ROOT / "tests"))
+    import _oracle as orc
+    L = orc.lib(fast=True)
+    n = cfg["n"]
+    m_s = min(cfg["m"], 2000 if cfg["layout"] == "ragged" else 4000)
+    iters = 0
+    t_total = 0.0
+    mk = 0
+    if cfg["layout"] == "ragged":
+        r = data["ragged"]
+
+        class Sub:
+            pass
+        sub = Sub()
+        sub.ones_off = np.ascontiguousarray(r.ones_off[: m_s + 1])
+        sub.twos_off = np.ascontiguousarray(r.twos_off[: m_s + 1])
+        sub.miss_off = np.ascontiguousarray(r.miss_off[: m_s + 1])
+        sub.ones, sub.twos, sub.miss = r.ones, r.twos, r.miss
+        sub.stats = np.ascontiguousarray(r.stats[:m_s])
+        prob = orc.OracleProblem(n, m_s, orc.RAGGED, 4, 1, S_DEFAULT, ragged=sub)
+    else:
+        prob = orc.OracleProblem(n, m_s, orc.DENSE, 4, 1, S_DEFAULT, bed=data["bed"][:m_s],
+                                 lut=data["lut"][:m_s])
+    y = np.ascontiguousarray(data["y"])
+    order = np.arange(m_s, dtype=np.uint32)
+    hist = np.zeros((1, 5))
+    # one iteration at a time until the budget is spent (same chain, continued)
+    L.orc_init(C.byref(prob.data), C.byref(prob.model), C.byref(prob.state), orc.p(y, C.c_double), seed, 0)
+    u = np.zeros(m_s)
+    z = np.zeros(m_s)
+    while t_total < budget_s and iters < cfg["chain"]:
+        t0 = time.perf_counter()
+        L.orc_update_mu(C.byref(prob.data), C.byref(prob.state), seed, iters, 0)
+        L.orc_fill_marker_uz(seed, iters, 0, m_s, orc.p(u, C.c_double), orc.p(z, C.c_double))
+        L.orc_shuffle_philox(seed, iters, 0, m_s, orc.p(order, C.c_uint32))
+        L.orc_sweep(C.byref(prob.data), C.byref(prob.model), C.byref(prob.state), orc.p(order, C.c_uint32),
+                    orc.p(u, C.c_double), orc.p(z, C.c_double), None)
+        L.orc_post_sweep(C.byref(prob.data), C.byref(prob.model), C.byref(prob.state), seed, iters, 0)
+        t_total += time.perf_counter() - t0
+        iters += 1
+        mk += m_s
+    return dict(value=mk / t_total, unit="marker-updates/s", cores=1, kind="port",
+                sample=(f"oracle (FP64 restatement of the reference sweep, -O3 -ffast-math "
+                        f"-funroll-loops, 1 thread): first {m_s} markers of the same {cfg['layout']} "
+                        f"workload at N={n}, {iters} chain iterations from init, {t_total:.1f}s"))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None, help="timed Gibbs iterations (default: chain length)")
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--seed", type=int, default=20261202)
+    ap.add_argument("--workgroups", type=int, default=0)
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py measures one GPU only: this build has no multi-GPU path "
+              f"(--gpus {args.gpus}, WORLD_SIZE {world}). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import bayesrrcmd_amd as brr
+    cfg = CONFIGS[args.config]
+    steps = args.steps if args.steps is not None else cfg["chain"]
+    n, m = cfg["n"], cfg["m"]
+    data = make_data(cfg, args.seed, args.threads)
+
+    layout = brr.LAYOUT_RAGGED if cfg["layout"] == "ragged" else brr.LAYOUT_DENSE_2BIT
+    h = brr.HipSweep(n, m, 4, 1, layout, workgroups=args.workgroups, block_markers=args.block)
+    t0 = time.time()
+    if data["ragged"] is not None:
+        h.upload_ragged(data["ragged"])
+    else:
+        h.upload_dense(data["bed"], data["lut"])
+    h.set_model(S_DEFAULT)
+    log(f"uploaded in {time.time() - t0:.1f}s")
+
+    def fresh_chain():
+        eps, sc = brr.bayesr_init(data["y"], 1, 4, args.seed)
+        h.set_state(eps, np.zeros(m), np.zeros(m, np.int32))
+        h.set_scalars(sc["mu"], sc["sigmaE"], sc["epsilon_sum"], sc["sigmaG"], sc["pi"])
+
+    order = np.arange(m, dtype=np.uint32)
+    # warm-up: a short separate chain (code paths, caches), then a fresh chain is timed
+    fresh_chain()
+    for it in range(args.warmup):
+        brr.shuffle(args.seed + 1, it, 0, order)
+        h.run_chain(args.seed + 1, 0, it, order)
+    order = np.arange(m, dtype=np.uint32)
+    fresh_chain()
+    orders = np.empty((steps, m), dtype=np.uint32)
+    for it in range(steps):
+        brr.shuffle(args.seed, it, 0, order)
+        orders[it] = order
+
+    kernel_ms, changes, rounds, phases = [], [], [], []
+    hist = []
+    t_start = time.perf_counter()
+    for it in range(steps):
+        hist.append(h.run_chain(args.seed, 0, it, orders[it])[0])
+        s = h.stats()
+        kernel_ms.append(s["kernel_ms"])
+        changes.append(s["changes"])
+        rounds.append(s["rounds"])
+        phases.append(list(s["phase_ms"].values()))
+        if it % 20 == 0 or it == steps - 1:
+            ph = ", ".join(f"{k} {v:.1f}" for k, v in s["phase_ms"].items())
+            log(f"iteration {it}: {s['kernel_ms']:.2f} ms, rounds {s['rounds']}, changes {s['changes']} "
+                f"| phases ms: {ph}")
+    wall = time.perf_counter() - t_start
+    hist = np.array(hist)
+    stats = h.stats()
+
+    total_updates = float(steps) * m
+    value = total_updates / wall
+    kernel_ms = np.array(kernel_ms)
+    avg_ms = float(kernel_ms.mean())
+    bytes_per_launch = np.mean([algorithmic_bytes(cfg, data, m, c) for c in changes])
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    sigmaE, sigmaG = hist[:, 1], hist[:, 4]
+    burn = min(steps // 2, 50)
+    h2 = float(np.mean(sigmaG[burn:] / (sigmaG[burn:] + sigmaE[burn:]))) if steps > 1 else float("nan")
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "marker-updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (Philox-seeded PLINK genotypes, HWE, y = X beta + e)",
+        "config": {
+            "workload": cfg["workload"], "N": n, "M": m, "layout": cfg["layout"],
+            "maf": list(cfg["maf"]), "causal_fraction": cfg["causal"], "h2_simulated": cfg["h2"],
+            "S": S_DEFAULT.tolist(), "timed_iterations": steps, "precision": "fp64",
+            "workgroups": stats["workgroups"], "block_markers": stats["block_markers"],
+            "slice": stats["slice"], "parallelism": "single GPU, persistent cooperative kernel",
+            "h2_posterior_mean": h2,
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "kernel": "brr::bayesr_sweep_kernel (one launch per iteration)",
+            "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": bytes_per_launch,
+            "rounds_per_launch": float(np.mean(rounds)), "changes_per_launch": float(np.mean(changes)),
+            "phase_ms_per_launch": dict(zip(brr.HipSweep.PHASES, np.mean(phases, axis=0).round(3).tolist())),
+        },
+    }
+    if not args.no_cpu:
+        log("timing the CPU reference sweep ...")
+        line["cpu_baseline"] = cpu_baseline(cfg, data, args.seed, args.cpu_budget)
+    print(json.dumps(line), flush=True)
+    h.close()
+
+
+if __name__ == "__main__":
+    main()

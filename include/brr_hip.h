@@ -156,6 +156,11 @@ int brr_hip_epsilon_change(brr_hip_ctx* ctx, uint32_t marker, double beta_old, d
 
 int brr_hip_last_stats(const brr_hip_ctx* ctx, brr_hip_stats* stats);
 
+/* Device time of workgroup 0 in each phase of the last launch, ms: block
+ * load, partial dots, exchange, decisions, commit/update, per-iteration
+ * scalars (n <= 6 values written). */
+int brr_hip_last_phase_ms(const brr_hip_ctx* ctx, double* out, uint32_t n);
+
 /* ---- host-side marker building (MarkerBuilder semantics, no device) ---- */
 
 /* PLINK .bed magic check (src/preprocessgraph.cpp:195-200): 0 if ok. */
