@@ -22,11 +22,12 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _abi
-from ._abi import LAYOUT_DENSE_2BIT, LAYOUT_RAGGED
+from ._abi import LAYOUT_DENSE_2BIT, LAYOUT_RAGGED, STORAGE_AUTO, STORAGE_CODES, STORAGE_LISTS
 
 __all__ = [
     "BrrHipError", "HipSweep", "RaggedArena", "build_dense", "build_ragged", "synth_bed",
     "bayesr_init", "shuffle", "marker_uz", "LAYOUT_DENSE_2BIT", "LAYOUT_RAGGED", "lib",
+    "STORAGE_AUTO", "STORAGE_CODES", "STORAGE_LISTS",
 ]
 
 
@@ -151,10 +152,11 @@ class HipSweep:
 
     def __init__(self, n: int, m: int, n_components: int, n_groups: int = 1,
                  layout: int = LAYOUT_RAGGED, workgroups: int = 0, block_markers: int = 0,
-                 device: int = 0):
+                 device: int = 0, ragged_storage: int = 0):
         self.n, self.m, self.K, self.G = n, m, n_components, n_groups
         self.layout = layout
-        cfg = _abi.Config(n, m, n_components, n_groups, layout, workgroups, block_markers, device)
+        cfg = _abi.Config(n, m, n_components, n_groups, layout, workgroups, block_markers, device,
+                          ragged_storage)
         ctx = _abi.CTX()
         rc = lib().brr_hip_create(C.byref(cfg), C.byref(ctx))
         if rc != 0:
@@ -270,7 +272,7 @@ class HipSweep:
     def stats(self) -> dict:
         s = _abi.Stats()
         self._ck(lib().brr_hip_last_stats(self.ctx, C.byref(s)), "brr_hip_last_stats")
-        out = {f: getattr(s, f) for f, _ in _abi.Stats._fields_ if f != "reserved"}
+        out = {f: getattr(s, f) for f, _ in _abi.Stats._fields_}
         ph = np.zeros(6)
         self._ck(lib().brr_hip_last_phase_ms(self.ctx, _p(ph, C.c_double), 6), "brr_hip_last_phase_ms")
         out["phase_ms"] = dict(zip(self.PHASES, ph.tolist()))

@@ -13,6 +13,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libbrr_hip.so"
 OK, EINVAL, ENOMEM, EDEVICE, ESTATE, ETIMEOUT = 0, 1, 2, 3, 4, 5
 LAYOUT_DENSE_2BIT = 1
 LAYOUT_RAGGED = 2
+STORAGE_AUTO, STORAGE_LISTS, STORAGE_CODES = 0, 1, 2
 
 P_u8 = C.POINTER(C.c_uint8)
 P_i32 = C.POINTER(C.c_int32)
@@ -25,7 +26,7 @@ class Config(C.Structure):
     _fields_ = [
         ("num_inds", C.c_uint32), ("num_snps", C.c_uint32), ("num_components", C.c_uint32),
         ("num_groups", C.c_uint32), ("layout", C.c_uint32), ("workgroups", C.c_uint32),
-        ("block_markers", C.c_uint32), ("device", C.c_uint32),
+        ("block_markers", C.c_uint32), ("device", C.c_uint32), ("ragged_storage", C.c_uint32),
     ]
 
 
@@ -42,7 +43,7 @@ class Scalars(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("markers", C.c_uint64), ("rounds", C.c_uint64), ("changes", C.c_uint64),
                 ("kernel_ms", C.c_double), ("workgroups", C.c_uint32), ("block_markers", C.c_uint32),
-                ("slice", C.c_uint32), ("reserved", C.c_uint32)]
+                ("slice", C.c_uint32), ("storage", C.c_uint32)]
 
 
 CTX = C.c_void_p

@@ -22,6 +22,7 @@ struct brr_hip_ctx {
     std::string err;
     uint32_t P = 0, B = 0, S = 0, R = 0;
     int layout = 0;
+    bool codes = false;          // columns held as 2-bit codes on the device
     bool have_markers = false;
     bool have_model = false;
     hipStream_t stream = nullptr;
@@ -241,14 +242,45 @@ SweepParams base_params(brr_hip_ctx* c)
     return p;
 }
 
-const void* kernel_for(uint32_t R, int layout)
+template <int R>
+const void* kernel_r(int layout, bool codes)
 {
-    const bool d = layout == kLayoutDense;
+    if (layout == kLayoutDense) return (const void*)bayesr_sweep_kernel<R, true, true>;
+    return codes ? (const void*)bayesr_sweep_kernel<R, false, true> : (const void*)bayesr_sweep_kernel<R, false, false>;
+}
+
+const void* kernel_for(uint32_t R, int layout, bool codes)
+{
     switch (R) {
-    case 1: return d ? (const void*)bayesr_sweep_kernel<1, true> : (const void*)bayesr_sweep_kernel<1, false>;
-    case 2: return d ? (const void*)bayesr_sweep_kernel<2, true> : (const void*)bayesr_sweep_kernel<2, false>;
-    case 4: return d ? (const void*)bayesr_sweep_kernel<4, true> : (const void*)bayesr_sweep_kernel<4, false>;
+    case 1: return kernel_r<1>(layout, codes);
+    case 2: return kernel_r<2>(layout, codes);
+    case 4: return kernel_r<4>(layout, codes);
     default: return nullptr;
+    }
+}
+
+// Ragged lists -> 2-bit PLINK codes, one workgroup per column: every call
+// starts as code 3 (genotype 0), ones become 2, twos 0, missing 1.
+__global__ void ragged_to_codes_kernel(const int32_t* ones, const uint64_t* ones_off,
+                                       const int32_t* twos, const uint64_t* twos_off,
+                                       const int32_t* miss, const uint64_t* miss_off,
+                                       uint64_t col_words, uint32_t* codes)
+{
+    const uint32_t j = blockIdx.x;
+    uint32_t* col = codes + (size_t)j * col_words;
+    for (uint64_t t = threadIdx.x; t < col_words; t += blockDim.x) col[t] = 0xFFFFFFFFu;
+    __syncthreads();
+    for (uint64_t q = ones_off[j] + threadIdx.x; q < ones_off[j + 1]; q += blockDim.x) {
+        const uint32_t i = (uint32_t)ones[q];
+        atomicAnd(col + (i >> 4), ~(1u << (2u * (i & 15u))));
+    }
+    for (uint64_t q = twos_off[j] + threadIdx.x; q < twos_off[j + 1]; q += blockDim.x) {
+        const uint32_t i = (uint32_t)twos[q];
+        atomicAnd(col + (i >> 4), ~(3u << (2u * (i & 15u))));
+    }
+    for (uint64_t q = miss_off[j] + threadIdx.x; q < miss_off[j + 1]; q += blockDim.x) {
+        const uint32_t i = (uint32_t)miss[q];
+        atomicAnd(col + (i >> 4), ~(2u << (2u * (i & 15u))));
     }
 }
 
@@ -263,7 +295,7 @@ int reset_exchange(brr_hip_ctx* c)
 
 int launch(brr_hip_ctx* c, SweepParams& p)
 {
-    const void* fn = kernel_for(c->R, c->layout);
+    const void* fn = kernel_for(c->R, c->layout, c->codes);
     if (!fn) return fail(c, BRR_HIP_EINVAL, "unsupported slice size");
     HIP_TRY(c, hipMemsetAsync(c->d_ctl + kCtlAbort, 0, sizeof(uint32_t) * 2, c->stream));
     HIP_TRY(c, hipMemsetAsync(c->d_counters, 0, sizeof(unsigned long long) * 8, c->stream));
@@ -285,6 +317,7 @@ int launch(brr_hip_ctx* c, SweepParams& p)
     c->stats.workgroups = c->P;
     c->stats.block_markers = c->B;
     c->stats.slice = c->S;
+    c->stats.storage = c->codes ? 2u : 1u;
     if (ctl[kCtlAbort]) {
         reset_exchange(c);
         return fail(c, BRR_HIP_ETIMEOUT, "device exchange wait exceeded its bound (site " +
@@ -325,6 +358,7 @@ int brr_hip_create(const brr_hip_config* cfg, brr_hip_ctx** out)
     if (cfg->num_groups < 1 || cfg->num_groups > (uint32_t)kMaxG) return BRR_HIP_EINVAL;
     if (cfg->layout != BRR_HIP_LAYOUT_DENSE_2BIT && cfg->layout != BRR_HIP_LAYOUT_RAGGED) return BRR_HIP_EINVAL;
     if (cfg->block_markers > (uint32_t)kMaxB) return BRR_HIP_EINVAL;
+    if (cfg->ragged_storage > 2) return BRR_HIP_EINVAL;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return BRR_HIP_EDEVICE;
     if ((int)cfg->device >= ndev) return BRR_HIP_EINVAL;
@@ -333,6 +367,7 @@ int brr_hip_create(const brr_hip_config* cfg, brr_hip_ctx** out)
     c->cfg = *cfg;
     c->device = (int)cfg->device;
     c->layout = (int)cfg->layout;
+    c->codes = c->layout == kLayoutDense;
     if (hipSetDevice(c->device) != hipSuccess) { delete c; return BRR_HIP_EDEVICE; }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, c->device) != hipSuccess) { delete c; return BRR_HIP_EDEVICE; }
@@ -353,10 +388,10 @@ int brr_hip_create(const brr_hip_config* cfg, brr_hip_ctx** out)
     c->S = S;
     c->R = R;
     c->P = (N + S - 1) / S;
-    c->B = cfg->block_markers ? cfg->block_markers : 32u;
+    c->B = cfg->block_markers ? cfg->block_markers : 64u;
 
     int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel_for(R, (int)cfg->layout), kThreads, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel_for(R, (int)cfg->layout, false), kThreads, 0) != hipSuccess ||
         (uint64_t)occ * cus < c->P) {
         delete c;
         return BRR_HIP_EDEVICE;
@@ -481,6 +516,19 @@ int brr_hip_upload_ragged(brr_hip_ctx* c, const int32_t* ones, const uint64_t* o
     build_seg_kernel<<<blocks, 256, 0, c->stream>>>(c->d_ones, c->d_ones_off, c->d_twos, c->d_twos_off,
                                                    c->d_miss, c->d_miss_off, M, c->P, c->S, c->d_seg);
     HIP_TRY(c, hipGetLastError());
+    // device copy as 2-bit codes when asked, or when smaller than the lists
+    const uint64_t list_bytes = 4ull * (n1 + n2 + nm);
+    const uint64_t code_bytes = (uint64_t)M * (((uint64_t)N + 3) / 4);
+    const uint32_t st = c->cfg.ragged_storage;
+    c->codes = st == 2 || (st == 0 && code_bytes < list_bytes);
+    if (c->codes) {
+        c->col_bytes = ((uint64_t)N + 3) / 4;
+        c->col_words = (c->col_bytes + 15) / 16 * 4;
+        if ((rc = dalloc(c, &c->d_dense, (size_t)c->col_words * M))) return rc;
+        ragged_to_codes_kernel<<<M, 256, 0, c->stream>>>(c->d_ones, c->d_ones_off, c->d_twos, c->d_twos_off,
+                                                        c->d_miss, c->d_miss_off, c->col_words, c->d_dense);
+        HIP_TRY(c, hipGetLastError());
+    }
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->have_markers = true;
     return BRR_HIP_OK;

@@ -559,7 +559,9 @@ __device__ __forceinline__ void ragged_store(const SweepParams& p, Shared<R>& L,
 
 // -------------------------------------------------------------------- kernel --
 
-template <int R, bool kDense>
+// kDenseMath: DenseBayesRRmz formulas (else SparseBayesRRG / ragged formulas).
+// kCodes: columns stored as 2-bit codes (else as ragged index lists).
+template <int R, bool kDenseMath, bool kCodes>
 __global__ void __launch_bounds__(kThreads)
 bayesr_sweep_kernel(SweepParams p)
 {
@@ -574,7 +576,7 @@ bayesr_sweep_kernel(SweepParams p)
     const ScalarLayout SL{p.G, p.K};
     const double dN = (double)p.N;
     const double NM1 = (double)(p.N - 1u);
-    constexpr bool dense = kDense;
+    constexpr bool dense = kDenseMath;
 
     // eps slice in registers: individual lo + 512 r + 8 lane + e
     double eps[R][8];
@@ -669,7 +671,7 @@ bayesr_sweep_kernel(SweepParams p)
             __syncthreads();
             MetaNext m{};
             if (tid < nb0) meta_issue(p, L.mk_next[tid], iteration, m);
-            if constexpr (kDense) {
+            if constexpr (kCodes) {
                 uint32_t words[8 * R];
                 dense_issue<R>(p, L, nb0, lo, tid, words);
                 dense_store<R>(L, nb0, tid, words);
@@ -689,7 +691,7 @@ bayesr_sweep_kernel(SweepParams p)
                 const int nb1 = (int)min(p.B, p.M - p.B);
                 if (tid < nb1) L.mk_next[tid] = order[p.B + tid];
                 __syncthreads();
-                if (!dense) {
+                if constexpr (!kCodes) {
                     uint64_t st = 0;
                     uint32_t ln = 0;
                     if (tid < 3 * nb1) seg_issue(p, L.mk_next[tid / 3], tid % 3, w, st, ln);
@@ -714,12 +716,12 @@ bayesr_sweep_kernel(SweepParams p)
             uint32_t jnext2 = 0;
             uint64_t st2 = 0;
             uint32_t ln2 = 0;
-            uint32_t words[kDense ? 8 * R : 1];
-            int32_t idx[kDense ? 1 : kIdxCap];
-            uint32_t itm[kDense ? 1 : kIdxCap / 4];
+            uint32_t words[kCodes ? 8 * R : 1];
+            int32_t idx[kCodes ? 1 : kIdxCap];
+            uint32_t itm[kCodes ? 1 : kIdxCap / 4];
             if (has1) {
                 if (tid < nb1) meta_issue(p, L.mk_next[tid], iteration, mnext);
-                if constexpr (kDense) dense_issue<R>(p, L, nb1, lo, tid, words);
+                if constexpr (kCodes) dense_issue<R>(p, L, nb1, lo, tid, words);
                 else idx_issue<R>(p, L, nb1, tid, idx, itm);
             }
             if (has2 && tid < nb2) jnext2 = order[b0 + 2 * p.B + tid];
@@ -867,7 +869,7 @@ bayesr_sweep_kernel(SweepParams p)
                     first_round = false;
                     // ragged stage 1 of block blk+2: segment bounds to registers,
                     // stored once block blk+1's segments have been consumed
-                    if (!dense && has2 && tid < 3 * nb2) seg_issue(p, L.mk_next2[tid / 3], tid % 3, w, st2, ln2);
+                    if (!kCodes && has2 && tid < 3 * nb2) seg_issue(p, L.mk_next2[tid / 3], tid % 3, w, st2, ln2);
                 }
                 const int c = L.change;
                 const int last = (c < nb) ? c : nb - 1;
@@ -894,13 +896,13 @@ bayesr_sweep_kernel(SweepParams p)
 
             // ---- commit the prefetch: block blk+1 becomes current
             if (has1) {
-                if constexpr (kDense) dense_store<R>(L, nb1, tid, words);
+                if constexpr (kCodes) dense_store<R>(L, nb1, tid, words);
                 else ragged_store<R>(p, L, nb1, lo, tid, idx, itm);
                 if (tid < nb1) meta_store<R>(p, L, tid, mnext);
                 __syncthreads();
                 if (has2) {
                     if (tid < nb2) L.mk_next[tid] = jnext2;
-                    if (!dense) seg_store_prefix<R>(L, nb2, tid, wave, lane, st2, ln2);
+                    if constexpr (!kCodes) seg_store_prefix<R>(L, nb2, tid, wave, lane, st2, ln2);
                     else __syncthreads();
                 }
             }

@@ -74,17 +74,18 @@ def make_data(cfg, seed, threads):
     return dict(bed=bed, y=y, ragged=None, lut=lut)
 
 
-def algorithmic_bytes(cfg, data, markers_per_step, changes):
-    """Per step: sum_j B_j with B_j = G_j + 4N (1 + delta_j) (FP64 eps counted
-    as 8N), G_j = 4 (n1 + n2 + nmiss) + 32 for ragged, ceil(N/4) for dense."""
+def algorithmic_bytes(cfg, data, storage, markers_per_step, changes, eps_elem=8):
+    """Per step: sum_j B_j with B_j = G_j + e N (1 + delta_j): G_j the column
+    bytes of the device copy (index lists: 4 (n1 + n2 + nmiss) + 32; 2-bit
+    codes: ceil(N/4)), e N one read of eps (e = 8 bytes in FP64), delta_j = 1
+    when marker j's beta changed (one more eps pass, the update)."""
     n, m = cfg["n"], cfg["m"]
-    if cfg["layout"] == "ragged":
+    if cfg["layout"] == "ragged" and storage == 1:
         r = data["ragged"]
         g_total = 4.0 * (r.ones.size + r.twos.size + r.miss.size) + 32.0 * m
     else:
         g_total = float(m * ((n + 3) // 4))
-    eps_bytes = 8.0 * n
-    return g_total * (markers_per_step / m) + eps_bytes * (markers_per_step + changes)
+    return g_total * (markers_per_step / m) + eps_elem * n * (markers_per_step + changes)
 
 
 def cpu_baseline(cfg, data, seed, budget_s):
@@ -150,6 +151,10 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="run a 1 GiB device copy first (reference kernel for HBM counters)")
+    ap.add_argument("--ragged-storage", default="auto", choices=["auto", "lists", "codes"],
+                    help="device copy of ragged columns: smaller of lists / 2-bit codes, or forced")
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
     args = ap.parse_args()
 
@@ -160,13 +165,27 @@ def main():
         sys.exit(2)
 
     import bayesrrcmd_amd as brr
+    if args.calibrate:
+        # a plain device copy of known size: FETCH_SIZE / WRITE_SIZE of this
+        # kernel give the counter scale for the sweep's HBM traffic
+        import torch
+        a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log("calibration copy: 1 GiB read + 1 GiB written")
+        del a, b
     cfg = CONFIGS[args.config]
     steps = args.steps if args.steps is not None else cfg["chain"]
     n, m = cfg["n"], cfg["m"]
     data = make_data(cfg, args.seed, args.threads)
 
     layout = brr.LAYOUT_RAGGED if cfg["layout"] == "ragged" else brr.LAYOUT_DENSE_2BIT
-    h = brr.HipSweep(n, m, 4, 1, layout, workgroups=args.workgroups, block_markers=args.block)
+    storage = {"auto": brr.STORAGE_AUTO, "lists": brr.STORAGE_LISTS, "codes": brr.STORAGE_CODES}[args.ragged_storage]
+    h = brr.HipSweep(n, m, 4, 1, layout, workgroups=args.workgroups, block_markers=args.block,
+                     ragged_storage=storage)
     t0 = time.time()
     if data["ragged"] is not None:
         h.upload_ragged(data["ragged"])
@@ -215,7 +234,7 @@ def main():
     value = total_updates / wall
     kernel_ms = np.array(kernel_ms)
     avg_ms = float(kernel_ms.mean())
-    bytes_per_launch = np.mean([algorithmic_bytes(cfg, data, m, c) for c in changes])
+    bytes_per_launch = np.mean([algorithmic_bytes(cfg, data, stats["storage"], m, c) for c in changes])
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     sigmaE, sigmaG = hist[:, 1], hist[:, 4]
     burn = min(steps // 2, 50)
@@ -236,6 +255,7 @@ def main():
         "data": "synthetic (Philox-seeded PLINK genotypes, HWE, y = X beta + e)",
         "config": {
             "workload": cfg["workload"], "N": n, "M": m, "layout": cfg["layout"],
+            "device_storage": "2-bit codes" if stats["storage"] == 2 else "index lists",
             "maf": list(cfg["maf"]), "causal_fraction": cfg["causal"], "h2_simulated": cfg["h2"],
             "S": S_DEFAULT.tolist(), "timed_iterations": steps, "precision": "fp64",
             "workgroups": stats["workgroups"], "block_markers": stats["block_markers"],

@@ -63,6 +63,9 @@ typedef struct {
     uint32_t workgroups;     /* persistent workgroups (0 = choose) */
     uint32_t block_markers;  /* markers decided per speculation block, 1..64 (0 = choose) */
     uint32_t device;         /* HIP device ordinal */
+    uint32_t ragged_storage; /* ragged layout only: device copy of the columns as
+                                0 = whichever is smaller (index lists or 2-bit codes),
+                                1 = index lists, 2 = 2-bit codes built on the device */
 } brr_hip_config;
 
 /* Model constants: Options::S (G x (K-1), row-major) and the priors
@@ -93,7 +96,7 @@ typedef struct {
     uint32_t workgroups;   /* persistent workgroups used */
     uint32_t block_markers;
     uint32_t slice;        /* individuals per workgroup */
-    uint32_t reserved;
+    uint32_t storage;      /* device copy of the columns: 1 index lists, 2 2-bit codes */
 } brr_hip_stats;
 
 int brr_hip_abi_version(void);

@@ -11,7 +11,7 @@
  * by a counter-based Philox4x32-10 stream whose exact definition is given in
  * oracle/rng.c (the product implements the same stream independently).
  *
- * Pinning status of each part is listed in oracle/README.md.
+ * The pinning status of each part is listed in README.md next to this file.
  */
 #ifndef BRR_ORACLE_H
 #define BRR_ORACLE_H

@@ -42,13 +42,14 @@ def test_create_rejects_bad_config():
     lib = brr.lib()
     ctx = _abi.CTX()
     bad = [
-        _abi.Config(0, 10, 4, 1, 2, 0, 0, 0),     # N < 2
-        _abi.Config(10, 0, 4, 1, 2, 0, 0, 0),     # M = 0
-        _abi.Config(10, 10, 1, 1, 2, 0, 0, 0),    # K < 2
-        _abi.Config(10, 10, 9, 1, 2, 0, 0, 0),    # K > 8
-        _abi.Config(10, 10, 4, 0, 2, 0, 0, 0),    # G = 0
-        _abi.Config(10, 10, 4, 1, 7, 0, 0, 0),    # unknown layout
-        _abi.Config(10, 10, 4, 1, 2, 0, 65, 0),   # block > 64
+        _abi.Config(0, 10, 4, 1, 2, 0, 0, 0, 0),     # N < 2
+        _abi.Config(10, 0, 4, 1, 2, 0, 0, 0, 0),     # M = 0
+        _abi.Config(10, 10, 1, 1, 2, 0, 0, 0, 0),    # K < 2
+        _abi.Config(10, 10, 9, 1, 2, 0, 0, 0, 0),    # K > 8
+        _abi.Config(10, 10, 4, 0, 2, 0, 0, 0, 0),    # G = 0
+        _abi.Config(10, 10, 4, 1, 7, 0, 0, 0, 0),    # unknown layout
+        _abi.Config(10, 10, 4, 1, 2, 0, 65, 0, 0),   # block > 64
+        _abi.Config(10, 10, 4, 1, 2, 0, 0, 0, 3),    # unknown ragged storage
     ]
     for cfg in bad:
         assert lib.brr_hip_create(C.byref(cfg), C.byref(ctx)) == _abi.EINVAL
@@ -60,7 +61,7 @@ def test_create_without_device_fails_cleanly():
     assert lib.brr_hip_device_count(C.byref(n)) == 0
     if n.value == 0:
         ctx = _abi.CTX()
-        cfg = _abi.Config(100, 10, 4, 1, 2, 0, 0, 0)
+        cfg = _abi.Config(100, 10, 4, 1, 2, 0, 0, 0, 0)
         assert lib.brr_hip_create(C.byref(cfg), C.byref(ctx)) == _abi.EDEVICE
         assert not ctx.value
 

@@ -30,14 +30,21 @@ def layout_code(layout):
     return brr.LAYOUT_DENSE_2BIT if layout == orc.DENSE else brr.LAYOUT_RAGGED
 
 
-def make_pair(pr, layout, K=4, G=1, group=None, cva=S, workgroups=0, block=0, seed=5):
+# (oracle layout, device ragged storage): dense, ragged kept as index lists,
+# ragged converted to 2-bit codes on the device
+LAYOUTS = [(orc.DENSE, brr.STORAGE_AUTO), (orc.RAGGED, brr.STORAGE_LISTS), (orc.RAGGED, brr.STORAGE_CODES)]
+LAYOUT_IDS = ["dense", "ragged-lists", "ragged-codes"]
+
+
+def make_pair(pr, layout, K=4, G=1, group=None, cva=S, workgroups=0, block=0, seed=5, storage=0):
     cv = np.tile(cva, G)
     if layout == orc.DENSE:
         o = orc.OracleProblem(pr.n, pr.m, orc.DENSE, K, G, cv, group=group, bed=pr.bed, lut=pr.lut)
     else:
         o = orc.OracleProblem(pr.n, pr.m, orc.RAGGED, K, G, cv, group=group, ragged=pr.ragged)
     o.init(pr.y, seed)
-    h = brr.HipSweep(pr.n, pr.m, K, G, layout_code(layout), workgroups=workgroups, block_markers=block)
+    h = brr.HipSweep(pr.n, pr.m, K, G, layout_code(layout), workgroups=workgroups, block_markers=block,
+                     ragged_storage=storage)
     if layout == orc.DENSE:
         h.upload_dense(pr.bed, pr.lut)
     else:
@@ -72,11 +79,11 @@ def sweep_inputs(m, it, seed=77):
     return order, u, z
 
 
-@pytest.mark.parametrize("layout", [orc.DENSE, orc.RAGGED])
-@pytest.mark.parametrize("workgroups,block", [(0, 0), (3, 1), (17, 7), (64, 64), (0, 13)])
-def test_sweep_parity(gpu_lib, layout, workgroups, block):
+@pytest.mark.parametrize("layout,storage", LAYOUTS, ids=LAYOUT_IDS)
+@pytest.mark.parametrize("workgroups,block", [(0, 0), (3, 1), (17, 7), (64, 64), (0, 13), (0, 63)])
+def test_sweep_parity(gpu_lib, layout, storage, workgroups, block):
     pr = make_problem(3001, 700, seed=101, h2=0.5, causal_frac=0.02)
-    o, h = make_pair(pr, layout, workgroups=workgroups, block=block)
+    o, h = make_pair(pr, layout, workgroups=workgroups, block=block, storage=storage)
     for it in range(3):
         order, u, z = sweep_inputs(pr.m, it)
         o.sweep(order, u, z, trace=False)
@@ -84,16 +91,17 @@ def test_sweep_parity(gpu_lib, layout, workgroups, block):
         compare_state(o, h, layout, f"iteration {it}")
     s = h.stats()
     assert s["markers"] == pr.m and s["rounds"] >= (pr.m + s["block_markers"] - 1) // s["block_markers"]
+    assert s["storage"] == (2 if layout == orc.DENSE else storage)
     h.close()
 
 
-@pytest.mark.parametrize("layout", [orc.DENSE, orc.RAGGED])
-def test_sweep_parity_missing_and_groups(gpu_lib, layout):
+@pytest.mark.parametrize("layout,storage", LAYOUTS, ids=LAYOUT_IDS)
+def test_sweep_parity_missing_and_groups(gpu_lib, layout, storage):
     pr = make_problem(2503, 400, seed=103, miss=0.05, causal_frac=0.05)
     G = 3
     group = (np.arange(pr.m) % G).astype(np.int32)
     cva = np.array([0.05, 0.005, 0.0005])
-    o, h = make_pair(pr, layout, G=G, group=group, cva=cva)
+    o, h = make_pair(pr, layout, G=G, group=group, cva=cva, storage=storage)
     for it in range(2):
         order, u, z = sweep_inputs(pr.m, it, seed=9)
         o.sweep(order, u, z, trace=False)
@@ -102,12 +110,12 @@ def test_sweep_parity_missing_and_groups(gpu_lib, layout):
     h.close()
 
 
-@pytest.mark.parametrize("layout", [orc.DENSE, orc.RAGGED])
-@pytest.mark.parametrize("K", [2, 3, 5])
-def test_sweep_parity_component_counts(gpu_lib, layout, K):
+@pytest.mark.parametrize("layout,storage", LAYOUTS, ids=LAYOUT_IDS)
+@pytest.mark.parametrize("K", [2, 3, 5, 8])
+def test_sweep_parity_component_counts(gpu_lib, layout, storage, K):
     pr = make_problem(1500, 300, seed=105, causal_frac=0.05)
-    cva = np.array([0.02, 0.002, 0.0002, 0.00002])[: K - 1]
-    o, h = make_pair(pr, layout, K=K, cva=cva)
+    cva = np.array([0.02, 0.002, 0.0002, 0.00002, 2e-6, 2e-7, 2e-8])[: K - 1]
+    o, h = make_pair(pr, layout, K=K, cva=cva, storage=storage)
     order, u, z = sweep_inputs(pr.m, 0)
     o.sweep(order, u, z, trace=False)
     h.sweep(order, u, z)
@@ -118,8 +126,8 @@ def test_sweep_parity_component_counts(gpu_lib, layout, K):
 @pytest.mark.parametrize("n,m", [(37, 5), (513, 1), (1025, 64), (2048, 9)])
 def test_sweep_parity_edge_sizes(gpu_lib, n, m):
     pr = make_problem(n, m, seed=107 + n, causal_frac=0.2)
-    for layout in (orc.DENSE, orc.RAGGED):
-        o, h = make_pair(pr, layout)
+    for layout, storage in LAYOUTS:
+        o, h = make_pair(pr, layout, storage=storage)
         order, u, z = sweep_inputs(pr.m, 0)
         o.sweep(order, u, z, trace=False)
         h.sweep(order, u, z)
@@ -127,10 +135,11 @@ def test_sweep_parity_edge_sizes(gpu_lib, n, m):
         h.close()
 
 
-def test_sweep_parity_every_marker_changes(gpu_lib):
+@pytest.mark.parametrize("storage", [brr.STORAGE_LISTS, brr.STORAGE_CODES])
+def test_sweep_parity_every_marker_changes(gpu_lib, storage):
     """pi_0 tiny: nearly every decision changes beta, one exchange round each."""
     pr = make_problem(2000, 300, seed=109)
-    o, h = make_pair(pr, orc.RAGGED, block=32)
+    o, h = make_pair(pr, orc.RAGGED, block=32, storage=storage)
     pi = np.array([[1e-6, 0.4, 0.3, 0.3]])
     o.set_scalars(o.mu, o.sigmaE, o.epsSum, o.sigmaG, pi)
     h.set_scalars(o.mu, o.sigmaE, o.epsSum, o.sigmaG, pi)
@@ -143,10 +152,10 @@ def test_sweep_parity_every_marker_changes(gpu_lib):
     h.close()
 
 
-@pytest.mark.parametrize("layout", [orc.DENSE, orc.RAGGED])
-def test_per_marker_kernels(gpu_lib, layout):
+@pytest.mark.parametrize("layout,storage", LAYOUTS, ids=LAYOUT_IDS)
+def test_per_marker_kernels(gpu_lib, layout, storage):
     pr = make_problem(3000, 50, seed=111, miss=0.02)
-    o, h = make_pair(pr, layout)
+    o, h = make_pair(pr, layout, storage=storage)
     rng = np.random.default_rng(0)
     for j in rng.choice(pr.m, 12, replace=False):
         j = int(j)
@@ -163,8 +172,8 @@ def test_per_marker_kernels(gpu_lib, layout):
     h.close()
 
 
-@pytest.mark.parametrize("layout", [orc.DENSE, orc.RAGGED])
-def test_device_chain_matches_oracle_gibbs(gpu_lib, layout):
+@pytest.mark.parametrize("layout,storage", LAYOUTS, ids=LAYOUT_IDS)
+def test_device_chain_matches_oracle_gibbs(gpu_lib, layout, storage):
     pr = make_problem(2500, 500, seed=113, h2=0.5, causal_frac=0.02)
     seed, iters = 20261202, 6
     o = orc.OracleProblem(pr.n, pr.m, layout, 4, 1, S, bed=pr.bed, lut=pr.lut) if layout == orc.DENSE \
@@ -178,7 +187,7 @@ def test_device_chain_matches_oracle_gibbs(gpu_lib, layout):
         brr.shuffle(seed, it, 0, cur)
         orders[it] = cur
     assert np.array_equal(orders[-1], order_o)
-    h = brr.HipSweep(pr.n, pr.m, 4, 1, layout_code(layout))
+    h = brr.HipSweep(pr.n, pr.m, 4, 1, layout_code(layout), ragged_storage=storage)
     h.upload_dense(pr.bed, pr.lut) if layout == orc.DENSE else h.upload_ragged(pr.ragged)
     h.set_model(S)
     h.set_state(eps, np.zeros(pr.m), np.zeros(pr.m, np.int32))
@@ -196,13 +205,14 @@ def test_device_chain_matches_oracle_gibbs(gpu_lib, layout):
     h.close()
 
 
-def test_large_ragged_residual_identity(gpu_lib):
+@pytest.mark.parametrize("storage", [brr.STORAGE_LISTS, brr.STORAGE_CODES])
+def test_large_ragged_residual_identity(gpu_lib, storage):
     """At N = 100,000: after a device chain eps equals y_std - mu - X beta
     (size-independent identity), recomputed on the host from the ragged lists."""
     pr = make_problem(100_000, 400, seed=115, maf=(0.005, 0.5), h2=0.4, causal_frac=0.02)
     seed = 3
     eps0, sc = brr.bayesr_init(pr.y, 1, 4, seed)
-    h = brr.HipSweep(pr.n, pr.m, 4, 1, brr.LAYOUT_RAGGED)
+    h = brr.HipSweep(pr.n, pr.m, 4, 1, brr.LAYOUT_RAGGED, ragged_storage=storage)
     h.upload_ragged(pr.ragged)
     h.set_model(S)
     h.set_state(eps0, np.zeros(pr.m), np.zeros(pr.m, np.int32))
