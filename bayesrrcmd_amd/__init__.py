@@ -152,16 +152,20 @@ class HipSweep:
 
     def __init__(self, n: int, m: int, n_components: int, n_groups: int = 1,
                  layout: int = LAYOUT_RAGGED, workgroups: int = 0, block_markers: int = 0,
-                 device: int = 0, ragged_storage: int = 0):
+                 device: int = 0, ragged_storage: int = 0, traits: int = 0):
+        """traits = 0: one chain; traits = T >= 1: T chains over the same columns and
+        order (state arrays gain a trailing trait axis)."""
         self.n, self.m, self.K, self.G = n, m, n_components, n_groups
         self.layout = layout
+        self.traits = traits
+        self.Tn = max(1, traits)
         cfg = _abi.Config(n, m, n_components, n_groups, layout, workgroups, block_markers, device,
-                          ragged_storage)
+                          ragged_storage, traits)
         ctx = _abi.CTX()
         rc = lib().brr_hip_create(C.byref(cfg), C.byref(ctx))
         if rc != 0:
             raise BrrHipError(f"brr_hip_create failed with status {rc} (N={n}, M={m}, K={n_components}, "
-                              f"G={n_groups}, layout={layout}, workgroups={workgroups})")
+                              f"G={n_groups}, layout={layout}, workgroups={workgroups}, traits={traits})")
         self.ctx = ctx
         self._keep = []
 
@@ -209,52 +213,67 @@ class HipSweep:
         self._ck(lib().brr_hip_set_model(self.ctx, C.byref(mdl)), "brr_hip_set_model")
 
     # state
+    def _shape(self, rows):
+        return (rows, self.traits) if self.traits else (rows,)
+
+    def _arr(self, a, rows, dtype, what):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, dtype=dtype)
+        if a.shape != self._shape(rows):
+            raise ValueError(f"{what}: shape {a.shape}, expected {self._shape(rows)}")
+        return a
+
     def set_state(self, eps=None, beta=None, comp=None):
-        e = None if eps is None else np.ascontiguousarray(eps, dtype=np.float64)
-        b = None if beta is None else np.ascontiguousarray(beta, dtype=np.float64)
-        c = None if comp is None else np.ascontiguousarray(comp, dtype=np.int32)
+        """eps (N,), beta (M,), comp (M,); (N, T) / (M, T) for a batched context."""
+        e = self._arr(eps, self.n, np.float64, "eps")
+        b = self._arr(beta, self.m, np.float64, "beta")
+        c = self._arr(comp, self.m, np.int32, "comp")
         self._ck(lib().brr_hip_set_state(self.ctx, _p(e, C.c_double), _p(b, C.c_double), _p(c, C.c_int32)),
                  "brr_hip_set_state")
 
-    def set_scalars(self, mu, sigmaE, epsilon_sum, sigmaG, pi):
+    def set_scalars(self, mu, sigmaE, epsilon_sum, sigmaG, pi, trait: int = 0):
         sg = np.ascontiguousarray(sigmaG, dtype=np.float64).reshape(self.G)
         pp = np.ascontiguousarray(pi, dtype=np.float64).reshape(self.G * self.K)
         s = _abi.Scalars(mu, sigmaE, epsilon_sum, _p(sg, C.c_double), _p(pp, C.c_double), None, None)
-        self._ck(lib().brr_hip_set_scalars(self.ctx, C.byref(s)), "brr_hip_set_scalars")
+        self._ck(lib().brr_hip_set_trait_scalars(self.ctx, trait, C.byref(s)), "brr_hip_set_trait_scalars")
 
     def get_state(self):
-        eps = np.empty(self.n, dtype=np.float64)
-        beta = np.empty(self.m, dtype=np.float64)
-        comp = np.empty(self.m, dtype=np.int32)
-        acum = np.empty(self.m, dtype=np.float64)
+        eps = np.empty(self._shape(self.n), dtype=np.float64)
+        beta = np.empty(self._shape(self.m), dtype=np.float64)
+        comp = np.empty(self._shape(self.m), dtype=np.int32)
+        acum = np.empty(self._shape(self.m), dtype=np.float64)
         self._ck(lib().brr_hip_get_state(self.ctx, _p(eps, C.c_double), _p(beta, C.c_double),
                                          _p(comp, C.c_int32), _p(acum, C.c_double)), "brr_hip_get_state")
         return dict(eps=eps, beta=beta, comp=comp, acum=acum)
 
-    def get_scalars(self):
+    def get_scalars(self, trait: int = 0):
         sg = np.empty(self.G, dtype=np.float64)
         pi = np.empty(self.G * self.K, dtype=np.float64)
         v = np.empty(self.G * self.K, dtype=np.float64)
         bs = np.empty(self.G, dtype=np.float64)
         s = _abi.Scalars(0.0, 0.0, 0.0, _p(sg, C.c_double), _p(pi, C.c_double), _p(v, C.c_double),
                          _p(bs, C.c_double))
-        self._ck(lib().brr_hip_get_scalars(self.ctx, C.byref(s)), "brr_hip_get_scalars")
+        self._ck(lib().brr_hip_get_trait_scalars(self.ctx, trait, C.byref(s)), "brr_hip_get_trait_scalars")
         return dict(mu=s.mu, sigmaE=s.sigmaE, epsilon_sum=s.epsilon_sum, sigmaG=sg,
                     pi=pi.reshape(self.G, self.K), v=v.reshape(self.G, self.K), betasqn=bs)
 
     # compute
     def sweep(self, order: np.ndarray, u: np.ndarray, z: np.ndarray):
+        """u, z by marker id: (M,), or (M, T) for a batched context."""
         o = np.ascontiguousarray(order, dtype=np.uint32)
-        uu = np.ascontiguousarray(u, dtype=np.float64)
-        zz = np.ascontiguousarray(z, dtype=np.float64)
+        uu = self._arr(u, self.m, np.float64, "u")
+        zz = self._arr(z, self.m, np.float64, "z")
         self._ck(lib().brr_hip_sweep(self.ctx, _p(o, C.c_uint32), _p(uu, C.c_double), _p(zz, C.c_double)),
                  "brr_hip_sweep")
 
     def run_chain(self, seed: int, chain: int, first_iteration: int, orders: np.ndarray) -> np.ndarray:
+        """hist: (iterations, 4 + G), or (iterations, T, 5) for a batched context."""
         o = np.ascontiguousarray(orders, dtype=np.uint32)
         iters = o.size // self.m
         assert iters * self.m == o.size
-        hist = np.empty((iters, 4 + self.G), dtype=np.float64)
+        shape = (iters, self.traits, 4 + self.G) if self.traits else (iters, 4 + self.G)
+        hist = np.empty(shape, dtype=np.float64)
         self._ck(lib().brr_hip_run_chain(self.ctx, seed, chain, first_iteration, iters, _p(o, C.c_uint32),
                                          _p(hist, C.c_double)), "brr_hip_run_chain")
         return hist

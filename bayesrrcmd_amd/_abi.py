@@ -27,6 +27,7 @@ class Config(C.Structure):
         ("num_inds", C.c_uint32), ("num_snps", C.c_uint32), ("num_components", C.c_uint32),
         ("num_groups", C.c_uint32), ("layout", C.c_uint32), ("workgroups", C.c_uint32),
         ("block_markers", C.c_uint32), ("device", C.c_uint32), ("ragged_storage", C.c_uint32),
+        ("num_traits", C.c_uint32),
     ]
 
 
@@ -63,6 +64,8 @@ SIGNATURES = {
     "brr_hip_set_scalars": (C.c_int, [CTX, C.POINTER(Scalars)]),
     "brr_hip_get_state": (C.c_int, [CTX, P_f64, P_f64, P_i32, P_f64]),
     "brr_hip_get_scalars": (C.c_int, [CTX, C.POINTER(Scalars)]),
+    "brr_hip_set_trait_scalars": (C.c_int, [CTX, C.c_uint32, C.POINTER(Scalars)]),
+    "brr_hip_get_trait_scalars": (C.c_int, [CTX, C.c_uint32, C.POINTER(Scalars)]),
     "brr_hip_sweep": (C.c_int, [CTX, P_u32, P_f64, P_f64]),
     "brr_hip_run_chain": (C.c_int, [CTX, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, P_u32, P_f64]),
     "brr_hip_compute_num": (C.c_int, [CTX, C.c_uint32, C.c_double, P_f64]),

@@ -13,6 +13,7 @@
 
 #include "brr_hip.h"
 #include "sweep_kernel.hpp"
+#include "batch_kernel.hpp"
 
 using namespace brr;
 
@@ -21,6 +22,10 @@ struct brr_hip_ctx {
     int device = 0;
     std::string err;
     uint32_t P = 0, B = 0, S = 0, R = 0;
+    uint32_t T = 0;              // traits of a batched context (0: one chain)
+    uint32_t Tn = 1;             // columns of the per-individual / per-marker state
+    uint32_t nt = 0, NT = 0;     // batched: 16-individual tiles per slice, kernel variant
+    size_t x1_words = 0, x2_words = 0;
     int layout = 0;
     bool codes = false;          // columns held as 2-bit codes on the device
     bool have_markers = false;
@@ -259,6 +264,54 @@ const void* kernel_for(uint32_t R, int layout, bool codes)
     }
 }
 
+// Batched kernels by register tiles per slice (4, 8, 13 or 16 tiles of 16
+// individuals; 13 tiles fit N = 50,000 on 241 workgroups).
+const void* batch_kernel_for(uint32_t NT, int layout)
+{
+    const bool d = layout == kLayoutDense;
+    switch (NT) {
+    case 4: return d ? (const void*)bayesr_batch_kernel<4, true> : (const void*)bayesr_batch_kernel<4, false>;
+    case 8: return d ? (const void*)bayesr_batch_kernel<8, true> : (const void*)bayesr_batch_kernel<8, false>;
+    case 13: return d ? (const void*)bayesr_batch_kernel<13, true> : (const void*)bayesr_batch_kernel<13, false>;
+    case 16: return d ? (const void*)bayesr_batch_kernel<16, true> : (const void*)bayesr_batch_kernel<16, false>;
+    default: return nullptr;
+    }
+}
+
+BatchParams batch_params(brr_hip_ctx* c)
+{
+    BatchParams p{};
+    p.N = c->cfg.num_inds;
+    p.M = c->cfg.num_snps;
+    p.K = c->cfg.num_components;
+    p.T = c->T;
+    p.P = c->P;
+    p.S = c->S;
+    p.nt = c->nt;
+    p.nq = (c->T + 15) / 16;
+    p.layout = c->layout;
+    p.codes = c->d_dense;
+    p.col_words = c->col_words;
+    p.col_bytes = c->col_bytes;
+    p.lut = c->d_lut;
+    p.stats = c->d_stats;
+    p.cva = c->d_cva;
+    p.v0E = c->model.v0E;
+    p.s02E = c->model.s02E;
+    p.v0G = c->model.v0G;
+    p.s02G = c->model.s02G;
+    p.eps = c->d_eps;
+    p.beta = c->d_beta;
+    p.comp = c->d_comp;
+    p.acum = c->d_acum;
+    p.scal = c->d_scal;
+    p.xchg1 = c->d_xchg1;
+    p.xchg2 = c->d_xchg2;
+    p.ctl = c->d_ctl;
+    p.counters = c->d_counters;
+    return p;
+}
+
 // Ragged lists -> 2-bit PLINK codes, one workgroup per column: every call
 // starts as code 3 (genotype 0), ones become 2, twos 0, missing 1.
 __global__ void ragged_to_codes_kernel(const int32_t* ones, const uint64_t* ones_off,
@@ -286,20 +339,20 @@ __global__ void ragged_to_codes_kernel(const int32_t* ones, const uint64_t* ones
 
 int reset_exchange(brr_hip_ctx* c)
 {
-    HIP_TRY(c, hipMemsetAsync(c->d_xchg1, 0, sizeof(uint64_t) * 2 * kMaxB * c->P * 2, c->stream));
-    HIP_TRY(c, hipMemsetAsync(c->d_xchg2, 0, sizeof(uint64_t) * 2 * kMaxB * 2, c->stream));
+    HIP_TRY(c, hipMemsetAsync(c->d_xchg1, 0, sizeof(uint64_t) * c->x1_words, c->stream));
+    HIP_TRY(c, hipMemsetAsync(c->d_xchg2, 0, sizeof(uint64_t) * c->x2_words, c->stream));
     HIP_TRY(c, hipMemsetAsync(c->d_ctl, 0, sizeof(uint32_t) * 8, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return BRR_HIP_OK;
 }
 
-int launch(brr_hip_ctx* c, SweepParams& p)
+// One cooperative launch of the context's kernel with argument block `param`.
+int launch_fn(brr_hip_ctx* c, const void* fn, void* param)
 {
-    const void* fn = kernel_for(c->R, c->layout, c->codes);
     if (!fn) return fail(c, BRR_HIP_EINVAL, "unsupported slice size");
     HIP_TRY(c, hipMemsetAsync(c->d_ctl + kCtlAbort, 0, sizeof(uint32_t) * 2, c->stream));
     HIP_TRY(c, hipMemsetAsync(c->d_counters, 0, sizeof(unsigned long long) * 8, c->stream));
-    void* args[] = {&p};
+    void* args[] = {param};
     HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
     HIP_TRY(c, hipLaunchCooperativeKernel(fn, dim3(c->P), dim3(kThreads), args, 0, c->stream));
     HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
@@ -315,7 +368,7 @@ int launch(brr_hip_ctx* c, SweepParams& p)
     c->stats.changes = counters[1];
     for (int q = 0; q < kPhases; ++q) c->phase_ms[q] = (double)counters[2 + q] * 1e-5;
     c->stats.workgroups = c->P;
-    c->stats.block_markers = c->B;
+    c->stats.block_markers = c->T ? (uint32_t)kBB : c->B;
     c->stats.slice = c->S;
     c->stats.storage = c->codes ? 2u : 1u;
     if (ctl[kCtlAbort]) {
@@ -324,6 +377,32 @@ int launch(brr_hip_ctx* c, SweepParams& p)
                                              std::to_string(ctl[kCtlAbortWhere]) + ")");
     }
     return BRR_HIP_OK;
+}
+
+int launch(brr_hip_ctx* c, SweepParams& p)
+{
+    return launch_fn(c, kernel_for(c->R, c->layout, c->codes), &p);
+}
+
+// Batched contexts: the same launch with the sweep-mode / chain-mode fields of p.
+int launch_batch(brr_hip_ctx* c, const SweepParams& p)
+{
+    BatchParams b = batch_params(c);
+    b.mode = p.mode;
+    b.iterations = p.iterations;
+    b.iter_base = p.iter_base;
+    b.chain = p.chain;
+    b.seed = p.seed;
+    b.order = p.order;
+    b.u = p.u;
+    b.z = p.z;
+    b.hist = p.hist;
+    return launch_fn(c, batch_kernel_for(c->NT, c->layout), &b);
+}
+
+int launch_any(brr_hip_ctx* c, SweepParams& p)
+{
+    return c->T ? launch_batch(c, p) : launch(c, p);
 }
 
 int check_ready(brr_hip_ctx* c)
@@ -359,6 +438,8 @@ int brr_hip_create(const brr_hip_config* cfg, brr_hip_ctx** out)
     if (cfg->layout != BRR_HIP_LAYOUT_DENSE_2BIT && cfg->layout != BRR_HIP_LAYOUT_RAGGED) return BRR_HIP_EINVAL;
     if (cfg->block_markers > (uint32_t)kMaxB) return BRR_HIP_EINVAL;
     if (cfg->ragged_storage > 2) return BRR_HIP_EINVAL;
+    if (cfg->num_traits > (uint32_t)kMaxT) return BRR_HIP_EINVAL;
+    if (cfg->num_traits > 0 && cfg->num_groups != 1) return BRR_HIP_EINVAL;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return BRR_HIP_EDEVICE;
     if ((int)cfg->device >= ndev) return BRR_HIP_EINVAL;
@@ -382,31 +463,49 @@ int brr_hip_create(const brr_hip_config* cfg, brr_hip_ctx** out)
     Preq = std::min(Preq, cus);
     uint32_t S = (N + Preq - 1) / Preq;
     S = (S + kSliceAlign - 1) / kSliceAlign * kSliceAlign;
-    uint32_t R = (S + kRun - 1) / kRun;
-    if (R > (uint32_t)kMaxRuns) { delete c; return BRR_HIP_EINVAL; }
-    if (R == 3) R = 4;
+    c->T = cfg->num_traits;
+    c->Tn = c->T ? c->T : 1u;
+    const void* probe = nullptr;
+    if (c->T) {
+        // batched traits: residuals of a slice in registers as 16-row tiles;
+        // ragged columns are held as 2-bit codes
+        c->nt = S / 16;
+        if (c->nt > (uint32_t)kMaxNT) { delete c; return BRR_HIP_EINVAL; }
+        c->NT = c->nt <= 4 ? 4u : (c->nt <= 8 ? 8u : (c->nt <= 13 ? 13u : 16u));
+        c->codes = true;
+        c->B = kBB;
+        probe = batch_kernel_for(c->NT, c->layout);
+    } else {
+        uint32_t R = (S + kRun - 1) / kRun;
+        if (R > (uint32_t)kMaxRuns) { delete c; return BRR_HIP_EINVAL; }
+        if (R == 3) R = 4;
+        c->R = R;
+        c->B = cfg->block_markers ? cfg->block_markers : 64u;
+        probe = kernel_for(R, (int)cfg->layout, false);
+    }
     c->S = S;
-    c->R = R;
     c->P = (N + S - 1) / S;
-    c->B = cfg->block_markers ? cfg->block_markers : 64u;
+    c->x1_words = c->T ? (size_t)2 * c->P * kBVmax * 2 : (size_t)2 * kMaxB * c->P * 2;
+    c->x2_words = c->T ? (size_t)2 * kBVmax * 2 : (size_t)2 * kMaxB * 2;
 
     int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel_for(R, (int)cfg->layout, false), kThreads, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, probe, kThreads, 0) != hipSuccess ||
         (uint64_t)occ * cus < c->P) {
         delete c;
         return BRR_HIP_EDEVICE;
     }
 
     const uint32_t M = cfg->num_snps, G = cfg->num_groups, K = cfg->num_components;
+    const size_t Tn = c->Tn;
     const ScalarLayout SL{G, K};
     int rc = BRR_HIP_OK;
-    if ((rc = dalloc(c, &c->d_eps, N)) || (rc = dalloc(c, &c->d_beta, M)) ||
-        (rc = dalloc(c, &c->d_comp, M)) || (rc = dalloc(c, &c->d_acum, M)) ||
-        (rc = dalloc(c, &c->d_scal, SL.size())) || (rc = dalloc(c, &c->d_cva, G * (K - 1))) ||
-        (rc = dalloc(c, &c->d_xchg1, (size_t)2 * kMaxB * c->P * 2)) ||
-        (rc = dalloc(c, &c->d_xchg2, (size_t)2 * kMaxB * 2)) || (rc = dalloc(c, &c->d_ctl, 8)) ||
+    if ((rc = dalloc(c, &c->d_eps, N * Tn)) || (rc = dalloc(c, &c->d_beta, M * Tn)) ||
+        (rc = dalloc(c, &c->d_comp, M * Tn)) || (rc = dalloc(c, &c->d_acum, M * Tn)) ||
+        (rc = dalloc(c, &c->d_scal, SL.size() * Tn)) || (rc = dalloc(c, &c->d_cva, G * (K - 1))) ||
+        (rc = dalloc(c, &c->d_xchg1, c->x1_words)) ||
+        (rc = dalloc(c, &c->d_xchg2, c->x2_words)) || (rc = dalloc(c, &c->d_ctl, 8)) ||
         (rc = dalloc(c, &c->d_counters, 8)) || (rc = dalloc(c, &c->d_tmp, 4)) ||
-        (rc = dalloc(c, &c->d_u, M)) || (rc = dalloc(c, &c->d_z, M))) {
+        (rc = dalloc(c, &c->d_u, M * Tn)) || (rc = dalloc(c, &c->d_z, M * Tn))) {
         brr_hip_destroy(c);
         return rc;
     }
@@ -415,11 +514,11 @@ int brr_hip_create(const brr_hip_config* cfg, brr_hip_ctx** out)
         brr_hip_destroy(c);
         return BRR_HIP_EDEVICE;
     }
-    (void)hipMemset(c->d_eps, 0, sizeof(double) * N);
-    (void)hipMemset(c->d_beta, 0, sizeof(double) * M);
-    (void)hipMemset(c->d_comp, 0, sizeof(int32_t) * M);
-    (void)hipMemset(c->d_acum, 0, sizeof(double) * M);
-    (void)hipMemset(c->d_scal, 0, sizeof(double) * SL.size());
+    (void)hipMemset(c->d_eps, 0, sizeof(double) * N * Tn);
+    (void)hipMemset(c->d_beta, 0, sizeof(double) * M * Tn);
+    (void)hipMemset(c->d_comp, 0, sizeof(int32_t) * M * Tn);
+    (void)hipMemset(c->d_acum, 0, sizeof(double) * M * Tn);
+    (void)hipMemset(c->d_scal, 0, sizeof(double) * SL.size() * Tn);
     if (reset_exchange(c) != BRR_HIP_OK) { brr_hip_destroy(c); return BRR_HIP_EDEVICE; }
     *out = c;
     return BRR_HIP_OK;
@@ -520,7 +619,7 @@ int brr_hip_upload_ragged(brr_hip_ctx* c, const int32_t* ones, const uint64_t* o
     const uint64_t list_bytes = 4ull * (n1 + n2 + nm);
     const uint64_t code_bytes = (uint64_t)M * (((uint64_t)N + 3) / 4);
     const uint32_t st = c->cfg.ragged_storage;
-    c->codes = st == 2 || (st == 0 && code_bytes < list_bytes);
+    c->codes = c->T > 0 || st == 2 || (st == 0 && code_bytes < list_bytes);
     if (c->codes) {
         c->col_bytes = ((uint64_t)N + 3) / 4;
         c->col_words = (c->col_bytes + 15) / 16 * 4;
@@ -567,16 +666,22 @@ int brr_hip_set_state(brr_hip_ctx* c, const double* eps, const double* beta, con
 {
     if (!c) return BRR_HIP_EINVAL;
     HIP_TRY(c, hipSetDevice(c->device));
-    const uint32_t N = c->cfg.num_inds, M = c->cfg.num_snps;
-    if (eps) HIP_TRY(c, hipMemcpy(c->d_eps, eps, sizeof(double) * N, hipMemcpyHostToDevice));
-    if (beta) HIP_TRY(c, hipMemcpy(c->d_beta, beta, sizeof(double) * M, hipMemcpyHostToDevice));
-    if (comp) HIP_TRY(c, hipMemcpy(c->d_comp, comp, sizeof(int32_t) * M, hipMemcpyHostToDevice));
+    const size_t N = c->cfg.num_inds, M = c->cfg.num_snps, Tn = c->Tn;
+    if (eps) HIP_TRY(c, hipMemcpy(c->d_eps, eps, sizeof(double) * N * Tn, hipMemcpyHostToDevice));
+    if (beta) HIP_TRY(c, hipMemcpy(c->d_beta, beta, sizeof(double) * M * Tn, hipMemcpyHostToDevice));
+    if (comp) HIP_TRY(c, hipMemcpy(c->d_comp, comp, sizeof(int32_t) * M * Tn, hipMemcpyHostToDevice));
     return BRR_HIP_OK;
 }
 
 int brr_hip_set_scalars(brr_hip_ctx* c, const brr_hip_scalars* s)
 {
+    return brr_hip_set_trait_scalars(c, 0, s);
+}
+
+int brr_hip_set_trait_scalars(brr_hip_ctx* c, uint32_t trait, const brr_hip_scalars* s)
+{
     if (!c || !s || !s->sigmaG || !s->pi) return fail(c, BRR_HIP_EINVAL, "null argument");
+    if (trait >= c->Tn) return fail(c, BRR_HIP_EINVAL, "trait index out of range");
     HIP_TRY(c, hipSetDevice(c->device));
     const uint32_t G = c->cfg.num_groups, K = c->cfg.num_components;
     const ScalarLayout SL{G, K};
@@ -586,7 +691,8 @@ int brr_hip_set_scalars(brr_hip_ctx* c, const brr_hip_scalars* s)
     h[SL.epsSum()] = s->epsilon_sum;
     for (uint32_t g = 0; g < G; ++g) h[SL.sigmaG() + g] = s->sigmaG[g];
     for (uint32_t t = 0; t < G * K; ++t) h[SL.pi() + t] = s->pi[t];
-    HIP_TRY(c, hipMemcpy(c->d_scal, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->d_scal + (size_t)trait * SL.size(), h.data(), sizeof(double) * h.size(),
+                         hipMemcpyHostToDevice));
     return BRR_HIP_OK;
 }
 
@@ -594,22 +700,29 @@ int brr_hip_get_state(brr_hip_ctx* c, double* eps, double* beta, int32_t* comp, 
 {
     if (!c) return BRR_HIP_EINVAL;
     HIP_TRY(c, hipSetDevice(c->device));
-    const uint32_t N = c->cfg.num_inds, M = c->cfg.num_snps;
-    if (eps) HIP_TRY(c, hipMemcpy(eps, c->d_eps, sizeof(double) * N, hipMemcpyDeviceToHost));
-    if (beta) HIP_TRY(c, hipMemcpy(beta, c->d_beta, sizeof(double) * M, hipMemcpyDeviceToHost));
-    if (comp) HIP_TRY(c, hipMemcpy(comp, c->d_comp, sizeof(int32_t) * M, hipMemcpyDeviceToHost));
-    if (acum) HIP_TRY(c, hipMemcpy(acum, c->d_acum, sizeof(double) * M, hipMemcpyDeviceToHost));
+    const size_t N = c->cfg.num_inds, M = c->cfg.num_snps, Tn = c->Tn;
+    if (eps) HIP_TRY(c, hipMemcpy(eps, c->d_eps, sizeof(double) * N * Tn, hipMemcpyDeviceToHost));
+    if (beta) HIP_TRY(c, hipMemcpy(beta, c->d_beta, sizeof(double) * M * Tn, hipMemcpyDeviceToHost));
+    if (comp) HIP_TRY(c, hipMemcpy(comp, c->d_comp, sizeof(int32_t) * M * Tn, hipMemcpyDeviceToHost));
+    if (acum) HIP_TRY(c, hipMemcpy(acum, c->d_acum, sizeof(double) * M * Tn, hipMemcpyDeviceToHost));
     return BRR_HIP_OK;
 }
 
 int brr_hip_get_scalars(brr_hip_ctx* c, brr_hip_scalars* s)
 {
+    return brr_hip_get_trait_scalars(c, 0, s);
+}
+
+int brr_hip_get_trait_scalars(brr_hip_ctx* c, uint32_t trait, brr_hip_scalars* s)
+{
     if (!c || !s) return BRR_HIP_EINVAL;
+    if (trait >= c->Tn) return fail(c, BRR_HIP_EINVAL, "trait index out of range");
     HIP_TRY(c, hipSetDevice(c->device));
     const uint32_t G = c->cfg.num_groups, K = c->cfg.num_components;
     const ScalarLayout SL{G, K};
     std::vector<double> h(SL.size());
-    HIP_TRY(c, hipMemcpy(h.data(), c->d_scal, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(h.data(), c->d_scal + (size_t)trait * SL.size(), sizeof(double) * h.size(),
+                         hipMemcpyDeviceToHost));
     s->mu = h[SL.mu()];
     s->sigmaE = h[SL.sigmaE()];
     s->epsilon_sum = h[SL.epsSum()];
@@ -637,16 +750,16 @@ int brr_hip_sweep(brr_hip_ctx* c, const uint32_t* order, const double* u, const 
         c->order_cap = M;
     }
     HIP_TRY(c, hipMemcpy(c->d_order, order, sizeof(uint32_t) * M, hipMemcpyHostToDevice));
-    HIP_TRY(c, hipMemcpy(c->d_u, u, sizeof(double) * M, hipMemcpyHostToDevice));
-    HIP_TRY(c, hipMemcpy(c->d_z, z, sizeof(double) * M, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->d_u, u, sizeof(double) * M * c->Tn, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->d_z, z, sizeof(double) * M * c->Tn, hipMemcpyHostToDevice));
     SweepParams p = base_params(c);
     p.mode = kModeSweep;
     p.iterations = 1;
     p.order = c->d_order;
     p.u = c->d_u;
     p.z = c->d_z;
-    rc = launch(c, p);
-    c->stats.markers = M;
+    rc = launch_any(c, p);
+    c->stats.markers = (uint64_t)M * c->Tn;
     return rc;
 }
 
@@ -672,7 +785,7 @@ int brr_hip_run_chain(brr_hip_ctx* c, uint64_t seed, uint32_t chain, uint32_t fi
         if ((rc = dalloc(c, &c->d_order, total))) return rc;
         c->order_cap = total;
     }
-    const size_t hsz = (size_t)iterations * (4 + G);
+    const size_t hsz = (size_t)iterations * c->Tn * (4 + G);
     if (c->hist_cap < hsz) {
         if ((rc = dalloc(c, &c->d_hist, hsz))) return rc;
         c->hist_cap = hsz;
@@ -686,8 +799,8 @@ int brr_hip_run_chain(brr_hip_ctx* c, uint64_t seed, uint32_t chain, uint32_t fi
     p.seed = seed;
     p.order = c->d_order;
     p.hist = c->d_hist;
-    rc = launch(c, p);
-    c->stats.markers = total;
+    rc = launch_any(c, p);
+    c->stats.markers = total * c->Tn;
     if (rc) return rc;
     if (hist) HIP_TRY(c, hipMemcpy(hist, c->d_hist, sizeof(double) * hsz, hipMemcpyDeviceToHost));
     return BRR_HIP_OK;
@@ -697,6 +810,7 @@ int brr_hip_compute_num(brr_hip_ctx* c, uint32_t marker, double beta_old, double
 {
     int rc = check_ready(c);
     if (rc) return rc;
+    if (c->T) return fail(c, BRR_HIP_ESTATE, "per-marker calls need a single-chain context");
     if (!num || marker >= c->cfg.num_snps) return fail(c, BRR_HIP_EINVAL, "bad marker or null output");
     HIP_TRY(c, hipSetDevice(c->device));
     SweepParams p = base_params(c);
@@ -711,6 +825,7 @@ int brr_hip_epsilon_change(brr_hip_ctx* c, uint32_t marker, double beta_old, dou
 {
     int rc = check_ready(c);
     if (rc) return rc;
+    if (c->T) return fail(c, BRR_HIP_ESTATE, "per-marker calls need a single-chain context");
     if (marker >= c->cfg.num_snps) return fail(c, BRR_HIP_EINVAL, "bad marker");
     HIP_TRY(c, hipSetDevice(c->device));
     SweepParams p = base_params(c);
@@ -728,6 +843,7 @@ int brr_hip_debug_block_codes(brr_hip_ctx* c, const uint32_t* order, const doubl
 {
     int rc = check_ready(c);
     if (rc) return rc;
+    if (c->T) return fail(c, BRR_HIP_ESTATE, "per-marker calls need a single-chain context");
     const uint32_t M = c->cfg.num_snps;
     const uint64_t nblocks = (M + c->B - 1) / c->B;
     const uint64_t need = nblocks * c->P * kMaxB * c->R * 32;

@@ -1,0 +1,783 @@
+// batch_kernel.hpp -- persistent gfx950 kernel for T BayesR chains (traits)
+// that share one genotype matrix.
+//
+// What it computes: for every trait t < T an independent BayesRBase::processColumn
+// sweep (BayesRBase.cpp:287-392) over one marker order, each trait with its own
+// residual, beta, components and chain scalars. That is what T separate runs of
+// the reference over T phenotypes compute when they are handed the same order
+// (each run is a valid random-scan chain; the order stream is shared).
+//
+// How (MI355X-first): one cooperative launch of P workgroups, one per CU. A
+// workgroup owns a slice of at most 16 * kMaxNT individuals and keeps its
+// residuals for all traits in registers (FP64). Markers go in blocks of 32:
+//   1. matrix cores (v_mfma_f64_16x16x4f64) form the slice's partial dots
+//      C[k][t] = x_k . eps_t for the 32 markers x T traits, and the partial Gram
+//      G[i][k] = u_i . w_k (i < k) of the block, where w_k is marker k's dot
+//      weight and u_i the residual change per unit beta change of marker i;
+//   2. one deterministic reduce-scatter / all-gather (agent-coherent tagged
+//      words) makes every workgroup hold the full C and G;
+//   3. every workgroup evaluates the same FP64 decisions: all pending
+//      (marker, trait) pairs in parallel; per trait the pairs up to its first
+//      beta change are accepted, the dots of that trait's later markers are
+//      corrected with the change times the Gram row, and the rest is decided
+//      again -- the one-marker-at-a-time result, with the dot of a later marker
+//      formed as C + sum(delta * G) instead of a fresh x . eps;
+//   4. matrix cores apply eps_t += sum_k delta_{k,t} u_k to the slice (only
+//      4-marker steps that hold a change are issued).
+// So one exchange per block of 32 markers serves all T traits.
+#pragma once
+
+#include "sweep_kernel.hpp"
+
+namespace brr {
+
+constexpr int kBB = 32;                      // markers per block (two 16-marker tiles)
+constexpr int kMaxT = 64;                    // traits per launch
+constexpr int kMaxNT = 16;                   // 16-individual tiles per workgroup slice
+constexpr int kBRow = 4 * kMaxNT + 4;        // LDS bytes per code row (padded: rows fall in distinct banks)
+constexpr int kBGram = kBB * (kBB - 1) / 2;  // strictly-upper Gram entries of a block
+constexpr uint32_t kBVmax = (uint32_t)(kBB * kMaxT + kBGram);   // values exchanged per block
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+struct BatchParams {
+    uint32_t N, M, K, T;
+    uint32_t P, S, nt, nq;            // workgroups, slice, 16-row tiles per slice, 16-trait tiles
+    int layout, mode;
+    uint32_t iterations, iter_base, chain, pad0;
+    uint64_t seed;
+    const uint32_t* codes;            // 2-bit PLINK columns, column j at codes + j * col_words
+    uint64_t col_words, col_bytes;
+    const double* lut;                // dense: M x 4 by dosage 0, 1, 2, missing
+    const double* stats;              // ragged: M x 4 {mean, sd, sqrdZ, Zsum}
+    const double* cva;                // K - 1 mixture variances
+    double v0E, s02E, v0G, s02G;
+    const uint32_t* order;            // M (sweep) or iterations x M (chain)
+    const double* u;                  // sweep mode: M x T, by marker id
+    const double* z;
+    double* eps;                      // N x T (trait fastest)
+    double* beta;                     // M x T (agent-coherent words)
+    int32_t* comp;                    // M x T
+    double* acum;                     // M x T
+    double* scal;                     // T scalar blocks of ScalarLayout{1, K}
+    double* hist;                     // iterations x T x 5 or null
+    uint64_t* xchg1;                  // [2][P][kBVmax][2] tagged partials
+    uint64_t* xchg2;                  // [2][kBVmax][2] tagged totals
+    uint32_t* ctl;
+    unsigned long long* counters;     // [0] rounds, [1] changes, [2..7] phase ticks
+};
+
+// Per-block inputs, double-buffered: the current block reads buffer cb while
+// the next block's data is written to cb ^ 1.
+struct BBlock {
+    uint8_t codes[kBB][kBRow];        // byte b of row k: individuals lo + 4b .. 4b + 3 of marker k
+    double wt[kBB][4];                // dot weight per 2-bit code (0 for markers past the block)
+    double ut[kBB][4];                // residual change per unit beta change, per code
+    double mstat[kBB][3];             // ragged: mean, sd, Zsum
+    double bo[kBB][kMaxT];            // beta before the block
+    uint32_t mk[kBB];                 // marker ids
+};
+
+struct BShared {
+    BBlock blk[2];
+    double cd[kBB][kMaxT];            // dots, corrected dots, then the accepted beta deltas
+    double gm[kBB][kBB];              // Gram u_i . w_k, i < k
+    double bn[kBB][kMaxT];            // decided beta
+    double ac[kBB][kMaxT];            // decided acum
+    int8_t sl[kBB][kMaxT];            // decided component (-1: none selected)
+    double gc[kMaxK][kMaxT];          // log pi_k - 0.5 log(logLScale S_k + 1)
+    double gden[kMaxK][kMaxT];        // denom_k
+    double gbsd[kMaxK][kMaxT];        // sqrt(sigmaE / denom_k)
+    double mu[kMaxT], sigmaE[kMaxT], epsSum[kMaxT], sigmaG[kMaxT], betasqn[kMaxT];
+    double pi[kMaxK][kMaxT];
+    double v[kMaxK][kMaxT];
+    double tvec[kMaxT];               // per-trait all-reduce results / new mu
+    double cva[kMaxK];                // S_k for k >= 1
+    double red[kWaves][8];
+    int pos[kMaxT];                   // first undecided marker of the block, per trait
+    int fc[kMaxT];                    // first change among this round's pairs, per trait
+    int pref[kMaxT + 1];              // exclusive prefix of pending pairs over traits
+    uint32_t flag[kBB / 4];           // bit q: 4-marker step holds a change in trait tile q
+    uint64_t tk[kPhases];             // phase timers of workgroup 0 (100 MHz ticks)
+    uint64_t tlast;
+    int abort;
+};
+
+// Phase timer of workgroup 0, thread 0, kept in LDS.
+#define BRR_BTICK(slot)                                                          \
+    do {                                                                         \
+        if (tid == 0) {                                                          \
+            const uint64_t t_now_ = __builtin_amdgcn_s_memrealtime();           \
+            L.tk[(slot)] += t_now_ - L.tlast;                                    \
+            L.tlast = t_now_;                                                    \
+        }                                                                        \
+    } while (0)
+
+__device__ __forceinline__ uint64_t* bslot1(const BatchParams& p, uint32_t rnd, uint32_t w, uint32_t v)
+{
+    return p.xchg1 + ((((size_t)(rnd & 1u) * p.P + w) * kBVmax + v) * 2u);
+}
+
+__device__ __forceinline__ uint64_t* bslot2(const BatchParams& p, uint32_t rnd, uint32_t v)
+{
+    return p.xchg2 + (((size_t)(rnd & 1u) * kBVmax + v) * 2u);
+}
+
+__device__ __forceinline__ dbl4 mfma16x16x4(double a, double b, dbl4 c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// Reduce-scatter of the V values published at round rnd. Chunks of 8
+// consecutive values belong to workgroup (chunk % P); thread (g, e) of the
+// owner sums partial (w2, 8 chunk + e) for w2 = g, g + 32, ... in order, the
+// 8 lane groups of a wave are folded by a fixed butterfly, the 4 waves in
+// index order, and the total is published. Every total is formed once, so all
+// workgroups read the same bits. Returns false (in all threads) on abort.
+__device__ __forceinline__ bool breduce(const BatchParams& p, BShared& L, uint32_t rnd, uint32_t V,
+                                        uint32_t w, int tid)
+{
+    const int lane = tid & 63, wave = tid >> 6;
+    const uint32_t nch = (V + 7u) / 8u;
+    for (uint32_t c = w; c < nch; c += p.P) {
+        const uint32_t v = 8u * c + (uint32_t)(tid & 7);
+        double acc = 0.0;
+        bool ok = true;
+        if (v < V) {
+            for (uint32_t w2 = (uint32_t)(tid >> 3); w2 < p.P; w2 += 32u) {
+                double x;
+                if (!wait_pair(bslot1(p, rnd, w2, v), rnd, p.ctl, &x)) { ok = false; break; }
+                acc += x;
+            }
+        }
+        if (!ok) { L.abort = 1; raise_abort(p, 11); }
+        acc += __shfl_xor(acc, 8, 64);
+        acc += __shfl_xor(acc, 16, 64);
+        acc += __shfl_xor(acc, 32, 64);
+        if (lane < 8) L.red[wave][lane] = acc;
+        __syncthreads();
+        if (L.abort) return false;
+        if (tid < 8 && v < V)
+            publish_pair(bslot2(p, rnd, v), rnd, ((L.red[0][tid] + L.red[1][tid]) + L.red[2][tid]) + L.red[3][tid]);
+        __syncthreads();
+    }
+    return true;
+}
+
+// All-reduce of one value per trait (published at v = trait): totals to L.tvec.
+__device__ __forceinline__ bool ballreduce_traits(const BatchParams& p, BShared& L, uint32_t rnd, uint32_t w,
+                                                  int tid)
+{
+    if (!breduce(p, L, rnd, p.T, w, tid)) return false;
+    if (tid < (int)p.T) {
+        double x;
+        if (!wait_pair(bslot2(p, rnd, (uint32_t)tid), rnd, p.ctl, &x)) { L.abort = 1; raise_abort(p, 12); }
+        else L.tvec[tid] = x;
+    }
+    __syncthreads();
+    return L.abort == 0;
+}
+
+// ------------------------------------------------------------ block loading --
+
+// Everything of the next block that can be fetched before it is needed, in
+// registers: code words, per-marker tables and beta of (marker, trait) pairs.
+struct BNext {
+    uint32_t words[2];          // kBB * kMaxNT words over 256 threads
+    double m0, m1, m2, m3;      // dense: table by dosage; ragged: mean, sd, Zsum
+    double b[8];                // beta of pair u * 256 + tid = (k = e >> 6, t = e & 63)
+};
+
+__device__ __forceinline__ void bissue(const BatchParams& p, const uint32_t* ids, int nb, uint32_t lo, int tid,
+                                       BNext& x)
+{
+    const uint32_t per = p.nt;   // 32-bit words (16 individuals each) per marker row
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const uint32_t t = (uint32_t)u * kThreads + (uint32_t)tid;
+        x.words[u] = 0u;
+        if (t < (uint32_t)nb * per) {
+            const uint32_t k = t / per, rem = t % per;
+            const uint64_t byte_off = (uint64_t)(lo >> 2) + 4ull * rem;
+            if (byte_off < p.col_bytes) x.words[u] = p.codes[(size_t)ids[k] * p.col_words + (lo >> 4) + rem];
+        }
+    }
+    x.m0 = x.m1 = x.m2 = x.m3 = 0.0;
+    if (tid < nb) {
+        const uint32_t j = ids[tid];
+        if (p.layout == kLayoutDense) {
+            const double* lt = p.lut + 4 * (size_t)j;
+            x.m0 = lt[0]; x.m1 = lt[1]; x.m2 = lt[2]; x.m3 = lt[3];
+        } else {
+            const double* st = p.stats + 4 * (size_t)j;
+            x.m0 = st[0]; x.m1 = st[1]; x.m2 = st[3];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        const int e = u * kThreads + tid;
+        const int k = e >> 6, t = e & 63;
+        x.b[u] = 0.0;
+        if (k < nb && t < (int)p.T)
+            x.b[u] = bits_as_double(ld_agent((const uint64_t*)(p.beta + (size_t)ids[k] * p.T + t)));
+    }
+}
+
+__device__ __forceinline__ void bstore(const BatchParams& p, BBlock& L, const BNext& x, int nb, int tid)
+{
+    const uint32_t per = p.nt;
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const uint32_t t = (uint32_t)u * kThreads + (uint32_t)tid;
+        if (t < (uint32_t)nb * per) *(uint32_t*)&L.codes[t / per][4u * (t % per)] = x.words[u];
+    }
+    if (tid < kBB) {
+        const int k = tid;
+        if (k >= nb) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) { L.wt[k][c] = 0.0; L.ut[k][c] = 0.0; }
+            L.mstat[k][0] = 0.0; L.mstat[k][1] = 1.0; L.mstat[k][2] = 0.0;
+        } else if (p.layout == kLayoutDense) {
+            // 2-bit code v -> dosage: 0->2, 1->missing, 2->1, 3->0
+            L.wt[k][0] = x.m2; L.wt[k][1] = x.m3; L.wt[k][2] = x.m1; L.wt[k][3] = x.m0;
+            L.ut[k][0] = x.m2; L.ut[k][1] = x.m3; L.ut[k][2] = x.m1; L.ut[k][3] = x.m0;
+        } else {
+            // ragged dot over the ones/twos lists (missing counts 0), and the
+            // residual change per unit beta change (raggedbayesrkernel.cpp:10-30)
+            const double mean = x.m0, sd = x.m1;
+            L.mstat[k][0] = mean; L.mstat[k][1] = sd; L.mstat[k][2] = x.m2;
+            L.wt[k][0] = 2.0; L.wt[k][1] = 0.0; L.wt[k][2] = 1.0; L.wt[k][3] = 0.0;
+            const double meanAdjustment = mean / sd;
+            const double oneAdjustment = 1.0 / sd;
+            L.ut[k][0] = -meanAdjustment + 2 * oneAdjustment;
+            L.ut[k][1] = 0.0;
+            L.ut[k][2] = -meanAdjustment + oneAdjustment;
+            L.ut[k][3] = -meanAdjustment;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        const int e = u * kThreads + tid;
+        const int k = e >> 6, t = e & 63;
+        if (k < nb && t < (int)p.T) L.bo[k][t] = x.b[u];
+    }
+}
+
+// ------------------------------------------------------------- decisions --
+
+// processColumn's draw for marker slot k of trait t (BayesRBase.cpp:300-366)
+// from the current dot in L.cd; results to L.bn / L.ac / L.sl.
+template <bool kDenseMath>
+__device__ __forceinline__ void bdecide(const BatchParams& p, BShared& L, const BBlock& B, int k, int t,
+                                        uint32_t iteration, double dN, double NM1)
+{
+    const int K = (int)p.K;
+    const uint32_t j = B.mk[k];
+    const double beta_old = B.bo[k][t];
+    double num;
+    if (kDenseMath) {
+        num = L.cd[k][t] + beta_old * NM1;
+    } else {
+        const double mean = B.mstat[k][0], sd = B.mstat[k][1];
+        num = beta_old * (dN - 1.0) - mean * L.epsSum[t] / sd + L.cd[k][t] / sd;
+    }
+    const double sigmaE = L.sigmaE[t];
+    double logL[kMaxK], muk[kMaxK];
+    logL[0] = L.gc[0][t];
+    muk[0] = 0.0;
+#pragma unroll
+    for (int c = 1; c < kMaxK; ++c) {
+        muk[c] = 0.0;
+        logL[c] = 0.0;
+        if (c < K) {
+            muk[c] = num / L.gden[c][t];
+            logL[c] = L.gc[c][t] + 0.5 * (muk[c] * num) / sigmaE;
+        }
+    }
+    double acum = guard700(logL, K, logL[0]) ? 0.0 : inv_sum_exp(logL, K, logL[0]);
+    const double acum0 = acum;
+    U4 xr{};
+    double uu;
+    if (p.mode == kModeChain) {
+        xr = draw_block(p.seed, j, iteration, kStreamMarker, p.chain + (uint32_t)t);
+        uu = unit_u53(xr.x[0], xr.x[1]);
+    } else {
+        uu = p.u[(size_t)j * p.T + t];
+    }
+    int sel = -1;
+#pragma unroll
+    for (int c = 0; c < kMaxK; ++c) {
+        if (c < K && sel < 0) {
+            if (uu <= acum) {
+                sel = c;
+            } else if (c + 1 < K) {
+                const double ref = logL[c + 1 < kMaxK ? c + 1 : 0];
+                if (!guard700(logL, K, ref)) acum += inv_sum_exp(logL, K, ref);
+            }
+        }
+    }
+    double bnew = beta_old;
+    if (sel == 0) {
+        bnew = 0.0;
+    } else if (sel > 0) {
+        double m = 0.0;
+#pragma unroll
+        for (int c = 1; c < kMaxK; ++c)
+            if (c == sel) m = muk[c];
+        const double zz = (p.mode == kModeChain) ? unit_normal(xr.x[2], xr.x[3]) : p.z[(size_t)j * p.T + t];
+        bnew = zz * L.gbsd[sel][t] + m;
+    }
+    L.bn[k][t] = bnew;
+    L.ac[k][t] = acum0;
+    L.sl[k][t] = (int8_t)sel;
+    if (bnew != beta_old) atomicMin(&L.fc[t], k);
+}
+
+// -------------------------------------------------------------------- kernel --
+
+template <int NT, bool kDenseMath>
+__global__ void __launch_bounds__(kThreads)
+bayesr_batch_kernel(BatchParams p)
+{
+    __shared__ BShared L;
+    const uint32_t w = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int tq = lane & 15;          // matrix-core row / column index of this lane
+    const int kq = lane >> 4;          // matrix-core k index of this lane
+    const uint32_t lo = w * p.S;
+    const uint32_t nloc = min(p.S, p.N - lo);
+    const int T = (int)p.T;
+    const int K = (int)p.K;
+    const int nt = (int)p.nt;
+    const int tl = 16 * wave + tq;     // trait of this lane's residual column
+    const bool tval = tl < T;
+    const ScalarLayout SL{1u, p.K};
+    const uint32_t sls = SL.size();
+    const double dN = (double)p.N;
+    const double NM1 = (double)(p.N - 1u);
+    constexpr bool dense = kDenseMath;
+
+    // residuals in the matrix-core accumulator layout (lane = 16 * (row % 4) + column,
+    // entry r = row / 4): E[s][r] = eps of individual lo + 16 s + 4 r + kq, trait tl.
+    // As the B operand of step (s, r) the same registers hold individuals
+    // 16 s + 4 r + 0..3 (k index kq), so a tile feeds the dots without shuffles.
+    dbl4 E[NT];
+#pragma unroll
+    for (int s = 0; s < NT; ++s)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const uint32_t i = 16u * (uint32_t)s + 4u * (uint32_t)r + (uint32_t)kq;
+            double e = 0.0;
+            if (s < nt && i < nloc && tval) e = p.eps[(size_t)(lo + i) * T + tl];
+            E[s][r] = e;
+        }
+
+    if (tid < kMaxT) {
+        const int t = tid;
+        L.mu[t] = 0.0; L.sigmaE[t] = 1.0; L.epsSum[t] = 0.0; L.sigmaG[t] = 1.0; L.betasqn[t] = 0.0;
+        for (int k = 0; k < kMaxK; ++k) { L.pi[k][t] = 0.0; L.v[k][t] = 0.0; }
+        if (t < T) {
+            const double* sc = p.scal + (size_t)t * sls;
+            L.mu[t] = sc[SL.mu()];
+            L.sigmaE[t] = sc[SL.sigmaE()];
+            L.epsSum[t] = sc[SL.epsSum()];
+            L.sigmaG[t] = sc[SL.sigmaG()];
+            for (int k = 0; k < K; ++k) L.pi[k][t] = sc[SL.pi() + k];
+        }
+    }
+    if (tid < kMaxK) L.cva[tid] = (tid >= 1 && tid < K) ? p.cva[tid - 1] : 0.0;
+    for (int e = tid; e < kBB * kMaxT; e += kThreads) (&L.cd[0][0])[e] = 0.0;
+    if (tid == 0) L.abort = 0;
+    uint32_t rnd = __hip_atomic_load(p.ctl + kCtlRound, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned long long n_rounds = 0, n_changes = 0;
+    if (tid < kPhases) L.tk[tid] = 0;
+    if (tid == 0) L.tlast = __builtin_amdgcn_s_memrealtime();
+    int cb = 0;     // current block buffer
+    __syncthreads();
+
+    const uint32_t iters = (p.mode == kModeChain) ? p.iterations : 1u;
+    const uint32_t nblocks = (p.M + kBB - 1) / kBB;
+    for (uint32_t it = 0; it < iters; ++it) {
+        const uint32_t iteration = p.iter_base + it;
+        const uint32_t* order = p.order + (size_t)((p.mode == kModeChain) ? it : 0u) * p.M;
+
+        if (p.mode == kModeChain) {
+            // ---- updateMu per trait (DenseBayesRRmz.cpp:36-42 / SparseBayesRRG.cpp:106-113)
+            const double om = L.mu[tl];
+#pragma unroll
+            for (int s = 0; s < NT; ++s)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const uint32_t i = 16u * (uint32_t)s + 4u * (uint32_t)r + (uint32_t)kq;
+                    if (s < nt && i < nloc && tval) E[s][r] = E[s][r] + om;
+                }
+            if (dense) {
+                double part = 0.0;
+#pragma unroll
+                for (int s = 0; s < NT; ++s)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) part += E[s][r];
+                part += __shfl_xor(part, 16, 64);
+                part += __shfl_xor(part, 32, 64);
+                ++rnd;
+                ++n_rounds;
+                if (kq == 0 && tval) publish_pair(bslot1(p, rnd, w, (uint32_t)tl), rnd, part);
+                if (!ballreduce_traits(p, L, rnd, w, tid)) goto done;
+            } else {
+                __syncthreads();
+                if (tid < T) L.epsSum[tid] += L.mu[tid] * dN;
+                __syncthreads();
+            }
+            if (tid < T) {
+                const double mean = dense ? L.tvec[tid] / dN : L.epsSum[tid] / dN;
+                L.tvec[tid] = norm_rng(p.seed, iteration, kDrawMu, p.chain + (uint32_t)tid, mean,
+                                       L.sigmaE[tid] / dN);
+            }
+            __syncthreads();
+            const double nm = L.tvec[tl];
+#pragma unroll
+            for (int s = 0; s < NT; ++s)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const uint32_t i = 16u * (uint32_t)s + 4u * (uint32_t)r + (uint32_t)kq;
+                    if (s < nt && i < nloc && tval) E[s][r] = E[s][r] - nm;
+                }
+            __syncthreads();
+            if (tid < T) {
+                L.mu[tid] = L.tvec[tid];
+                if (!dense) L.epsSum[tid] -= L.tvec[tid] * dN;
+                L.betasqn[tid] = 0.0;
+            }
+            for (int e = tid; e < kMaxK * kMaxT; e += kThreads) (&L.v[0][0])[e] = 0.0;
+            __syncthreads();
+        }
+
+        // ---- per-iteration constants of processColumn (BayesRBase.cpp:309-333)
+        for (int e = tid; e < T * K; e += kThreads) {
+            const int t = e / K, k = e % K;
+            const double sigmaG = L.sigmaG[t], sigmaE = L.sigmaE[t];
+            if (k == 0) {
+                L.gc[0][t] = log(L.pi[0][t]);
+                L.gden[0][t] = 0.0;
+                L.gbsd[0][t] = 0.0;
+            } else {
+                const double S = L.cva[k];
+                const double sigmaEOverSigmaG = sigmaE / sigmaG;
+                const double den = NM1 + sigmaEOverSigmaG * (1.0 / S);
+                const double logLScale = sigmaG / sigmaE * NM1;
+                L.gden[k][t] = den;
+                L.gc[k][t] = log(L.pi[k][t]) - 0.5 * log(logLScale * S + 1.0);
+                L.gbsd[k][t] = __dsqrt_rn(sigmaE / den);
+            }
+        }
+        // ---- block 0 loaded in place; ids of block 1 staged
+        {
+            const int nb0 = (int)min((uint32_t)kBB, p.M);
+            if (tid < nb0) L.blk[cb].mk[tid] = order[tid];
+            __syncthreads();
+            BNext x;
+            bissue(p, L.blk[cb].mk, nb0, lo, tid, x);
+            bstore(p, L.blk[cb], x, nb0, tid);
+            if (nblocks > 1) {
+                const int nb1 = (int)min((uint32_t)kBB, p.M - kBB);
+                if (tid < nb1) L.blk[cb ^ 1].mk[tid] = order[kBB + tid];
+            }
+            __syncthreads();
+        }
+        BRR_BTICK(5);
+
+        for (uint32_t blk = 0; blk < nblocks; ++blk) {
+            const uint32_t b0 = blk * kBB;
+            const int nb = (int)min((uint32_t)kBB, p.M - b0);
+            const bool has1 = blk + 1 < nblocks;
+            const int nb1 = has1 ? (int)min((uint32_t)kBB, p.M - b0 - kBB) : 0;
+            const bool has2 = blk + 2 < nblocks;
+            const int nb2 = has2 ? (int)min((uint32_t)kBB, p.M - b0 - 2 * kBB) : 0;
+
+            BBlock& B = L.blk[cb];
+            // prefetch of block blk+1 into registers (stored to the other buffer
+            // once the matrix-core phase has hidden the latency), ids of blk+2
+            BNext x;
+            if (has1) bissue(p, L.blk[cb ^ 1].mk, nb1, lo, tid, x);
+            uint32_t jn2 = 0;
+            if (has2 && tid < nb2) jn2 = order[b0 + 2 * kBB + tid];
+            ++rnd;
+            ++n_rounds;
+            BRR_BTICK(0);
+
+            // ---- 1. partial dots C[k][t] over this slice: wave q = traits 16q..16q+15
+            if (wave < (int)p.nq) {
+                dbl4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+                const bool two = nb > 16;
+#pragma unroll
+                for (int s = 0; s < NT; ++s) {
+                    if (s < nt) {
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const uint32_t by0 = B.codes[tq][4 * s + r];
+                            const uint32_t by1 = B.codes[16 + tq][4 * s + r];
+                            acc0 = mfma16x16x4(B.wt[tq][(by0 >> (2 * kq)) & 3u], E[s][r], acc0);
+                            if (two) acc1 = mfma16x16x4(B.wt[16 + tq][(by1 >> (2 * kq)) & 3u], E[s][r], acc1);
+                        }
+                    }
+                }
+                if (tval) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int k0 = kq + 4 * r;
+                        if (k0 < nb) publish_pair(bslot1(p, rnd, w, (uint32_t)(k0 * T + tl)), rnd, acc0[r]);
+                        if (16 + k0 < nb) publish_pair(bslot1(p, rnd, w, (uint32_t)((16 + k0) * T + tl)), rnd, acc1[r]);
+                    }
+                }
+            }
+            // ---- partial Gram tiles (0,0), (0,1), (1,1) on waves 0, 1, 2
+            if (wave < 3) {
+                const int mi = (wave == 2) ? 1 : 0;
+                const int mc = (wave == 0) ? 0 : 1;
+                if (16 * mc < nb) {
+                    dbl4 g = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                    for (int s = 0; s < NT; ++s) {
+                        if (s < nt) {
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) {
+                                const uint32_t i = 16u * (uint32_t)s + 4u * (uint32_t)r + (uint32_t)kq;
+                                const uint32_t byi = B.codes[16 * mi + tq][4 * s + r];
+                                const uint32_t byk = B.codes[16 * mc + tq][4 * s + r];
+                                const double a = (i < nloc) ? B.ut[16 * mi + tq][(byi >> (2 * kq)) & 3u] : 0.0;
+                                g = mfma16x16x4(a, B.wt[16 * mc + tq][(byk >> (2 * kq)) & 3u], g);
+                            }
+                        }
+                    }
+                    const int CT = nb * T;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int i = 16 * mi + kq + 4 * r, k = 16 * mc + tq;
+                        if (i < k && k < nb)
+                            publish_pair(bslot1(p, rnd, w, (uint32_t)(CT + k * (k - 1) / 2 + i)), rnd, g[r]);
+                    }
+                }
+            }
+            // the prefetched block goes to the other buffer (read after this block)
+            if (has1) bstore(p, L.blk[cb ^ 1], x, nb1, tid);
+            BRR_BTICK(1);
+
+            // ---- 2. exchange: every workgroup ends with the full C and G
+            {
+                const uint32_t CT = (uint32_t)nb * (uint32_t)T;
+                const uint32_t V = CT + (uint32_t)(nb * (nb - 1) / 2);
+                if (!breduce(p, L, rnd, V, w, tid)) goto done;
+                for (uint32_t v = (uint32_t)tid; v < V; v += kThreads) {
+                    double xv;
+                    if (!wait_pair(bslot2(p, rnd, v), rnd, p.ctl, &xv)) { L.abort = 1; raise_abort(p, 13); break; }
+                    if (v < CT) {
+                        const uint32_t k = v / (uint32_t)T;
+                        L.cd[k][v - k * (uint32_t)T] = xv;
+                    } else {
+                        const uint32_t e = v - CT;
+                        uint32_t k = (uint32_t)((1.0f + sqrtf(1.0f + 8.0f * (float)e)) * 0.5f);
+                        while (k > 1 && k * (k - 1) / 2 > e) --k;
+                        while ((k + 1) * k / 2 <= e) ++k;
+                        L.gm[e - k * (k - 1) / 2][k] = xv;
+                    }
+                }
+                if (tid < kMaxT) { L.pos[tid] = (tid < T) ? 0 : nb; L.fc[tid] = nb; }
+                if (tid < kBB / 4) L.flag[tid] = 0u;
+                __syncthreads();
+                if (L.abort) goto done;
+            }
+            BRR_BTICK(2);
+
+            // ---- 3. decisions: rounds over the pending (marker, trait) pairs
+            while (true) {
+                if (wave == 0) {
+                    const int cnt = nb - L.pos[lane];
+                    int incl = cnt;
+#pragma unroll
+                    for (int off = 1; off < 64; off <<= 1) {
+                        const int y = __shfl_up(incl, off, 64);
+                        if (lane >= off) incl += y;
+                    }
+                    L.pref[lane] = incl - cnt;
+                    if (lane == 63) L.pref[kMaxT] = incl;
+                }
+                __syncthreads();
+                const int total = L.pref[kMaxT];
+                if (total == 0) break;
+                for (int e = tid; e < total; e += kThreads) {
+                    int a = 0, b = kMaxT - 1;   // last trait whose prefix <= e
+                    while (a < b) {
+                        const int mid = (a + b + 1) >> 1;
+                        if (L.pref[mid] <= e) a = mid; else b = mid - 1;
+                    }
+                    bdecide<kDenseMath>(p, L, B, L.pos[a] + (e - L.pref[a]), a, iteration, dN, NM1);
+                }
+                __syncthreads();
+                // accept each trait's pairs up to its first change; correct the
+                // dots of its later pairs by delta * Gram row (BayesRBase.cpp:370-378)
+                if (lane < T) {
+                    const int t = lane, c = L.fc[t], p0 = L.pos[t];
+                    const double dl = (c < nb) ? B.bo[c][t] - L.bn[c][t] : 0.0;
+                    for (int k = p0 + wave; k < nb; k += kWaves) {
+                        if (k <= c) {
+                            const int sel = L.sl[k][t];
+                            if (sel >= 0) atomicAdd(&L.v[sel][t], 1.0);
+                            if (w == 0) {
+                                const size_t o = (size_t)B.mk[k] * p.T + (size_t)t;
+                                st_agent((uint64_t*)(p.beta + o), double_as_bits(L.bn[k][t]));
+                                if (sel >= 0) p.comp[o] = sel;
+                                p.acum[o] = L.ac[k][t];
+                            }
+                            L.cd[k][t] = (k == c) ? dl : 0.0;
+                            if (k == c) atomicOr(&L.flag[k >> 2], 1u << (t >> 4));
+                        } else {
+                            L.cd[k][t] += dl * L.gm[c][k];
+                        }
+                    }
+                }
+                __syncthreads();
+                if (wave == 0) {
+                    const int t = lane;
+                    const int c = L.fc[t];
+                    const bool act = L.pos[t] < nb;
+                    if (act && c < nb) {
+                        if (!dense) {
+                            // epsilonSum (sparsebayesrkernel.cpp:29-34)
+                            const double dBeta = B.bo[c][t] - L.bn[c][t];
+                            const double mean = B.mstat[c][0], sd = B.mstat[c][1], Zsum = B.mstat[c][2];
+                            L.epsSum[t] += dBeta * Zsum / sd - dBeta * mean * dN / sd;
+                        }
+                        L.pos[t] = c + 1;
+                    } else if (act) {
+                        L.pos[t] = nb;
+                    }
+                    L.fc[t] = nb;
+                    n_changes += (unsigned long long)__popcll(__ballot(act && c < nb));
+                }
+            }
+            BRR_BTICK(3);
+
+            // ---- betasqnG += beta^2 in marker order (BayesRBase.cpp:353)
+            if (tid < T) {
+                for (int k = 0; k < nb; ++k)
+                    if (L.sl[k][tid] > 0) L.betasqn[tid] += L.bn[k][tid] * L.bn[k][tid];
+            }
+            // ---- 4. eps_t += sum_k delta_{k,t} u_k on the matrix cores: per
+            // residual tile, the 4-marker steps holding a change of this wave's traits
+            if (wave < (int)p.nq) {
+                uint32_t fmask = 0u;
+#pragma unroll
+                for (int p4 = 0; p4 < kBB / 4; ++p4)
+                    if ((L.flag[p4] >> wave) & 1u) fmask |= 1u << p4;
+                if (fmask) {
+#pragma unroll
+                    for (int s = 0; s < NT; ++s) {
+                        if (s < nt) {
+                            const uint32_t i = 16u * (uint32_t)s + (uint32_t)tq;
+                            dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+                            for (uint32_t m = fmask; m; m &= m - 1u) {
+                                const int km = 4 * __builtin_ctz(m) + kq;
+                                const double bv = (km < nb) ? L.cd[km][16 * wave + tq] : 0.0;
+                                const uint32_t by = B.codes[km][4 * s + (tq >> 2)];
+                                const double a = (i < nloc && km < nb) ? B.ut[km][(by >> (2 * (tq & 3))) & 3u] : 0.0;
+                                acc = mfma16x16x4(a, bv, acc);
+                            }
+                            E[s] += acc;
+                        }
+                    }
+                }
+            }
+            // workgroup 0's beta stores complete before later rounds let others read them
+            if (w == 0) __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+            BRR_BTICK(4);
+
+            // ---- block blk+1 becomes current; its successor's ids go to the freed buffer
+            cb ^= 1;
+            if (has2 && tid < nb2) L.blk[cb ^ 1].mk[tid] = jn2;
+            __syncthreads();
+            BRR_BTICK(0);
+        }
+
+        if (p.mode == kModeChain) {
+            // ---- sigmaE, sigmaG, pi draws per trait (BayesRBase.cpp:237-247)
+            double part = 0.0;
+#pragma unroll
+            for (int s = 0; s < NT; ++s)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) part += E[s][r] * E[s][r];
+            part += __shfl_xor(part, 16, 64);
+            part += __shfl_xor(part, 32, 64);
+            ++rnd;
+            ++n_rounds;
+            if (kq == 0 && tval) publish_pair(bslot1(p, rnd, w, (uint32_t)tl), rnd, part);
+            if (!ballreduce_traits(p, L, rnd, w, tid)) goto done;
+            if (tid < T) {
+                const int t = tid;
+                const uint32_t ch = p.chain + (uint32_t)t;
+                const double e2 = L.tvec[t];
+                L.sigmaE[t] = inv_scaled_chisq_rng(p.seed, iteration, kDrawSigmaE, ch, p.v0E + p.N,
+                                                   (e2 + p.v0E * p.s02E) / (p.v0E + p.N));
+                double vsum = 0.0;
+                for (int k = 0; k < K; ++k) vsum += L.v[k][t];
+                const double m0 = vsum - L.v[0][t];
+                L.sigmaG[t] = inv_scaled_chisq_rng(p.seed, iteration, kDrawSigmaG0, ch, p.v0G + m0,
+                                                   (L.betasqn[t] * m0 + p.v0G * p.s02G) / (p.v0G + m0));
+                // Dirichlet(v + 1): draw ids 2 + G + k with one group
+                double sum = 0.0;
+                for (int k = 0; k < K; ++k) {
+                    L.pi[k][t] = rgamma(p.seed, iteration, 3u + (uint32_t)k, ch, L.v[k][t] + 1.0, 1.0);
+                    sum += L.pi[k][t];
+                }
+                for (int k = 0; k < K; ++k) L.pi[k][t] = L.pi[k][t] / sum;
+                if (w == 0 && p.hist) {
+                    double* h = p.hist + ((size_t)it * p.T + (size_t)t) * 5u;
+                    h[0] = L.mu[t];
+                    h[1] = L.sigmaE[t];
+                    h[2] = m0;
+                    h[3] = e2;
+                    h[4] = L.sigmaG[t];
+                }
+            }
+            __syncthreads();
+            BRR_BTICK(5);
+        }
+    }
+
+done:
+    __syncthreads();
+    if (L.abort == 0) {
+#pragma unroll
+        for (int s = 0; s < NT; ++s)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const uint32_t i = 16u * (uint32_t)s + 4u * (uint32_t)r + (uint32_t)kq;
+                if (s < nt && i < nloc && tval) p.eps[(size_t)(lo + i) * T + tl] = E[s][r];
+            }
+        if (w == 0) {
+            if (tid < T) {
+                const int t = tid;
+                double* sc = p.scal + (size_t)t * sls;
+                sc[SL.mu()] = L.mu[t];
+                sc[SL.sigmaE()] = L.sigmaE[t];
+                sc[SL.epsSum()] = L.epsSum[t];
+                sc[SL.sigmaG()] = L.sigmaG[t];
+                sc[SL.betasqn()] = L.betasqn[t];
+                for (int k = 0; k < K; ++k) {
+                    sc[SL.pi() + k] = L.pi[k][t];
+                    sc[SL.v() + k] = L.v[k][t];
+                }
+            }
+            if (tid == 0) {
+                __hip_atomic_store(p.ctl + kCtlRound, rnd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                p.counters[0] = n_rounds;
+                p.counters[1] = n_changes;
+                for (int q = 0; q < kPhases; ++q) p.counters[2 + q] = L.tk[q];
+            }
+        }
+    }
+}
+
+} // namespace brr

@@ -226,7 +226,8 @@ __device__ __forceinline__ bool wait_pair(const uint64_t* s, uint32_t rnd, const
     return true;
 }
 
-__device__ __forceinline__ void raise_abort(const SweepParams& p, uint32_t where)
+template <typename Params>
+__device__ __forceinline__ void raise_abort(const Params& p, uint32_t where)
 {
     __hip_atomic_store(p.ctl + kCtlAbort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(p.ctl + kCtlAbortWhere, where, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

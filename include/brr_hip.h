@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define BRR_HIP_ABI_VERSION 1
+#define BRR_HIP_ABI_VERSION 2
 
 enum brr_hip_status {
     BRR_HIP_OK = 0,
@@ -66,6 +66,12 @@ typedef struct {
     uint32_t ragged_storage; /* ragged layout only: device copy of the columns as
                                 0 = whichever is smaller (index lists or 2-bit codes),
                                 1 = index lists, 2 = 2-bit codes built on the device */
+    uint32_t num_traits;     /* 0 = one chain. 1..64 = T chains (traits, one phenotype
+                                each) over the same genotype columns and marker order,
+                                swept together; needs num_groups == 1 and at most
+                                16 * 16 individuals per workgroup. The per-individual and
+                                per-marker arrays of the state calls are then N x T and
+                                M x T, trait fastest. */
 } brr_hip_config;
 
 /* Model constants: Options::S (G x (K-1), row-major) and the priors
@@ -135,23 +141,30 @@ int brr_hip_set_scalars(brr_hip_ctx* ctx, const brr_hip_scalars* s);
 int brr_hip_get_state(brr_hip_ctx* ctx, double* epsilon, double* beta,
                       int32_t* components, double* acum);
 int brr_hip_get_scalars(brr_hip_ctx* ctx, brr_hip_scalars* s);
+/* Scalars of one trait of a batched context (trait 0 is what set/get_scalars
+ * address). One BayesRBase instance per trait holds these in the reference. */
+int brr_hip_set_trait_scalars(brr_hip_ctx* ctx, uint32_t trait, const brr_hip_scalars* s);
+int brr_hip_get_trait_scalars(brr_hip_ctx* ctx, uint32_t trait, brr_hip_scalars* s);
 
 /* One AnalysisGraph::exec: processColumn over order[0..M) in sequence.
  * u, z: the per-marker uniform and standard normal of this iteration, indexed
- * by marker id (BayesRBase.cpp:343-344). Uses and updates the device state. */
+ * by marker id (BayesRBase.cpp:343-344); M x T for a batched context. Uses and
+ * updates the device state. */
 int brr_hip_sweep(brr_hip_ctx* ctx, const uint32_t* order, const double* u, const double* z);
 
 /* A device-resident chain: `iterations` full runGibbs iterations (updateMu,
  * sweep, sigmaE / sigmaG / pi draws) in one persistent launch, drawing u, z
  * and the scalars from the counter-based stream (seed, chain). orders holds
  * the marker order of each iteration (iterations x M). hist (may be NULL)
- * receives iterations x (4 + G) doubles {mu, sigmaE, m0, |eps|^2, sigmaG[G]}. */
+ * receives iterations x (4 + G) doubles {mu, sigmaE, m0, |eps|^2, sigmaG[G]}.
+ * Batched context: trait t draws from stream (seed, chain + t) and hist is
+ * iterations x T x 5. */
 int brr_hip_run_chain(brr_hip_ctx* ctx, uint64_t seed, uint32_t chain,
                       uint32_t first_iteration, uint32_t iterations,
                       const uint32_t* orders, double* hist);
 
-/* Per-marker kernel calls on the device epsilon (correctness path: one launch
- * per call). compute_num: BayesRKernel::computeNum with the context's
+/* Per-marker kernel calls on the device epsilon of a single-chain context
+ * (correctness path: one launch per call). compute_num: BayesRKernel::computeNum with the context's
  * epsilon_sum. epsilon_change: applies calculateEpsilonChange(beta_old, beta)
  * to the device epsilon (and epsilon_sum for ragged). */
 int brr_hip_compute_num(brr_hip_ctx* ctx, uint32_t marker, double beta_old, double* num);

@@ -35,7 +35,7 @@ def test_every_declared_symbol_is_exported():
 
 
 def test_abi_version():
-    assert brr.lib().brr_hip_abi_version() == 1
+    assert brr.lib().brr_hip_abi_version() == 2
 
 
 def test_create_rejects_bad_config():
