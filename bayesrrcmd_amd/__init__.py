@@ -25,7 +25,7 @@ from . import _abi
 from ._abi import LAYOUT_DENSE_2BIT, LAYOUT_RAGGED, STORAGE_AUTO, STORAGE_CODES, STORAGE_LISTS
 
 __all__ = [
-    "BrrHipError", "HipSweep", "RaggedArena", "build_dense", "build_ragged", "synth_bed",
+    "BrrHipError", "HipSweep", "RaggedArena", "build_dense", "build_ragged", "synth_bed", "synth_xb",
     "bayesr_init", "shuffle", "marker_uz", "LAYOUT_DENSE_2BIT", "LAYOUT_RAGGED", "lib",
     "STORAGE_AUTO", "STORAGE_CODES", "STORAGE_LISTS",
 ]
@@ -71,6 +71,20 @@ def synth_bed(seed: int, n: int, m: int, maf_lo: float, maf_hi: float, miss_rate
     if rc != 0:
         raise BrrHipError(f"brr_synth_bed failed with status {rc}")
     return bed, maf
+
+
+def synth_xb(bed: np.ndarray, n: int, lut: np.ndarray, beta: np.ndarray, threads: int | None = None):
+    """X beta over the standardised columns: (n,) for beta (m,), (n, T) for beta (m, T)."""
+    bed = np.ascontiguousarray(bed, dtype=np.uint8)
+    lut = np.ascontiguousarray(lut, dtype=np.float64)
+    b = np.ascontiguousarray(beta, dtype=np.float64)
+    T = 1 if b.ndim == 1 else b.shape[1]
+    out = np.empty((n, T), dtype=np.float64)
+    rc = lib().brr_synth_xb(_p(bed, C.c_uint8), bed.shape[1], n, bed.shape[0], _p(lut, C.c_double), T,
+                            _p(b, C.c_double), _p(out, C.c_double), threads or _threads())
+    if rc != 0:
+        raise BrrHipError(f"brr_synth_xb failed with status {rc}")
+    return out[:, 0] if b.ndim == 1 else out
 
 
 def build_dense(bed: np.ndarray, n: int, threads: int | None = None) -> np.ndarray:
@@ -287,12 +301,16 @@ class HipSweep:
         self._ck(lib().brr_hip_epsilon_change(self.ctx, marker, beta_old, beta), "brr_hip_epsilon_change")
 
     PHASES = ("block_load", "dots", "exchange", "decisions", "commit_update", "iteration_scalars")
+    BATCH_PHASES = ("block_commit", "dot_tiles", "exchange_gather", "decision_draws", "residual_update",
+                    "iteration_scalars", "gram_tiles", "uz_draws", "exchange_reduce", "accept_correct",
+                    "block_end_barrier")
 
     def stats(self) -> dict:
         s = _abi.Stats()
         self._ck(lib().brr_hip_last_stats(self.ctx, C.byref(s)), "brr_hip_last_stats")
         out = {f: getattr(s, f) for f, _ in _abi.Stats._fields_}
-        ph = np.zeros(6)
-        self._ck(lib().brr_hip_last_phase_ms(self.ctx, _p(ph, C.c_double), 6), "brr_hip_last_phase_ms")
-        out["phase_ms"] = dict(zip(self.PHASES, ph.tolist()))
+        ph = np.zeros(12)
+        self._ck(lib().brr_hip_last_phase_ms(self.ctx, _p(ph, C.c_double), 12), "brr_hip_last_phase_ms")
+        names = self.BATCH_PHASES if self.traits else self.PHASES
+        out["phase_ms"] = dict(zip(names, ph.tolist()))
         return out

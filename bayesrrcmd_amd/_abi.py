@@ -44,7 +44,8 @@ class Scalars(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("markers", C.c_uint64), ("rounds", C.c_uint64), ("changes", C.c_uint64),
                 ("kernel_ms", C.c_double), ("workgroups", C.c_uint32), ("block_markers", C.c_uint32),
-                ("slice", C.c_uint32), ("storage", C.c_uint32)]
+                ("slice", C.c_uint32), ("storage", C.c_uint32),
+                ("decision_rounds", C.c_uint64)]
 
 
 CTX = C.c_void_p
@@ -85,6 +86,8 @@ SIGNATURES = {
     "brr_marker_uz": (C.c_int, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, P_f64, P_f64]),
     "brr_synth_bed": (C.c_int, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double,
                                 P_u8, C.c_uint64, P_f64, C.c_uint32]),
+    "brr_synth_xb": (C.c_int, [P_u8, C.c_uint64, C.c_uint32, C.c_uint32, P_f64, C.c_uint32, P_f64, P_f64,
+                               C.c_uint32]),
 }
 
 _LIB = None

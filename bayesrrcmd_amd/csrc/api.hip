@@ -17,6 +17,10 @@
 
 using namespace brr;
 
+// device counters of a launch: [0] rounds, [1] changes, [2..7] phase ticks,
+// batched kernel also [8] decision rounds (summed over waves), [10..15] phase ticks 6..11
+constexpr int kCounters = 16;
+
 struct brr_hip_ctx {
     brr_hip_config cfg{};
     int device = 0;
@@ -35,7 +39,7 @@ struct brr_hip_ctx {
     brr_hip_model model{};
     std::vector<double> cva;
     brr_hip_stats stats{};
-    double phase_ms[kPhases] = {0, 0, 0, 0, 0, 0};
+    double phase_ms[kBPhases] = {};
     // dense arena
     uint32_t* d_dense = nullptr;
     uint64_t col_words = 0, col_bytes = 0;
@@ -351,7 +355,7 @@ int launch_fn(brr_hip_ctx* c, const void* fn, void* param)
 {
     if (!fn) return fail(c, BRR_HIP_EINVAL, "unsupported slice size");
     HIP_TRY(c, hipMemsetAsync(c->d_ctl + kCtlAbort, 0, sizeof(uint32_t) * 2, c->stream));
-    HIP_TRY(c, hipMemsetAsync(c->d_counters, 0, sizeof(unsigned long long) * 8, c->stream));
+    HIP_TRY(c, hipMemsetAsync(c->d_counters, 0, sizeof(unsigned long long) * kCounters, c->stream));
     void* args[] = {param};
     HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
     HIP_TRY(c, hipLaunchCooperativeKernel(fn, dim3(c->P), dim3(kThreads), args, 0, c->stream));
@@ -360,13 +364,15 @@ int launch_fn(brr_hip_ctx* c, const void* fn, void* param)
     float ms = 0.f;
     HIP_TRY(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     uint32_t ctl[4] = {0, 0, 0, 0};
-    unsigned long long counters[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    unsigned long long counters[kCounters] = {};
     HIP_TRY(c, hipMemcpy(ctl, c->d_ctl, sizeof(ctl), hipMemcpyDeviceToHost));
     HIP_TRY(c, hipMemcpy(counters, c->d_counters, sizeof(counters), hipMemcpyDeviceToHost));
     c->stats.kernel_ms = ms;
     c->stats.rounds = counters[0];
     c->stats.changes = counters[1];
-    for (int q = 0; q < kPhases; ++q) c->phase_ms[q] = (double)counters[2 + q] * 1e-5;
+    c->stats.decision_rounds = counters[8];
+    for (int q = 0; q < 6; ++q) c->phase_ms[q] = (double)counters[2 + q] * 1e-5;
+    for (int q = 6; q < kBPhases; ++q) c->phase_ms[q] = (double)counters[4 + q] * 1e-5;
     c->stats.workgroups = c->P;
     c->stats.block_markers = c->T ? (uint32_t)kBB : c->B;
     c->stats.slice = c->S;
@@ -504,7 +510,7 @@ int brr_hip_create(const brr_hip_config* cfg, brr_hip_ctx** out)
         (rc = dalloc(c, &c->d_scal, SL.size() * Tn)) || (rc = dalloc(c, &c->d_cva, G * (K - 1))) ||
         (rc = dalloc(c, &c->d_xchg1, c->x1_words)) ||
         (rc = dalloc(c, &c->d_xchg2, c->x2_words)) || (rc = dalloc(c, &c->d_ctl, 8)) ||
-        (rc = dalloc(c, &c->d_counters, 8)) || (rc = dalloc(c, &c->d_tmp, 4)) ||
+        (rc = dalloc(c, &c->d_counters, kCounters)) || (rc = dalloc(c, &c->d_tmp, 4)) ||
         (rc = dalloc(c, &c->d_u, M * Tn)) || (rc = dalloc(c, &c->d_z, M * Tn))) {
         brr_hip_destroy(c);
         return rc;
@@ -881,7 +887,7 @@ int brr_hip_debug_block_codes(brr_hip_ctx* c, const uint32_t* order, const doubl
 int brr_hip_last_phase_ms(const brr_hip_ctx* c, double* out, uint32_t n)
 {
     if (!c || !out) return BRR_HIP_EINVAL;
-    for (uint32_t q = 0; q < n && q < (uint32_t)kPhases; ++q) out[q] = c->phase_ms[q];
+    for (uint32_t q = 0; q < n && q < (uint32_t)kBPhases; ++q) out[q] = c->phase_ms[q];
     return BRR_HIP_OK;
 }
 

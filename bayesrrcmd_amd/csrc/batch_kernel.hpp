@@ -37,6 +37,10 @@ constexpr int kMaxNT = 16;                   // 16-individual tiles per workgrou
 constexpr int kBRow = 4 * kMaxNT + 4;        // LDS bytes per code row (padded: rows fall in distinct banks)
 constexpr int kBGram = kBB * (kBB - 1) / 2;  // strictly-upper Gram entries of a block
 constexpr uint32_t kBVmax = (uint32_t)(kBB * kMaxT + kBGram);   // values exchanged per block
+// phase timers: 0 block commit, 1 dot tiles, 2 exchange gather, 3 decision draws,
+// 4 residual update, 5 iteration scalars, 6 Gram tiles, 7 u / z draws,
+// 8 exchange reduce, 9 accept / correct, 10 block-end barrier, 11 spare
+constexpr int kBPhases = 12;
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
@@ -83,11 +87,15 @@ struct BShared {
     double cd[kBB][kMaxT];            // dots, corrected dots, then the accepted beta deltas
     double gm[kBB][kBB];              // Gram u_i . w_k, i < k
     double bn[kBB][kMaxT];            // decided beta
-    double ac[kBB][kMaxT];            // decided acum
     int8_t sl[kBB][kMaxT];            // decided component (-1: none selected)
+    double uu[kBB][kMaxT];            // uniform of (marker slot, trait), drawn while the exchange runs
+    uint64_t zw[kBB][kMaxT];          // its normal: Philox words (chain) or the given z's bits
+    uint16_t plist[kBB * kMaxT];      // pending pairs of a round after the first: k << 8 | t
     double gc[kMaxK][kMaxT];          // log pi_k - 0.5 log(logLScale S_k + 1)
     double gden[kMaxK][kMaxT];        // denom_k
     double gbsd[kMaxK][kMaxT];        // sqrt(sigmaE / denom_k)
+    double dg[kMaxK][kMaxT];          // logL_k - logL_0 = dg_k + num^2 hk_k
+    double hk[kMaxK][kMaxT];
     double mu[kMaxT], sigmaE[kMaxT], epsSum[kMaxT], sigmaG[kMaxT], betasqn[kMaxT];
     double pi[kMaxK][kMaxT];
     double v[kMaxK][kMaxT];
@@ -98,7 +106,8 @@ struct BShared {
     int fc[kMaxT];                    // first change among this round's pairs, per trait
     int pref[kMaxT + 1];              // exclusive prefix of pending pairs over traits
     uint32_t flag[kBB / 4];           // bit q: 4-marker step holds a change in trait tile q
-    uint64_t tk[kPhases];             // phase timers of workgroup 0 (100 MHz ticks)
+    uint64_t tk[kBPhases];            // phase timers of workgroup 0 (100 MHz ticks)
+    unsigned long long sum_changes, sum_drounds;
     uint64_t tlast;
     int abort;
 };
@@ -121,6 +130,14 @@ __device__ __forceinline__ uint64_t* bslot1(const BatchParams& p, uint32_t rnd, 
 __device__ __forceinline__ uint64_t* bslot2(const BatchParams& p, uint32_t rnd, uint32_t v)
 {
     return p.xchg2 + (((size_t)(rnd & 1u) * kBVmax + v) * 2u);
+}
+
+// Orders the LDS accesses of one wave across its lanes: the wave's LDS
+// operations complete in issue order, so a compiler-level barrier suffices.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
 }
 
 __device__ __forceinline__ dbl4 mfma16x16x4(double a, double b, dbl4 c)
@@ -265,24 +282,22 @@ __device__ __forceinline__ void bstore(const BatchParams& p, BBlock& L, const BN
 
 // ------------------------------------------------------------- decisions --
 
-// processColumn's draw for marker slot k of trait t (BayesRBase.cpp:300-366)
-// from the current dot in L.cd; results to L.bn / L.ac / L.sl.
+// num of processColumn for slot k of trait t from the current dot in L.cd
+// (densebayesrkernel.cpp:10-15; sparsebayesrkernel.cpp:24-27)
 template <bool kDenseMath>
-__device__ __forceinline__ void bdecide(const BatchParams& p, BShared& L, const BBlock& B, int k, int t,
-                                        uint32_t iteration, double dN, double NM1)
+__device__ __forceinline__ double bnum(const BShared& L, const BBlock& B, int k, int t, double dN, double NM1)
 {
-    const int K = (int)p.K;
-    const uint32_t j = B.mk[k];
     const double beta_old = B.bo[k][t];
-    double num;
-    if (kDenseMath) {
-        num = L.cd[k][t] + beta_old * NM1;
-    } else {
-        const double mean = B.mstat[k][0], sd = B.mstat[k][1];
-        num = beta_old * (dN - 1.0) - mean * L.epsSum[t] / sd + L.cd[k][t] / sd;
-    }
+    if (kDenseMath) return L.cd[k][t] + beta_old * NM1;
+    const double mean = B.mstat[k][0], sd = B.mstat[k][1];
+    return beta_old * (dN - 1.0) - mean * L.epsSum[t] / sd + L.cd[k][t] / sd;
+}
+
+// logL and mu of every component (BayesRBase.cpp:309-333)
+__device__ __forceinline__ void blogl(const BShared& L, int t, int K, double num, double (&logL)[kMaxK],
+                                      double (&muk)[kMaxK])
+{
     const double sigmaE = L.sigmaE[t];
-    double logL[kMaxK], muk[kMaxK];
     logL[0] = L.gc[0][t];
     muk[0] = 0.0;
 #pragma unroll
@@ -294,43 +309,81 @@ __device__ __forceinline__ void bdecide(const BatchParams& p, BShared& L, const 
             logL[c] = L.gc[c][t] + 0.5 * (muk[c] * num) / sigmaE;
         }
     }
-    double acum = guard700(logL, K, logL[0]) ? 0.0 : inv_sum_exp(logL, K, logL[0]);
-    const double acum0 = acum;
-    U4 xr{};
-    double uu;
-    if (p.mode == kModeChain) {
-        xr = draw_block(p.seed, j, iteration, kStreamMarker, p.chain + (uint32_t)t);
-        uu = unit_u53(xr.x[0], xr.x[1]);
-    } else {
-        uu = p.u[(size_t)j * p.T + t];
-    }
-    int sel = -1;
+}
+
+// The draw of processColumn (BayesRBase.cpp:335-366) from num, u and z. With
+// d_k = logL_k - logL_0 = dg_k + num^2 hk_k, the probability terms
+// 1 / sum_i exp(logL_i - logL_c) equal exp(d_c) / sum_i exp(d_i); while every
+// |d_k| <= 350 no 700 guard can fire and no exponential overflows, so they are
+// formed from one set of K - 1 exponentials. Otherwise the reference's own
+// formulation, guards included, is evaluated.
+struct BDraw {
+    double bnew, acum0;
+    int sel;
+};
+
+__device__ __forceinline__ BDraw bdraw(const BShared& L, int t, int K, double num, double beta_old, double u,
+                                       uint64_t zw, bool zwords)
+{
+    const double n2 = num * num;
+    double d[kMaxK];
+    d[0] = 0.0;
+    bool far = false;
 #pragma unroll
-    for (int c = 0; c < kMaxK; ++c) {
-        if (c < K && sel < 0) {
-            if (uu <= acum) {
-                sel = c;
-            } else if (c + 1 < K) {
-                const double ref = logL[c + 1 < kMaxK ? c + 1 : 0];
+    for (int c = 1; c < kMaxK; ++c) {
+        d[c] = 0.0;
+        if (c < K) {
+            d[c] = L.dg[c][t] + n2 * L.hk[c][t];
+            far = far || fabs(d[c]) > 350.0;
+        }
+    }
+    BDraw r{beta_old, 0.0, -1};
+    if (!far) {
+        double e[kMaxK];
+        double S = 1.0;
+        e[0] = 1.0;
+#pragma unroll
+        for (int c = 1; c < kMaxK; ++c) {
+            e[c] = 0.0;
+            if (c < K) {
+                e[c] = exp(d[c]);
+                S += e[c];
+            }
+        }
+        const double invS = 1.0 / S;
+        double acum = invS;
+        r.acum0 = acum;
+#pragma unroll
+        for (int c = 0; c < kMaxK; ++c) {
+            if (c < K && r.sel < 0) {
+                if (u <= acum) r.sel = c;
+                else if (c + 1 < kMaxK && c + 1 < K) acum += e[c + 1 < kMaxK ? c + 1 : 0] * invS;
+            }
+        }
+    } else {
+        double logL[kMaxK], muk[kMaxK];
+        blogl(L, t, K, num, logL, muk);
+        double acum = guard700(logL, K, logL[0]) ? 0.0 : inv_sum_exp(logL, K, logL[0]);
+        r.acum0 = acum;
+#pragma unroll 1
+        for (int c = 0; c < K; ++c) {
+            if (u <= acum) { r.sel = c; break; }
+            if (c + 1 < K) {
+                double ref = 0.0;
+#pragma unroll
+                for (int q = 1; q < kMaxK; ++q)
+                    if (q == c + 1) ref = logL[q];
                 if (!guard700(logL, K, ref)) acum += inv_sum_exp(logL, K, ref);
             }
         }
     }
-    double bnew = beta_old;
-    if (sel == 0) {
-        bnew = 0.0;
-    } else if (sel > 0) {
-        double m = 0.0;
-#pragma unroll
-        for (int c = 1; c < kMaxK; ++c)
-            if (c == sel) m = muk[c];
-        const double zz = (p.mode == kModeChain) ? unit_normal(xr.x[2], xr.x[3]) : p.z[(size_t)j * p.T + t];
-        bnew = zz * L.gbsd[sel][t] + m;
+    if (r.sel == 0) {
+        r.bnew = 0.0;
+    } else if (r.sel > 0) {
+        const double z = zwords ? unit_normal((uint32_t)(zw >> 32), (uint32_t)zw) : bits_as_double(zw);
+        r.bnew = z * L.gbsd[r.sel][t] + num / L.gden[r.sel][t];
     }
-    L.bn[k][t] = bnew;
-    L.ac[k][t] = acum0;
-    L.sl[k][t] = (int8_t)sel;
-    if (bnew != beta_old) atomicMin(&L.fc[t], k);
+    return r;
 }
 
 // -------------------------------------------------------------------- kernel --
@@ -391,8 +444,9 @@ bayesr_batch_kernel(BatchParams p)
     for (int e = tid; e < kBB * kMaxT; e += kThreads) (&L.cd[0][0])[e] = 0.0;
     if (tid == 0) L.abort = 0;
     uint32_t rnd = __hip_atomic_load(p.ctl + kCtlRound, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    unsigned long long n_rounds = 0, n_changes = 0;
-    if (tid < kPhases) L.tk[tid] = 0;
+    unsigned long long n_rounds = 0, n_changes = 0, n_drounds = 0;
+    if (tid < kBPhases) L.tk[tid] = 0;
+    if (tid == 0) { L.sum_changes = 0; L.sum_drounds = 0; }
     if (tid == 0) L.tlast = __builtin_amdgcn_s_memrealtime();
     int cb = 0;     // current block buffer
     __syncthreads();
@@ -470,7 +524,13 @@ bayesr_batch_kernel(BatchParams p)
                 L.gden[k][t] = den;
                 L.gc[k][t] = log(L.pi[k][t]) - 0.5 * log(logLScale * S + 1.0);
                 L.gbsd[k][t] = __dsqrt_rn(sigmaE / den);
+                L.hk[k][t] = 0.5 / (den * sigmaE);
             }
+        }
+        __syncthreads();
+        for (int e = tid; e < T * K; e += kThreads) {
+            const int t = e / K, k = e % K;
+            L.dg[k][t] = (k == 0) ? 0.0 : L.gc[k][t] - L.gc[0][t];
         }
         // ---- block 0 loaded in place; ids of block 1 staged
         {
@@ -507,68 +567,107 @@ bayesr_batch_kernel(BatchParams p)
             ++n_rounds;
             BRR_BTICK(0);
 
-            // ---- 1. partial dots C[k][t] over this slice: wave q = traits 16q..16q+15
-            if (wave < (int)p.nq) {
-                dbl4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
-                const bool two = nb > 16;
+            // ---- 1. partial dots C[k][t] over this slice (wave q: traits 16q..16q+15)
+            // and the partial Gram tiles (0,0), (0,1), (1,1) on waves 0, 1, 2. The
+            // lane's code rows (markers tq and 16 + tq) and their tables sit in
+            // registers, so each matrix-core step decodes its operand with a few
+            // VALU selects instead of dependent LDS reads.
+            {
+                uint32_t rw0[NT], rw1[NT];
 #pragma unroll
                 for (int s = 0; s < NT; ++s) {
-                    if (s < nt) {
-#pragma unroll
-                        for (int r = 0; r < 4; ++r) {
-                            const uint32_t by0 = B.codes[tq][4 * s + r];
-                            const uint32_t by1 = B.codes[16 + tq][4 * s + r];
-                            acc0 = mfma16x16x4(B.wt[tq][(by0 >> (2 * kq)) & 3u], E[s][r], acc0);
-                            if (two) acc1 = mfma16x16x4(B.wt[16 + tq][(by1 >> (2 * kq)) & 3u], E[s][r], acc1);
-                        }
-                    }
+                    rw0[s] = (s < nt) ? *(const uint32_t*)&B.codes[tq][4 * s] : 0u;
+                    rw1[s] = (s < nt) ? *(const uint32_t*)&B.codes[16 + tq][4 * s] : 0u;
                 }
-                if (tval) {
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int k0 = kq + 4 * r;
-                        if (k0 < nb) publish_pair(bslot1(p, rnd, w, (uint32_t)(k0 * T + tl)), rnd, acc0[r]);
-                        if (16 + k0 < nb) publish_pair(bslot1(p, rnd, w, (uint32_t)((16 + k0) * T + tl)), rnd, acc1[r]);
-                    }
-                }
-            }
-            // ---- partial Gram tiles (0,0), (0,1), (1,1) on waves 0, 1, 2
-            if (wave < 3) {
-                const int mi = (wave == 2) ? 1 : 0;
-                const int mc = (wave == 0) ? 0 : 1;
-                if (16 * mc < nb) {
-                    dbl4 g = {0.0, 0.0, 0.0, 0.0};
+                const double w00 = B.wt[tq][0], w01 = B.wt[tq][1], w02 = B.wt[tq][2], w03 = B.wt[tq][3];
+                const double w10 = B.wt[16 + tq][0], w11 = B.wt[16 + tq][1], w12 = B.wt[16 + tq][2],
+                             w13 = B.wt[16 + tq][3];
+                if (wave < (int)p.nq) {
+                    // two accumulator chains per marker tile (even / odd residual tiles)
+                    dbl4 acc0[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+                    dbl4 acc1[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+                    // straight-line code (no per-step branches, which would force the
+                    // accumulators out of the matrix-core registers at each join):
+                    // tiles past the slice hold zero residuals, slots past the block
+                    // zero weights
 #pragma unroll
                     for (int s = 0; s < NT; ++s) {
-                        if (s < nt) {
 #pragma unroll
-                            for (int r = 0; r < 4; ++r) {
-                                const uint32_t i = 16u * (uint32_t)s + 4u * (uint32_t)r + (uint32_t)kq;
-                                const uint32_t byi = B.codes[16 * mi + tq][4 * s + r];
-                                const uint32_t byk = B.codes[16 * mc + tq][4 * s + r];
-                                const double a = (i < nloc) ? B.ut[16 * mi + tq][(byi >> (2 * kq)) & 3u] : 0.0;
-                                g = mfma16x16x4(a, B.wt[16 * mc + tq][(byk >> (2 * kq)) & 3u], g);
-                            }
+                        for (int r = 0; r < 4; ++r) {
+                            const uint32_t sh = 8u * (uint32_t)r + 2u * (uint32_t)kq;
+                            acc0[s & 1] = mfma16x16x4(wt_of((rw0[s] >> sh) & 3u, w00, w01, w02, w03), E[s][r], acc0[s & 1]);
+                            acc1[s & 1] = mfma16x16x4(wt_of((rw1[s] >> sh) & 3u, w10, w11, w12, w13), E[s][r], acc1[s & 1]);
                         }
                     }
-                    const int CT = nb * T;
+                    const dbl4 c0 = acc0[0] + acc0[1], c1 = acc1[0] + acc1[1];
+                    if (tval) {
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int i = 16 * mi + kq + 4 * r, k = 16 * mc + tq;
-                        if (i < k && k < nb)
-                            publish_pair(bslot1(p, rnd, w, (uint32_t)(CT + k * (k - 1) / 2 + i)), rnd, g[r]);
+                        for (int r = 0; r < 4; ++r) {
+                            const int k0 = kq + 4 * r;
+                            if (k0 < nb) publish_pair(bslot1(p, rnd, w, (uint32_t)(k0 * T + tl)), rnd, c0[r]);
+                            if (16 + k0 < nb)
+                                publish_pair(bslot1(p, rnd, w, (uint32_t)((16 + k0) * T + tl)), rnd, c1[r]);
+                        }
+                    }
+                }
+                BRR_BTICK(1);
+                if (wave < 3) {
+                    const bool mi = wave == 2;        // row tile of u (A operand)
+                    const bool mc = wave != 0;        // column tile of w (B operand)
+                    if (16 * (int)mc < nb) {
+                        const int ra = 16 * (int)mi + tq;
+                        const double u0 = B.ut[ra][0], u1 = B.ut[ra][1], u2 = B.ut[ra][2], u3 = B.ut[ra][3];
+                        const double b0 = mc ? w10 : w00, b1 = mc ? w11 : w01, b2 = mc ? w12 : w02,
+                                     b3 = mc ? w13 : w03;
+                        dbl4 gg[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+#pragma unroll
+                        for (int s = 0; s < NT; ++s) {
+                            const uint32_t rowa = mi ? rw1[s] : rw0[s];
+                            const uint32_t rowb = mc ? rw1[s] : rw0[s];
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) {
+                                const uint32_t sh = 8u * (uint32_t)r + 2u * (uint32_t)kq;
+                                const uint32_t i = 16u * (uint32_t)s + 4u * (uint32_t)r + (uint32_t)kq;
+                                const double av = (i < nloc) ? wt_of((rowa >> sh) & 3u, u0, u1, u2, u3) : 0.0;
+                                gg[r & 1] = mfma16x16x4(av, wt_of((rowb >> sh) & 3u, b0, b1, b2, b3), gg[r & 1]);
+                            }
+                        }
+                        const dbl4 g = gg[0] + gg[1];
+                        const int CT = nb * T;
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const int i = 16 * (int)mi + kq + 4 * r, k = 16 * (int)mc + tq;
+                            if (i < k && k < nb)
+                                publish_pair(bslot1(p, rnd, w, (uint32_t)(CT + k * (k - 1) / 2 + i)), rnd, g[r]);
+                        }
                     }
                 }
             }
+            BRR_BTICK(6);
             // the prefetched block goes to the other buffer (read after this block)
             if (has1) bstore(p, L.blk[cb ^ 1], x, nb1, tid);
-            BRR_BTICK(1);
+            // u and z of every (slot, trait) pair, drawn while the partials travel
+            for (int e = tid; e < nb * kMaxT; e += kThreads) {
+                const int k = e >> 6, t = e & 63;
+                if (t >= T) continue;
+                const uint32_t j = B.mk[k];
+                if (p.mode == kModeChain) {
+                    const U4 xr = draw_block(p.seed, j, iteration, kStreamMarker, p.chain + (uint32_t)t);
+                    L.uu[k][t] = unit_u53(xr.x[0], xr.x[1]);
+                    L.zw[k][t] = ((uint64_t)xr.x[2] << 32) | xr.x[3];   // z formed when it is needed
+                } else {
+                    L.uu[k][t] = p.u[(size_t)j * p.T + t];
+                    L.zw[k][t] = double_as_bits(p.z[(size_t)j * p.T + t]);
+                }
+            }
+            BRR_BTICK(7);
 
             // ---- 2. exchange: every workgroup ends with the full C and G
             {
                 const uint32_t CT = (uint32_t)nb * (uint32_t)T;
                 const uint32_t V = CT + (uint32_t)(nb * (nb - 1) / 2);
                 if (!breduce(p, L, rnd, V, w, tid)) goto done;
+                BRR_BTICK(8);
                 for (uint32_t v = (uint32_t)tid; v < V; v += kThreads) {
                     double xv;
                     if (!wait_pair(bslot2(p, rnd, v), rnd, p.ctl, &xv)) { L.abort = 1; raise_abort(p, 13); break; }
@@ -590,109 +689,155 @@ bayesr_batch_kernel(BatchParams p)
             }
             BRR_BTICK(2);
 
-            // ---- 3. decisions: rounds over the pending (marker, trait) pairs
-            while (true) {
-                if (wave == 0) {
-                    const int cnt = nb - L.pos[lane];
+            // ---- 3. decisions, wave q for its traits 16q .. 16q+15 (traits are
+            // independent chains, so each wave resolves its tile without
+            // workgroup barriers). Rounds: every pending (slot, trait) pair is
+            // drawn in parallel; each trait accepts its pairs up to its first beta
+            // change, the dots of its later slots are corrected by delta * Gram
+            // row (BayesRBase.cpp:370-378) and those slots are drawn again.
+            if (wave < (int)p.nq) {
+                const int tt = 16 * wave + tq;             // trait of this lane's accept work
+                const bool tok = tt < T;
+                const bool mine = tok && ((uint32_t)tt + blk * 7u) % p.P == w;   // output workgroup
+                uint16_t* plist = L.plist + wave * (kBB * 16);
+                for (bool first = true;; first = false) {
+                    const int p0 = tok ? L.pos[tt] : nb;
+                    const int cnt = nb - p0;
                     int incl = cnt;
 #pragma unroll
-                    for (int off = 1; off < 64; off <<= 1) {
-                        const int y = __shfl_up(incl, off, 64);
-                        if (lane >= off) incl += y;
+                    for (int off = 1; off < 16; off <<= 1) {
+                        const int y = __shfl_up(incl, off, 16);
+                        if (tq >= off) incl += y;
                     }
-                    L.pref[lane] = incl - cnt;
-                    if (lane == 63) L.pref[kMaxT] = incl;
-                }
-                __syncthreads();
-                const int total = L.pref[kMaxT];
-                if (total == 0) break;
-                for (int e = tid; e < total; e += kThreads) {
-                    int a = 0, b = kMaxT - 1;   // last trait whose prefix <= e
-                    while (a < b) {
-                        const int mid = (a + b + 1) >> 1;
-                        if (L.pref[mid] <= e) a = mid; else b = mid - 1;
-                    }
-                    bdecide<kDenseMath>(p, L, B, L.pos[a] + (e - L.pref[a]), a, iteration, dN, NM1);
-                }
-                __syncthreads();
-                // accept each trait's pairs up to its first change; correct the
-                // dots of its later pairs by delta * Gram row (BayesRBase.cpp:370-378)
-                if (lane < T) {
-                    const int t = lane, c = L.fc[t], p0 = L.pos[t];
-                    const double dl = (c < nb) ? B.bo[c][t] - L.bn[c][t] : 0.0;
-                    for (int k = p0 + wave; k < nb; k += kWaves) {
-                        if (k <= c) {
-                            const int sel = L.sl[k][t];
-                            if (sel >= 0) atomicAdd(&L.v[sel][t], 1.0);
-                            if (w == 0) {
-                                const size_t o = (size_t)B.mk[k] * p.T + (size_t)t;
-                                st_agent((uint64_t*)(p.beta + o), double_as_bits(L.bn[k][t]));
-                                if (sel >= 0) p.comp[o] = sel;
-                                p.acum[o] = L.ac[k][t];
-                            }
-                            L.cd[k][t] = (k == c) ? dl : 0.0;
-                            if (k == c) atomicOr(&L.flag[k >> 2], 1u << (t >> 4));
+                    const int total = __shfl(incl, 15, 64);
+                    if (total == 0) break;
+                    if (!first && kq == 0)
+                        for (int i = 0; i < cnt; ++i) plist[incl - cnt + i] = (uint16_t)(((p0 + i) << 8) | tt);
+                    wave_lds_sync();
+                    const int span = first ? nb * 16 : total;
+                    for (int e = lane; e < span; e += 64) {
+                        int k, t;
+                        if (first) {
+                            k = e >> 4;
+                            t = 16 * wave + (e & 15);
+                            if (t >= T) continue;
                         } else {
-                            L.cd[k][t] += dl * L.gm[c][k];
+                            const int kt = plist[e];
+                            k = kt >> 8;
+                            t = kt & 0xFF;
+                        }
+                        const double beta_old = B.bo[k][t];
+                        const BDraw r = bdraw(L, t, K, bnum<kDenseMath>(L, B, k, t, dN, NM1), beta_old,
+                                              L.uu[k][t], L.zw[k][t], p.mode == kModeChain);
+                        L.bn[k][t] = r.bnew;
+                        L.sl[k][t] = (int8_t)r.sel;
+                        if (r.bnew != beta_old) atomicMin(&L.fc[t], k);
+                    }
+                    ++n_drounds;
+                    wave_lds_sync();
+                    BRR_BTICK(3);
+                    // accept the trait's slots up to its first change; correct later dots
+                    if (tok) {
+                        const int c = L.fc[tt];
+                        const double dl = (c < nb) ? B.bo[c][tt] - L.bn[c][tt] : 0.0;
+                        for (int k = p0 + kq; k < nb; k += 4) {
+                            if (k <= c) {
+                                const int sel = L.sl[k][tt];
+                                if (sel >= 0) atomicAdd(&L.v[sel][tt], 1.0);
+                                if (mine) {
+                                    const size_t o = (size_t)B.mk[k] * p.T + (size_t)tt;
+                                    const double ac = bdraw(L, tt, K, bnum<kDenseMath>(L, B, k, tt, dN, NM1),
+                                                            B.bo[k][tt], 2.0, 0ull, false).acum0;
+                                    // agent-scope stores: the same slot may be written by
+                                    // another workgroup (another XCD's L2) in a later iteration
+                                    st_agent((uint64_t*)(p.beta + o), double_as_bits(L.bn[k][tt]));
+                                    if (sel >= 0)
+                                        __hip_atomic_store(p.comp + o, sel, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                                    st_agent((uint64_t*)(p.acum + o), double_as_bits(ac));
+                                }
+                                L.cd[k][tt] = (k == c) ? dl : 0.0;
+                                if (k == c) atomicOr(&L.flag[k >> 2], 1u << wave);
+                            } else {
+                                L.cd[k][tt] += dl * L.gm[c][k];
+                            }
                         }
                     }
-                }
-                __syncthreads();
-                if (wave == 0) {
-                    const int t = lane;
-                    const int c = L.fc[t];
-                    const bool act = L.pos[t] < nb;
-                    if (act && c < nb) {
-                        if (!dense) {
-                            // epsilonSum (sparsebayesrkernel.cpp:29-34)
-                            const double dBeta = B.bo[c][t] - L.bn[c][t];
-                            const double mean = B.mstat[c][0], sd = B.mstat[c][1], Zsum = B.mstat[c][2];
-                            L.epsSum[t] += dBeta * Zsum / sd - dBeta * mean * dN / sd;
+                    wave_lds_sync();
+                    if (kq == 0 && tok) {
+                        const int c = L.fc[tt];
+                        if (c < nb) {
+                            if (!dense) {
+                                // epsilonSum (sparsebayesrkernel.cpp:29-34)
+                                const double dBeta = B.bo[c][tt] - L.bn[c][tt];
+                                const double mean = B.mstat[c][0], sd = B.mstat[c][1], Zsum = B.mstat[c][2];
+                                L.epsSum[tt] += dBeta * Zsum / sd - dBeta * mean * dN / sd;
+                            }
+                            L.pos[tt] = c + 1;
+                            ++n_changes;
+                        } else {
+                            L.pos[tt] = nb;
                         }
-                        L.pos[t] = c + 1;
-                    } else if (act) {
-                        L.pos[t] = nb;
+                        L.fc[tt] = nb;
                     }
-                    L.fc[t] = nb;
-                    n_changes += (unsigned long long)__popcll(__ballot(act && c < nb));
+                    wave_lds_sync();
+                    BRR_BTICK(9);
                 }
-            }
-            BRR_BTICK(3);
-
-            // ---- betasqnG += beta^2 in marker order (BayesRBase.cpp:353)
-            if (tid < T) {
-                for (int k = 0; k < nb; ++k)
-                    if (L.sl[k][tid] > 0) L.betasqn[tid] += L.bn[k][tid] * L.bn[k][tid];
-            }
-            // ---- 4. eps_t += sum_k delta_{k,t} u_k on the matrix cores: per
-            // residual tile, the 4-marker steps holding a change of this wave's traits
-            if (wave < (int)p.nq) {
+                // betasqnG += beta^2 in marker order (BayesRBase.cpp:353)
+                if (kq == 0 && tok) {
+                    for (int k = 0; k < nb; ++k)
+                        if (L.sl[k][tt] > 0) L.betasqn[tt] += L.bn[k][tt] * L.bn[k][tt];
+                }
+                // ---- 4. eps_t += sum_k delta_{k,t} u_k on the matrix cores, for the
+                // 4-marker steps holding a change of this wave's traits
                 uint32_t fmask = 0u;
 #pragma unroll
                 for (int p4 = 0; p4 < kBB / 4; ++p4)
                     if ((L.flag[p4] >> wave) & 1u) fmask |= 1u << p4;
                 if (fmask) {
+                    // two halves of the slice's tiles, so the per-step code rows and
+                    // tables are loaded once per 4-marker step into registers
+                    constexpr int HS = (NT + 1) / 2;
 #pragma unroll
-                    for (int s = 0; s < NT; ++s) {
-                        if (s < nt) {
-                            const uint32_t i = 16u * (uint32_t)s + (uint32_t)tq;
-                            dbl4 acc = {0.0, 0.0, 0.0, 0.0};
-                            for (uint32_t m = fmask; m; m &= m - 1u) {
-                                const int km = 4 * __builtin_ctz(m) + kq;
-                                const double bv = (km < nb) ? L.cd[km][16 * wave + tq] : 0.0;
-                                const uint32_t by = B.codes[km][4 * s + (tq >> 2)];
-                                const double a = (i < nloc && km < nb) ? B.ut[km][(by >> (2 * (tq & 3))) & 3u] : 0.0;
-                                acc = mfma16x16x4(a, bv, acc);
+                    for (int h = 0; h < 2; ++h) {
+                        dbl4 acc[HS];
+#pragma unroll
+                        for (int i2 = 0; i2 < HS; ++i2) acc[i2] = dbl4{0.0, 0.0, 0.0, 0.0};
+                        for (uint32_t m = fmask; m; m &= m - 1u) {
+                            const int km = 4 * __builtin_ctz(m) + kq;
+                            const bool kv = km < nb;
+                            uint32_t row[HS];
+#pragma unroll
+                            for (int i2 = 0; i2 < HS; ++i2) {
+                                const int s2 = h * HS + i2;
+                                row[i2] = (s2 < NT && s2 < nt && kv) ? *(const uint32_t*)&B.codes[km][4 * s2] : 0u;
                             }
-                            E[s] += acc;
+                            const double u0 = B.ut[km][0], u1 = B.ut[km][1], u2 = B.ut[km][2], u3 = B.ut[km][3];
+                            const double bv = kv ? L.cd[km][16 * wave + tq] : 0.0;
+                            // straight-line: rows past the slice are masked to zero by i < nloc
+#pragma unroll
+                            for (int i2 = 0; i2 < HS; ++i2) {
+                                const int s2 = h * HS + i2;
+                                if (s2 < NT) {
+                                    const uint32_t i = 16u * (uint32_t)s2 + (uint32_t)tq;
+                                    const double av =
+                                        (i < nloc) ? wt_of((row[i2] >> (2u * (uint32_t)tq)) & 3u, u0, u1, u2, u3) : 0.0;
+                                    acc[i2] = mfma16x16x4(av, bv, acc[i2]);
+                                }
+                            }
+                        }
+#pragma unroll
+                        for (int i2 = 0; i2 < HS; ++i2) {
+                            const int s2 = h * HS + i2;
+                            if (s2 < NT) E[s2 < NT ? s2 : 0] += acc[i2];
                         }
                     }
                 }
+                BRR_BTICK(4);
             }
-            // workgroup 0's beta stores complete before later rounds let others read them
-            if (w == 0) __builtin_amdgcn_s_waitcnt(0);
+            // this workgroup's beta stores complete before later rounds let others read them
+            __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
-            BRR_BTICK(4);
+            BRR_BTICK(10);
 
             // ---- block blk+1 becomes current; its successor's ids go to the freed buffer
             cb ^= 1;
@@ -748,6 +893,12 @@ bayesr_batch_kernel(BatchParams p)
 
 done:
     __syncthreads();
+    if (w == 0) {
+        // per-lane change counts (lanes kq == 0 own a trait) and per-wave round counts
+        atomicAdd(&L.sum_changes, n_changes);
+        if (lane == 0) atomicAdd(&L.sum_drounds, n_drounds);
+    }
+    __syncthreads();
     if (L.abort == 0) {
 #pragma unroll
         for (int s = 0; s < NT; ++s)
@@ -773,8 +924,10 @@ done:
             if (tid == 0) {
                 __hip_atomic_store(p.ctl + kCtlRound, rnd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 p.counters[0] = n_rounds;
-                p.counters[1] = n_changes;
-                for (int q = 0; q < kPhases; ++q) p.counters[2 + q] = L.tk[q];
+                p.counters[1] = L.sum_changes;
+                for (int q = 0; q < 6; ++q) p.counters[2 + q] = L.tk[q];
+                p.counters[8] = L.sum_drounds;
+                for (int q = 6; q < kBPhases; ++q) p.counters[4 + q] = L.tk[q];
             }
         }
     }

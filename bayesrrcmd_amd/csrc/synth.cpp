@@ -9,7 +9,9 @@
 // Every value is a function of (seed, j, i) through Philox4x32-10, so any
 // slice of the matrix can be regenerated independently.
 #include <cstdint>
+#include <algorithm>
 #include <cstring>
+#include <utility>
 #include <thread>
 #include <vector>
 
@@ -81,5 +83,43 @@ extern "C" int brr_synth_bed(uint64_t seed, uint32_t num_inds, uint32_t num_snps
     if (maf)
         for (uint32_t j = 0; j < num_snps; ++j)
             if (maf[j] < 0.0) return 2;
+    return 0;
+}
+
+// y(:, t) = X beta(:, t) for T effect vectors over the standardised columns
+// (value lut[j][dosage], dosage 3 = missing), threads over individual ranges.
+extern "C" int brr_synth_xb(const uint8_t* bed, uint64_t column_stride, uint32_t num_inds,
+                            uint32_t num_snps, const double* lut, uint32_t num_traits,
+                            const double* beta, double* out, uint32_t threads)
+{
+    if (!bed || !lut || !beta || !out || num_traits == 0 || column_stride < ((uint64_t)num_inds + 3) / 4)
+        return 1;
+    const uint32_t T = num_traits;
+    std::vector<std::pair<uint32_t, uint32_t>> nz;     // (marker, trait) with beta != 0
+    for (uint32_t j = 0; j < num_snps; ++j)
+        for (uint32_t t = 0; t < T; ++t)
+            if (beta[(size_t)j * T + t] != 0.0) nz.emplace_back(j, t);
+    std::memset(out, 0, sizeof(double) * num_inds * T);
+    if (threads == 0) threads = 1;
+    const uint32_t chunks = (num_inds + 1023) / 1024;
+    if (threads > chunks) threads = chunks;
+    std::vector<std::thread> pool;
+    for (uint32_t th = 0; th < threads; ++th) {
+        pool.emplace_back([=, &nz]() {
+            for (uint32_t c = th; c < chunks; c += threads) {
+                const uint32_t i0 = c * 1024, i1 = std::min<uint32_t>(num_inds, i0 + 1024);
+                for (const auto& e : nz) {
+                    const uint8_t* col = bed + (size_t)e.first * column_stride;
+                    const double* lt = lut + 4 * (size_t)e.first;
+                    // 2-bit code v -> dosage: 0->2, 1->missing, 2->1, 3->0
+                    const double wv[4] = {lt[2], lt[3], lt[1], lt[0]};
+                    const double b = beta[(size_t)e.first * T + e.second];
+                    for (uint32_t i = i0; i < i1; ++i)
+                        out[(size_t)i * T + e.second] += b * wv[(col[i >> 2] >> (2 * (i & 3))) & 3];
+                }
+            }
+        });
+    }
+    for (auto& t : pool) t.join();
     return 0;
 }

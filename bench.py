@@ -8,11 +8,15 @@ data from a fixed seed. A step is one Gibbs iteration of the chain (updateMu,
 the sweep over all M markers, the sigmaE / sigmaG / pi draws), run by the
 persistent device kernel with genotypes and chain state resident in HBM.
 `--config c1` is the small dense case (N = 5,000, M = 10,000 .bed 2-bit).
+`--config c3` is the batched case: T = 64 traits (one phenotype and chain each,
+h2 ~ U(0.1, 0.8), causal fraction ~ U(0.1%, 2%)) over one dense 2-bit matrix of
+N = 50,000 x M = 50,000, chain length 100, all traits in one persistent launch.
 
 Prints one JSON line: marker-updates/s (whole job), the roofline of the sweep
 kernel measured with HIP events on its stream, and the CPU reference sweep
 (the FP64 oracle built with the reference's Release flags) timed on a bounded
-sample of the same workload on this host.
+sample of the same workload on this host. The default line also carries a
+"batched" object: the c3 workload measured the same way (batched vs plain).
 """
 from __future__ import annotations
 
@@ -41,6 +45,10 @@ CONFIGS = {
                      chain=200),
     "c1": dict(workload="plain BayesR ppbayes, dense .bed 2-bit, N=5000 M=10000, single chain",
                n=5_000, m=10_000, layout="dense", maf=(0.01, 0.5), causal=0.01, h2=0.5, chain=10),
+    "c3": dict(workload="batched traits: 64 BayesR chains (one phenotype each) over one dense .bed 2-bit "
+                        "matrix, N=50000 M=50000, shared marker order",
+               n=50_000, m=50_000, layout="dense", maf=(0.01, 0.5), traits=64, causal=(0.001, 0.02),
+               h2=(0.1, 0.8), chain=100),
 }
 
 
@@ -72,6 +80,30 @@ def make_data(cfg, seed, threads):
     lut = brr.build_dense(bed, n, threads=threads)
     log(f"dense tables built in {time.time() - t0:.1f}s")
     return dict(bed=bed, y=y, ragged=None, lut=lut)
+
+
+def make_batch_data(cfg, seed, threads):
+    """Genotypes plus T phenotypes: per trait h2 ~ U(h2), causal fraction ~ U(causal),
+    causal set uniform, beta ~ N(0, h2 / Mc), y = X beta + e, e ~ N(0, 1 - h2)."""
+    import bayesrrcmd_amd as brr
+    n, m, T = cfg["n"], cfg["m"], cfg["traits"]
+    t0 = time.time()
+    bed, _ = brr.synth_bed(seed, n, m, cfg["maf"][0], cfg["maf"][1], 0.0, threads=threads)
+    lut = brr.build_dense(bed, n, threads=threads)
+    log(f"genotypes {m} x {n} and tables generated in {time.time() - t0:.1f}s")
+    rng = np.random.default_rng(seed + 3)
+    beta = np.zeros((m, T))
+    h2 = rng.uniform(cfg["h2"][0], cfg["h2"][1], T)
+    frac = rng.uniform(cfg["causal"][0], cfg["causal"][1], T)
+    for t in range(T):
+        mc = max(1, int(round(frac[t] * m)))
+        causal = rng.choice(m, mc, replace=False)
+        beta[causal, t] = rng.normal(0.0, np.sqrt(h2[t] / mc), mc)
+    t0 = time.time()
+    g = brr.synth_xb(bed, n, lut, beta, threads=threads)
+    y = (g + rng.normal(0.0, 1.0, (n, T)) * np.sqrt(1.0 - h2)).astype(np.float32).astype(np.float64)
+    log(f"{T} phenotypes in {time.time() - t0:.1f}s")
+    return dict(bed=bed, lut=lut, y=y, h2=h2, frac=frac)
 
 
 def algorithmic_bytes(cfg, data, storage, markers_per_step, changes, eps_elem=8):
@@ -140,6 +172,145 @@ def cpu_baseline(cfg, data, seed, budget_s):
                         f"workload at N={n}, {iters} chain iterations from init, {t_total:.1f}s"))
 
 
+def cpu_baseline_batched(cfg, data, seed, budget_s, workers):
+    """T independent reference chains on the host, as T brr processes would run:
+    `workers` threads (ctypes releases the GIL) each drive one oracle chain
+    (-O3 -ffast-math, FP64) over the first markers of the batched workload."""
+    import ctypes as C
+    import threading
+    sys.path.insert(0, str(ROOT / "tests"))
+    import _oracle as orc
+    L = orc.lib(fast=True)
+    n = cfg["n"]
+    m_s = min(cfg["m"], 2000)
+    bed = np.ascontiguousarray(data["bed"][:m_s])
+    lut = np.ascontiguousarray(data["lut"][:m_s])
+    done = [0] * workers
+    spent = [0.0] * workers
+
+    def worker(t):
+        prob = orc.OracleProblem(n, m_s, orc.DENSE, 4, 1, S_DEFAULT, bed=bed, lut=lut)
+        y = np.ascontiguousarray(data["y"][:, t])
+        L.orc_init(C.byref(prob.data), C.byref(prob.model), C.byref(prob.state), orc.p(y, C.c_double), seed, t)
+        order = np.arange(m_s, dtype=np.uint32)
+        u, z = np.zeros(m_s), np.zeros(m_s)
+        it = 0
+        while spent[t] < budget_s and it < cfg["chain"]:
+            t0 = time.perf_counter()
+            L.orc_update_mu(C.byref(prob.data), C.byref(prob.state), seed, it, t)
+            L.orc_fill_marker_uz(seed, it, t, m_s, orc.p(u, C.c_double), orc.p(z, C.c_double))
+            L.orc_shuffle_philox(seed, it, 0, m_s, orc.p(order, C.c_uint32))
+            L.orc_sweep(C.byref(prob.data), C.byref(prob.model), C.byref(prob.state), orc.p(order, C.c_uint32),
+                        orc.p(u, C.c_double), orc.p(z, C.c_double), None)
+            L.orc_post_sweep(C.byref(prob.data), C.byref(prob.model), C.byref(prob.state), seed, it, t)
+            spent[t] += time.perf_counter() - t0
+            it += 1
+            done[t] += m_s
+
+    t0 = time.perf_counter()
+    ths = [threading.Thread(target=worker, args=(t,)) for t in range(workers)]
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
+    wall = time.perf_counter() - t0
+    return dict(value=sum(done) / wall, unit="marker-updates/s", cores=workers, kind="port",
+                sample=(f"oracle (FP64 restatement of the reference sweep, -O3 -ffast-math -funroll-loops): "
+                        f"{workers} chains of the batched workload on {workers} host threads, one chain per "
+                        f"thread, first {m_s} markers at N={n}, {sum(done) // m_s} chain iterations in total, "
+                        f"{wall:.1f}s wall"))
+
+
+def run_batched(cfg, args, with_cpu):
+    """The batched workload on the device: returns the fields of its JSON line."""
+    import bayesrrcmd_amd as brr
+    n, m, T = cfg["n"], cfg["m"], cfg["traits"]
+    steps = args.steps if (args.steps is not None and args.config == "c3") else cfg["chain"]
+    data = make_batch_data(cfg, args.seed, args.threads)
+    h = brr.HipSweep(n, m, 4, 1, brr.LAYOUT_DENSE_2BIT, workgroups=args.workgroups, traits=T)
+    t0 = time.time()
+    h.upload_dense(data["bed"], data["lut"])
+    h.set_model(S_DEFAULT)
+    log(f"batched: uploaded in {time.time() - t0:.1f}s")
+
+    def fresh_chain(seed):
+        eps = np.empty((n, T))
+        for t in range(T):
+            e, sc = brr.bayesr_init(data["y"][:, t], 1, 4, seed, chain=t)
+            eps[:, t] = e
+            h.set_scalars(sc["mu"], sc["sigmaE"], sc["epsilon_sum"], sc["sigmaG"], sc["pi"], trait=t)
+        h.set_state(eps, np.zeros((m, T)), np.zeros((m, T), np.int32))
+
+    order = np.arange(m, dtype=np.uint32)
+    fresh_chain(args.seed + 1)
+    for it in range(args.warmup):
+        brr.shuffle(args.seed + 1, it, 0, order)
+        h.run_chain(args.seed + 1, 0, it, order)
+    fresh_chain(args.seed)
+    order = np.arange(m, dtype=np.uint32)
+    orders = np.empty((steps, m), dtype=np.uint32)
+    for it in range(steps):
+        brr.shuffle(args.seed, it, 0, order)
+        orders[it] = order
+    kernel_ms, changes, rounds, phases, hist = [], [], [], [], []
+    t_start = time.perf_counter()
+    for it in range(steps):
+        hist.append(h.run_chain(args.seed, 0, it, orders[it])[0])
+        s = h.stats()
+        kernel_ms.append(s["kernel_ms"])
+        changes.append(s["changes"])
+        rounds.append(s["rounds"])
+        phases.append(list(s["phase_ms"].values()))
+        if it % 20 == 0 or it == steps - 1:
+            ph = ", ".join(f"{k} {v:.1f}" for k, v in s["phase_ms"].items())
+            log(f"batched iteration {it}: {s['kernel_ms']:.2f} ms, rounds {s['rounds']}, changes {s['changes']}, "
+                f"decision rounds {s['decision_rounds']} | phases ms: {ph}")
+    wall = time.perf_counter() - t_start
+    stats = h.stats()
+    hist = np.array(hist)                                  # (steps, T, 5)
+    avg_ms = float(np.mean(kernel_ms))
+    # B_j = G_j + 8 N T + 8 N (changes of marker j over the traits): the column
+    # once for all traits, one FP64 eps read per trait, one more per change
+    g_col = float((n + 3) // 4)
+    bytes_per_launch = float(np.mean([m * g_col + 8.0 * n * (T * m + c) for c in changes]))
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    # matrix-core work per launch: dots 2 N T per marker, Gram 2 N per pair of a 32-marker block
+    flops = 2.0 * n * T * m + 2.0 * n * (m / 32.0) * (32 * 31 / 2)
+    burn = min(steps // 2, 30)
+    sE, sG = hist[burn:, :, 1], hist[burn:, :, 4]
+    h2_post = (sG / (sG + sE)).mean(axis=0) if steps > 1 else np.full(T, np.nan)
+    out = {
+        "value": float(steps) * m * T / wall,
+        "unit": "marker-updates/s",
+        "steps": steps,
+        "ms_per_step": wall * 1e3 / steps,
+        "dtype": "f64",
+        "config": {
+            "workload": cfg["workload"], "N": n, "M": m, "traits": T, "layout": "dense 2-bit",
+            "maf": list(cfg["maf"]), "causal_fraction_range": list(cfg["causal"]), "h2_range": list(cfg["h2"]),
+            "S": S_DEFAULT.tolist(), "timed_iterations": steps, "precision": "fp64",
+            "workgroups": stats["workgroups"], "block_markers": stats["block_markers"], "slice": stats["slice"],
+            "parallelism": "single GPU, persistent cooperative kernel, all traits in one launch",
+            "h2_abs_err_mean": float(np.mean(np.abs(h2_post - data["h2"]))),
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "kernel": "brr::bayesr_batch_kernel (one launch per iteration)",
+            "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": bytes_per_launch,
+            "mfma_fp64_tflops": flops / (avg_ms * 1e-3) / 1e12, "mfma_fp64_peak_tflops": 78.6,
+            "rounds_per_launch": float(np.mean(rounds)), "changes_per_launch": float(np.mean(changes)),
+            "phase_ms_per_launch": dict(zip(brr.HipSweep.BATCH_PHASES, np.mean(phases, axis=0).round(3).tolist())),
+        },
+    }
+    h.close()
+    if with_cpu:
+        log("timing the CPU reference chains (batched) ...")
+        workers = max(1, min(T, args.threads))
+        out["cpu_baseline"] = cpu_baseline_batched(cfg, data, args.seed, args.cpu_budget, workers)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -156,6 +327,7 @@ def main():
     ap.add_argument("--ragged-storage", default="auto", choices=["auto", "lists", "codes"],
                     help="device copy of ragged columns: smaller of lists / 2-bit codes, or forced")
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched (c3) leg of the default line")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -178,6 +350,17 @@ def main():
         log("calibration copy: 1 GiB read + 1 GiB written")
         del a, b
     cfg = CONFIGS[args.config]
+    if args.config == "c3":
+        b = run_batched(cfg, args, not args.no_cpu)
+        line = {"metric": METRIC, "value": b["value"], "unit": b["unit"], "n_gpus": 1, "steps": b["steps"],
+                "warmup": args.warmup, "ms_per_step": b["ms_per_step"], "higher_is_better": True,
+                "scaling": "weak", "vs_baseline": None, "dtype": b["dtype"],
+                "data": "synthetic (Philox-seeded PLINK genotypes, HWE, 64 phenotypes y = X beta + e)",
+                "config": b["config"], "roofline": b["roofline"]}
+        if "cpu_baseline" in b:
+            line["cpu_baseline"] = b["cpu_baseline"]
+        print(json.dumps(line), flush=True)
+        return
     steps = args.steps if args.steps is not None else cfg["chain"]
     n, m = cfg["n"], cfg["m"]
     data = make_data(cfg, args.seed, args.threads)
@@ -271,11 +454,14 @@ def main():
             "phase_ms_per_launch": dict(zip(brr.HipSweep.PHASES, np.mean(phases, axis=0).round(3).tolist())),
         },
     }
+    h.close()
     if not args.no_cpu:
         log("timing the CPU reference sweep ...")
         line["cpu_baseline"] = cpu_baseline(cfg, data, args.seed, args.cpu_budget)
+    if args.config == "c2" and not args.no_batched:
+        del data
+        line["batched"] = run_batched(CONFIGS["c3"], args, not args.no_cpu)
     print(json.dumps(line), flush=True)
-    h.close()
 
 
 if __name__ == "__main__":

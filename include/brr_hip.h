@@ -103,6 +103,7 @@ typedef struct {
     uint32_t block_markers;
     uint32_t slice;        /* individuals per workgroup */
     uint32_t storage;      /* device copy of the columns: 1 index lists, 2 2-bit codes */
+    uint64_t decision_rounds;  /* batched: decision rounds, summed over the waves of workgroup 0 */
 } brr_hip_stats;
 
 int brr_hip_abi_version(void);
@@ -172,9 +173,12 @@ int brr_hip_epsilon_change(brr_hip_ctx* ctx, uint32_t marker, double beta_old, d
 
 int brr_hip_last_stats(const brr_hip_ctx* ctx, brr_hip_stats* stats);
 
-/* Device time of workgroup 0 in each phase of the last launch, ms: block
- * load, partial dots, exchange, decisions, commit/update, per-iteration
- * scalars (n <= 6 values written). */
+/* Device time of workgroup 0 in each phase of the last launch, ms (n <= 12
+ * values written). Single chain: block load, partial dots, exchange,
+ * decisions, commit/update, per-iteration scalars. Batched: block commit, dot
+ * tiles, exchange gather, decision draws, residual update, per-iteration
+ * scalars, Gram tiles, u/z draws, exchange reduce, accept/correct, block-end
+ * barrier, spare. */
 int brr_hip_last_phase_ms(const brr_hip_ctx* ctx, double* out, uint32_t n);
 
 /* ---- host-side marker building (MarkerBuilder semantics, no device) ---- */

@@ -19,6 +19,13 @@ extern "C" {
 int brr_synth_bed(uint64_t seed, uint32_t num_inds, uint32_t num_snps, double maf_lo, double maf_hi,
                   double miss_rate, uint8_t* bed, uint64_t column_stride, double* maf, uint32_t threads);
 
+/* Genetic values of synthetic phenotypes: out (num_inds x num_traits, trait
+ * fastest) = X beta, X the standardised columns (lut: num_snps x 4 doubles for
+ * dosage 0, 1, 2, missing, as brr_dense_luts returns them) and beta
+ * num_snps x num_traits (mostly zero). Returns 0, or 1 on bad arguments. */
+int brr_synth_xb(const uint8_t* bed, uint64_t column_stride, uint32_t num_inds, uint32_t num_snps,
+                 const double* lut, uint32_t num_traits, const double* beta, double* out, uint32_t threads);
+
 #ifdef __cplusplus
 }
 #endif

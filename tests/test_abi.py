@@ -50,6 +50,8 @@ def test_create_rejects_bad_config():
         _abi.Config(10, 10, 4, 1, 7, 0, 0, 0, 0),    # unknown layout
         _abi.Config(10, 10, 4, 1, 2, 0, 65, 0, 0),   # block > 64
         _abi.Config(10, 10, 4, 1, 2, 0, 0, 0, 3),    # unknown ragged storage
+        _abi.Config(10, 10, 4, 1, 2, 0, 0, 0, 0, 65),  # more than 64 traits
+        _abi.Config(10, 10, 4, 2, 2, 0, 0, 0, 0, 4),   # traits need one group
     ]
     for cfg in bad:
         assert lib.brr_hip_create(C.byref(cfg), C.byref(ctx)) == _abi.EINVAL
@@ -84,3 +86,21 @@ def test_host_init_matches_reference_formulas():
     assert abs(s["sigmaE"] - (eps @ eps) / y.size * 0.5) < 1e-14
     assert np.allclose(s["pi"], [[0.5, 0.125, 0.125, 0.125]] * 2)
     assert np.all((s["sigmaG"] >= 0) & (s["sigmaG"] < 1))
+
+
+def test_synth_xb_matches_dense_product():
+    """X beta of the phenotype generator against the explicit standardised matrix."""
+    import sys
+    sys.path.insert(0, str(ROOT / "tests"))
+    from _data import spec_dosage
+    n, m, T = 1234, 57, 5
+    bed, _ = brr.synth_bed(3, n, m, 0.05, 0.5, 0.02)
+    lut = brr.build_dense(bed, n)
+    rng = np.random.default_rng(4)
+    beta = rng.normal(size=(m, T)) * (rng.random((m, T)) < 0.3)
+    d = spec_dosage(bed, n)
+    x = np.take_along_axis(lut, np.where(d < 0, 3, d), axis=1)      # (m, n)
+    got = brr.synth_xb(bed, n, lut, beta, threads=3)
+    assert got.shape == (n, T)
+    np.testing.assert_allclose(got, x.T @ beta, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(brr.synth_xb(bed, n, lut, beta[:, 2]), x.T @ beta[:, 2], rtol=1e-12, atol=1e-12)
