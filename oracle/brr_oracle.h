@@ -195,6 +195,81 @@ enum { ORC_DRAW_MU = 0, ORC_DRAW_SIGMAE = 1, ORC_DRAW_SIGMAG0 = 2 };
 #define ORC_DRAW_PI0(G) (2u + (uint32_t)(G))
 #define ORC_INIT_ITERATION 0xFFFFFFFFu
 
+/* ------------------------------------------------------ BayesW (a11-a13) -- */
+
+/* ARMS uniforms: stream tag of ctr word 2 (see bayesw.c). */
+enum { ORC_STREAM_ARMS = 4 };
+#define ORC_ARMS_NPOINT 100
+typedef double (*orc_logdens)(double x, void* ctx);
+typedef struct {
+    uint64_t seed;
+    uint32_t site;        /* marker id, or ORC_BW_SITE_MU / _ALPHA */
+    uint32_t iteration;
+    uint32_t chain;
+} orc_arms_rng;
+#define ORC_BW_SITE_MU 0xFFFFFFF0u
+#define ORC_BW_SITE_ALPHA 0xFFFFFFF1u
+/* scalar draws of a BayesW iteration: sigma_b, then the Dirichlet gammas */
+enum { ORC_BW_DRAW_SIGMAB = 0, ORC_BW_DRAW_PI0 = 1 };
+#define ORC_BW_MAXK 8
+
+/* a13: one ARMS draw (BayesW_arms.cpp:133-240) with nsamp = 1 and no
+ * Metropolis step. Returns 0 or the reference's error code (1001-1008, 2000). */
+int orc_arms(const double* xinit, int ninit, double xl, double xr, orc_logdens f, void* ctx, int npoint,
+             const orc_arms_rng* rng, double* xsamp, int* neval);
+
+/* a11: the tabulated adaptive Gauss-Hermite rule of order n (3..15, odd):
+ * n-1 off-centre nodes x / weights w in the reference's order and the centre
+ * weight; returns n-1 (0 for an unsupported n). */
+int orc_bw_gh_rule(int n, double* x, double* w, double* wc);
+/* gauss_hermite_adaptive_integral for class C_k; vi = {vi_sum, vi_0, vi_1, vi_2}. */
+double orc_bw_gh_integral(int n, double sigma, double C_k, double sigma_b, double alpha, double sum_failure,
+                          double mean, double sd, const double vi[4]);
+/* calculateSumFailure of a ragged marker over the failure indicator vector. */
+double orc_bw_sum_failure(const orc_data* d, uint32_t j, const double* fail);
+
+typedef struct {
+    int K;                      /* classes including the zero one, <= ORC_BW_MAXK */
+    const double* C;            /* K-1 mixture variances */
+    int quad_points;            /* 3..15 odd */
+    const double* sum_failure;  /* M, orc_bw_sum_failure per marker */
+} orc_bw_model;
+
+typedef struct {
+    double* eps;                /* N */
+    double* vi;                 /* N: exp(alpha eps - EulerGamma) */
+    double* beta;               /* M */
+    int32_t* comp;              /* M */
+    double* v;                  /* K counts (1 + assignments) */
+    double* pi;                 /* K */
+    double mu, alpha, sigma_b, d;
+} orc_bw_state;
+
+typedef struct {
+    double vi_sum, vi_1, vi_2, acum;
+    double ml[ORC_BW_MAXK];
+} orc_bw_trace;
+
+/* a11: BayesWBase::processColumn on the ragged layout; p is the marker's
+ * uniform. Returns 0 or an ARMS error code. */
+int orc_bw_process_column(const orc_data* d, const orc_bw_model* m, orc_bw_state* s, uint32_t j, double p,
+                          uint64_t seed, uint32_t iteration, uint32_t chain, orc_bw_trace* tr);
+/* processColumn over order with u indexed by marker id. */
+int orc_bw_sweep(const orc_data* d, const orc_bw_model* m, orc_bw_state* s, const uint32_t* order,
+                 const double* u, uint64_t seed, uint32_t iteration, uint32_t chain);
+/* a12: BayesWBase::init, sampleMu, sampleAlpha, the sigma_b / pi_L draws and runGibbs. */
+void orc_bw_init(const orc_data* d, const orc_bw_model* m, orc_bw_state* s, const double* y,
+                 const double* fail);
+int orc_bw_sample_mu(const orc_data* d, orc_bw_state* s, uint64_t seed, uint32_t iteration, uint32_t chain);
+int orc_bw_sample_alpha(const orc_data* d, orc_bw_state* s, const double* fail, uint64_t seed,
+                        uint32_t iteration, uint32_t chain);
+void orc_bw_post_sweep(const orc_data* d, const orc_bw_model* m, orc_bw_state* s, uint64_t seed,
+                       uint32_t iteration, uint32_t chain);
+/* hist: iterations x 5 {mu, alpha, sigma_b, nonzero markers, |beta|^2}. */
+int orc_bw_run_gibbs(const orc_data* d, const orc_bw_model* m, orc_bw_state* s, const double* y,
+                     const double* fail, uint64_t seed, uint32_t chain, uint32_t iterations, uint32_t* order,
+                     double* hist);
+
 #ifdef __cplusplus
 }
 #endif

@@ -53,3 +53,45 @@ def make_problem(n, m, seed, h2=0.5, causal_frac=0.01, maf=(0.01, 0.5), miss=0.0
     beta_true = np.zeros(m)
     beta_true[causal] = b
     return Problem(n, m, bed, lut, ragged, y, causal, beta_true, mafs)
+
+
+@dataclass
+class BwProblem:
+    n: int
+    m: int
+    bed: np.ndarray
+    ragged: brr.RaggedArena
+    y: np.ndarray          # log event / censoring times (rounded through float)
+    fail: np.ndarray       # 1 = event observed, 0 = right-censored
+    causal: np.ndarray
+    beta_true: np.ndarray
+
+
+def make_bw_problem(n, m, seed, h2=0.5, causal_frac=0.05, censor=0.1, alpha=3.0, mu=4.0,
+                    maf=(0.05, 0.5)) -> BwProblem:
+    """Weibull survival data: log T = mu + g + (log E + EulerGamma) / alpha,
+    E ~ Exp(1); var(g) / (var(g) + pi^2 / (6 alpha^2)) = h2; a fraction of the
+    individuals is right-censored at C ~ U(0, T) (fail = 0, time = C)."""
+    bed, _ = brr.synth_bed(seed, n, m, maf[0], maf[1], 0.0)
+    ragged = brr.build_ragged(bed, n)
+    lut = brr.build_dense(bed, n)
+    rng = np.random.default_rng(seed + 1)
+    mc = max(1, int(round(causal_frac * m)))
+    causal = np.sort(rng.choice(m, mc, replace=False))
+    b = rng.normal(0.0, 1.0, mc)
+    d = spec_dosage(bed[causal], n)
+    x = np.take_along_axis(lut[causal], np.where(d < 0, 3, d), axis=1)
+    g = b @ x
+    ve = np.pi ** 2 / (6 * alpha ** 2)
+    scale = np.sqrt(h2 / (1 - h2) * ve / max(g.var(), 1e-12))
+    g *= scale
+    b *= scale
+    logt = mu + g + (np.log(rng.exponential(1.0, n)) + 0.5772156649015329) / alpha
+    fail = np.ones(n)
+    cens = rng.random(n) < censor
+    fail[cens] = 0.0
+    logt[cens] = np.log(rng.uniform(0.0, 1.0, cens.sum())) + logt[cens]
+    y = logt.astype(np.float32).astype(np.float64)
+    beta_true = np.zeros(m)
+    beta_true[causal] = b
+    return BwProblem(n, m, bed, ragged, y, fail, causal, beta_true)

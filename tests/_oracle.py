@@ -46,6 +46,29 @@ class OrcTrace(C.Structure):
     _fields_ = [("num", P_f64), ("acum", P_f64), ("comp", P_i32), ("beta", P_f64)]
 
 
+LOGDENS = C.CFUNCTYPE(C.c_double, C.c_double, C.c_void_p)
+
+
+class OrcArmsRng(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("site", C.c_uint32), ("iteration", C.c_uint32), ("chain", C.c_uint32)]
+
+
+class OrcBwModel(C.Structure):
+    _fields_ = [("K", C.c_int), ("C", P_f64), ("quad_points", C.c_int), ("sum_failure", P_f64)]
+
+
+class OrcBwState(C.Structure):
+    _fields_ = [("eps", P_f64), ("vi", P_f64), ("beta", P_f64), ("comp", P_i32), ("v", P_f64), ("pi", P_f64),
+                ("mu", C.c_double), ("alpha", C.c_double), ("sigma_b", C.c_double), ("d", C.c_double)]
+
+
+class OrcBwTrace(C.Structure):
+    _fields_ = [("vi_sum", C.c_double), ("vi_1", C.c_double), ("vi_2", C.c_double), ("acum", C.c_double),
+                ("ml", C.c_double * 8)]
+
+SITE_MU, SITE_ALPHA = 0xFFFFFFF0, 0xFFFFFFF1
+
+
 _LIB = None
 
 
@@ -98,6 +121,26 @@ def lib(fast: bool = False):
                                   C.c_uint32, C.c_uint32]),
         "orc_run_gibbs": (None, [C.POINTER(OrcData), C.POINTER(OrcModel), C.POINTER(OrcState), P_f64,
                                  C.c_uint64, C.c_uint32, C.c_uint32, P_u32, C.c_int, P_f64]),
+        "orc_arms": (C.c_int, [P_f64, C.c_int, C.c_double, C.c_double, LOGDENS, C.c_void_p, C.c_int,
+                               C.POINTER(OrcArmsRng), P_f64, C.POINTER(C.c_int)]),
+        "orc_bw_gh_rule": (C.c_int, [C.c_int, P_f64, P_f64, P_f64]),
+        "orc_bw_gh_integral": (C.c_double, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                            C.c_double, C.c_double, P_f64]),
+        "orc_bw_sum_failure": (C.c_double, [C.POINTER(OrcData), C.c_uint32, P_f64]),
+        "orc_bw_init": (None, [C.POINTER(OrcData), C.POINTER(OrcBwModel), C.POINTER(OrcBwState), P_f64, P_f64]),
+        "orc_bw_process_column": (C.c_int, [C.POINTER(OrcData), C.POINTER(OrcBwModel), C.POINTER(OrcBwState),
+                                            C.c_uint32, C.c_double, C.c_uint64, C.c_uint32, C.c_uint32,
+                                            C.POINTER(OrcBwTrace)]),
+        "orc_bw_sweep": (C.c_int, [C.POINTER(OrcData), C.POINTER(OrcBwModel), C.POINTER(OrcBwState), P_u32, P_f64,
+                                   C.c_uint64, C.c_uint32, C.c_uint32]),
+        "orc_bw_sample_mu": (C.c_int, [C.POINTER(OrcData), C.POINTER(OrcBwState), C.c_uint64, C.c_uint32,
+                                       C.c_uint32]),
+        "orc_bw_sample_alpha": (C.c_int, [C.POINTER(OrcData), C.POINTER(OrcBwState), P_f64, C.c_uint64,
+                                          C.c_uint32, C.c_uint32]),
+        "orc_bw_post_sweep": (None, [C.POINTER(OrcData), C.POINTER(OrcBwModel), C.POINTER(OrcBwState), C.c_uint64,
+                                     C.c_uint32, C.c_uint32]),
+        "orc_bw_run_gibbs": (C.c_int, [C.POINTER(OrcData), C.POINTER(OrcBwModel), C.POINTER(OrcBwState), P_f64,
+                                       P_f64, C.c_uint64, C.c_uint32, C.c_uint32, P_u32, P_f64]),
     }
     for n, (r, a) in sig.items():
         f = getattr(L, n)
@@ -269,3 +312,64 @@ class OracleProblem:
                             seed, chain, iterations, p(order, C.c_uint32), shuffle_mode,
                             p(hist, C.c_double))
         return hist, order
+
+
+@dataclass
+class BayesWProblem:
+    """Ragged markers + failure indicators + BayesW state for the oracle."""
+    n: int
+    m: int
+    ragged: object
+    fail: np.ndarray
+    K: int = 4
+    C: tuple = (0.001, 0.01, 0.1)
+    quad_points: int = 7
+
+    def __post_init__(self):
+        r = self.ragged
+        d = OrcData()
+        d.N, d.M, d.layout = self.n, self.m, RAGGED
+        d.ones, d.ones_off = p(r.ones, C.c_int32), p(r.ones_off, C.c_uint64)
+        d.twos, d.twos_off = p(r.twos, C.c_int32), p(r.twos_off, C.c_uint64)
+        d.miss, d.miss_off = p(r.miss, C.c_int32), p(r.miss_off, C.c_uint64)
+        self._stats = np.ascontiguousarray(r.stats, dtype=np.float64)
+        d.stats = p(self._stats, C.c_double)
+        self.data = d
+        self.fail = np.ascontiguousarray(self.fail, dtype=np.float64)
+        self.sum_failure = np.array([lib().orc_bw_sum_failure(C.byref(d), j, p(self.fail, C.c_double))
+                                     for j in range(self.m)])
+        self._C = np.ascontiguousarray(self.C, dtype=np.float64)
+        self.model = OrcBwModel(self.K, p(self._C, C.c_double), self.quad_points, p(self.sum_failure, C.c_double))
+        self.eps = np.zeros(self.n)
+        self.vi = np.zeros(self.n)
+        self.beta = np.zeros(self.m)
+        self.comp = np.zeros(self.m, dtype=np.int32)
+        self.v = np.ones(self.K)
+        self.pi = np.zeros(self.K)
+        st = OrcBwState()
+        st.eps, st.vi, st.beta = p(self.eps, C.c_double), p(self.vi, C.c_double), p(self.beta, C.c_double)
+        st.comp, st.v, st.pi = p(self.comp, C.c_int32), p(self.v, C.c_double), p(self.pi, C.c_double)
+        self.state = st
+
+    def init(self, y):
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        lib().orc_bw_init(C.byref(self.data), C.byref(self.model), C.byref(self.state), p(y, C.c_double),
+                          p(self.fail, C.c_double))
+
+    def refresh_vi(self):
+        self.vi[:] = np.exp(self.state.alpha * self.eps - 0.577215664901532)
+
+    def sweep(self, order, u, seed, iteration, chain=0):
+        order = np.ascontiguousarray(order, dtype=np.uint32)
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        return lib().orc_bw_sweep(C.byref(self.data), C.byref(self.model), C.byref(self.state),
+                                  p(order, C.c_uint32), p(u, C.c_double), seed, iteration, chain)
+
+    def run_gibbs(self, y, seed, iterations, chain=0):
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        order = np.arange(self.m, dtype=np.uint32)
+        hist = np.zeros((iterations, 5))
+        err = lib().orc_bw_run_gibbs(C.byref(self.data), C.byref(self.model), C.byref(self.state), p(y, C.c_double),
+                                     p(self.fail, C.c_double), seed, chain, iterations, p(order, C.c_uint32),
+                                     p(hist, C.c_double))
+        return err, hist, order
