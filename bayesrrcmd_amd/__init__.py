@@ -12,6 +12,9 @@ Reference correspondence (BayesRRcmd src/):
   bayesr_init           BayesRBase::init
   shuffle               std::random_shuffle of the marker order
   marker_uz             BayesRBase::prepareForAnylsis
+  HipBayesW.sweep       AnalysisGraph::exec -> BayesWBase::processColumn (ragged)
+  HipBayesW.run_chain   BayesWBase::runGibbs iterations, device-resident
+  bayesw_init           BayesWBase::init
 """
 from __future__ import annotations
 
@@ -25,7 +28,7 @@ from . import _abi
 from ._abi import LAYOUT_DENSE_2BIT, LAYOUT_RAGGED, STORAGE_AUTO, STORAGE_CODES, STORAGE_LISTS
 
 __all__ = [
-    "BrrHipError", "HipSweep", "RaggedArena", "build_dense", "build_ragged", "synth_bed", "synth_xb",
+    "BrrHipError", "HipSweep", "HipBayesW", "bayesw_init", "RaggedArena", "build_dense", "build_ragged", "synth_bed", "synth_xb",
     "bayesr_init", "shuffle", "marker_uz", "LAYOUT_DENSE_2BIT", "LAYOUT_RAGGED", "lib",
     "STORAGE_AUTO", "STORAGE_CODES", "STORAGE_LISTS",
 ]
@@ -142,6 +145,20 @@ def bayesr_init(y: np.ndarray, n_groups: int, n_components: int, seed: int, chai
                                  _p(eps, C.c_double), C.byref(s)), "brr_bayesr_init")
     return eps, dict(mu=s.mu, sigmaE=s.sigmaE, epsilon_sum=s.epsilon_sum, sigmaG=sigmaG,
                      pi=pi.reshape(n_groups, n_components))
+
+
+def bayesw_init(y: np.ndarray, fail: np.ndarray, m: int, n_components: int):
+    """BayesWBase::init: returns (eps, scalars dict with mu, alpha, sigma_b, d, pi, v)."""
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    f = np.ascontiguousarray(fail, dtype=np.float64)
+    n = y.size
+    eps = np.empty(n, dtype=np.float64)
+    pi = np.empty(n_components, dtype=np.float64)
+    v = np.empty(n_components, dtype=np.float64)
+    s = _abi.BwScalars(0.0, 0.0, 0.0, 0.0, _p(pi, C.c_double), _p(v, C.c_double), 0.0)
+    _check(lib().brr_bayesw_init(_p(y, C.c_double), _p(f, C.c_double), n, m, n_components, _p(eps, C.c_double),
+                                 C.byref(s)), "brr_bayesw_init")
+    return eps, dict(mu=s.mu, alpha=s.alpha, sigma_b=s.sigma_b, d=s.d, pi=pi, v=v)
 
 
 def shuffle(seed: int, iteration: int, chain: int, order: np.ndarray, mode: int = 0) -> np.ndarray:
@@ -314,3 +331,114 @@ class HipSweep:
         names = self.BATCH_PHASES if self.traits else self.PHASES
         out["phase_ms"] = dict(zip(names, ph.tolist()))
         return out
+
+
+# ------------------------------------------------------------ BayesW sweep --
+
+class HipBayesW:
+    """Device context of the BayesW (Weibull survival) sweep on ragged markers."""
+
+    def __init__(self, n: int, m: int, n_components: int, quad_points: int = 7, workgroups: int = 0,
+                 device: int = 0):
+        self.n, self.m, self.K, self.nq = n, m, n_components, quad_points
+        cfg = _abi.BwConfig(n, m, n_components, quad_points, workgroups, device)
+        ctx = _abi.CTX()
+        rc = lib().brr_hip_bw_create(C.byref(cfg), C.byref(ctx))
+        if rc != 0:
+            raise BrrHipError(f"brr_hip_bw_create failed with status {rc} (N={n}, M={m}, K={n_components}, "
+                              f"quad_points={quad_points}, workgroups={workgroups})")
+        self.ctx = ctx
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            lib().brr_hip_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _ck(self, rc, what):
+        _check(rc, what, self.ctx)
+
+    def upload_ragged(self, a: RaggedArena):
+        self._ck(lib().brr_hip_upload_ragged(
+            self.ctx, _p(a.ones, C.c_int32), _p(a.ones_off, C.c_uint64), _p(a.twos, C.c_int32),
+            _p(a.twos_off, C.c_uint64), _p(a.miss, C.c_int32), _p(a.miss_off, C.c_uint64),
+            _p(np.ascontiguousarray(a.stats), C.c_double)), "brr_hip_upload_ragged")
+
+    def set_model(self, mixture_C: np.ndarray, fail: np.ndarray):
+        cc = np.ascontiguousarray(mixture_C, dtype=np.float64).reshape(self.K - 1)
+        f = np.ascontiguousarray(fail, dtype=np.float64).reshape(self.n)
+        self._ck(lib().brr_hip_bw_set_model(self.ctx, _p(cc, C.c_double), _p(f, C.c_double)),
+                 "brr_hip_bw_set_model")
+
+    def set_state(self, eps=None, beta=None, comp=None):
+        e = None if eps is None else np.ascontiguousarray(eps, dtype=np.float64).reshape(self.n)
+        b = None if beta is None else np.ascontiguousarray(beta, dtype=np.float64).reshape(self.m)
+        c = None if comp is None else np.ascontiguousarray(comp, dtype=np.int32).reshape(self.m)
+        self._ck(lib().brr_hip_set_state(self.ctx, _p(e, C.c_double), _p(b, C.c_double), _p(c, C.c_int32)),
+                 "brr_hip_set_state")
+
+    def get_state(self):
+        eps = np.empty(self.n, dtype=np.float64)
+        beta = np.empty(self.m, dtype=np.float64)
+        comp = np.empty(self.m, dtype=np.int32)
+        self._ck(lib().brr_hip_get_state(self.ctx, _p(eps, C.c_double), _p(beta, C.c_double),
+                                         _p(comp, C.c_int32), None), "brr_hip_get_state")
+        return dict(eps=eps, beta=beta, comp=comp)
+
+    def set_scalars(self, mu, alpha, sigma_b, d, pi, v=None):
+        pp = np.ascontiguousarray(pi, dtype=np.float64).reshape(self.K)
+        vv = None if v is None else np.ascontiguousarray(v, dtype=np.float64).reshape(self.K)
+        s = _abi.BwScalars(mu, alpha, sigma_b, d, _p(pp, C.c_double), _p(vv, C.c_double), 0.0)
+        self._ck(lib().brr_hip_bw_set_scalars(self.ctx, C.byref(s)), "brr_hip_bw_set_scalars")
+
+    def get_scalars(self):
+        pi = np.empty(self.K, dtype=np.float64)
+        v = np.empty(self.K, dtype=np.float64)
+        s = _abi.BwScalars(0.0, 0.0, 0.0, 0.0, _p(pi, C.c_double), _p(v, C.c_double), 0.0)
+        self._ck(lib().brr_hip_bw_get_scalars(self.ctx, C.byref(s)), "brr_hip_bw_get_scalars")
+        return dict(mu=s.mu, alpha=s.alpha, sigma_b=s.sigma_b, d=s.d, pi=pi, v=v, betasq=s.betasq)
+
+    def sweep(self, order: np.ndarray, u: np.ndarray, seed: int, iteration: int, chain: int = 0):
+        o = np.ascontiguousarray(order, dtype=np.uint32)
+        uu = np.ascontiguousarray(u, dtype=np.float64).reshape(self.m)
+        self._ck(lib().brr_hip_bw_sweep(self.ctx, _p(o, C.c_uint32), _p(uu, C.c_double), seed, iteration, chain),
+                 "brr_hip_bw_sweep")
+
+    def run_chain(self, seed: int, chain: int, first_iteration: int, orders: np.ndarray) -> np.ndarray:
+        """hist: (iterations, 5) {mu, alpha, sigma_b, M - v0 + 1, sum beta^2}."""
+        o = np.ascontiguousarray(orders, dtype=np.uint32)
+        iters = o.size // self.m
+        assert iters * self.m == o.size
+        hist = np.empty((iters, 5), dtype=np.float64)
+        self._ck(lib().brr_hip_bw_run_chain(self.ctx, seed, chain, first_iteration, iters, _p(o, C.c_uint32),
+                                            _p(hist, C.c_double)), "brr_hip_bw_run_chain")
+        return hist
+
+    def set_vi(self, marker: int, alpha: float) -> np.ndarray:
+        out = np.empty(4, dtype=np.float64)
+        self._ck(lib().brr_hip_bw_set_vi(self.ctx, marker, alpha, _p(out, C.c_double)), "brr_hip_bw_set_vi")
+        return out
+
+    def residual_update(self, marker: int, beta: float, sign: int):
+        self._ck(lib().brr_hip_bw_residual_update(self.ctx, marker, beta, sign), "brr_hip_bw_residual_update")
+
+    def sum_failure(self) -> np.ndarray:
+        out = np.empty(self.m, dtype=np.float64)
+        self._ck(lib().brr_hip_bw_sum_failure(self.ctx, _p(out, C.c_double)), "brr_hip_bw_sum_failure")
+        return out
+
+    def stats(self) -> dict:
+        s = _abi.Stats()
+        self._ck(lib().brr_hip_last_stats(self.ctx, C.byref(s)), "brr_hip_last_stats")
+        return {f: getattr(s, f) for f, _ in _abi.Stats._fields_}

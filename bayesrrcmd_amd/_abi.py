@@ -10,7 +10,7 @@ from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libbrr_hip.so"
 
-OK, EINVAL, ENOMEM, EDEVICE, ESTATE, ETIMEOUT = 0, 1, 2, 3, 4, 5
+OK, EINVAL, ENOMEM, EDEVICE, ESTATE, ETIMEOUT, ESAMPLER = 0, 1, 2, 3, 4, 5, 6
 LAYOUT_DENSE_2BIT = 1
 LAYOUT_RAGGED = 2
 STORAGE_AUTO, STORAGE_LISTS, STORAGE_CODES = 0, 1, 2
@@ -48,6 +48,16 @@ class Stats(C.Structure):
                 ("decision_rounds", C.c_uint64)]
 
 
+class BwConfig(C.Structure):
+    _fields_ = [("num_inds", C.c_uint32), ("num_snps", C.c_uint32), ("num_components", C.c_uint32),
+                ("quad_points", C.c_uint32), ("workgroups", C.c_uint32), ("device", C.c_uint32)]
+
+
+class BwScalars(C.Structure):
+    _fields_ = [("mu", C.c_double), ("alpha", C.c_double), ("sigma_b", C.c_double), ("d", C.c_double),
+                ("pi", P_f64), ("v", P_f64), ("betasq", C.c_double)]
+
+
 CTX = C.c_void_p
 
 # name -> (restype, argtypes)
@@ -73,6 +83,17 @@ SIGNATURES = {
     "brr_hip_epsilon_change": (C.c_int, [CTX, C.c_uint32, C.c_double, C.c_double]),
     "brr_hip_last_stats": (C.c_int, [CTX, C.POINTER(Stats)]),
     "brr_hip_last_phase_ms": (C.c_int, [CTX, P_f64, C.c_uint32]),
+    "brr_hip_bw_create": (C.c_int, [C.POINTER(BwConfig), C.POINTER(CTX)]),
+    "brr_hip_bw_set_model": (C.c_int, [CTX, P_f64, P_f64]),
+    "brr_hip_bw_set_scalars": (C.c_int, [CTX, C.POINTER(BwScalars)]),
+    "brr_hip_bw_get_scalars": (C.c_int, [CTX, C.POINTER(BwScalars)]),
+    "brr_hip_bw_sweep": (C.c_int, [CTX, P_u32, P_f64, C.c_uint64, C.c_uint32, C.c_uint32]),
+    "brr_hip_bw_run_chain": (C.c_int, [CTX, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, P_u32, P_f64]),
+    "brr_hip_bw_set_vi": (C.c_int, [CTX, C.c_uint32, C.c_double, P_f64]),
+    "brr_hip_bw_residual_update": (C.c_int, [CTX, C.c_uint32, C.c_double, C.c_int]),
+    "brr_hip_bw_sum_failure": (C.c_int, [CTX, P_f64]),
+    "brr_bayesw_init": (C.c_int, [P_f64, P_f64, C.c_uint32, C.c_uint32, C.c_uint32, P_f64,
+                                  C.POINTER(BwScalars)]),
     "brr_bed_check_magic": (C.c_int, [P_u8]),
     "brr_dense_lut": (C.c_int, [P_u8, C.c_uint32, P_f64]),
     "brr_dense_luts": (C.c_int, [P_u8, C.c_uint64, C.c_uint32, C.c_uint32, P_f64, C.c_uint32]),

@@ -14,6 +14,7 @@
 #include "brr_hip.h"
 #include "sweep_kernel.hpp"
 #include "batch_kernel.hpp"
+#include "bayesw_kernel.hpp"
 
 using namespace brr;
 
@@ -31,6 +32,8 @@ struct brr_hip_ctx {
     uint32_t nt = 0, NT = 0;     // batched: 16-individual tiles per slice, kernel variant
     size_t x1_words = 0, x2_words = 0;
     int layout = 0;
+    bool bw = false;             // BayesW context (brr_hip_bw_create)
+    uint32_t nq = 0;             // BayesW Gauss-Hermite points
     bool codes = false;          // columns held as 2-bit codes on the device
     bool have_markers = false;
     bool have_model = false;
@@ -72,6 +75,10 @@ struct brr_hip_ctx {
     uint32_t* d_ctl = nullptr;
     unsigned long long* d_counters = nullptr;
     double* d_tmp = nullptr;
+    // BayesW
+    double* d_fail = nullptr;
+    double* d_sumf = nullptr;
+    unsigned long long last_counters[kCounters] = {};
 };
 
 namespace {
@@ -282,6 +289,44 @@ const void* batch_kernel_for(uint32_t NT, int layout)
     }
 }
 
+const void* bw_kernel_for(uint32_t R)
+{
+    switch (R) {
+    case 1: return (const void*)bayesw_sweep_kernel<1>;
+    case 2: return (const void*)bayesw_sweep_kernel<2>;
+    case 4: return (const void*)bayesw_sweep_kernel<4>;
+    default: return nullptr;
+    }
+}
+
+BwParams bw_params(brr_hip_ctx* c)
+{
+    BwParams p{};
+    p.N = c->cfg.num_inds;
+    p.M = c->cfg.num_snps;
+    p.K = c->cfg.num_components;
+    p.P = c->P;
+    p.S = c->S;
+    p.nq = c->nq;
+    p.codes = c->d_dense;
+    p.col_words = c->col_words;
+    p.col_bytes = c->col_bytes;
+    p.stats = c->d_stats;
+    p.sumf = c->d_sumf;
+    p.fail = c->d_fail;
+    p.cls = c->d_cva;
+    p.u = c->d_u;
+    p.eps = c->d_eps;
+    p.beta = c->d_beta;
+    p.comp = c->d_comp;
+    p.scal = c->d_scal;
+    p.xchg1 = c->d_xchg1;
+    p.xchg2 = c->d_xchg2;
+    p.ctl = c->d_ctl;
+    p.counters = c->d_counters;
+    return p;
+}
+
 BatchParams batch_params(brr_hip_ctx* c)
 {
     BatchParams p{};
@@ -367,6 +412,7 @@ int launch_fn(brr_hip_ctx* c, const void* fn, void* param)
     unsigned long long counters[kCounters] = {};
     HIP_TRY(c, hipMemcpy(ctl, c->d_ctl, sizeof(ctl), hipMemcpyDeviceToHost));
     HIP_TRY(c, hipMemcpy(counters, c->d_counters, sizeof(counters), hipMemcpyDeviceToHost));
+    std::memcpy(c->last_counters, counters, sizeof(counters));
     c->stats.kernel_ms = ms;
     c->stats.rounds = counters[0];
     c->stats.changes = counters[1];
@@ -379,6 +425,11 @@ int launch_fn(brr_hip_ctx* c, const void* fn, void* param)
     c->stats.storage = c->codes ? 2u : 1u;
     if (ctl[kCtlAbort]) {
         reset_exchange(c);
+        if (c->bw && ctl[kCtlAbortWhere] >= 30 && ctl[kCtlAbortWhere] <= 32)
+            return fail(c, BRR_HIP_ESAMPLER, std::string("adaptive rejection sampler failed for ") +
+                                                 (ctl[kCtlAbortWhere] == 30 ? "mu" :
+                                                  ctl[kCtlAbortWhere] == 31 ? "beta" : "alpha") +
+                                                 " with error " + std::to_string(counters[9]));
         return fail(c, BRR_HIP_ETIMEOUT, "device exchange wait exceeded its bound (site " +
                                              std::to_string(ctl[kCtlAbortWhere]) + ")");
     }
@@ -415,7 +466,31 @@ int check_ready(brr_hip_ctx* c)
 {
     if (!c) return BRR_HIP_EINVAL;
     if (!c->have_markers) return fail(c, BRR_HIP_ESTATE, "markers not uploaded");
-    if (!c->have_model) return fail(c, BRR_HIP_ESTATE, "model not set (brr_hip_set_model)");
+    if (!c->have_model) return fail(c, BRR_HIP_ESTATE, "model not set (brr_hip_set_model / brr_hip_bw_set_model)");
+    return BRR_HIP_OK;
+}
+
+// BayesR calls on a BayesW context and the reverse are refused.
+int need_kind(brr_hip_ctx* c, bool bw)
+{
+    if (!c) return BRR_HIP_EINVAL;
+    if (c->bw != bw)
+        return fail(c, BRR_HIP_ESTATE, bw ? "BayesW call on a BayesR context" : "BayesR call on a BayesW context");
+    return BRR_HIP_OK;
+}
+
+int check_order(brr_hip_ctx* c, const uint32_t* orders, uint32_t count)
+{
+    const uint32_t M = c->cfg.num_snps;
+    std::vector<uint8_t> seen(M);
+    for (uint32_t it = 0; it < count; ++it) {
+        std::fill(seen.begin(), seen.end(), 0);
+        const uint32_t* o = orders + (size_t)it * M;
+        for (uint32_t i = 0; i < M; ++i) {
+            if (o[i] >= M || seen[o[i]]) return fail(c, BRR_HIP_EINVAL, "order is not a permutation of 0..M-1");
+            seen[o[i]] = 1;
+        }
+    }
     return BRR_HIP_OK;
 }
 
@@ -543,6 +618,7 @@ void brr_hip_destroy(brr_hip_ctx* c)
     dfree(c->d_eps); dfree(c->d_beta); dfree(c->d_comp); dfree(c->d_acum);
     dfree(c->d_scal); dfree(c->d_hist); dfree(c->d_order); dfree(c->d_u); dfree(c->d_z);
     dfree(c->d_xchg1); dfree(c->d_xchg2); dfree(c->d_ctl); dfree(c->d_counters); dfree(c->d_tmp);
+    dfree(c->d_fail); dfree(c->d_sumf);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -607,7 +683,7 @@ int brr_hip_upload_ragged(brr_hip_ctx* c, const int32_t* ones, const uint64_t* o
         (rc = dalloc(c, &c->d_miss, nm)) || (rc = dalloc(c, &c->d_ones_off, M + 1)) ||
         (rc = dalloc(c, &c->d_twos_off, M + 1)) || (rc = dalloc(c, &c->d_miss_off, M + 1)) ||
         (rc = dalloc(c, &c->d_stats, (size_t)4 * M)) ||
-        (rc = dalloc(c, &c->d_seg, (size_t)M * 3 * (c->P + 1))))
+        (rc = dalloc(c, &c->d_seg, c->bw ? (size_t)1 : (size_t)M * 3 * (c->P + 1))))
         return rc;
     if (n1) HIP_TRY(c, hipMemcpy(c->d_ones, ones, sizeof(int32_t) * n1, hipMemcpyHostToDevice));
     if (n2) HIP_TRY(c, hipMemcpy(c->d_twos, twos, sizeof(int32_t) * n2, hipMemcpyHostToDevice));
@@ -616,16 +692,18 @@ int brr_hip_upload_ragged(brr_hip_ctx* c, const int32_t* ones, const uint64_t* o
     HIP_TRY(c, hipMemcpy(c->d_twos_off, twos_off, sizeof(uint64_t) * (M + 1), hipMemcpyHostToDevice));
     HIP_TRY(c, hipMemcpy(c->d_miss_off, miss_off, sizeof(uint64_t) * (M + 1), hipMemcpyHostToDevice));
     HIP_TRY(c, hipMemcpy(c->d_stats, stats, sizeof(double) * 4 * M, hipMemcpyHostToDevice));
-    const uint64_t total = (uint64_t)M * 3 * (c->P + 1);
-    const uint32_t blocks = (uint32_t)((total + 255) / 256);
-    build_seg_kernel<<<blocks, 256, 0, c->stream>>>(c->d_ones, c->d_ones_off, c->d_twos, c->d_twos_off,
-                                                   c->d_miss, c->d_miss_off, M, c->P, c->S, c->d_seg);
-    HIP_TRY(c, hipGetLastError());
+    if (!c->bw) {
+        const uint64_t total = (uint64_t)M * 3 * (c->P + 1);
+        const uint32_t blocks = (uint32_t)((total + 255) / 256);
+        build_seg_kernel<<<blocks, 256, 0, c->stream>>>(c->d_ones, c->d_ones_off, c->d_twos, c->d_twos_off,
+                                                       c->d_miss, c->d_miss_off, M, c->P, c->S, c->d_seg);
+        HIP_TRY(c, hipGetLastError());
+    }
     // device copy as 2-bit codes when asked, or when smaller than the lists
     const uint64_t list_bytes = 4ull * (n1 + n2 + nm);
     const uint64_t code_bytes = (uint64_t)M * (((uint64_t)N + 3) / 4);
     const uint32_t st = c->cfg.ragged_storage;
-    c->codes = c->T > 0 || st == 2 || (st == 0 && code_bytes < list_bytes);
+    c->codes = c->T > 0 || c->bw || st == 2 || (st == 0 && code_bytes < list_bytes);
     if (c->codes) {
         c->col_bytes = ((uint64_t)N + 3) / 4;
         c->col_words = (c->col_bytes + 15) / 16 * 4;
@@ -635,12 +713,18 @@ int brr_hip_upload_ragged(brr_hip_ctx* c, const int32_t* ones, const uint64_t* o
         HIP_TRY(c, hipGetLastError());
     }
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (c->bw) {
+        // the BayesW sweep reads the codes only
+        dfree(c->d_ones); dfree(c->d_twos); dfree(c->d_miss);
+    }
     c->have_markers = true;
+    c->have_model = c->have_model && !c->bw;   // sum_failure depends on the columns
     return BRR_HIP_OK;
 }
 
 int brr_hip_set_groups(brr_hip_ctx* c, const int32_t* group)
 {
+    if (int k = need_kind(c, false)) return k;
     if (!c || !group) return fail(c, BRR_HIP_EINVAL, "null argument");
     const uint32_t M = c->cfg.num_snps;
     for (uint32_t j = 0; j < M; ++j)
@@ -655,6 +739,7 @@ int brr_hip_set_groups(brr_hip_ctx* c, const int32_t* group)
 
 int brr_hip_set_model(brr_hip_ctx* c, const brr_hip_model* m)
 {
+    if (int k = need_kind(c, false)) return k;
     if (!c || !m || !m->cva) return fail(c, BRR_HIP_EINVAL, "null argument");
     const uint32_t G = c->cfg.num_groups, K = c->cfg.num_components;
     for (uint32_t t = 0; t < G * (K - 1); ++t)
@@ -686,6 +771,7 @@ int brr_hip_set_scalars(brr_hip_ctx* c, const brr_hip_scalars* s)
 
 int brr_hip_set_trait_scalars(brr_hip_ctx* c, uint32_t trait, const brr_hip_scalars* s)
 {
+    if (int k = need_kind(c, false)) return k;
     if (!c || !s || !s->sigmaG || !s->pi) return fail(c, BRR_HIP_EINVAL, "null argument");
     if (trait >= c->Tn) return fail(c, BRR_HIP_EINVAL, "trait index out of range");
     HIP_TRY(c, hipSetDevice(c->device));
@@ -707,6 +793,7 @@ int brr_hip_get_state(brr_hip_ctx* c, double* eps, double* beta, int32_t* comp, 
     if (!c) return BRR_HIP_EINVAL;
     HIP_TRY(c, hipSetDevice(c->device));
     const size_t N = c->cfg.num_inds, M = c->cfg.num_snps, Tn = c->Tn;
+    if (acum && c->bw) return fail(c, BRR_HIP_EINVAL, "a BayesW context keeps no acum");
     if (eps) HIP_TRY(c, hipMemcpy(eps, c->d_eps, sizeof(double) * N * Tn, hipMemcpyDeviceToHost));
     if (beta) HIP_TRY(c, hipMemcpy(beta, c->d_beta, sizeof(double) * M * Tn, hipMemcpyDeviceToHost));
     if (comp) HIP_TRY(c, hipMemcpy(comp, c->d_comp, sizeof(int32_t) * M * Tn, hipMemcpyDeviceToHost));
@@ -721,6 +808,7 @@ int brr_hip_get_scalars(brr_hip_ctx* c, brr_hip_scalars* s)
 
 int brr_hip_get_trait_scalars(brr_hip_ctx* c, uint32_t trait, brr_hip_scalars* s)
 {
+    if (int k = need_kind(c, false)) return k;
     if (!c || !s) return BRR_HIP_EINVAL;
     if (trait >= c->Tn) return fail(c, BRR_HIP_EINVAL, "trait index out of range");
     HIP_TRY(c, hipSetDevice(c->device));
@@ -743,6 +831,7 @@ int brr_hip_sweep(brr_hip_ctx* c, const uint32_t* order, const double* u, const 
 {
     int rc = check_ready(c);
     if (rc) return rc;
+    if ((rc = need_kind(c, false))) return rc;
     if (!order || !u || !z) return fail(c, BRR_HIP_EINVAL, "null argument");
     const uint32_t M = c->cfg.num_snps;
     std::vector<uint8_t> seen(M, 0);
@@ -774,6 +863,7 @@ int brr_hip_run_chain(brr_hip_ctx* c, uint64_t seed, uint32_t chain, uint32_t fi
 {
     int rc = check_ready(c);
     if (rc) return rc;
+    if ((rc = need_kind(c, false))) return rc;
     if (!orders || iterations == 0) return fail(c, BRR_HIP_EINVAL, "null orders or zero iterations");
     const uint32_t M = c->cfg.num_snps, G = c->cfg.num_groups;
     const size_t total = (size_t)iterations * M;
@@ -816,6 +906,7 @@ int brr_hip_compute_num(brr_hip_ctx* c, uint32_t marker, double beta_old, double
 {
     int rc = check_ready(c);
     if (rc) return rc;
+    if ((rc = need_kind(c, false))) return rc;
     if (c->T) return fail(c, BRR_HIP_ESTATE, "per-marker calls need a single-chain context");
     if (!num || marker >= c->cfg.num_snps) return fail(c, BRR_HIP_EINVAL, "bad marker or null output");
     HIP_TRY(c, hipSetDevice(c->device));
@@ -831,6 +922,7 @@ int brr_hip_epsilon_change(brr_hip_ctx* c, uint32_t marker, double beta_old, dou
 {
     int rc = check_ready(c);
     if (rc) return rc;
+    if ((rc = need_kind(c, false))) return rc;
     if (c->T) return fail(c, BRR_HIP_ESTATE, "per-marker calls need a single-chain context");
     if (marker >= c->cfg.num_snps) return fail(c, BRR_HIP_EINVAL, "bad marker");
     HIP_TRY(c, hipSetDevice(c->device));
@@ -849,6 +941,7 @@ int brr_hip_debug_block_codes(brr_hip_ctx* c, const uint32_t* order, const doubl
 {
     int rc = check_ready(c);
     if (rc) return rc;
+    if ((rc = need_kind(c, false))) return rc;
     if (c->T) return fail(c, BRR_HIP_ESTATE, "per-marker calls need a single-chain context");
     const uint32_t M = c->cfg.num_snps;
     const uint64_t nblocks = (M + c->B - 1) / c->B;
@@ -895,6 +988,252 @@ int brr_hip_last_stats(const brr_hip_ctx* c, brr_hip_stats* s)
 {
     if (!c || !s) return BRR_HIP_EINVAL;
     *s = c->stats;
+    return BRR_HIP_OK;
+}
+
+// ---------------------------------------------------------------- BayesW --
+
+int brr_hip_bw_create(const brr_hip_bw_config* cfg, brr_hip_ctx** out)
+{
+    if (!cfg || !out) return BRR_HIP_EINVAL;
+    *out = nullptr;
+    if (cfg->num_inds < 2 || cfg->num_snps < 1) return BRR_HIP_EINVAL;
+    if (cfg->num_components < 2 || cfg->num_components > (uint32_t)kMaxK) return BRR_HIP_EINVAL;
+    if (cfg->quad_points < 3 || cfg->quad_points > 15 || (cfg->quad_points % 2) == 0) return BRR_HIP_EINVAL;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return BRR_HIP_EDEVICE;
+    if ((int)cfg->device >= ndev) return BRR_HIP_EINVAL;
+
+    brr_hip_ctx* c = new brr_hip_ctx();
+    c->cfg.num_inds = cfg->num_inds;
+    c->cfg.num_snps = cfg->num_snps;
+    c->cfg.num_components = cfg->num_components;
+    c->cfg.num_groups = 1;
+    c->cfg.layout = BRR_HIP_LAYOUT_RAGGED;
+    c->cfg.workgroups = cfg->workgroups;
+    c->cfg.device = cfg->device;
+    c->cfg.ragged_storage = 2;
+    c->device = (int)cfg->device;
+    c->layout = kLayoutRagged;
+    c->bw = true;
+    c->codes = true;
+    c->nq = cfg->quad_points;
+    if (hipSetDevice(c->device) != hipSuccess) { delete c; return BRR_HIP_EDEVICE; }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, c->device) != hipSuccess) { delete c; return BRR_HIP_EDEVICE; }
+    int coop = 0;
+    (void)hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, c->device);
+    if (!coop) { delete c; return BRR_HIP_EDEVICE; }
+
+    const uint32_t N = cfg->num_inds, M = cfg->num_snps, K = cfg->num_components;
+    const uint32_t cus = (uint32_t)prop.multiProcessorCount;
+    uint32_t Preq = cfg->workgroups ? std::min(cfg->workgroups, cus) : cus;
+    uint32_t S = (N + Preq - 1) / Preq;
+    S = (S + kSliceAlign - 1) / kSliceAlign * kSliceAlign;
+    uint32_t R = (S + kRun - 1) / kRun;
+    if (R > (uint32_t)kMaxRuns) { delete c; return BRR_HIP_EINVAL; }
+    if (R == 3) R = 4;
+    c->R = R;
+    c->S = S;
+    c->P = (N + S - 1) / S;
+    c->B = kBwB;
+    c->x1_words = (size_t)2 * kBwV * c->P * 2;
+    c->x2_words = (size_t)2 * kBwV * 2;
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, bw_kernel_for(R), kThreads, 0) != hipSuccess ||
+        (uint64_t)occ * cus < c->P) {
+        delete c;
+        return BRR_HIP_EDEVICE;
+    }
+    const size_t nscal = 5 + 2 * (size_t)K;
+    int rc = BRR_HIP_OK;
+    if ((rc = dalloc(c, &c->d_eps, N)) || (rc = dalloc(c, &c->d_beta, M)) || (rc = dalloc(c, &c->d_comp, M)) ||
+        (rc = dalloc(c, &c->d_acum, 1)) || (rc = dalloc(c, &c->d_scal, nscal)) ||
+        (rc = dalloc(c, &c->d_cva, K - 1)) || (rc = dalloc(c, &c->d_xchg1, c->x1_words)) ||
+        (rc = dalloc(c, &c->d_xchg2, c->x2_words)) || (rc = dalloc(c, &c->d_ctl, 8)) ||
+        (rc = dalloc(c, &c->d_counters, kCounters)) || (rc = dalloc(c, &c->d_tmp, 4)) ||
+        (rc = dalloc(c, &c->d_u, M)) || (rc = dalloc(c, &c->d_z, 1)) || (rc = dalloc(c, &c->d_fail, N)) ||
+        (rc = dalloc(c, &c->d_sumf, M))) {
+        brr_hip_destroy(c);
+        return rc;
+    }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        brr_hip_destroy(c);
+        return BRR_HIP_EDEVICE;
+    }
+    (void)hipMemset(c->d_eps, 0, sizeof(double) * N);
+    (void)hipMemset(c->d_beta, 0, sizeof(double) * M);
+    (void)hipMemset(c->d_comp, 0, sizeof(int32_t) * M);
+    (void)hipMemset(c->d_scal, 0, sizeof(double) * nscal);
+    if (reset_exchange(c) != BRR_HIP_OK) { brr_hip_destroy(c); return BRR_HIP_EDEVICE; }
+    *out = c;
+    return BRR_HIP_OK;
+}
+
+int brr_hip_bw_set_model(brr_hip_ctx* c, const double* mixture_C, const double* failure)
+{
+    int rc;
+    if ((rc = need_kind(c, true))) return rc;
+    if (!mixture_C || !failure) return fail(c, BRR_HIP_EINVAL, "null argument");
+    if (!c->have_markers) return fail(c, BRR_HIP_ESTATE, "markers not uploaded");
+    const uint32_t N = c->cfg.num_inds, M = c->cfg.num_snps, K = c->cfg.num_components;
+    for (uint32_t k = 0; k + 1 < K; ++k)
+        if (!(mixture_C[k] > 0.0)) return fail(c, BRR_HIP_EINVAL, "mixture variances must be > 0");
+    for (uint32_t i = 0; i < N; ++i)
+        if (failure[i] != 0.0 && failure[i] != 1.0)
+            return fail(c, BRR_HIP_EINVAL, "failure indicators must be 0 or 1");
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipMemcpy(c->d_cva, mixture_C, sizeof(double) * (K - 1), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->d_fail, failure, sizeof(double) * N, hipMemcpyHostToDevice));
+    bw_sum_failure_kernel<<<M, kThreads, 0, c->stream>>>(c->d_dense, c->col_words, N, c->d_fail, c->d_stats,
+                                                         c->d_sumf);
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->have_model = true;
+    return BRR_HIP_OK;
+}
+
+int brr_hip_bw_set_scalars(brr_hip_ctx* c, const brr_hip_bw_scalars* s)
+{
+    int rc;
+    if ((rc = need_kind(c, true))) return rc;
+    if (!s || !s->pi) return fail(c, BRR_HIP_EINVAL, "null argument");
+    const uint32_t K = c->cfg.num_components;
+    std::vector<double> h(5 + 2 * (size_t)K, 0.0);
+    h[kBwMu] = s->mu;
+    h[kBwAlpha] = s->alpha;
+    h[kBwSigmaB] = s->sigma_b;
+    h[kBwD] = s->d;
+    for (uint32_t k = 0; k < K; ++k) {
+        h[kBwPi + k] = s->pi[k];
+        h[kBwPi + K + k] = s->v ? s->v[k] : 1.0;
+    }
+    if (!(s->alpha > 0.0) || !(s->sigma_b > 0.0)) return fail(c, BRR_HIP_EINVAL, "alpha and sigma_b must be > 0");
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipMemcpy(c->d_scal, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+    return BRR_HIP_OK;
+}
+
+int brr_hip_bw_get_scalars(brr_hip_ctx* c, brr_hip_bw_scalars* s)
+{
+    int rc;
+    if ((rc = need_kind(c, true))) return rc;
+    if (!s) return fail(c, BRR_HIP_EINVAL, "null argument");
+    const uint32_t K = c->cfg.num_components;
+    std::vector<double> h(5 + 2 * (size_t)K);
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipMemcpy(h.data(), c->d_scal, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+    s->mu = h[kBwMu];
+    s->alpha = h[kBwAlpha];
+    s->sigma_b = h[kBwSigmaB];
+    s->d = h[kBwD];
+    s->betasq = h[kBwBetaSq];
+    if (s->pi) for (uint32_t k = 0; k < K; ++k) s->pi[k] = h[kBwPi + k];
+    if (s->v) for (uint32_t k = 0; k < K; ++k) s->v[k] = h[kBwPi + K + k];
+    return BRR_HIP_OK;
+}
+
+int brr_hip_bw_sweep(brr_hip_ctx* c, const uint32_t* order, const double* u, uint64_t seed, uint32_t iteration,
+                     uint32_t chain)
+{
+    int rc;
+    if ((rc = need_kind(c, true)) || (rc = check_ready(c))) return rc;
+    if (!order || !u) return fail(c, BRR_HIP_EINVAL, "null argument");
+    if ((rc = check_order(c, order, 1))) return rc;
+    const uint32_t M = c->cfg.num_snps;
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (c->order_cap < M) {
+        if ((rc = dalloc(c, &c->d_order, M))) return rc;
+        c->order_cap = M;
+    }
+    HIP_TRY(c, hipMemcpy(c->d_order, order, sizeof(uint32_t) * M, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->d_u, u, sizeof(double) * M, hipMemcpyHostToDevice));
+    BwParams p = bw_params(c);
+    p.mode = kModeSweep;
+    p.iterations = 1;
+    p.iter_base = iteration;
+    p.chain = chain;
+    p.seed = seed;
+    p.order = c->d_order;
+    rc = launch_fn(c, bw_kernel_for(c->R), &p);
+    c->stats.markers = M;
+    return rc;
+}
+
+int brr_hip_bw_run_chain(brr_hip_ctx* c, uint64_t seed, uint32_t chain, uint32_t first_iteration,
+                         uint32_t iterations, const uint32_t* orders, double* hist)
+{
+    int rc;
+    if ((rc = need_kind(c, true)) || (rc = check_ready(c))) return rc;
+    if (!orders || iterations == 0) return fail(c, BRR_HIP_EINVAL, "null orders or zero iterations");
+    if ((rc = check_order(c, orders, iterations))) return rc;
+    const uint32_t M = c->cfg.num_snps;
+    const size_t total = (size_t)iterations * M;
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (c->order_cap < total) {
+        if ((rc = dalloc(c, &c->d_order, total))) return rc;
+        c->order_cap = total;
+    }
+    const size_t hsz = (size_t)iterations * 5;
+    if (c->hist_cap < hsz) {
+        if ((rc = dalloc(c, &c->d_hist, hsz))) return rc;
+        c->hist_cap = hsz;
+    }
+    HIP_TRY(c, hipMemcpy(c->d_order, orders, sizeof(uint32_t) * total, hipMemcpyHostToDevice));
+    BwParams p = bw_params(c);
+    p.mode = kModeChain;
+    p.iterations = iterations;
+    p.iter_base = first_iteration;
+    p.chain = chain;
+    p.seed = seed;
+    p.order = c->d_order;
+    p.hist = c->d_hist;
+    rc = launch_fn(c, bw_kernel_for(c->R), &p);
+    c->stats.markers = total;
+    if (rc) return rc;
+    if (hist) HIP_TRY(c, hipMemcpy(hist, c->d_hist, sizeof(double) * hsz, hipMemcpyDeviceToHost));
+    return BRR_HIP_OK;
+}
+
+int brr_hip_bw_set_vi(brr_hip_ctx* c, uint32_t marker, double alpha, double* out4)
+{
+    int rc;
+    if ((rc = need_kind(c, true))) return rc;
+    if (!c->have_markers) return fail(c, BRR_HIP_ESTATE, "markers not uploaded");
+    if (!out4 || marker >= c->cfg.num_snps) return fail(c, BRR_HIP_EINVAL, "bad marker or null output");
+    HIP_TRY(c, hipSetDevice(c->device));
+    bw_set_vi_kernel<<<1, kThreads, 0, c->stream>>>(c->d_dense, c->col_words, c->cfg.num_inds, c->d_eps, alpha,
+                                                    marker, c->d_tmp);
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(out4, c->d_tmp, sizeof(double) * 4, hipMemcpyDeviceToHost));
+    return BRR_HIP_OK;
+}
+
+int brr_hip_bw_residual_update(brr_hip_ctx* c, uint32_t marker, double beta, int sign)
+{
+    int rc;
+    if ((rc = need_kind(c, true))) return rc;
+    if (!c->have_markers) return fail(c, BRR_HIP_ESTATE, "markers not uploaded");
+    if (marker >= c->cfg.num_snps || (sign != 1 && sign != -1)) return fail(c, BRR_HIP_EINVAL, "bad marker or sign");
+    HIP_TRY(c, hipSetDevice(c->device));
+    const uint32_t blocks = (c->cfg.num_inds + kThreads - 1) / kThreads;
+    bw_residual_kernel<<<blocks, kThreads, 0, c->stream>>>(c->d_dense, c->col_words, c->cfg.num_inds, c->d_stats,
+                                                           marker, beta, sign, c->d_eps);
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return BRR_HIP_OK;
+}
+
+int brr_hip_bw_sum_failure(brr_hip_ctx* c, double* out)
+{
+    int rc;
+    if ((rc = need_kind(c, true))) return rc;
+    if (!c->have_model) return fail(c, BRR_HIP_ESTATE, "model not set (brr_hip_bw_set_model)");
+    if (!out) return fail(c, BRR_HIP_EINVAL, "null argument");
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipMemcpy(out, c->d_sumf, sizeof(double) * c->cfg.num_snps, hipMemcpyDeviceToHost));
     return BRR_HIP_OK;
 }
 

@@ -4,6 +4,7 @@
 //   init                 BayesRBase.cpp:39-99
 //   random numbers       prepareForAnylsis, BayesRBase.cpp:101-110 (per marker id)
 //   marker order         std::random_shuffle, BayesRBase.cpp:198
+//   BayesW init          BayesWBase::init, bayeswbase.cpp:469-540
 // They are restated here over the counter-based stream of brr_rng.h so a host
 // driver and the device-resident chain consume the same numbers.
 #include <cmath>
@@ -75,6 +76,32 @@ int brr_marker_uz(uint64_t seed, uint32_t iteration, uint32_t chain, uint32_t M,
 {
     if (!u || !z) return BRR_HIP_EINVAL;
     for (uint32_t j = 0; j < M; ++j) brr::marker_uz(seed, iteration, j, chain, &u[j], &z[j]);
+    return BRR_HIP_OK;
+}
+
+int brr_bayesw_init(const double* y, const double* fail, uint32_t N, uint32_t M, uint32_t K, double* eps,
+                    brr_hip_bw_scalars* s)
+{
+    // BayesWBase::init (bayeswbase.cpp:469-540) without fixed effects
+    if (!y || !fail || !eps || !s || !s->pi || N < 2 || M < 1 || K < 2 || K > 8) return BRR_HIP_EINVAL;
+    const uint32_t km1 = K - 1;
+    s->pi[0] = 0.99;
+    for (uint32_t k = 1; k < K; ++k) s->pi[k] = (1 - s->pi[0]) / km1;
+    if (s->v)
+        for (uint32_t k = 0; k < K; ++k) s->v[k] = 1.0;
+    double sy = 0.0;
+    for (uint32_t i = 0; i < N; ++i) sy += y[i];
+    s->mu = sy / N;
+    double dd = 0.0;
+    for (uint32_t i = 0; i < N; ++i) dd += fail[i];
+    s->d = dd;
+    double ss = 0.0;
+    for (uint32_t i = 0; i < N; ++i) ss += (y[i] - s->mu) * (y[i] - s->mu);
+    const double denominator = (6 * ss / (N - 1));
+    s->alpha = 3.14159 / std::sqrt(denominator);     // PI of bayeswbase.cpp
+    for (uint32_t i = 0; i < N; ++i) eps[i] = y[i] - s->mu;
+    s->sigma_b = 6.283185 / (6 * (s->alpha * s->alpha) * M);   // PI2
+    s->betasq = 0.0;
     return BRR_HIP_OK;
 }
 

@@ -25,6 +25,14 @@
  *     with updateMu (src/DenseBayesRRmz.cpp:36-42, src/SparseBayesRRG.cpp:106-113)
  *     and the sigmaE / sigmaG / pi draws (src/BayesRBase.cpp:237-247), held on
  *     the device for a whole chain  -> brr_hip_run_chain
+ *   - BayesW (Weibull survival) on ragged markers: AnalysisGraph::exec driving
+ *     BayesWBase::processColumn (src/bayeswbase.cpp:604-704) with
+ *     RaggedBayesWKernel (src/raggedbayeswkernel.cpp:10-77)  -> brr_hip_bw_sweep;
+ *     BayesWKernel::setVi / calculateSumFailure / calculateResidualUpdate
+ *     (src/bayeswkernel.h:18-26)  -> brr_hip_bw_set_vi / brr_hip_bw_sum_failure /
+ *     brr_hip_bw_residual_update; BayesWBase::runGibbs (src/bayeswbase.cpp:737-830)
+ *     with sampleMu / sampleAlpha (:542-569, :707-734) and the sigma_b / pi_L
+ *     draws (:798-806)  -> brr_hip_bw_run_chain
  */
 #ifndef BRR_HIP_H
 #define BRR_HIP_H
@@ -35,7 +43,7 @@
 extern "C" {
 #endif
 
-#define BRR_HIP_ABI_VERSION 2
+#define BRR_HIP_ABI_VERSION 3
 
 enum brr_hip_status {
     BRR_HIP_OK = 0,
@@ -43,7 +51,9 @@ enum brr_hip_status {
     BRR_HIP_ENOMEM = 2,   /* device allocation failed */
     BRR_HIP_EDEVICE = 3,  /* HIP runtime error or no device */
     BRR_HIP_ESTATE = 4,   /* call out of order (e.g. sweep before upload) */
-    BRR_HIP_ETIMEOUT = 5  /* a device-side wait exceeded its bound */
+    BRR_HIP_ETIMEOUT = 5, /* a device-side wait exceeded its bound */
+    BRR_HIP_ESAMPLER = 6  /* BayesW: the adaptive rejection sampler failed (its
+                             error code is in brr_hip_last_error) */
 };
 
 /* Marker layouts (PreprocessDataType, src/common.h:27-32). */
@@ -180,6 +190,75 @@ int brr_hip_last_stats(const brr_hip_ctx* ctx, brr_hip_stats* stats);
  * scalars, Gram tiles, u/z draws, exchange reduce, accept/correct, block-end
  * barrier, spare. */
 int brr_hip_last_phase_ms(const brr_hip_ctx* ctx, double* out, uint32_t n);
+
+/* ---- BayesW: Weibull survival model on ragged markers ---- */
+
+/* A BayesW context holds N x {epsilon, failure indicator}, the ragged columns as
+ * 2-bit codes, beta and the mixture component per marker. Markers are uploaded
+ * with brr_hip_upload_ragged; brr_hip_set_state / brr_hip_get_state move
+ * epsilon, beta and components (acum is not kept: pass NULL). */
+typedef struct {
+    uint32_t num_inds;       /* N */
+    uint32_t num_snps;       /* M */
+    uint32_t num_components; /* K = number of mixture classes + the zero class, 2..8 */
+    uint32_t quad_points;    /* Gauss-Hermite points n_q: 3, 5, ..., 15 (Options::quad_points) */
+    uint32_t workgroups;     /* persistent workgroups (0 = choose) */
+    uint32_t device;         /* HIP device ordinal */
+} brr_hip_bw_config;
+
+/* Chain scalars of BayesWBase (src/bayeswbase.h:48-67). pi and v have K entries;
+ * in set calls v is the class count a sweep starts from (runGibbs sets it to
+ * ones before each sweep, src/bayeswbase.cpp:789) and betasq is ignored. */
+typedef struct {
+    double mu;        /* m_mu */
+    double alpha;     /* m_alpha, the Weibull shape */
+    double sigma_b;   /* m_sigma_b */
+    double d;         /* d, the number of events (sum of the failure indicators) */
+    double* pi;       /* m_pi_L, K */
+    double* v;        /* m_v, K */
+    double betasq;    /* out: sum of beta^2 drawn in the last sweep */
+} brr_hip_bw_scalars;
+
+int brr_hip_bw_create(const brr_hip_bw_config* cfg, brr_hip_ctx** out);
+
+/* The mixture variances C_k (Options::S, K-1 values > 0) and the failure
+ * indicator of each individual (N doubles, 0 or 1). Needs the markers: the
+ * per-marker sum_failure (calculateSumFailure) is computed here. */
+int brr_hip_bw_set_model(brr_hip_ctx* ctx, const double* mixture_C, const double* failure);
+int brr_hip_bw_set_scalars(brr_hip_ctx* ctx, const brr_hip_bw_scalars* s);
+int brr_hip_bw_get_scalars(brr_hip_ctx* ctx, brr_hip_bw_scalars* s);
+
+/* One AnalysisGraph::exec of BayesW: processColumn over order[0..M) in
+ * sequence with the uniform u[j] of marker j (indexed by marker id) choosing
+ * the class; a nonzero class draws beta by adaptive rejection sampling from
+ * the stream (seed, chain) at `iteration`. Uses and updates the device state. */
+int brr_hip_bw_sweep(brr_hip_ctx* ctx, const uint32_t* order, const double* u,
+                     uint64_t seed, uint32_t iteration, uint32_t chain);
+
+/* `iterations` whole BayesWBase::runGibbs iterations in one persistent launch:
+ * sampleMu, the sweep (u from the counter-based stream), sampleAlpha, then
+ * sigma_b and pi_L. orders: iterations x M marker orders. hist (may be NULL):
+ * iterations x 5 doubles {mu, alpha, sigma_b, M - v0 + 1, sum beta^2}. */
+int brr_hip_bw_run_chain(brr_hip_ctx* ctx, uint64_t seed, uint32_t chain,
+                         uint32_t first_iteration, uint32_t iterations,
+                         const uint32_t* orders, double* hist);
+
+/* Per-marker kernel calls on the device epsilon (correctness path).
+ * set_vi: RaggedBayesWKernel::setVi of vi = exp(alpha eps - EulerGamma):
+ *   out[4] = {vi_sum, vi_0, vi_1, vi_2}.
+ * residual_update: epsilon += sign * calculateResidualUpdate(beta), sign = +1 / -1.
+ * sum_failure: calculateSumFailure of every marker (M doubles). */
+int brr_hip_bw_set_vi(brr_hip_ctx* ctx, uint32_t marker, double alpha, double* out4);
+int brr_hip_bw_residual_update(brr_hip_ctx* ctx, uint32_t marker, double beta, int sign);
+int brr_hip_bw_sum_failure(brr_hip_ctx* ctx, double* out);
+
+/* BayesWBase::init (src/bayeswbase.cpp:469-540, no fixed effects): from the
+ * log survival times y (N, as the reference stores them, i.e. through float)
+ * and failure indicators: eps = y - mu, mu = mean(y), alpha = 3.14159 /
+ * sqrt(6 var(y)), sigma_b = 6.283185 / (6 alpha^2 M), pi = {0.99, 0.01/(K-1), ...},
+ * v = ones, d = sum(failure). s->pi and s->v must hold K values. */
+int brr_bayesw_init(const double* y, const double* failure, uint32_t num_inds, uint32_t num_snps,
+                    uint32_t num_components, double* epsilon, brr_hip_bw_scalars* s);
 
 /* ---- host-side marker building (MarkerBuilder semantics, no device) ---- */
 

@@ -8,6 +8,7 @@ import re
 from pathlib import Path
 
 import numpy as np
+import pytest
 
 import bayesrrcmd_amd as brr
 from bayesrrcmd_amd import _abi
@@ -35,7 +36,7 @@ def test_every_declared_symbol_is_exported():
 
 
 def test_abi_version():
-    assert brr.lib().brr_hip_abi_version() == 2
+    assert brr.lib().brr_hip_abi_version() == 3
 
 
 def test_create_rejects_bad_config():
@@ -55,6 +56,45 @@ def test_create_rejects_bad_config():
     ]
     for cfg in bad:
         assert lib.brr_hip_create(C.byref(cfg), C.byref(ctx)) == _abi.EINVAL
+
+
+def test_bw_create_rejects_bad_config():
+    lib = brr.lib()
+    ctx = _abi.CTX()
+    bad = [
+        _abi.BwConfig(1, 10, 4, 7, 0, 0),      # N < 2
+        _abi.BwConfig(10, 0, 4, 7, 0, 0),      # M = 0
+        _abi.BwConfig(10, 10, 1, 7, 0, 0),     # K < 2
+        _abi.BwConfig(10, 10, 9, 7, 0, 0),     # K > 8
+        _abi.BwConfig(10, 10, 4, 8, 0, 0),     # even number of quadrature points
+        _abi.BwConfig(10, 10, 4, 17, 0, 0),    # more than 15 points
+        _abi.BwConfig(10, 10, 4, 1, 0, 0),     # fewer than 3 points
+    ]
+    for cfg in bad:
+        assert lib.brr_hip_bw_create(C.byref(cfg), C.byref(ctx)) == _abi.EINVAL
+    assert lib.brr_hip_bw_sweep(None, None, None, 0, 0, 0) == _abi.EINVAL
+
+
+def test_bayesw_init_matches_oracle():
+    """BayesWBase::init of the host library against the oracle's restatement."""
+    import sys
+    sys.path.insert(0, str(ROOT / "tests"))
+    import _oracle as orc
+    from _data import make_bw_problem
+    pr = make_bw_problem(1500, 40, seed=12)
+    w = orc.BayesWProblem(pr.n, pr.m, pr.ragged, pr.fail)
+    w.init(pr.y)
+    eps, s = brr.bayesw_init(pr.y, pr.fail, pr.m, 4)
+    st = w.state
+    assert (s["mu"], s["alpha"], s["sigma_b"], s["d"]) == (st.mu, st.alpha, st.sigma_b, st.d)
+    np.testing.assert_array_equal(eps, w.eps)
+    np.testing.assert_array_equal(s["pi"], w.pi)
+    np.testing.assert_array_equal(s["v"], np.ones(4))
+    # the closed forms of bayeswbase.cpp:494-526
+    assert s["alpha"] == pytest.approx(3.14159 / np.sqrt(6 * np.sum((pr.y - pr.y.mean()) ** 2) / (pr.n - 1)),
+                                       rel=1e-13)
+    assert s["sigma_b"] == pytest.approx(6.283185 / (6 * s["alpha"] ** 2 * pr.m), rel=1e-14)
+    np.testing.assert_allclose(s["pi"], [0.99, 0.01 / 3, 0.01 / 3, 0.01 / 3], rtol=1e-15)
 
 
 def test_create_without_device_fails_cleanly():
