@@ -438,7 +438,16 @@ class HipBayesW:
         self._ck(lib().brr_hip_bw_sum_failure(self.ctx, _p(out, C.c_double)), "brr_hip_bw_sum_failure")
         return out
 
+    PHASES = ("block_load", "round_sums", "exchange", "decisions", "beta_draws", "residual_update",
+              "iteration_scalars", "in_model_shift")
+
     def stats(self) -> dict:
         s = _abi.Stats()
         self._ck(lib().brr_hip_last_stats(self.ctx, C.byref(s)), "brr_hip_last_stats")
-        return {f: getattr(s, f) for f, _ in _abi.Stats._fields_}
+        out = {f: getattr(s, f) for f, _ in _abi.Stats._fields_}
+        ph = np.zeros(12)
+        self._ck(lib().brr_hip_last_phase_ms(self.ctx, _p(ph, C.c_double), 12), "brr_hip_last_phase_ms")
+        out["phase_ms"] = dict(zip(self.PHASES, ph.tolist()))
+        # phase slots 8 and 9 carry the shader-clock and reference-clock ticks of the launch
+        out["core_clock_mhz"] = 100.0 * ph[8] / ph[9] if ph[9] > 0 else float("nan")
+        return out

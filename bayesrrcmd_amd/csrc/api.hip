@@ -541,7 +541,7 @@ int brr_hip_create(const brr_hip_config* cfg, brr_hip_ctx** out)
     const uint32_t N = cfg->num_inds;
     const uint32_t cus = (uint32_t)prop.multiProcessorCount;
     uint32_t Preq = cfg->workgroups ? cfg->workgroups : cus;
-    Preq = std::min(Preq, cus);
+    Preq = std::min(std::min(Preq, cus), kMaxP);
     uint32_t S = (N + Preq - 1) / Preq;
     S = (S + kSliceAlign - 1) / kSliceAlign * kSliceAlign;
     c->T = cfg->num_traits;
@@ -1027,7 +1027,7 @@ int brr_hip_bw_create(const brr_hip_bw_config* cfg, brr_hip_ctx** out)
 
     const uint32_t N = cfg->num_inds, M = cfg->num_snps, K = cfg->num_components;
     const uint32_t cus = (uint32_t)prop.multiProcessorCount;
-    uint32_t Preq = cfg->workgroups ? std::min(cfg->workgroups, cus) : cus;
+    uint32_t Preq = std::min(cfg->workgroups ? std::min(cfg->workgroups, cus) : cus, kMaxP);
     uint32_t S = (N + Preq - 1) / Preq;
     S = (S + kSliceAlign - 1) / kSliceAlign * kSliceAlign;
     uint32_t R = (S + kRun - 1) / kRun;

@@ -27,7 +27,8 @@
 namespace brr {
 
 constexpr int kBwB = 32;                       // slots per block
-constexpr int kBwV = 1 + 2 * kBwB;             // values per round: vi_sum, (vi_1, vi_2) per slot
+constexpr int kBwV = 2 * kBwB + 4;             // values per round: vi_sum, (vi_1, vi_2) per slot,
+                                               // and the shifted (vi_sum, vi_1, vi_2) of a speculated slot
 constexpr int kArmsN = 100;                    // ARMS envelope points (npoint of bayeswbase.cpp)
 constexpr uint32_t kStreamArms = 4;
 constexpr uint32_t kSiteMu = 0xFFFFFFF0u;
@@ -63,30 +64,34 @@ struct BwParams {
 
 enum : int { kBwMu = 0, kBwAlpha = 1, kBwSigmaB = 2, kBwD = 3, kBwBetaSq = 4, kBwPi = 5 };
 
-struct ArmsPoint {
-    double x, y, ey, cum;
-    int f, pl, pr, pad;
-};
-
-// One sampler run: the envelope (kArmsN points + the working point), the
-// resumption state and the random stream position. Lives in LDS, one per wave.
+// One sampler run, one per wave in LDS. The envelope is kept sorted by x:
+// position 0 is the left bound, odd positions are evaluated points, even ones
+// intersections of chords, position n - 1 the right bound (the index-linked
+// list of BayesW_arms.cpp in its traversal order). Lane l works on positions l
+// and l + 64.
+constexpr int kArmsCap = 128;
 struct ArmsEnv {
-    ArmsPoint p[kArmsN + 1];
+    double x[kArmsCap], y[kArmsCap], ey[kArmsCap], cum[kArmsCap], area[kArmsCap];
     double xinit[4];
     double ymax, yu, ynew;
+    double wx, wy, wey;           // the working point
     double xout;                  // point to evaluate next, or the sample
     uint64_t seed;
     uint32_t site, iteration, chain, k;
-    int cpoint, state, kinit, q, guard, pad;
+    int n, state, kinit, q, wq, guard;
 };
 
 template <int R>
 struct BwShared {
     uint32_t codes[kBwB][R][32];
     uint32_t mk[kBwB];
+    uint32_t ord[2][kBwB];        // marker ids of the next two blocks, by block parity
     double bold[kBwB], mean[kBwB], sd[kBwB], sumf[kBwB], pu[kBwB];
     double xv[2][kBwV];           // round totals, by round parity
     double cls[kMaxK], pi[kMaxK], v[kMaxK];
+    double gv[kBwB][kMaxK - 1][14];   // weighted integrand terms per slot, slab class and node
+    double ml[kBwB][kMaxK];            // class likelihoods per slot
+    double ghx[14], ghw[14], ghwc;     // the rule's nodes in summation order
     double mu, alpha, sigma_b, d, visum, betasq;
     int stop;                     // first slot that changes eps this round
     int sel;                      // its class
@@ -117,118 +122,151 @@ __device__ __forceinline__ double arms_expshift(double y, double y0)
 
 __device__ __forceinline__ double arms_logshift(double y, double y0) { return log(y) + y0 - 50.0; }
 
-#define AP(i) (e.p[(i)])
-
-// intersection of the chords either side of intersection point q
-// (BayesW_arms.cpp:707-808); nonzero on a convexity violation
-__device__ __forceinline__ int arms_meet(ArmsEnv& e, int q)
+// meet (BayesW_arms.cpp:707-808) of the intersection at even position q: its
+// x and y from the chords through the evaluated points q-3, q-1 and q+1, q+3.
+// Nonzero on a convexity violation or a misplaced intersection. Reads only
+// evaluated points, so every intersection may be recomputed at once: one whose
+// neighbours did not change gets the same value again.
+__device__ __forceinline__ int arms_meet_at(ArmsEnv& e, int q)
 {
+    const int n = e.n;
     double gl = 0, gr = 0, grl = 0, dl = 0, dr = 0;
-    int il = 0, ir = 0, irl = 0;
-    if (AP(q).f) return 3;
-    const int ql = AP(q).pl, qr = AP(q).pr;
-    if (ql >= 0 && AP(ql).pl >= 0 && AP(AP(ql).pl).pl >= 0) {
-        const int q3 = AP(AP(ql).pl).pl;
-        gl = (AP(ql).y - AP(q3).y) / (AP(ql).x - AP(q3).x);
-        il = 1;
-    }
-    if (qr >= 0 && AP(qr).pr >= 0 && AP(AP(qr).pr).pr >= 0) {
-        const int q3 = AP(AP(qr).pr).pr;
-        gr = (AP(qr).y - AP(q3).y) / (AP(qr).x - AP(q3).x);
-        ir = 1;
-    }
-    if (ql >= 0 && qr >= 0) {
-        grl = (AP(qr).y - AP(ql).y) / (AP(qr).x - AP(ql).x);
-        irl = 1;
-    }
+    const bool il = q - 3 >= 0, ir = q + 3 <= n - 1, irl = q - 1 >= 0 && q + 1 <= n - 1;
+    if (il) gl = (e.y[q - 1] - e.y[q - 3]) / (e.x[q - 1] - e.x[q - 3]);
+    if (ir) gr = (e.y[q + 1] - e.y[q + 3]) / (e.x[q + 1] - e.x[q + 3]);
+    if (irl) grl = (e.y[q + 1] - e.y[q - 1]) / (e.x[q + 1] - e.x[q - 1]);
     if (irl && il && gl < grl) return 1;
     if (irl && ir && gr > grl) return 1;
     if (il && irl) {
-        dr = (gl - grl) * (AP(qr).x - AP(ql).x);
+        dr = (gl - grl) * (e.x[q + 1] - e.x[q - 1]);
         if (dr < 0.1) dr = 0.1;
     }
     if (ir && irl) {
-        dl = (grl - gr) * (AP(qr).x - AP(ql).x);
+        dl = (grl - gr) * (e.x[q + 1] - e.x[q - 1]);
         if (dl < 0.1) dl = 0.1;
     }
+    double x = e.x[q], y;
     if (il && ir && irl) {
-        AP(q).x = (dl * AP(qr).x + dr * AP(ql).x) / (dl + dr);
-        AP(q).y = (dl * AP(qr).y + dr * AP(ql).y + dl * dr) / (dl + dr);
+        x = (dl * e.x[q + 1] + dr * e.x[q - 1]) / (dl + dr);
+        y = (dl * e.y[q + 1] + dr * e.y[q - 1] + dl * dr) / (dl + dr);
     } else if (il && irl) {
-        AP(q).x = AP(qr).x;
-        AP(q).y = AP(qr).y + dr;
+        x = e.x[q + 1];
+        y = e.y[q + 1] + dr;
     } else if (ir && irl) {
-        AP(q).x = AP(ql).x;
-        AP(q).y = AP(ql).y + dl;
+        x = e.x[q - 1];
+        y = e.y[q - 1] + dl;
     } else if (il) {
-        AP(q).y = AP(ql).y + gl * (AP(q).x - AP(ql).x);
+        y = e.y[q - 1] + gl * (x - e.x[q - 1]);
     } else if (ir) {
-        AP(q).y = AP(qr).y - gr * (AP(qr).x - AP(q).x);
+        y = e.y[q + 1] - gr * (e.x[q + 1] - x);
     } else {
         return 4;
     }
-    if ((ql >= 0 && AP(q).x < AP(ql).x) || (qr >= 0 && AP(q).x > AP(qr).x)) return 5;
+    e.x[q] = x;
+    e.y[q] = y;
+    if ((q - 1 >= 0 && x < e.x[q - 1]) || (q + 1 <= n - 1 && x > e.x[q + 1])) return 5;
     return 0;
 }
 
-// exponentiate and integrate the envelope (BayesW_arms.cpp:672-703, 812-834)
-__device__ __forceinline__ void arms_cumulate(ArmsEnv& e)
+// every intersection of the wave's envelope; nonzero if any fails
+__device__ __forceinline__ int arms_meet_all(ArmsEnv& e, int lane)
 {
-    int lm = 0;
-    while (AP(lm).pl >= 0) lm = AP(lm).pl;
-    double ymax = AP(lm).y;
-    for (int q = AP(lm).pr; q >= 0; q = AP(q).pr)
-        if (AP(q).y > ymax) ymax = AP(q).y;
-    e.ymax = ymax;
-    for (int q = lm; q >= 0; q = AP(q).pr) AP(q).ey = arms_expshift(AP(q).y, ymax);
-    AP(lm).cum = 0.0;
-    for (int q = AP(lm).pr; q >= 0; q = AP(q).pr) {
-        const int l = AP(q).pl;
-        double a;
-        if (AP(l).x == AP(q).x) a = 0.0;
-        else if (fabs(AP(q).y - AP(l).y) < 0.1) a = 0.5 * (AP(q).ey + AP(l).ey) * (AP(q).x - AP(l).x);
-        else a = ((AP(q).ey - AP(l).ey) / (AP(q).y - AP(l).y)) * (AP(q).x - AP(l).x);
-        AP(q).cum = AP(l).cum + a;
+    bool bad = false;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int q = lane + 64 * h;
+        if (q < e.n && (q & 1) == 0) bad |= arms_meet_at(e, q) != 0;
     }
+    return __any(bad) ? 1 : 0;
+}
+
+// exponentiate and integrate the envelope (BayesW_arms.cpp:672-703, 812-834):
+// ey and the areas in parallel, the running sum in position order as the
+// reference accumulates it
+__device__ __forceinline__ void arms_cumulate(ArmsEnv& e, int lane)
+{
+    const int n = e.n;
+    double m = -__builtin_inf();
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int q = lane + 64 * h;
+        if (q < n) m = fmax(m, e.y[q]);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    const double ymax = m;
+    double eyq[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int q = lane + 64 * h;
+        eyq[h] = 0.0;
+        if (q < n) {
+            eyq[h] = arms_expshift(e.y[q], ymax);
+            e.ey[q] = eyq[h];
+        }
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int q = lane + 64 * h;
+        if (q >= 1 && q < n) {
+            const double eyl = arms_expshift(e.y[q - 1], ymax);   // the value its left neighbour stored
+            double a;
+            if (e.x[q - 1] == e.x[q]) a = 0.0;
+            else if (fabs(e.y[q] - e.y[q - 1]) < 0.1) a = 0.5 * (eyq[h] + eyl) * (e.x[q] - e.x[q - 1]);
+            else a = ((eyq[h] - eyl) / (e.y[q] - e.y[q - 1])) * (e.x[q] - e.x[q - 1]);
+            e.area[q] = a;
+        }
+    }
+    double c = 0.0;
+    e.cum[0] = 0.0;
+    for (int q = 1; q < n; ++q) {
+        c = c + e.area[q];
+        e.cum[q] = c;
+    }
+    e.ymax = ymax;
 }
 
 // the x whose envelope cumulative probability is prob, into the working point
-// w (BayesW_arms.cpp:364-425)
-__device__ __forceinline__ int arms_invert(ArmsEnv& e, double prob, int w)
+// (BayesW_arms.cpp:364-425); it falls between positions q - 1 and q, q = e.wq
+__device__ __forceinline__ int arms_invert(ArmsEnv& e, double prob, int lane)
 {
-    int q = 0;
-    while (AP(q).pr >= 0) q = AP(q).pr;
-    const double u = prob * AP(q).cum;
-    while (AP(AP(q).pl).cum > u) q = AP(q).pl;
-    const int l = AP(q).pl;
-    AP(w).pl = l;
-    AP(w).pr = q;
-    AP(w).f = 0;
-    AP(w).cum = u;
-    const double prop = (u - AP(l).cum) / (AP(q).cum - AP(l).cum);
-    double xl = AP(q).x, xr = AP(q).x;
-    if (AP(l).x == AP(q).x) {
-        AP(w).x = AP(q).x;
-        AP(w).y = AP(q).y;
-        AP(w).ey = AP(q).ey;
+    const int n = e.n;
+    const double u = prob * e.cum[n - 1];
+    // q - 1 = the last position below n - 1 whose running sum is <= u
+    // (the reference walks left from the end while the left neighbour's sum is > u)
+    const unsigned long long b0 = __ballot(lane <= n - 2 && !(e.cum[lane] > u));
+    const unsigned long long b1 = __ballot(lane + 64 <= n - 2 && !(e.cum[lane + 64] > u));
+    if (!b0 && !b1) return 6;   // (cum[0] = 0 <= u always holds)
+    const int last = b1 ? 64 + 63 - __clzll((long long)b1) : 63 - __clzll((long long)b0);
+    const int q = last + 1, l = last;
+    e.wq = q;
+    const double prop = (u - e.cum[l]) / (e.cum[q] - e.cum[l]);
+    double xl = e.x[q], xr = e.x[q], wx, wy, wey;
+    if (e.x[l] == e.x[q]) {
+        wx = e.x[q];
+        wy = e.y[q];
+        wey = e.ey[q];
     } else {
-        xl = AP(l).x;
-        xr = AP(q).x;
-        const double yl = AP(l).y, yr = AP(q).y, eyl = AP(l).ey, eyr = AP(q).ey;
+        xl = e.x[l];
+        xr = e.x[q];
+        const double yl = e.y[l], yr = e.y[q], eyl = e.ey[l], eyr = e.ey[q];
         if (fabs(yr - yl) < 0.1) {
             if (fabs(eyr - eyl) > 0.001 * fabs(eyr + eyl))
-                AP(w).x = xl + ((xr - xl) / (eyr - eyl)) * (-eyl + sqrt((1. - prop) * eyl * eyl + prop * eyr * eyr));
+                wx = xl + ((xr - xl) / (eyr - eyl)) * (-eyl + sqrt((1. - prop) * eyl * eyl + prop * eyr * eyr));
             else
-                AP(w).x = xl + (xr - xl) * prop;
-            AP(w).ey = ((AP(w).x - xl) / (xr - xl)) * (eyr - eyl) + eyl;
-            AP(w).y = arms_logshift(AP(w).ey, e.ymax);
+                wx = xl + (xr - xl) * prop;
+            wey = ((wx - xl) / (xr - xl)) * (eyr - eyl) + eyl;
+            wy = arms_logshift(wey, e.ymax);
         } else {
-            AP(w).x = xl + ((xr - xl) / (yr - yl)) * (-yl + arms_logshift(((1. - prop) * eyl + prop * eyr), e.ymax));
-            AP(w).y = ((AP(w).x - xl) / (xr - xl)) * (yr - yl) + yl;
-            AP(w).ey = arms_expshift(AP(w).y, e.ymax);
+            wx = xl + ((xr - xl) / (yr - yl)) * (-yl + arms_logshift(((1. - prop) * eyl + prop * eyr), e.ymax));
+            wy = ((wx - xl) / (xr - xl)) * (yr - yl) + yl;
+            wey = arms_expshift(wy, e.ymax);
         }
     }
-    if (AP(w).x < xl || AP(w).x > xr) return 6;
+    e.wx = wx;
+    e.wy = wy;
+    e.wey = wey;
+    if (wx < xl || wx > xr) return 6;
     return 0;
 }
 
@@ -238,21 +276,17 @@ enum : int { kAsInit = 0, kAsSample = 1, kAsEvalW = 2, kAsEvalQ = 3 };
 // Sets up one draw (arms with ninit = 4, npoint = 100, nsamp = 1 and no
 // Metropolis step, BayesW_arms.cpp:133-340); 0 or the reference's error code.
 __device__ __forceinline__ int arms_begin(ArmsEnv& e, const double (&xinit)[4], double xl, double xr, uint64_t seed,
-                                          uint32_t site, uint32_t iteration, uint32_t chain)
+                                          uint32_t site, uint32_t iteration, uint32_t chain, int lane)
 {
     constexpr int ninit = 4, mpoint = 2 * ninit + 1;
     if (xinit[0] <= xl || xinit[ninit - 1] >= xr) return 1003;
     for (int i = 1; i < ninit; ++i)
         if (xinit[i] <= xinit[i - 1]) return 1004;
-    for (int j = 0; j < mpoint; ++j) {
-        AP(j).pl = j - 1;
-        AP(j).pr = (j + 1 < mpoint) ? j + 1 : -1;
-        AP(j).f = (j & 1);
-        AP(j).y = 0.0;
-        AP(j).x = (j & 1) ? xinit[j >> 1] : 0.0;
+    if (lane < mpoint) {
+        e.x[lane] = (lane & 1) ? xinit[lane >> 1] : (lane == 0 ? xl : (lane == mpoint - 1 ? xr : 0.0));
+        e.y[lane] = 0.0;
     }
-    AP(0).x = xl;
-    AP(mpoint - 1).x = xr;
+    e.n = mpoint;
     e.seed = seed;
     e.site = site;
     e.iteration = iteration;
@@ -268,114 +302,119 @@ __device__ __forceinline__ int arms_begin(ArmsEnv& e, const double (&xinit)[4], 
 // asked for. Returns kArmsEval with e.xout the next point to evaluate, kArmsDone
 // with e.xout the sample, or an error code (>= 1000). Every lane of a wave calls
 // it with the same arguments on the wave's own envelope.
-__device__ __noinline__ int arms_step(ArmsEnv& e, double y)
+__device__ __forceinline__ int arms_step(ArmsEnv& e, double y, int lane)
 {
-    constexpr int mpoint = 9, w = kArmsN;
+    constexpr int mpoint = 9;
     if (e.state == kAsInit) {
-        if (e.kinit > 0) AP(2 * e.kinit - 1).y = y;
+        if (e.kinit > 0) e.y[2 * e.kinit - 1] = y;
         if (e.kinit < 4) {
-            e.xout = AP(2 * e.kinit + 1).x;
+            e.xout = e.x[2 * e.kinit + 1];
             e.kinit++;
             return kArmsEval;
         }
-        for (int j = 0; j < mpoint; j += 2)
-            if (arms_meet(e, j)) return 2000;
-        arms_cumulate(e);
-        e.cpoint = mpoint;
+        if (arms_meet_all(e, lane)) return 2000;
+        arms_cumulate(e, lane);
         e.state = kAsSample;
     } else {
         if (e.state == kAsEvalW) {
             // the working point was evaluated (BayesW_arms.cpp:505-536)
             e.ynew = y;
-            AP(w).y = y;
-            AP(w).ey = arms_expshift(y, e.ymax);
-            AP(w).f = 1;
-            if (e.cpoint <= kArmsN - 2) {
-                // update: insert it with a new intersection (BayesW_arms.cpp:540-668)
-                const int q = e.cpoint++;
-                AP(q).x = AP(w).x;
-                AP(q).y = AP(w).y;
-                AP(q).f = 1;
-                const int m = e.cpoint++;
-                AP(m).f = 0;
-                const int wl = AP(w).pl, wr = AP(w).pr;
-                if (AP(wl).f && !AP(wr).f) {
-                    AP(m).pl = wl; AP(m).pr = q; AP(q).pl = m; AP(q).pr = wr; AP(wl).pr = m; AP(wr).pl = q;
-                } else if (!AP(wl).f && AP(wr).f) {
-                    AP(m).pr = wr; AP(m).pl = q; AP(q).pr = m; AP(q).pl = wl; AP(wr).pl = m; AP(wl).pr = q;
-                } else {
-                    return 2000;
+            e.wy = y;
+            e.wey = arms_expshift(y, e.ymax);
+            if (e.n <= kArmsN - 2) {
+                // update: insert it with a new intersection (BayesW_arms.cpp:540-668).
+                // It lies between positions wq - 1 and wq; the evaluated one of the two
+                // keeps an intersection between itself and the new point.
+                const int n = e.n, wq = e.wq;
+                double sx[2], sy[2];
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int q = lane + 64 * h;
+                    sx[h] = (q < n) ? e.x[q] : 0.0;
+                    sy[h] = (q < n) ? e.y[q] : 0.0;
                 }
-                e.q = q;
-                const int ql = (AP(AP(q).pl).pl >= 0) ? AP(AP(q).pl).pl : AP(q).pl;
-                const int qr = (AP(AP(q).pr).pr >= 0) ? AP(AP(q).pr).pr : AP(q).pr;
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int q = lane + 64 * h;
+                    if (q >= wq && q < n) {
+                        e.x[q + 2] = sx[h];
+                        e.y[q + 2] = sy[h];
+                    }
+                }
+                // positions wq - 1 (wl) and wq + 2 (wr, shifted): wl evaluated <=> wq - 1 odd
+                const int qn = ((wq - 1) & 1) ? wq + 1 : wq;    // the new evaluated point
+                e.x[qn] = e.wx;
+                e.y[qn] = e.wy;
+                e.n = n + 2;
+                e.q = qn;
+                const int ql = (qn - 2 >= 0) ? qn - 2 : qn - 1;
+                const int qr = (qn + 2 <= e.n - 1) ? qn + 2 : qn + 1;
                 double xa = 0.0;
                 bool move = false;
-                if (AP(q).x < (1. - 0.00001) * AP(ql).x + 0.00001 * AP(qr).x) {
-                    xa = (1. - 0.00001) * AP(ql).x + 0.00001 * AP(qr).x;
+                if (e.x[qn] < (1. - 0.00001) * e.x[ql] + 0.00001 * e.x[qr]) {
+                    xa = (1. - 0.00001) * e.x[ql] + 0.00001 * e.x[qr];
                     move = true;
-                } else if (AP(q).x > 0.00001 * AP(ql).x + (1. - 0.00001) * AP(qr).x) {
-                    xa = 0.00001 * AP(ql).x + (1. - 0.00001) * AP(qr).x;
+                } else if (e.x[qn] > 0.00001 * e.x[ql] + (1. - 0.00001) * e.x[qr]) {
+                    xa = 0.00001 * e.x[ql] + (1. - 0.00001) * e.x[qr];
                     move = true;
                 }
                 if (move) {
-                    AP(q).x = xa;
+                    e.x[qn] = xa;
                     e.xout = xa;
                     e.state = kAsEvalQ;
                     return kArmsEval;
                 }
                 e.state = kAsEvalQ;
-                y = AP(q).y;
+                y = e.y[qn];
             }
         }
         if (e.state == kAsEvalQ) {
-            const int q = e.q;
-            AP(q).y = y;
-            if (arms_meet(e, AP(q).pl)) return 2000;
-            if (arms_meet(e, AP(q).pr)) return 2000;
-            if (AP(AP(q).pl).pl >= 0 && arms_meet(e, AP(AP(AP(q).pl).pl).pl)) return 2000;
-            if (AP(AP(q).pr).pr >= 0 && arms_meet(e, AP(AP(AP(q).pr).pr).pr)) return 2000;
-            arms_cumulate(e);
+            e.y[e.q] = y;
+            if (arms_meet_all(e, lane)) return 2000;
+            arms_cumulate(e, lane);
         }
         // rejection test (BayesW_arms.cpp:529-535)
         if (!(e.yu >= e.ynew)) {
-            e.xout = AP(w).x;
+            e.xout = e.wx;
             return kArmsDone;
         }
         e.state = kAsSample;
     }
     // sample and squeeze (BayesW_arms.cpp:344-360, 429-503)
     if (++e.guard > 100000) return 2001;
-    const int err = arms_invert(e, arms_u(e), w);
+    const int err = arms_invert(e, arms_u(e), lane);
     if (err) return err;
-    e.yu = arms_logshift(arms_u(e) * AP(w).ey, e.ymax);
-    const int wl = AP(w).pl, wr = AP(w).pr;
-    if (AP(wl).pl >= 0 && AP(wr).pr >= 0) {
-        const int ql = AP(wl).f ? wl : AP(wl).pl;
-        const int qr = AP(wr).f ? wr : AP(wr).pr;
-        const double ysq = (AP(qr).y * (AP(w).x - AP(ql).x) + AP(ql).y * (AP(qr).x - AP(w).x)) / (AP(qr).x - AP(ql).x);
+    e.yu = arms_logshift(arms_u(e) * e.wey, e.ymax);
+    const int wq = e.wq, n = e.n;
+    if (wq - 2 >= 0 && wq + 1 <= n - 1) {
+        const int ql = ((wq - 1) & 1) ? wq - 1 : wq - 2;
+        const int qr = (wq & 1) ? wq : wq + 1;
+        const double ysq = (e.y[qr] * (e.wx - e.x[ql]) + e.y[ql] * (e.x[qr] - e.wx)) / (e.x[qr] - e.x[ql]);
         if (e.yu <= ysq) {
-            e.xout = AP(w).x;
+            e.xout = e.wx;
             return kArmsDone;
         }
     }
-    e.xout = AP(w).x;
+    e.xout = e.wx;
     e.state = kAsEvalW;
     return kArmsEval;
 }
 
-#undef AP
+// The same step behind a call, for the draws of the per-iteration scalars (one
+// copy of the sampler's code for them; the per-marker draws inline it).
+__device__ __noinline__ int arms_step_call(ArmsEnv& e, double y, int lane) { return arms_step(e, y, lane); }
 
 // One draw from log density f (inlined at the call site, where it may run
 // grid-wide sums: every thread of the grid takes the same path). 0 or an error.
-template <class F>
+template <bool kInline, class F>
 __device__ __forceinline__ int arms_draw(ArmsEnv& e, const double (&xinit)[4], double xl, double xr, uint64_t seed,
                                          uint32_t site, uint32_t iteration, uint32_t chain, F&& f, double* out)
 {
-    int st = arms_begin(e, xinit, xl, xr, seed, site, iteration, chain);
+    const int lane = threadIdx.x & 63;
+    int st = arms_begin(e, xinit, xl, xr, seed, site, iteration, chain, lane);
     if (st) return st;
     double y = 0.0;
-    while ((st = arms_step(e, y)) == kArmsEval) y = f(e.xout);
+    while ((st = kInline ? arms_step(e, y, lane) : arms_step_call(e, y, lane)) == kArmsEval) y = f(e.xout);
     if (st != kArmsDone) return st;
     *out = e.xout;
     return 0;
@@ -422,41 +461,32 @@ __device__ __forceinline__ double bw_integrand(double s, double alpha, double sq
     return exp(temp);
 }
 
-__device__ __forceinline__ double bw_gh(int nq, double sigma, double Ck, double sigma_b, double alpha, double sumf,
-                                        double mean, double sd, const BwSums& v)
+// Nodes and weights of the n-point adaptive rule in the order the reference
+// sums them (+x1, -x1, +x3, -x3, ...; the sixth node is -x3 for n >= 9), the
+// centre weight last.
+__device__ __forceinline__ void bw_gh_nodes(int nq, int n, double* x, double* w)
 {
-    const int r = (nq - 3) >> 1, h = (nq - 1) >> 1;
-    const double sq = sqrt(2 * Ck * sigma_b);
-    const double msr = mean / sd;
-    double temp = 0.0;
-    for (int i = 0; i < h; ++i) {
-        const double xp = kGhX[r][i];
-        const double xm = (nq >= 9 && i == 2) ? -kGhX[r][1] : -xp;   // x6 = -x3 for n >= 9
-        const double a = kGhW[r][i] * bw_integrand(sigma * xp, alpha, sq, sumf, msr, sd, v);
-        temp = (i == 0) ? a : temp + a;
-        temp = temp + kGhW[r][i] * bw_integrand(sigma * xm, alpha, sq, sumf, msr, sd, v);
-    }
-    temp = temp + kGhWc[r];
-    return sigma * temp;
+    const int r = (nq - 3) >> 1, i = n >> 1;
+    const double xp = kGhX[r][i];
+    *x = (n & 1) ? ((nq >= 9 && n == 5) ? -kGhX[r][1] : -xp) : xp;
+    *w = kGhW[r][i];
 }
 
-// class of processColumn (bayeswbase.cpp:624-688): -1 when K < 2 (never)
-template <int R>
-__device__ __forceinline__ int bw_select(const BwParams& p, const BwShared<R>& L, int k, const BwSums& v)
+// exp_sum and the slab scale sigma of class i (bayeswbase.cpp:624-640)
+__device__ __forceinline__ double bw_sigma(double alpha, double sigma_b, double Ck, double mean, double sd,
+                                           const BwSums& v)
 {
-    const int K = (int)p.K;
-    const double mean = L.mean[k], sd = L.sd[k];
-    double ml[kMaxK];
-    ml[0] = L.pi[0] * kSqrtPi;
     const double exp_sum = (v.vi_1 * (1 - 2 * mean) + 4 * (1 - mean) * v.vi_2 + v.vi_sum * mean * mean) / (sd * sd);
-    for (int i = 0; i < K - 1; ++i) {
-        const double sigma = 1.0 / sqrt(1 + L.alpha * L.alpha * L.sigma_b * L.cls[i] * exp_sum);
-        ml[i + 1] = L.pi[i + 1] * bw_gh((int)p.nq, sigma, L.cls[i], L.sigma_b, L.alpha, L.sumf[k], mean, sd, v);
-    }
+    return 1.0 / sqrt(1 + alpha * alpha * sigma_b * Ck * exp_sum);
+}
+
+// class picked by u from the class likelihoods ml[0..K) (bayeswbase.cpp:641-688;
+// the last class gets acum = 1)
+__device__ __forceinline__ int bw_pick(int K, const double* ml, double u)
+{
     double ml_sum = 0.0;
     for (int i = 0; i < K; ++i) ml_sum += ml[i];
     double acum = ml[0] / ml_sum;
-    const double u = L.pu[k];
     for (int c = 0; c < K; ++c) {
         if (u <= acum) return c;
         if ((c + 1) == (K - 1)) acum = 1;
@@ -491,11 +521,7 @@ __device__ __forceinline__ bool bw_exchange(const BwParams& p, BwShared<R>& L, u
         if ((uint32_t)v % p.P != w) continue;
         if ((cnt++ % kWaves) != wave) continue;
         double acc = 0.0;
-        for (uint32_t w2 = (uint32_t)lane; w2 < p.P; w2 += 64u) {
-            double x;
-            if (!wait_pair(bw_slot1(p, rnd, v, w2), rnd, p.ctl, &x)) { ok = false; break; }
-            acc += x;
-        }
+        ok = wait_sum_lanes([&](uint32_t w2) { return bw_slot1(p, rnd, v, w2); }, p.P, rnd, p.ctl, lane, &acc);
         if (__any(!ok)) { ok = false; break; }
         acc = wave_sum(acc);
         if (lane == 0) publish_pair(bw_slot2(p, rnd, v), rnd, acc);
@@ -528,6 +554,19 @@ __device__ __forceinline__ void bw_delta(double mean, double sd, double beta, do
     dv[3] = -meanAdjustment;                   // code 3: genotype 0
 }
 
+constexpr int kBwPhases = 8;   // block load, round sums, exchange, decisions, beta draws,
+                               // residual update, iteration scalars, in-model shifts
+
+// Phase timer of thread 0 (s_memrealtime: 100 MHz constant clock).
+#define BW_TICK(slot)                                                            \
+    do {                                                                         \
+        if (tid == 0) {                                                          \
+            const uint64_t t_now_ = __builtin_amdgcn_s_memrealtime();           \
+            tk[(slot)] += t_now_ - t_last;                                       \
+            t_last = t_now_;                                                     \
+        }                                                                        \
+    } while (0)
+
 template <int R>
 __global__ void __launch_bounds__(kThreads)
 bayesw_sweep_kernel(BwParams p)
@@ -559,6 +598,8 @@ bayesw_sweep_kernel(BwParams p)
         L.d = p.scal[kBwD];
         L.betasq = 0.0;
     }
+    if (tid < (int)p.nq - 1) bw_gh_nodes((int)p.nq, tid, &L.ghx[tid], &L.ghw[tid]);
+    if (tid == 0) L.ghwc = kGhWc[(p.nq - 3) >> 1];
     if (tid < kMaxK) {
         L.cls[tid] = (tid < K - 1) ? p.cls[tid] : 0.0;
         L.pi[tid] = (tid < K) ? p.scal[kBwPi + tid] : 0.0;
@@ -566,6 +607,9 @@ bayesw_sweep_kernel(BwParams p)
     }
     uint32_t rnd = __hip_atomic_load(p.ctl + kCtlRound, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     unsigned long long n_rounds = 0, n_changes = 0;
+    uint64_t tk[kBwPhases] = {0, 0, 0, 0, 0, 0, 0, 0};
+    uint64_t t_last = __builtin_amdgcn_s_memrealtime();
+    const uint64_t clk0 = __builtin_amdgcn_s_memtime(), rt0 = t_last;
     __syncthreads();
 
     // one value per workgroup: every wave holds the whole slice, so wave 0's sum
@@ -630,7 +674,7 @@ bayesw_sweep_kernel(BwParams p)
                 const double xinit[4] = {0.995 * mu, mu, 1.005 * mu, 1.01 * mu};
                 double xs = mu;
                 // mu_dens (bayeswbase.cpp:66-77), sigma_mu = 100
-                const int err = arms_draw(L.env[wave], xinit, 2.0, 5.0, p.seed, kSiteMu, iteration, p.chain,
+                const int err = arms_draw<false>(L.env[wave], xinit, 2.0, 5.0, p.seed, kSiteMu, iteration, p.chain,
                                           [&](double x) { return -a * x * d - exp(-a * x) * smu - x * x / (2 * 100.0); },
                                           &xs);
                 if (err) {
@@ -653,51 +697,99 @@ bayesw_sweep_kernel(BwParams p)
         }
         refresh_vi();
         __syncthreads();
+        BW_TICK(6);
 
-        // ---- the sweep (AnalysisGraph::exec over the order)
-        for (uint32_t blk = 0; blk < nblocks; ++blk) {
-            const uint32_t b0 = blk * kBwB;
-            const int nb = (int)min((uint32_t)kBwB, p.M - b0);
-            // block load: codes of the slice and per-marker constants
-            if (tid < nb) {
-                const uint32_t j = order[b0 + tid];
-                L.mk[tid] = j;
-                L.bold[tid] = bits_as_double(ld_agent((const uint64_t*)(p.beta + j)));
-                L.mean[tid] = p.stats[4 * (size_t)j];
-                L.sd[tid] = p.stats[4 * (size_t)j + 1];
-                L.sumf[tid] = p.sumf[j];
+        // ---- the sweep (AnalysisGraph::exec over the order). Block b + 1's code
+        // words and per-marker constants are loaded into registers while block b
+        // runs and written to LDS when it ends; its marker ids come from LDS,
+        // loaded one block earlier still.
+        constexpr int kWords = kBwB * R * 32 / kThreads;
+        uint32_t nw[kWords];
+        uint32_t nmk = 0, noj = 0;
+        double nbold = 0.0, nmean = 0.0, nsd = 0.0, nsumf = 0.0, npu = 0.0;
+        auto issue = [&](uint32_t nblk) {
+            const uint32_t c0 = nblk * kBwB;
+            const int cnb = (int)min((uint32_t)kBwB, p.M - c0);
+            const uint32_t* ids = L.ord[nblk & 1u];
+            if (tid < cnb) {
+                const uint32_t j = ids[tid];
+                nmk = j;
+                nbold = bits_as_double(ld_agent((const uint64_t*)(p.beta + j)));
+                nmean = p.stats[4 * (size_t)j];
+                nsd = p.stats[4 * (size_t)j + 1];
+                nsumf = p.sumf[j];
                 if (p.mode == kModeChain) {
                     const U4 x = draw_block(p.seed, j, iteration, kStreamMarker, p.chain);
-                    L.pu[tid] = unit_u53(x.x[0], x.x[1]);
+                    npu = unit_u53(x.x[0], x.x[1]);
                 } else {
-                    L.pu[tid] = p.u[j];
+                    npu = p.u[j];
                 }
             }
-            __syncthreads();
-            for (uint32_t t = tid; t < (uint32_t)nb * R * 32u; t += kThreads) {
+#pragma unroll
+            for (int q = 0; q < kWords; ++q) {
+                const uint32_t t = (uint32_t)tid + (uint32_t)q * kThreads;
                 const uint32_t k = t / (R * 32u), rem = t % (R * 32u);
                 const uint64_t byte_off = (uint64_t)(lo >> 2) + 4ull * rem;
                 uint32_t word = 0xFFFFFFFFu;
-                if (byte_off < p.col_bytes) word = p.codes[(size_t)L.mk[k] * p.col_words + (lo >> 4) + rem];
-                (&L.codes[0][0][0])[t] = word;
+                if ((int)k < cnb && byte_off < p.col_bytes)
+                    word = p.codes[(size_t)ids[k] * p.col_words + (lo >> 4) + rem];
+                nw[q] = word;
             }
-            __syncthreads();
+        };
+        auto commit = [&]() {
+            if (tid < kBwB) {
+                L.mk[tid] = nmk;
+                L.bold[tid] = nbold;
+                L.mean[tid] = nmean;
+                L.sd[tid] = nsd;
+                L.sumf[tid] = nsumf;
+                L.pu[tid] = npu;
+            }
+#pragma unroll
+            for (int q = 0; q < kWords; ++q) (&L.codes[0][0][0])[tid + q * kThreads] = nw[q];
+        };
+        if (tid < 2 * kBwB) {
+            const uint32_t idx = (uint32_t)tid;
+            L.ord[idx / kBwB][idx % kBwB] = idx < p.M ? order[idx] : 0u;
+        }
+        __syncthreads();
+        issue(0);
+        commit();
+        __syncthreads();
+        BW_TICK(0);
+        for (uint32_t blk = 0; blk < nblocks; ++blk) {
+            const uint32_t b0 = blk * kBwB;
+            const int nb = (int)min((uint32_t)kBwB, p.M - b0);
+            if (blk + 1 < nblocks) issue(blk + 1);
+            if (blk + 2 < nblocks && tid < kBwB) {
+                const uint32_t idx = (blk + 2) * kBwB + (uint32_t)tid;
+                noj = idx < p.M ? order[idx] : 0u;
+            }
 
             int pos = 0;
             while (pos < nb) {
-                const bool breaker = L.bold[pos] != 0.0;
+                // h: the first slot from pos whose marker is in the model (beta_old != 0).
+                // Its step starts from eps + x beta_old. When h == pos the round is
+                // that marker's own; otherwise the round publishes the sums of slots
+                // [pos, h) and, speculatively, h's sums after the shift, used when no
+                // slot before h changes eps.
+                const unsigned long long hm = __ballot(pos + lane < nb && L.bold[pos + lane] != 0.0);
+                const int h = __builtin_amdgcn_readfirstlane(hm ? pos + (int)__ffsll((long long)hm) - 1 : nb);
+                const bool breaker = h == pos;
+                const bool spec = !breaker && h < nb;
+                const int V0 = breaker ? 0 : 1 + 2 * (h - pos);   // first value of the shifted sums
                 ++rnd;
                 ++n_rounds;
                 int V;
                 if (!breaker) {
-                    // vi_sum of the slice and (vi_1, vi_2) of slots pos..nb-1: wave q
+                    // vi_sum of the slice and (vi_1, vi_2) of slots pos..h-1: wave q
                     // takes slots pos + q + 4 i, values 2 i (genotype 1) and 2 i + 1
                     double part[16];
 #pragma unroll
                     for (int i = 0; i < 8; ++i) {
                         const int k = pos + wave + kWaves * i;
                         double s1 = 0.0, s2 = 0.0;
-                        if (k < nb) {
+                        if (k < h) {
 #pragma unroll
                             for (int r = 0; r < R; ++r) {
                                 const uint32_t bits = L.codes[k][r][lane >> 1] >> (16 * (lane & 1));
@@ -715,7 +807,7 @@ bayesw_sweep_kernel(BwParams p)
                     const double tot = transpose_reduce16(part, lane);
                     const int vv = (lane >> 2) & 15;
                     const int k = pos + wave + kWaves * (vv >> 1);
-                    if ((lane & 3) == 0 && k < nb)
+                    if ((lane & 3) == 0 && k < h)
                         publish_pair(bw_slot1(p, rnd, 1 + 2 * (k - pos) + (vv & 1), w), rnd, tot);
                     double sv = 0.0;
 #pragma unroll
@@ -724,10 +816,43 @@ bayesw_sweep_kernel(BwParams p)
                         for (int e = 0; e < 8; ++e) sv += vi[r][e];
                     sv = wave_sum(sv);
                     if (tid == 0) publish_pair(bw_slot1(p, rnd, 0, w), rnd, sv);
-                    V = 1 + 2 * (nb - pos);
+                    V = V0;
+                    if (spec) {
+                        // h's sums of vi' = exp(alpha (eps + delta_h) - EulerGamma) (wave 3)
+                        if (wave == kWaves - 1) {
+                            double dv[4];
+                            bw_delta(L.mean[h], L.sd[h], L.bold[h], dv);
+                            const double a = L.alpha;
+                            double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll
+                            for (int r = 0; r < R; ++r) {
+                                const uint32_t bits = L.codes[h][r][lane >> 1] >> (16 * (lane & 1));
+#pragma unroll
+                                for (int e = 0; e < 8; ++e) {
+                                    const uint32_t idx = lo + (uint32_t)r * kRun + 8u * (uint32_t)lane + (uint32_t)e;
+                                    const uint32_t c = (bits >> (2 * e)) & 3u;
+                                    const double es = eps[r][e] + wt_of(c, dv[0], dv[1], dv[2], dv[3]);
+                                    const double x = (idx < hi) ? exp(a * es - kEuler) : 0.0;
+                                    s0 += x;
+                                    s1 += (c == 2u) ? x : 0.0;
+                                    s2 += (c == 0u) ? x : 0.0;
+                                }
+                            }
+                            s0 = wave_sum(s0);
+                            s1 = wave_sum(s1);
+                            s2 = wave_sum(s2);
+                            if (lane == 0) {
+                                publish_pair(bw_slot1(p, rnd, V0, w), rnd, s0);
+                                publish_pair(bw_slot1(p, rnd, V0 + 1, w), rnd, s1);
+                                publish_pair(bw_slot1(p, rnd, V0 + 2, w), rnd, s2);
+                            }
+                        }
+                        V = V0 + 3;
+                    }
                 } else {
                     // a marker in the model: eps += x beta_old, vi re-formed, then its sums
                     apply(pos, L.bold[pos], 1.0);
+                    BW_TICK(7);
                     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
 #pragma unroll
                     for (int r = 0; r < R; ++r) {
@@ -750,60 +875,103 @@ bayesw_sweep_kernel(BwParams p)
                     }
                     V = 3;
                 }
+                BW_TICK(1);
                 if (!bw_exchange<R>(p, L, rnd, V, w, wave, lane)) goto done;
                 const double* xv = L.xv[rnd & 1u];
+                BW_TICK(2);
 
-                // ---- decisions (wave 0, lane = slot)
-                if (wave == 0) {
-                    const int k = pos + lane;
-                    // a marker in the model stops a normal round: its step starts from eps + x beta_old
-                    const bool held = !breaker && k < nb && L.bold[k] != 0.0;
-                    const bool active = breaker ? lane == 0 : (k < nb && !held);
-                    int sel = -1;
-                    if (active) {
-                        const int vi1 = breaker ? 1 : 1 + 2 * lane;
-                        const double s1 = xv[vi1], s2 = xv[vi1 + 1];
-                        const BwSums v{xv[0], xv[0] - s1 - s2, s1, s2};
-                        sel = bw_select<R>(p, L, k, v);
+                // ---- decisions. Class likelihoods of the slots [pos, h] (or h alone in
+                // its own round): first the weighted integrand terms (slot, slab class,
+                // node) over all threads, then each (slot, class) sum in the reference's
+                // order, then the pick (wave 0, lane = slot). Slot h always stops the
+                // round: its step changes eps whatever its class.
+                // Sums of slot sl: (vi_sum, vi_1, vi_2) at xv[0], xv[1 + 2 sl], xv[2 + 2 sl],
+                // or the shifted sums at xv[V0 ..] for the speculative slot h.
+                {
+                    const int ncls = K - 1, nn = (int)p.nq - 1, per = ncls * nn;
+                    const int ns = breaker ? 1 : (spec ? h + 1 - pos : nb - pos);
+                    for (int t = tid; t < ns * per; t += kThreads) {
+                        const int sl = t / per, rem = t - sl * per, i = rem / nn, n = rem - i * nn;
+                        const int k = pos + sl;
+                        const bool sh = spec && k == h;
+                        const double vs = sh ? xv[V0] : xv[0];
+                        const double s1 = sh ? xv[V0 + 1] : xv[1 + 2 * sl], s2 = sh ? xv[V0 + 2] : xv[2 + 2 * sl];
+                        const BwSums v{vs, vs - s1 - s2, s1, s2};
+                        const double mean = L.mean[k], sd = L.sd[k], Ck = L.cls[i];
+                        const double sigma = bw_sigma(L.alpha, L.sigma_b, Ck, mean, sd, v);
+                        const double sq = sqrt(2 * Ck * L.sigma_b);
+                        L.gv[sl][i][n] = L.ghw[n] * bw_integrand(sigma * L.ghx[n], L.alpha, sq, L.sumf[k], mean / sd,
+                                                                  sd, v);
                     }
-                    const bool stop = held || (active && (breaker || sel > 0));
-                    const unsigned long long sm = __ballot(stop);
-                    const int c = sm ? pos + (int)__ffsll((long long)sm) - 1 : nb;
-                    const int cs = __shfl(sel, (c - pos) & 63, 64);
-                    // slots before c keep beta = 0 in class 0
-                    if (k < c && k < nb) {
-                        atomicAdd(&L.v[0], 1.0);
-                        if (w == 0) {
-                            st_agent((uint64_t*)(p.beta + L.mk[k]), double_as_bits(0.0));
-                            p.comp[L.mk[k]] = 0;
+                    __syncthreads();
+                    for (int t = tid; t < ns * ncls; t += kThreads) {
+                        const int sl = t / ncls, i = t - sl * ncls;
+                        const int k = pos + sl;
+                        const bool sh = spec && k == h;
+                        const double vs = sh ? xv[V0] : xv[0];
+                        const double s1 = sh ? xv[V0 + 1] : xv[1 + 2 * sl], s2 = sh ? xv[V0 + 2] : xv[2 + 2 * sl];
+                        const BwSums v{vs, vs - s1 - s2, s1, s2};
+                        const double sigma = bw_sigma(L.alpha, L.sigma_b, L.cls[i], L.mean[k], L.sd[k], v);
+                        double temp = L.gv[sl][i][0];
+                        for (int n = 1; n < nn; ++n) temp = temp + L.gv[sl][i][n];
+                        temp = temp + L.ghwc;
+                        L.ml[sl][i + 1] = L.pi[i + 1] * (sigma * temp);
+                    }
+                    __syncthreads();
+                    if (wave == 0) {
+                        const int k = pos + lane;
+                        const bool active = lane < ns;
+                        int sel = -1;
+                        if (active) {
+                            L.ml[lane][0] = L.pi[0] * kSqrtPi;
+                            sel = bw_pick(K, L.ml[lane], L.pu[k]);
                         }
-                    }
-                    if (lane == 0) {
-                        L.stop = c;
-                        L.sel = (c < nb) ? cs : -1;
+                        const bool stop = active && (breaker || k == h || sel > 0);
+                        const unsigned long long sm = __ballot(stop);
+                        const int c = sm ? pos + (int)__ffsll((long long)sm) - 1 : nb;
+                        const int cs = __shfl(sel, (c - pos) & 63, 64);
+                        // slots before c keep beta = 0 in class 0
+                        if (k < c && active) {
+                            atomicAdd(&L.v[0], 1.0);
+                            if (w == 0) {
+                                st_agent((uint64_t*)(p.beta + L.mk[k]), double_as_bits(0.0));
+                                p.comp[L.mk[k]] = 0;
+                            }
+                        }
+                        if (lane == 0) {
+                            L.stop = c;
+                            L.sel = (c < nb) ? cs : -1;
+                        }
                     }
                 }
                 __syncthreads();
+                BW_TICK(3);
                 {
-                    const int c = L.stop, sel = L.sel;
+                    const int c = __builtin_amdgcn_readfirstlane(L.stop);
+                    const int sel = __builtin_amdgcn_readfirstlane(L.sel);
                     if (c >= nb) {
                         pos = nb;
-                    } else if (!breaker && L.bold[c] != 0.0) {
-                        pos = c;                       // next round: the marker's own round
                     } else {
+                        // slot c's sums; a speculated in-model marker first takes its shift
+                        const bool sh = spec && c == h;
+                        if (sh) {
+                            apply(c, L.bold[c], 1.0);
+                            BW_TICK(7);
+                        }
+                        const int i1 = sh ? V0 + 1 : 1 + 2 * (c - pos);
                         // draw the slot's beta: 0 for class 0, else ARMS on beta_dens
                         double bnew = 0.0;
                         if (sel > 0) {
-                            const double s1 = breaker ? xv[1] : xv[1 + 2 * (c - pos)];
-                            const double s2 = breaker ? xv[2] : xv[2 + 2 * (c - pos)];
-                            const double vs = xv[0];
+                            const double s1 = xv[i1];
+                            const double s2 = xv[i1 + 1];
+                            const double vs = xv[sh ? V0 : 0];
                             const double v0 = vs - s1 - s2;
                             const double a = L.alpha, sb = L.sigma_b, Ck = L.cls[sel - 1];
                             const double sf = L.sumf[c], msr = L.mean[c] / L.sd[c], sd = L.sd[c];
                             const double bo = L.bold[c];
                             const double safe = 2 * sqrt(sb * Ck);
                             const double xinit[4] = {bo - safe / 10, bo, bo + safe / 20, bo + safe / 10};
-                            const int err = arms_draw(
+                            const int err = arms_draw<true>(
                                 L.env[wave], xinit, bo - safe, bo + safe, p.seed, L.mk[c], iteration, p.chain,
                                 [&](double x) {
                                     // beta_dens (sparsebayesw.cpp:26-37)
@@ -817,6 +985,7 @@ bayesw_sweep_kernel(BwParams p)
                                 __syncthreads();
                                 goto done;
                             }
+                            BW_TICK(4);
                         }
                         // skip iff beta_old == 0 && beta_new == 0 (bayeswbase.cpp:691-696)
                         if (!(L.bold[c] == 0.0 && bnew == 0.0)) {
@@ -824,6 +993,7 @@ bayesw_sweep_kernel(BwParams p)
                             ++n_changes;
                         }
                         __syncthreads();
+                        BW_TICK(5);
                         if (tid == 0) {
                             L.v[sel] += 1.0;
                             L.betasq += bnew * bnew;
@@ -839,6 +1009,10 @@ bayesw_sweep_kernel(BwParams p)
             }
             if (w == 0) __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
+            if (blk + 1 < nblocks) commit();
+            if (blk + 2 < nblocks && tid < kBwB) L.ord[blk & 1u][tid] = noj;   // block blk + 2 has blk's parity
+            __syncthreads();
+            BW_TICK(0);
         }
 
         if (p.mode == kModeChain) {
@@ -858,7 +1032,7 @@ bayesw_sweep_kernel(BwParams p)
             const double xinit[4] = {a0 * 0.5, a0, a0 * 1.15, a0 * 1.5};
             double xs = a0;
             // alpha_dens (bayeswbase.cpp:109-119), alpha_0 = kappa_0 = 0.01
-            const int err = arms_draw(
+            const int err = arms_draw<false>(
                 L.env[wave], xinit, 0.0, 20.0, p.seed, kSiteAlpha, iteration, p.chain,
                 [&](double x) {
                     double s = 0.0;
@@ -901,6 +1075,7 @@ bayesw_sweep_kernel(BwParams p)
                 }
             }
             __syncthreads();
+            BW_TICK(6);
         }
     }
 
@@ -928,6 +1103,11 @@ done:
             __hip_atomic_store(p.ctl + kCtlRound, rnd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             p.counters[0] = n_rounds;
             p.counters[1] = n_changes;
+            for (int q = 0; q < 6; ++q) p.counters[2 + q] = tk[q];
+            for (int q = 6; q < kBwPhases; ++q) p.counters[4 + q] = tk[q];
+            // shader clock and 100 MHz reference clock over the launch: the core clock
+            p.counters[12] = __builtin_amdgcn_s_memtime() - clk0;
+            p.counters[13] = __builtin_amdgcn_s_memrealtime() - rt0;
         }
     } else if (w == 0 && tid == 0) {
         p.counters[9] = (unsigned long long)L.arms_err;

@@ -45,6 +45,7 @@ constexpr int kMaxRuns = 4;    // slice <= 512 * kMaxRuns individuals
 constexpr uint32_t kRun = 512; // individuals per lane-run pass: 64 lanes x 8
 constexpr uint32_t kSliceAlign = 16;
 constexpr uint64_t kSpinLimit = 1ull << 22;
+constexpr uint32_t kMaxP = 256;   // workgroups of a launch (the exchange reducers read <= 4 per lane)
 constexpr int kIdxCap = 32;    // prefetched ragged indices per thread per block
 
 enum : int { kModeSweep = 0, kModeChain = 1 };
@@ -170,18 +171,26 @@ __device__ __forceinline__ double wave_sum(double x)
 // over their quarter, then a butterfly over xor 2, 1 completes it. 17 shuffles
 // instead of 16 x 6; the addition tree is fixed, so the result is the same in
 // every workgroup.
+// One halving stage with compile-time extents, so every array index is static
+// (a runtime index would turn each access into a 16-way select chain).
+template <int kHalf, int kBit>
+__device__ __forceinline__ void transpose_stage(double (&v)[16], int lane)
+{
+    const bool up = (lane & kBit) != 0;
+#pragma unroll
+    for (int i = 0; i < kHalf; ++i) {
+        const double send = up ? v[i] : v[i + kHalf];
+        const double recv = __shfl_xor(send, kBit, 64);
+        v[i] = (up ? v[i + kHalf] : v[i]) + recv;
+    }
+}
+
 __device__ __forceinline__ double transpose_reduce16(double (&v)[16], int lane)
 {
-#pragma unroll
-    for (int half = 8, bit = 32; half >= 1; half >>= 1, bit >>= 1) {
-        const bool up = (lane & bit) != 0;
-#pragma unroll
-        for (int i = 0; i < half; ++i) {
-            const double send = up ? v[i] : v[i + half];
-            const double recv = __shfl_xor(send, bit, 64);
-            v[i] = (up ? v[i + half] : v[i]) + recv;
-        }
-    }
+    transpose_stage<8, 32>(v, lane);
+    transpose_stage<4, 16>(v, lane);
+    transpose_stage<2, 8>(v, lane);
+    transpose_stage<1, 4>(v, lane);
     double x = v[0];
     x += __shfl_xor(x, 2, 64);
     x += __shfl_xor(x, 1, 64);
@@ -226,6 +235,67 @@ __device__ __forceinline__ bool wait_pair(const uint64_t* s, uint32_t rnd, const
     return true;
 }
 
+// Waits for the tagged pairs of workgroups w2 = first, first + stride, ... (< P,
+// at most kPer of them) and adds them to *acc in that order (acc = ((acc +
+// x_first) + x_first+stride) + ...). All of a thread's loads are in flight at
+// once; only pairs whose tag is stale are polled again. `slot(w2)` gives the
+// pair's address. False on timeout or abort.
+template <int kPer, class SlotFn>
+__device__ __forceinline__ bool wait_sum_strided(SlotFn slot, uint32_t first, uint32_t stride, uint32_t P,
+                                                 uint32_t rnd, const uint32_t* ctl, double* acc_io)
+{
+    uint64_t h[kPer], l[kPer];
+    uint32_t live = 0;
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+        const uint32_t w2 = first + stride * q;
+        h[q] = 0;
+        l[q] = 0;
+        if (w2 < P) {
+            const uint64_t* s = slot(w2);
+            h[q] = ld_agent(s);
+            l[q] = ld_agent(s + 1);
+            live |= 1u << q;
+        }
+    }
+    uint64_t spins = 0;
+    for (;;) {
+        uint32_t stale = 0;
+#pragma unroll
+        for (int q = 0; q < kPer; ++q)
+            if (((live >> q) & 1u) && ((uint32_t)(h[q] >> 32) != rnd || (uint32_t)(l[q] >> 32) != rnd))
+                stale |= 1u << q;
+        if (!stale) break;
+        if (++spins > kSpinLimit) return false;
+        if ((spins & 255u) == 0 &&
+            __hip_atomic_load(ctl + kCtlAbort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            return false;
+        __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+        for (int q = 0; q < kPer; ++q)
+            if ((stale >> q) & 1u) {
+                const uint64_t* s = slot(first + stride * q);
+                h[q] = ld_agent(s);
+                l[q] = ld_agent(s + 1);
+            }
+    }
+    double acc = *acc_io;
+#pragma unroll
+    for (int q = 0; q < kPer; ++q)
+        if ((live >> q) & 1u) acc += bits_as_double((h[q] << 32) | (l[q] & 0xFFFFFFFFull));
+    *acc_io = acc;
+    return true;
+}
+
+// One lane's share of a P-workgroup reduction: workgroups lane, lane + 64, ...
+template <class SlotFn>
+__device__ __forceinline__ bool wait_sum_lanes(SlotFn slot, uint32_t P, uint32_t rnd, const uint32_t* ctl, int lane,
+                                               double* out)
+{
+    *out = 0.0;
+    return wait_sum_strided<kMaxP / 64>(slot, (uint32_t)lane, 64u, P, rnd, ctl, out);
+}
+
 template <typename Params>
 __device__ __forceinline__ void raise_abort(const Params& p, uint32_t where)
 {
@@ -256,11 +326,7 @@ __device__ __forceinline__ bool exchange(const SweepParams& p, Shared<R>& L, uin
         if ((uint32_t)k % p.P != w) continue;
         if ((cnt++ % kWaves) != wave) continue;
         double acc = 0.0;
-        for (uint32_t w2 = (uint32_t)lane; w2 < p.P; w2 += 64u) {
-            double x;
-            if (!wait_pair(slot1(p, rnd, k, w2), rnd, p.ctl, &x)) { ok = false; break; }
-            acc += x;
-        }
+        ok = wait_sum_lanes([&](uint32_t w2) { return slot1(p, rnd, k, w2); }, p.P, rnd, p.ctl, lane, &acc);
         if (__any(!ok)) { ok = false; break; }
         acc = wave_sum(acc);
         if (lane == 0) publish_pair(slot2(p, rnd, k), rnd, acc);

@@ -11,12 +11,16 @@ persistent device kernel with genotypes and chain state resident in HBM.
 `--config c3` is the batched case: T = 64 traits (one phenotype and chain each,
 h2 ~ U(0.1, 0.8), causal fraction ~ U(0.1%, 2%)) over one dense 2-bit matrix of
 N = 50,000 x M = 50,000, chain length 100, all traits in one persistent launch.
+`--config c5` is BayesW (Weibull survival): N = 100,000, M = 50,000 ragged,
+0.5% causal, log T = 4 + g + (log E + EulerGamma) / 3, 10% right-censored,
+mixture variances 0.001, 0.01, 0.1, 7 Gauss-Hermite points, chain length 100.
 
 Prints one JSON line: marker-updates/s (whole job), the roofline of the sweep
 kernel measured with HIP events on its stream, and the CPU reference sweep
 (the FP64 oracle built with the reference's Release flags) timed on a bounded
 sample of the same workload on this host. The default line also carries a
-"batched" object: the c3 workload measured the same way (batched vs plain).
+"batched" object (the c3 workload measured the same way: batched vs plain) and
+a "bayesw" object (the c5 workload).
 """
 from __future__ import annotations
 
@@ -49,6 +53,9 @@ CONFIGS = {
                         "matrix, N=50000 M=50000, shared marker order",
                n=50_000, m=50_000, layout="dense", maf=(0.01, 0.5), traits=64, causal=(0.001, 0.02),
                h2=(0.1, 0.8), chain=100),
+    "c5": dict(workload="BayesW Weibull survival, ragged layout, N=100000 M=50000, single chain",
+               n=100_000, m=50_000, layout="ragged", maf=(0.005, 0.5), causal=0.005, h2=0.5, alpha=3.0,
+               mu=4.0, censor=0.1, C=(0.001, 0.01, 0.1), quad_points=7, chain=100),
 }
 
 
@@ -221,6 +228,171 @@ def cpu_baseline_batched(cfg, data, seed, budget_s, workers):
                         f"{wall:.1f}s wall"))
 
 
+def make_bw_data(cfg, seed, threads):
+    """Genotypes + Weibull survival: log T = mu + g + (log E + EulerGamma) / alpha,
+    E ~ Exp(1), var(g) / (var(g) + pi^2 / (6 alpha^2)) = h2; a uniform fraction
+    `censor` of individuals is right-censored at C ~ U(0, T) (fail = 0)."""
+    import bayesrrcmd_amd as brr
+    n, m = cfg["n"], cfg["m"]
+    t0 = time.time()
+    bed, _ = brr.synth_bed(seed, n, m, cfg["maf"][0], cfg["maf"][1], 0.0, threads=threads)
+    log(f"genotypes {m} x {n} generated in {time.time() - t0:.1f}s")
+    rng = np.random.default_rng(seed + 5)
+    mc = max(1, int(round(cfg["causal"] * m)))
+    causal = np.sort(rng.choice(m, mc, replace=False))
+    beta = np.zeros(m)
+    beta[causal] = rng.normal(0.0, 1.0, mc)
+    lut_c = brr.build_dense(bed[causal], n, threads=threads)
+    g = brr.synth_xb(bed[causal], n, lut_c, beta[causal], threads=threads)
+    ve = np.pi ** 2 / (6 * cfg["alpha"] ** 2)
+    g *= np.sqrt(cfg["h2"] / (1 - cfg["h2"]) * ve / g.var())
+    logt = cfg["mu"] + g + (np.log(rng.exponential(1.0, n)) + 0.5772156649015329) / cfg["alpha"]
+    fail = np.ones(n)
+    cens = rng.choice(n, int(round(cfg["censor"] * n)), replace=False)
+    fail[cens] = 0.0
+    logt[cens] += np.log(rng.uniform(0.0, 1.0, cens.size))
+    y = logt.astype(np.float32).astype(np.float64)
+    t0 = time.time()
+    arena = brr.build_ragged(bed, n, threads=threads)
+    log(f"ragged lists built in {time.time() - t0:.1f}s ({arena.nbytes() / 1e9:.2f} GB)")
+    return dict(bed=bed, y=y, fail=fail, ragged=arena)
+
+
+def cpu_baseline_bw(cfg, data, seed, budget_s):
+    """Reference-style BayesW chain on the host: the FP64 oracle built with -O3
+    -ffast-math -funroll-loops, one thread, over the first markers of the same
+    workload (sampleMu, sweep, sampleAlpha, sigma_b / pi per iteration)."""
+    import ctypes as C
+    sys.path.insert(0, str(ROOT / "tests"))
+    import _oracle as orc
+    L = orc.lib(fast=True)
+    n = cfg["n"]
+    m_s = min(cfg["m"], 1000)
+    r = data["ragged"]
+
+    class Sub:
+        pass
+    sub = Sub()
+    sub.ones_off = np.ascontiguousarray(r.ones_off[: m_s + 1])
+    sub.twos_off = np.ascontiguousarray(r.twos_off[: m_s + 1])
+    sub.miss_off = np.ascontiguousarray(r.miss_off[: m_s + 1])
+    sub.ones, sub.twos, sub.miss = r.ones, r.twos, r.miss
+    sub.stats = np.ascontiguousarray(r.stats[:m_s])
+    w = orc.BayesWProblem(n, m_s, sub, data["fail"], K=4, C=tuple(cfg["C"]), quad_points=cfg["quad_points"])
+    y = np.ascontiguousarray(data["y"])
+    fail = w.fail
+    L.orc_bw_init(C.byref(w.data), C.byref(w.model), C.byref(w.state), orc.p(y, C.c_double),
+                  orc.p(fail, C.c_double))
+    order = np.arange(m_s, dtype=np.uint32)
+    u, z = np.zeros(m_s), np.zeros(m_s)
+    iters, t_total = 0, 0.0
+    while t_total < budget_s and iters < cfg["chain"]:
+        t0 = time.perf_counter()
+        assert L.orc_bw_sample_mu(C.byref(w.data), C.byref(w.state), seed, iters, 0) == 0
+        w.vi[:] = np.exp(w.state.alpha * w.eps - 0.577215664901532)
+        L.orc_shuffle_philox(seed, iters, 0, m_s, orc.p(order, C.c_uint32))
+        w.v[:] = 1.0
+        L.orc_fill_marker_uz(seed, iters, 0, m_s, orc.p(u, C.c_double), orc.p(z, C.c_double))
+        assert L.orc_bw_sweep(C.byref(w.data), C.byref(w.model), C.byref(w.state), orc.p(order, C.c_uint32),
+                              orc.p(u, C.c_double), seed, iters, 0) == 0
+        assert L.orc_bw_sample_alpha(C.byref(w.data), C.byref(w.state), orc.p(fail, C.c_double), seed, iters,
+                                     0) == 0
+        L.orc_bw_post_sweep(C.byref(w.data), C.byref(w.model), C.byref(w.state), seed, iters, 0)
+        t_total += time.perf_counter() - t0
+        iters += 1
+    return dict(value=iters * m_s / t_total, unit="marker-updates/s", cores=1, kind="port",
+                sample=(f"oracle (FP64 restatement of the reference BayesW chain, -O3 -ffast-math "
+                        f"-funroll-loops, 1 thread): first {m_s} markers of the same ragged workload at N={n}, "
+                        f"{iters} chain iterations from init, {t_total:.1f}s"))
+
+
+def run_bayesw(cfg, args, with_cpu):
+    """The BayesW workload on the device: returns the fields of its JSON line."""
+    import bayesrrcmd_amd as brr
+    n, m = cfg["n"], cfg["m"]
+    steps = args.steps if (args.steps is not None and args.config == "c5") else cfg["chain"]
+    data = make_bw_data(cfg, args.seed, args.threads)
+    K = len(cfg["C"]) + 1
+    h = brr.HipBayesW(n, m, K, quad_points=cfg["quad_points"], workgroups=args.workgroups)
+    t0 = time.time()
+    h.upload_ragged(data["ragged"])
+    h.set_model(np.array(cfg["C"]), data["fail"])
+    log(f"bayesw: uploaded in {time.time() - t0:.1f}s")
+    nnz = data["ragged"].ones.size + data["ragged"].twos.size
+
+    def fresh_chain():
+        eps, sc = brr.bayesw_init(data["y"], data["fail"], m, K)
+        h.set_state(eps, np.zeros(m), np.zeros(m, np.int32))
+        h.set_scalars(sc["mu"], sc["alpha"], sc["sigma_b"], sc["d"], sc["pi"], sc["v"])
+
+    order = np.arange(m, dtype=np.uint32)
+    fresh_chain()
+    for it in range(args.warmup):
+        brr.shuffle(args.seed + 1, it, 0, order)
+        h.run_chain(args.seed + 1, 0, it, order)
+    fresh_chain()
+    order = np.arange(m, dtype=np.uint32)
+    orders = np.empty((steps, m), dtype=np.uint32)
+    for it in range(steps):
+        brr.shuffle(args.seed, it, 0, order)
+        orders[it] = order
+    kernel_ms, changes, rounds, hist, phases = [], [], [], [], []
+    t_start = time.perf_counter()
+    for it in range(steps):
+        hist.append(h.run_chain(args.seed, 0, it, orders[it])[0])
+        s = h.stats()
+        kernel_ms.append(s["kernel_ms"])
+        changes.append(s["changes"])
+        rounds.append(s["rounds"])
+        phases.append(list(s["phase_ms"].values()))
+        if it % 20 == 0 or it == steps - 1:
+            ph = ", ".join(f"{k} {v:.1f}" for k, v in s["phase_ms"].items())
+            log(f"bayesw iteration {it}: {s['kernel_ms']:.2f} ms, rounds {s['rounds']}, changes {s['changes']}, "
+                f"mu {hist[-1][0]:.4f} alpha {hist[-1][1]:.4f} nonzero {hist[-1][3]:.0f}, "
+                f"core clock {s['core_clock_mhz']:.0f} MHz | phases ms: {ph}")
+    wall = time.perf_counter() - t_start
+    stats = h.stats()
+    hist = np.array(hist)
+    avg_ms = float(np.mean(kernel_ms))
+    # B_j = G_j + 8 N + 24 N delta_j (FP64): the 2-bit column, one vi read, and
+    # on a change an eps read, an eps write and a vi write
+    g_col = float((n + 3) // 4)
+    bytes_per_launch = float(np.mean([m * g_col + 8.0 * n * m + 24.0 * n * c for c in changes]))
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    burn = min(steps // 2, 30)
+    out = {
+        "value": float(steps) * m / wall,
+        "unit": "marker-updates/s",
+        "steps": steps,
+        "ms_per_step": wall * 1e3 / steps,
+        "dtype": "f64",
+        "config": {
+            "workload": cfg["workload"], "N": n, "M": m, "layout": "ragged (held on the device as 2-bit codes)",
+            "maf": list(cfg["maf"]), "causal_fraction": cfg["causal"], "C": list(cfg["C"]),
+            "quad_points": cfg["quad_points"], "censored_fraction": cfg["censor"], "timed_iterations": steps,
+            "precision": "fp64", "workgroups": stats["workgroups"], "block_markers": stats["block_markers"],
+            "slice": stats["slice"], "ragged_list_bytes": 4 * nnz,
+            "parallelism": "single GPU, persistent cooperative kernel",
+            "alpha_posterior_mean": float(hist[burn:, 1].mean()) if steps > 1 else float("nan"),
+            "mu_posterior_mean": float(hist[burn:, 0].mean()) if steps > 1 else float("nan"),
+            "alpha_simulated": cfg["alpha"], "mu_simulated": cfg["mu"],
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "kernel": "brr::bayesw_sweep_kernel (one launch per iteration)",
+            "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": bytes_per_launch,
+            "rounds_per_launch": float(np.mean(rounds)), "changes_per_launch": float(np.mean(changes)),
+            "phase_ms_per_launch": dict(zip(brr.HipBayesW.PHASES, np.mean(phases, axis=0).round(3).tolist())),
+        },
+    }
+    h.close()
+    if with_cpu:
+        log("timing the CPU reference BayesW chain ...")
+        out["cpu_baseline"] = cpu_baseline_bw(cfg, data, args.seed, args.cpu_budget)
+    return out
+
+
 def run_batched(cfg, args, with_cpu):
     """The batched workload on the device: returns the fields of its JSON line."""
     import bayesrrcmd_amd as brr
@@ -328,6 +500,7 @@ def main():
                     help="device copy of ragged columns: smaller of lists / 2-bit codes, or forced")
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-batched", action="store_true", help="skip the batched (c3) leg of the default line")
+    ap.add_argument("--no-bayesw", action="store_true", help="skip the BayesW (c5) leg of the default line")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -350,12 +523,14 @@ def main():
         log("calibration copy: 1 GiB read + 1 GiB written")
         del a, b
     cfg = CONFIGS[args.config]
-    if args.config == "c3":
-        b = run_batched(cfg, args, not args.no_cpu)
+    if args.config in ("c3", "c5"):
+        b = (run_batched if args.config == "c3" else run_bayesw)(cfg, args, not args.no_cpu)
         line = {"metric": METRIC, "value": b["value"], "unit": b["unit"], "n_gpus": 1, "steps": b["steps"],
                 "warmup": args.warmup, "ms_per_step": b["ms_per_step"], "higher_is_better": True,
                 "scaling": "weak", "vs_baseline": None, "dtype": b["dtype"],
-                "data": "synthetic (Philox-seeded PLINK genotypes, HWE, 64 phenotypes y = X beta + e)",
+                "data": ("synthetic (Philox-seeded PLINK genotypes, HWE, 64 phenotypes y = X beta + e)"
+                         if args.config == "c3" else
+                         "synthetic (Philox-seeded PLINK genotypes, HWE, Weibull log-times, 10% censored)"),
                 "config": b["config"], "roofline": b["roofline"]}
         if "cpu_baseline" in b:
             line["cpu_baseline"] = b["cpu_baseline"]
@@ -460,7 +635,11 @@ def main():
         line["cpu_baseline"] = cpu_baseline(cfg, data, args.seed, args.cpu_budget)
     if args.config == "c2" and not args.no_batched:
         del data
+        data = None
         line["batched"] = run_batched(CONFIGS["c3"], args, not args.no_cpu)
+    if args.config == "c2" and not args.no_bayesw:
+        data = None
+        line["bayesw"] = run_bayesw(CONFIGS["c5"], args, not args.no_cpu)
     print(json.dumps(line), flush=True)
 
 
