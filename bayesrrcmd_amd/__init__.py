@@ -28,7 +28,7 @@ from . import _abi
 from ._abi import LAYOUT_DENSE_2BIT, LAYOUT_RAGGED, STORAGE_AUTO, STORAGE_CODES, STORAGE_LISTS
 
 __all__ = [
-    "BrrHipError", "HipSweep", "HipBayesW", "bayesw_init", "RaggedArena", "build_dense", "build_ragged", "synth_bed", "synth_xb",
+    "BrrHipError", "HipSweep", "HipBayesW", "bayesw_init", "RaggedArena", "read_ragged_sparsebed", "read_ppbed", "build_dense", "build_ragged", "synth_bed", "synth_xb",
     "bayesr_init", "shuffle", "marker_uz", "LAYOUT_DENSE_2BIT", "LAYOUT_RAGGED", "lib",
     "STORAGE_AUTO", "STORAGE_CODES", "STORAGE_LISTS",
 ]
@@ -129,6 +129,39 @@ def build_ragged(bed: np.ndarray, n: int, threads: int | None = None) -> RaggedA
            "brr_ragged_fill (a column has no genotype 1 or 2)")
     lists = [l[: int(o[-1])] for l, o in zip(lists, offs)]
     return RaggedArena(lists[0], offs[0], lists[1], offs[1], lists[2], offs[2], stats)
+
+
+def read_ragged_sparsebed(data: bytes | np.ndarray, index: np.ndarray, n: int, compressed: bool = False,
+                          threads: int | None = None) -> RaggedArena:
+    """A .ragged.sparsebed file's bytes and its index (M x 3 u64) -> RaggedArena."""
+    buf = np.frombuffer(data, dtype=np.uint8) if isinstance(data, (bytes, bytearray)) else np.ascontiguousarray(data, np.uint8)
+    idx = np.ascontiguousarray(index, dtype=np.uint64).reshape(-1, 3)
+    m = idx.shape[0]
+    t = threads or _threads()
+    offs = [np.zeros(m + 1, dtype=np.uint64) for _ in range(3)]
+    _check(lib().brr_ragged_read_counts(_p(buf, C.c_uint8), buf.size, _p(idx, C.c_uint64), m, int(compressed),
+                                        *(_p(o, C.c_uint64) for o in offs), t), "brr_ragged_read_counts")
+    lists = [np.empty(max(1, int(o[-1])), dtype=np.int32) for o in offs]
+    stats = np.empty((m, 4), dtype=np.float64)
+    _check(lib().brr_ragged_read_fill(_p(buf, C.c_uint8), buf.size, _p(idx, C.c_uint64), n, m, int(compressed),
+                                      *(_p(o, C.c_uint64) for o in offs), *(_p(l, C.c_int32) for l in lists),
+                                      _p(stats, C.c_double), t), "brr_ragged_read_fill")
+    lists = [l[: int(o[-1])] for l, o in zip(lists, offs)]
+    return RaggedArena(lists[0], offs[0], lists[1], offs[1], lists[2], offs[2], stats)
+
+
+def read_ppbed(data: bytes | np.ndarray, index: np.ndarray, n: int, compressed: bool = False,
+               threads: int | None = None):
+    """A .ppbed file's bytes and its index (M x 3 u64) -> (bed codes (M, ceil(n/4)), tables (M, 4))."""
+    buf = np.frombuffer(data, dtype=np.uint8) if isinstance(data, (bytes, bytearray)) else np.ascontiguousarray(data, np.uint8)
+    idx = np.ascontiguousarray(index, dtype=np.uint64).reshape(-1, 3)
+    m = idx.shape[0]
+    bed = np.empty((m, (n + 3) // 4), dtype=np.uint8)
+    lut = np.empty((m, 4), dtype=np.float64)
+    _check(lib().brr_dense_read_ppbed(_p(buf, C.c_uint8), buf.size, _p(idx, C.c_uint64), n, m, int(compressed),
+                                      _p(bed, C.c_uint8), _p(lut, C.c_double), threads or _threads()),
+           "brr_dense_read_ppbed")
+    return bed, lut
 
 
 # ---------------------------------------------------------- host driver --

@@ -94,6 +94,12 @@ SIGNATURES = {
     "brr_hip_bw_sum_failure": (C.c_int, [CTX, P_f64]),
     "brr_bayesw_init": (C.c_int, [P_f64, P_f64, C.c_uint32, C.c_uint32, C.c_uint32, P_f64,
                                   C.POINTER(BwScalars)]),
+    "brr_ragged_read_counts": (C.c_int, [P_u8, C.c_uint64, P_u64, C.c_uint32, C.c_int, P_u64, P_u64, P_u64,
+                                         C.c_uint32]),
+    "brr_ragged_read_fill": (C.c_int, [P_u8, C.c_uint64, P_u64, C.c_uint32, C.c_uint32, C.c_int, P_u64, P_u64,
+                                       P_u64, P_i32, P_i32, P_i32, P_f64, C.c_uint32]),
+    "brr_dense_read_ppbed": (C.c_int, [P_u8, C.c_uint64, P_u64, C.c_uint32, C.c_uint32, C.c_int, P_u8, P_f64,
+                                       C.c_uint32]),
     "brr_bed_check_magic": (C.c_int, [P_u8]),
     "brr_dense_lut": (C.c_int, [P_u8, C.c_uint32, P_f64]),
     "brr_dense_luts": (C.c_int, [P_u8, C.c_uint64, C.c_uint32, C.c_uint32, P_f64, C.c_uint32]),

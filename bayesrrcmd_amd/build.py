@@ -55,7 +55,7 @@ def build_lib() -> Path:
           str(CSRC / "api.hip"), "-o", str(dev_obj)])
     tmp = LIB.with_suffix(".so.tmp")
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", str(dev_obj), *host_objs,
-          "-pthread", "-o", str(tmp)])
+          "-pthread", "-lz", "-o", str(tmp)])
     os.replace(tmp, LIB)
     shutil.rmtree(objdir, ignore_errors=True)
     return LIB

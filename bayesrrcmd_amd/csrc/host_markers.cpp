@@ -163,3 +163,179 @@ int brr_ragged_fill(const uint8_t* bed, uint64_t stride, uint32_t N, uint32_t M,
 }
 
 } // extern "C"
+
+// ---------------------------------------------------------------------------
+// The reference's preprocessed files, read straight into the device arenas.
+// A preprocessed file is the markers' serialised bytes back to back; its index
+// holds {pos, compressedSize, originalSize} (three u64) per marker
+// (preprocessgraph.cpp:92-126, compression.cpp:137-154). With --compress each
+// marker is one zlib stream (deflateInit level -1, Z_FINISH, compression.cpp:5-20).
+#include <zlib.h>
+
+namespace {
+
+// The bytes of marker j: a view into data (uncompressed) or inflated into buf.
+// Returns false on a bad index entry or a failed inflate.
+bool marker_bytes(const uint8_t* data, uint64_t data_bytes, const uint64_t* index, uint32_t j, int compressed,
+                  std::vector<uint8_t>& buf, const uint8_t** out, uint64_t* size)
+{
+    const uint64_t pos = index[3 * (size_t)j], csize = index[3 * (size_t)j + 1], osize = index[3 * (size_t)j + 2];
+    if (pos > data_bytes || csize > data_bytes - pos) return false;
+    if (!compressed) {
+        if (csize != osize) return false;
+        *out = data + pos;
+        *size = csize;
+        return true;
+    }
+    if (osize > (uint64_t)UINT32_MAX || csize > (uint64_t)UINT32_MAX) return false;
+    buf.resize(osize ? osize : 1);
+    z_stream s;
+    std::memset(&s, 0, sizeof(s));
+    if (inflateInit(&s) != Z_OK) return false;
+    s.next_in = const_cast<Bytef*>(data + pos);
+    s.avail_in = (uInt)csize;
+    s.next_out = buf.data();
+    s.avail_out = (uInt)osize;
+    const int rc = inflate(&s, Z_FINISH);
+    const uint64_t got = osize - s.avail_out;
+    inflateEnd(&s);
+    if (rc != Z_STREAM_END || got != osize) return false;
+    *out = buf.data();
+    *size = osize;
+    return true;
+}
+
+// RaggedSparseMarker layout (raggedsparsemarker.cpp:17-66, sparsemarker.cpp:21-58):
+// {mean, sd, sqrdZ, Zsum} f64, then for ones, twos, missing a u64 count and
+// that many int32 indices. Sets the three counts and the list offsets in bytes.
+bool ragged_layout(const uint8_t* p, uint64_t size, uint64_t n[3], uint64_t at[3])
+{
+    uint64_t off = 4 * sizeof(double);
+    for (int l = 0; l < 3; ++l) {
+        if (size < off + sizeof(uint64_t)) return false;
+        std::memcpy(&n[l], p + off, sizeof(uint64_t));
+        off += sizeof(uint64_t);
+        if (n[l] > (size - off) / sizeof(int32_t)) return false;
+        at[l] = off;
+        off += n[l] * sizeof(int32_t);
+    }
+    return off == size;
+}
+
+} // namespace
+
+extern "C" {
+
+int brr_ragged_read_counts(const uint8_t* data, uint64_t data_bytes, const uint64_t* index, uint32_t M,
+                           int compressed, uint64_t* ones_off, uint64_t* twos_off, uint64_t* miss_off,
+                           uint32_t threads)
+{
+    if (!data || !index || !ones_off || !twos_off || !miss_off || M == 0) return BRR_HIP_EINVAL;
+    std::vector<uint8_t> bad(M, 0);
+    uint64_t* offs[3] = {ones_off, twos_off, miss_off};
+    parallel_for(M, threads, [&](uint32_t j) {
+        std::vector<uint8_t> buf;
+        const uint8_t* p;
+        uint64_t size, n[3], at[3];
+        if (!marker_bytes(data, data_bytes, index, j, compressed, buf, &p, &size) || !ragged_layout(p, size, n, at)) {
+            bad[j] = 1;
+            return;
+        }
+        for (int l = 0; l < 3; ++l) offs[l][j + 1] = n[l];
+    });
+    for (uint32_t j = 0; j < M; ++j)
+        if (bad[j]) return BRR_HIP_EINVAL;
+    for (int l = 0; l < 3; ++l) {
+        offs[l][0] = 0;
+        for (uint32_t j = 0; j < M; ++j) offs[l][j + 1] += offs[l][j];
+    }
+    return BRR_HIP_OK;
+}
+
+int brr_ragged_read_fill(const uint8_t* data, uint64_t data_bytes, const uint64_t* index, uint32_t N, uint32_t M,
+                         int compressed, const uint64_t* ones_off, const uint64_t* twos_off,
+                         const uint64_t* miss_off, int32_t* ones, int32_t* twos, int32_t* miss, double* stats,
+                         uint32_t threads)
+{
+    if (!data || !index || !ones_off || !twos_off || !miss_off || !stats || M == 0) return BRR_HIP_EINVAL;
+    const uint64_t* offs[3] = {ones_off, twos_off, miss_off};
+    int32_t* lists[3] = {ones, twos, miss};
+    for (int l = 0; l < 3; ++l)
+        if (offs[l][M] && !lists[l]) return BRR_HIP_EINVAL;
+    std::vector<uint8_t> bad(M, 0);
+    parallel_for(M, threads, [&](uint32_t j) {
+        std::vector<uint8_t> buf;
+        const uint8_t* p;
+        uint64_t size, n[3], at[3];
+        if (!marker_bytes(data, data_bytes, index, j, compressed, buf, &p, &size) || !ragged_layout(p, size, n, at)) {
+            bad[j] = 1;
+            return;
+        }
+        std::memcpy(stats + 4 * (size_t)j, p, 4 * sizeof(double));
+        for (int l = 0; l < 3; ++l) {
+            if (n[l] != offs[l][j + 1] - offs[l][j]) { bad[j] = 1; return; }
+            int32_t* dst = lists[l] + offs[l][j];
+            std::memcpy(dst, p + at[l], n[l] * sizeof(int32_t));
+            int64_t prev = -1;
+            for (uint64_t q = 0; q < n[l]; ++q) {
+                if (dst[q] <= prev || dst[q] >= (int64_t)N) { bad[j] = 1; return; }
+                prev = dst[q];
+            }
+        }
+    });
+    for (uint32_t j = 0; j < M; ++j)
+        if (bad[j]) return BRR_HIP_EINVAL;
+    return BRR_HIP_OK;
+}
+
+int brr_dense_read_ppbed(const uint8_t* data, uint64_t data_bytes, const uint64_t* index, uint32_t N, uint32_t M,
+                         int compressed, uint8_t* bed, double* lut, uint32_t threads)
+{
+    // DenseMarker (densemarker.cpp:34-49): N doubles, the standardised column.
+    // Its at most four distinct values become the column's table; individual i
+    // gets the 2-bit code of its value's slot, so lut[slot(code_i)] == Cx[i]
+    // bit for bit (slots 0, 1, 2, 3 take codes 11, 10, 00, 01: the table order
+    // of brr_hip_upload_dense).
+    if (!data || !index || !bed || !lut || N == 0 || M == 0) return BRR_HIP_EINVAL;
+    const uint64_t stride = ((uint64_t)N + 3) / 4;
+    std::vector<uint8_t> bad(M, 0);
+    parallel_for(M, threads, [&](uint32_t j) {
+        std::vector<uint8_t> buf;
+        const uint8_t* p;
+        uint64_t size;
+        if (!marker_bytes(data, data_bytes, index, j, compressed, buf, &p, &size) ||
+            size != (uint64_t)N * sizeof(double)) {
+            bad[j] = 1;
+            return;
+        }
+        static const unsigned kSlotCode[4] = {3u, 2u, 0u, 1u};
+        uint64_t vals[4];
+        int nv = 0;
+        uint8_t* col = bed + (size_t)j * stride;
+        std::memset(col, 0, stride);
+        for (uint32_t i = 0; i < N; ++i) {
+            uint64_t b;
+            std::memcpy(&b, p + (size_t)i * sizeof(double), sizeof(double));
+            int s = 0;
+            while (s < nv && vals[s] != b) ++s;
+            if (s == nv) {
+                if (nv == 4) { bad[j] = 1; return; }
+                vals[nv++] = b;
+            }
+            col[i >> 2] |= (uint8_t)(kSlotCode[s] << (2 * (i & 3u)));
+        }
+        // pad calls of the last byte: genotype 0 (code 11), as in a .bed
+        for (uint32_t i = N; i < stride * 4; ++i) col[i >> 2] |= (uint8_t)(3u << (2 * (i & 3u)));
+        double* lt = lut + 4 * (size_t)j;
+        for (int s = 0; s < 4; ++s) {
+            double v = 0.0;
+            if (s < nv) std::memcpy(&v, &vals[s], sizeof(double));
+            lt[s] = v;
+        }
+    });
+    for (uint32_t j = 0; j < M; ++j)
+        if (bad[j]) return BRR_HIP_EINVAL;
+    return BRR_HIP_OK;
+}
+
+} // extern "C"

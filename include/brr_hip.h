@@ -285,6 +285,26 @@ int brr_ragged_fill(const uint8_t* bed, uint64_t column_stride, uint32_t num_ind
 int brr_dense_luts(const uint8_t* bed, uint64_t column_stride, uint32_t num_inds,
                    uint32_t num_snps, double* lut, uint32_t threads);
 
+/* The reference's preprocessed files (src/preprocessgraph.cpp:92-126, src/compression.cpp:137-154):
+ * data = the file's bytes, index = its index file, M x {pos, compressedSize, originalSize} u64;
+ * compressed = Options::compress (each marker one zlib stream). BRR_HIP_EINVAL on a malformed
+ * entry. */
+/* .ragged.sparsebed (RaggedSparseMarker::write, src/raggedsparsemarker.cpp:43-66) -> the CSR
+ * arena of brr_hip_upload_ragged, in two passes like brr_ragged_count / brr_ragged_fill. */
+int brr_ragged_read_counts(const uint8_t* data, uint64_t data_bytes, const uint64_t* index,
+                           uint32_t num_snps, int compressed, uint64_t* ones_off, uint64_t* twos_off,
+                           uint64_t* miss_off, uint32_t threads);
+int brr_ragged_read_fill(const uint8_t* data, uint64_t data_bytes, const uint64_t* index,
+                         uint32_t num_inds, uint32_t num_snps, int compressed, const uint64_t* ones_off,
+                         const uint64_t* twos_off, const uint64_t* miss_off, int32_t* ones, int32_t* twos,
+                         int32_t* miss, double* stats, uint32_t threads);
+/* .ppbed (DenseMarker::write, src/densemarker.cpp:34-49, N doubles per column) -> 2-bit codes
+ * (M x ceil(N/4) bytes) and tables (M x 4) for brr_hip_upload_dense that reproduce every column
+ * value bit for bit; BRR_HIP_EINVAL if a column holds more than 4 distinct values. */
+int brr_dense_read_ppbed(const uint8_t* data, uint64_t data_bytes, const uint64_t* index,
+                         uint32_t num_inds, uint32_t num_snps, int compressed, uint8_t* bed, double* lut,
+                         uint32_t threads);
+
 /* ---- host half of runGibbs over the counter-based stream ---- */
 
 /* BayesRBase::init (src/BayesRBase.cpp:39-99): standardise y (N doubles, the
