@@ -28,7 +28,7 @@ from . import _abi
 from ._abi import LAYOUT_DENSE_2BIT, LAYOUT_RAGGED, STORAGE_AUTO, STORAGE_CODES, STORAGE_LISTS
 
 __all__ = [
-    "BrrHipError", "HipSweep", "HipBayesW", "bayesw_init", "RaggedArena", "read_ragged_sparsebed", "read_ppbed", "build_dense", "build_ragged", "synth_bed", "synth_xb",
+    "BrrHipError", "HipSweep", "HipBayesW", "bayesw_init", "RaggedArena", "read_ragged_sparsebed", "read_ppbed", "SampleWriter", "sample_due", "build_dense", "build_ragged", "synth_bed", "synth_xb",
     "bayesr_init", "shuffle", "marker_uz", "LAYOUT_DENSE_2BIT", "LAYOUT_RAGGED", "lib",
     "STORAGE_AUTO", "STORAGE_CODES", "STORAGE_LISTS",
 ]
@@ -162,6 +162,51 @@ def read_ppbed(data: bytes | np.ndarray, index: np.ndarray, n: int, compressed: 
                                       _p(bed, C.c_uint8), _p(lut, C.c_double), threads or _threads()),
            "brr_dense_read_ppbed")
     return bed, lut
+
+
+class SampleWriter:
+    """The reference's CSV sample file (SampleWriter): kind "bayesr" or "bayesw"."""
+
+    def __init__(self, path: str, kind: str, m: int, n: int = 0, groups: int = 1, fixed: int = 0):
+        self.kind = {"bayesr": 0, "bayesw": 1}[kind]
+        self.m, self.n, self.g, self.f = m, n, groups, fixed
+        h = C.c_void_p()
+        _check(lib().brr_samples_open(str(path).encode(), self.kind, m, n, groups, fixed, C.byref(h)),
+               "brr_samples_open")
+        self.h = h
+
+    def write_bayesr(self, iteration, mu, beta, sigmaE, sigmaG, comp, acum, eps, gamma=None):
+        b = np.ascontiguousarray(beta, np.float64)
+        sg = np.ascontiguousarray(sigmaG, np.float64).reshape(-1)
+        gm = None if gamma is None else np.ascontiguousarray(gamma, np.float64)
+        c = np.ascontiguousarray(comp, np.int32)
+        a = np.ascontiguousarray(acum, np.float64)
+        e = np.ascontiguousarray(eps, np.float64)
+        _check(lib().brr_samples_write_bayesr(self.h, iteration, mu, _p(b, C.c_double), sigmaE, _p(sg, C.c_double),
+                                              _p(gm, C.c_double), _p(c, C.c_int32), _p(a, C.c_double),
+                                              _p(e, C.c_double)), "brr_samples_write_bayesr")
+
+    def write_bayesw(self, iteration, alpha, mu, beta, comp, sigma_b, theta=None):
+        b = np.ascontiguousarray(beta, np.float64)
+        c = np.ascontiguousarray(comp, np.int32)
+        th = None if theta is None else np.ascontiguousarray(theta, np.float64)
+        _check(lib().brr_samples_write_bayesw(self.h, iteration, alpha, mu, _p(th, C.c_double), _p(b, C.c_double),
+                                              _p(c, C.c_int32), sigma_b), "brr_samples_write_bayesw")
+
+    def close(self):
+        if getattr(self, "h", None):
+            _check(lib().brr_samples_close(self.h), "brr_samples_close")
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def sample_due(iteration: int, burn_in: int, thinning: int) -> bool:
+    return bool(lib().brr_sample_due(iteration, burn_in, thinning))
 
 
 # ---------------------------------------------------------- host driver --

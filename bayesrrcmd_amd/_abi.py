@@ -100,6 +100,14 @@ SIGNATURES = {
                                        P_u64, P_i32, P_i32, P_i32, P_f64, C.c_uint32]),
     "brr_dense_read_ppbed": (C.c_int, [P_u8, C.c_uint64, P_u64, C.c_uint32, C.c_uint32, C.c_int, P_u8, P_f64,
                                        C.c_uint32]),
+    "brr_samples_open": (C.c_int, [C.c_char_p, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                   C.POINTER(C.c_void_p)]),
+    "brr_samples_write_bayesr": (C.c_int, [C.c_void_p, C.c_uint32, C.c_double, P_f64, C.c_double, P_f64, P_f64,
+                                           P_i32, P_f64, P_f64]),
+    "brr_samples_write_bayesw": (C.c_int, [C.c_void_p, C.c_uint32, C.c_double, C.c_double, P_f64, P_f64, P_i32,
+                                           C.c_double]),
+    "brr_samples_close": (C.c_int, [C.c_void_p]),
+    "brr_sample_due": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32]),
     "brr_bed_check_magic": (C.c_int, [P_u8]),
     "brr_dense_lut": (C.c_int, [P_u8, C.c_uint32, P_f64]),
     "brr_dense_luts": (C.c_int, [P_u8, C.c_uint64, C.c_uint32, C.c_uint32, P_f64, C.c_uint32]),

@@ -5,11 +5,17 @@
 //   random numbers       prepareForAnylsis, BayesRBase.cpp:101-110 (per marker id)
 //   marker order         std::random_shuffle, BayesRBase.cpp:198
 //   BayesW init          BayesWBase::init, bayeswbase.cpp:469-540
+//   sample output        SampleWriter / Writer (samplewriter.cpp, writer.cpp): the CSV
+//                        header and one row per kept iteration, printed like an
+//                        Eigen row vector with IOFormat(StreamPrecision, DontAlignCols,
+//                        ", ", ", ") on a default std::ofstream (6 significant digits)
 // They are restated here over the counter-based stream of brr_rng.h so a host
 // driver and the device-resident chain consume the same numbers.
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
+#include <fstream>
+#include <string>
 
 #include "brr_hip.h"
 #include "brr_rng.h"
@@ -103,6 +109,106 @@ int brr_bayesw_init(const double* y, const double* fail, uint32_t N, uint32_t M,
     s->sigma_b = 6.283185 / (6 * (s->alpha * s->alpha) * M);   // PI2
     s->betasq = 0.0;
     return BRR_HIP_OK;
+}
+
+} // extern "C"
+
+struct brr_samples {
+    std::ofstream out;
+    int kind = 0;
+    uint32_t M = 0, N = 0, G = 0, F = 0;
+};
+
+extern "C" {
+
+int brr_samples_open(const char* path, int kind, uint32_t M, uint32_t N, uint32_t G, uint32_t F, brr_samples** out)
+{
+    if (!path || !out || M == 0 || (kind != BRR_SAMPLES_BAYESR && kind != BRR_SAMPLES_BAYESW)) return BRR_HIP_EINVAL;
+    if (kind == BRR_SAMPLES_BAYESR && (N == 0 || G == 0)) return BRR_HIP_EINVAL;
+    *out = nullptr;
+    brr_samples* w = new brr_samples();
+    w->kind = kind;
+    w->M = M; w->N = N; w->G = G; w->F = F;
+    w->out.open(path);
+    if (!w->out) { delete w; return BRR_HIP_EINVAL; }
+    std::ofstream& o = w->out;
+    if (kind == BRR_SAMPLES_BAYESR) {
+        // SampleWriter::openGroups (samplewriter.cpp:82-118)
+        o << "iteration," << "mu,";
+        for (uint32_t i = 0; i < M; ++i) o << "beta[" << (i + 1) << "],";
+        o << "sigmaE,";
+        for (uint32_t i = 0; i < G; ++i) o << "sigmaG[" << (i + 1) << "],";
+        for (uint32_t i = 0; i < F; ++i) o << "gamma[" << (i + 1) << "],";
+        for (uint32_t i = 0; i < M; ++i) o << "comp[" << (i + 1) << "],";
+        for (uint32_t i = 0; i < M; ++i) o << "acum[" << (i + 1) << "],";
+        uint32_t i;
+        for (i = 0; i < N - 1; ++i) o << "epsilon[" << (i + 1) << "],";
+        o << "epsilon[" << (i + 1) << "]";
+    } else {
+        // SampleWriter::open_bayesW / open_bayesW_fixed (samplewriter.cpp:34-80)
+        o << "iteration," << "alpha," << "mu,";
+        for (uint32_t i = 0; i < F; ++i) o << "theta[" << (i + 1) << "],";
+        for (uint32_t i = 0; i < M; ++i) o << "beta[" << (i + 1) << "],";
+        for (uint32_t i = 0; i < M; ++i) o << "comp[" << (i + 1) << "],";
+        o << "sigma_b";
+    }
+    o << std::endl;
+    o.flush();
+    *out = w;
+    return o ? BRR_HIP_OK : BRR_HIP_EINVAL;
+}
+
+int brr_samples_write_bayesr(brr_samples* w, uint32_t iteration, double mu, const double* beta, double sigmaE,
+                             const double* sigmaG, const double* gamma, const int32_t* comp, const double* acum,
+                             const double* eps)
+{
+    // BayesRBase.cpp:251: iteration, mu, beta, sigmaE, sigmaG, gamma, components, acum, epsilon
+    if (!w || w->kind != BRR_SAMPLES_BAYESR || !beta || !sigmaG || !comp || !acum || !eps || (w->F && !gamma))
+        return BRR_HIP_EINVAL;
+    std::ofstream& o = w->out;
+    const char* sep = ", ";
+    o << (double)iteration << sep << mu;
+    for (uint32_t i = 0; i < w->M; ++i) o << sep << beta[i];
+    o << sep << sigmaE;
+    for (uint32_t i = 0; i < w->G; ++i) o << sep << sigmaG[i];
+    for (uint32_t i = 0; i < w->F; ++i) o << sep << gamma[i];
+    for (uint32_t i = 0; i < w->M; ++i) o << sep << (double)comp[i];
+    for (uint32_t i = 0; i < w->M; ++i) o << sep << acum[i];
+    for (uint32_t i = 0; i < w->N; ++i) o << sep << eps[i];
+    o << std::endl;
+    return o ? BRR_HIP_OK : BRR_HIP_EINVAL;
+}
+
+int brr_samples_write_bayesw(brr_samples* w, uint32_t iteration, double alpha, double mu, const double* theta,
+                             const double* beta, const int32_t* comp, double sigma_b)
+{
+    // bayeswbase.cpp:810-815: iteration, alpha, mu, (theta), beta, components, sigma_b
+    if (!w || w->kind != BRR_SAMPLES_BAYESW || !beta || !comp || (w->F && !theta)) return BRR_HIP_EINVAL;
+    std::ofstream& o = w->out;
+    const char* sep = ", ";
+    o << (double)iteration << sep << alpha << sep << mu;
+    for (uint32_t i = 0; i < w->F; ++i) o << sep << theta[i];
+    for (uint32_t i = 0; i < w->M; ++i) o << sep << beta[i];
+    for (uint32_t i = 0; i < w->M; ++i) o << sep << (double)comp[i];
+    o << sep << sigma_b;
+    o << std::endl;
+    return o ? BRR_HIP_OK : BRR_HIP_EINVAL;
+}
+
+int brr_samples_close(brr_samples* w)
+{
+    if (!w) return BRR_HIP_EINVAL;
+    w->out.flush();
+    const bool ok = (bool)w->out;
+    w->out.close();
+    delete w;
+    return ok ? BRR_HIP_OK : BRR_HIP_EINVAL;
+}
+
+int brr_sample_due(uint32_t iteration, uint32_t burn_in, uint32_t thinning)
+{
+    // BayesRBase.cpp:250, bayeswbase.cpp:810
+    return thinning != 0 && iteration >= burn_in && iteration % thinning == 0;
 }
 
 } // extern "C"

@@ -305,6 +305,26 @@ int brr_dense_read_ppbed(const uint8_t* data, uint64_t data_bytes, const uint64_
                          uint32_t num_inds, uint32_t num_snps, int compressed, uint8_t* bed, double* lut,
                          uint32_t threads);
 
+/* ---- sample output (SampleWriter, src/samplewriter.cpp, src/writer.cpp) ---- */
+
+/* The reference's CSV samples: the header of SampleWriter::openGroups (BayesR, kind 0) or
+ * open_bayesW[_fixed] (BayesW, kind 1), then one row per kept iteration, values separated by
+ * ", " and printed as a default std::ostream prints a double (6 significant digits). */
+enum brr_samples_kind { BRR_SAMPLES_BAYESR = 0, BRR_SAMPLES_BAYESW = 1 };
+typedef struct brr_samples brr_samples;
+int brr_samples_open(const char* path, int kind, uint32_t num_snps, uint32_t num_inds, uint32_t num_groups,
+                     uint32_t num_fixed, brr_samples** out);
+/* BayesRBase.cpp:251: iteration, mu, beta[M], sigmaE, sigmaG[G], gamma[F], comp[M], acum[M], epsilon[N]. */
+int brr_samples_write_bayesr(brr_samples* w, uint32_t iteration, double mu, const double* beta, double sigmaE,
+                             const double* sigmaG, const double* gamma, const int32_t* comp, const double* acum,
+                             const double* epsilon);
+/* bayeswbase.cpp:810-815: iteration, alpha, mu, theta[F], beta[M], comp[M], sigma_b. */
+int brr_samples_write_bayesw(brr_samples* w, uint32_t iteration, double alpha, double mu, const double* theta,
+                             const double* beta, const int32_t* comp, double sigma_b);
+int brr_samples_close(brr_samples* w);
+/* 1 if iteration is written: iteration >= burn_in && iteration % thinning == 0 (BayesRBase.cpp:250). */
+int brr_sample_due(uint32_t iteration, uint32_t burn_in, uint32_t thinning);
+
 /* ---- host half of runGibbs over the counter-based stream ---- */
 
 /* BayesRBase::init (src/BayesRBase.cpp:39-99): standardise y (N doubles, the
