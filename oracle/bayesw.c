@@ -10,6 +10,11 @@
  *                            including the x6 = -x3 entry of n = 9..15)
  *   init, mu, alpha, sigma_b, pi   bayeswbase.cpp:469-540, 542-569, 707-734,
  *                            777-809
+ *   dense layout             DenseBayesWKernel (densebayeswkernel.cpp:10-42) and
+ *                            DenseBayesW::beta_dens (densebayesw.cpp:25-33): the
+ *                            standardised column Cx (the dense table of each
+ *                            individual's call), every sum over individuals
+ *                            evaluated element by element as the reference does
  *   ARMS                     BayesW_arms.cpp:133-906, the derivative-free
  *                            adaptive rejection sampler of Gilks, Best & Tan
  *                            (1995), restated here with index-linked envelope
@@ -394,8 +399,28 @@ static vi_sums set_vi(const orc_data* d, uint32_t j, const double* vi)
     return s;
 }
 
+/* The dense column Cx of marker j: the table entry of each individual's call. */
+static void dense_cx(const orc_data* d, uint32_t j, double* cx)
+{
+    const uint8_t* col = d->bed + (size_t)j * ((d->N + 3u) >> 2);
+    const double* lut = d->lut + 4 * (size_t)j;
+    for (uint32_t i = 0; i < d->N; ++i) {
+        const unsigned c = (col[i >> 2] >> ((i & 3u) * 2u)) & 3u;
+        cx[i] = lut[(c == 0u) ? 2 : (c == 2u) ? 1 : (c == 3u) ? 0 : 3];
+    }
+}
+
 double orc_bw_sum_failure(const orc_data* d, uint32_t j, const double* fail)
 {
+    if (d->layout == ORC_DENSE) {
+        /* DenseBayesWKernel::calculateSumFailure (densebayeswkernel.cpp:15-18) */
+        double* cx = (double*)malloc(sizeof(double) * d->N);
+        dense_cx(d, j, cx);
+        double s = 0.0;
+        for (uint32_t i = 0; i < d->N; ++i) s += cx[i] * fail[i];
+        free(cx);
+        return s;
+    }
     /* calculateSumFailure (raggedbayeswkernel.cpp:18-25): the list sums go
      * through an int accumulator */
     int temp_sum = 0;
@@ -451,9 +476,61 @@ static double beta_dens(double x, void* c)
            x * x / (2 * p->used_mixture * p->sigma_b);
 }
 
-/* calculateResidualUpdate (raggedbayeswkernel.cpp:27-43), eps += sign * delta */
+typedef struct {
+    double alpha, sigma_b, sum_failure, used_mixture;
+    const double *cx, *eps, *vi;
+    uint32_t N;
+} dense_ctx;
+
+/* DenseBayesW beta_dens (densebayesw.cpp:25-33), every individual's term */
+static double dense_beta_dens(double x, void* c)
+{
+    const dense_ctx* p = (const dense_ctx*)c;
+    double s = 0.0;
+    for (uint32_t i = 0; i < p->N; ++i) s += exp(((p->eps[i] - p->cx[i] * x) * p->alpha) - BW_EULER);
+    return -p->alpha * x * p->sum_failure - s - x * x / (2 * p->used_mixture * p->sigma_b);
+}
+
+/* DenseBayesWKernel::integrand_adaptive (densebayeswkernel.cpp:36-42) */
+static double dense_integrand(double s, double alpha, double sq, double sum_failure, const double* cx,
+                              const double* vi, uint32_t N)
+{
+    double t = 0.0;
+    for (uint32_t i = 0; i < N; ++i) t += vi[i] * (1 - exp(-cx[i] * s * sq * alpha));
+    const double temp = -alpha * s * sum_failure * sq + t - s * s;
+    return exp(temp);
+}
+
+static double dense_gh_integral(int n, double sigma, double C_k, double sigma_b, double alpha, double sum_failure,
+                                const double* cx, const double* vi, uint32_t N)
+{
+    /* gauss_hermite_adaptive_integral (bayeswbase.cpp:126-457) with the dense integrand */
+    double x[14], w[14], wc;
+    const int m = orc_bw_gh_rule(n, x, w, &wc);
+    const double sq = sqrt(2 * C_k * sigma_b);
+    double temp = 0.0;
+    for (int i = 0; i < m; ++i) {
+        const double t = w[i] * dense_integrand(sigma * x[i], alpha, sq, sum_failure, cx, vi, N);
+        temp = (i == 0) ? t : temp + t;
+    }
+    temp = temp + wc;
+    return sigma * temp;
+}
+
+/* calculateResidualUpdate (raggedbayeswkernel.cpp:27-43; dense: Cx * beta,
+ * densebayeswkernel.cpp:20-23), eps += sign * delta */
 static void residual_update(const orc_data* d, uint32_t j, double* eps, double beta, int sign)
 {
+    if (d->layout == ORC_DENSE) {
+        double* cx = (double*)malloc(sizeof(double) * d->N);
+        dense_cx(d, j, cx);
+        for (uint32_t i = 0; i < d->N; ++i) {
+            const double delta = cx[i] * beta;
+            eps[i] = (sign > 0) ? eps[i] + delta : eps[i] - delta;
+        }
+        free(cx);
+        return;
+    }
     const double mean = d->stats[4 * (size_t)j], sd = d->stats[4 * (size_t)j + 1];
     const double mean_sd_ratio = mean / sd;
     const double meanAdjustment = mean_sd_ratio * beta;
@@ -483,18 +560,36 @@ int orc_bw_process_column(const orc_data* d, const orc_bw_model* m, orc_bw_state
         residual_update(d, j, s->eps, beta_old, +1);
         refresh_vi(d->N, s->eps, s->alpha, s->vi);
     }
-    const vi_sums v = set_vi(d, j, s->vi);
-    const double mean = d->stats[4 * (size_t)j], sd = d->stats[4 * (size_t)j + 1];
+    const int dense = d->layout == ORC_DENSE;
+    vi_sums v = {0, 0, 0, 0};
+    double mean = 0.0, sd = 1.0;
+    double* cx = NULL;
     const double sum_failure = m->sum_failure[j];
     const int K = m->K;
     double ml[ORC_BW_MAXK];
     ml[0] = s->pi[0] * BW_SQRT_PI;
-    const double exp_sum = (v.vi_1 * (1 - 2 * mean) + 4 * (1 - mean) * v.vi_2 + v.vi_sum * mean * mean) / (sd * sd);
+    double exp_sum;
+    if (dense) {
+        /* DenseBayesWKernel::exponent_sum (densebayeswkernel.cpp:30-34) */
+        cx = (double*)malloc(sizeof(double) * d->N);
+        dense_cx(d, j, cx);
+        exp_sum = 0.0;
+        for (uint32_t i = 0; i < d->N; ++i) exp_sum += s->vi[i] * cx[i] * cx[i];
+        for (uint32_t i = 0; i < d->N; ++i) v.vi_sum += s->vi[i];
+    } else {
+        v = set_vi(d, j, s->vi);
+        mean = d->stats[4 * (size_t)j];
+        sd = d->stats[4 * (size_t)j + 1];
+        exp_sum = (v.vi_1 * (1 - 2 * mean) + 4 * (1 - mean) * v.vi_2 + v.vi_sum * mean * mean) / (sd * sd);
+    }
     const double vv[4] = {v.vi_sum, v.vi_0, v.vi_1, v.vi_2};
     for (int i = 0; i < K - 1; ++i) {
         const double sigma = 1.0 / sqrt(1 + s->alpha * s->alpha * s->sigma_b * m->C[i] * exp_sum);
         ml[i + 1] = s->pi[i + 1] *
-                    orc_bw_gh_integral(m->quad_points, sigma, m->C[i], s->sigma_b, s->alpha, sum_failure, mean, sd, vv);
+                    (dense ? dense_gh_integral(m->quad_points, sigma, m->C[i], s->sigma_b, s->alpha, sum_failure, cx,
+                                               s->vi, d->N)
+                           : orc_bw_gh_integral(m->quad_points, sigma, m->C[i], s->sigma_b, s->alpha, sum_failure,
+                                                mean, sd, vv));
     }
     double ml_sum = 0.0;
     for (int i = 0; i < K; ++i) ml_sum += ml[i];
@@ -509,13 +604,16 @@ int orc_bw_process_column(const orc_data* d, const orc_bw_model* m, orc_bw_state
                 beta_new = 0;
             } else {
                 beta_ctx c = {s->alpha, s->sigma_b, sum_failure, m->C[k - 1], mean / sd, sd, v.vi_0, v.vi_1, v.vi_2};
+                dense_ctx dc = {s->alpha, s->sigma_b, sum_failure, m->C[k - 1], cx, s->eps, s->vi, d->N};
                 const double safe_limit = 2 * sqrt(s->sigma_b * m->C[k - 1]);
                 const double xinit[4] = {beta_old - safe_limit / 10, beta_old, beta_old + safe_limit / 20,
                                          beta_old + safe_limit / 10};
                 const orc_arms_rng r = {seed, j, iteration, chain};
                 double xs = beta_old;
-                err = orc_arms(xinit, 4, beta_old - safe_limit, beta_old + safe_limit, beta_dens, &c, 100, &r, &xs,
-                               NULL);
+                err = dense ? orc_arms(xinit, 4, beta_old - safe_limit, beta_old + safe_limit, dense_beta_dens, &dc,
+                                       100, &r, &xs, NULL)
+                            : orc_arms(xinit, 4, beta_old - safe_limit, beta_old + safe_limit, beta_dens, &c, 100,
+                                       &r, &xs, NULL);
                 beta_new = xs;
             }
             s->v[k] += 1.0;
@@ -534,6 +632,7 @@ int orc_bw_process_column(const orc_data* d, const orc_bw_model* m, orc_bw_state
     }
     s->comp[j] = component;
     s->beta[j] = beta_new;
+    free(cx);
     if (tr) {
         tr->vi_sum = v.vi_sum; tr->vi_1 = v.vi_1; tr->vi_2 = v.vi_2; tr->acum = acum0;
         for (int i = 0; i < K; ++i) tr->ml[i] = ml[i];

@@ -325,15 +325,25 @@ class BayesWProblem:
     C: tuple = (0.001, 0.01, 0.1)
     quad_points: int = 7
 
+    bed: object = None      # dense layout: M x ceil(N/4) codes and M x 4 tables instead of ragged
+    lut: object = None
+
     def __post_init__(self):
-        r = self.ragged
         d = OrcData()
-        d.N, d.M, d.layout = self.n, self.m, RAGGED
-        d.ones, d.ones_off = p(r.ones, C.c_int32), p(r.ones_off, C.c_uint64)
-        d.twos, d.twos_off = p(r.twos, C.c_int32), p(r.twos_off, C.c_uint64)
-        d.miss, d.miss_off = p(r.miss, C.c_int32), p(r.miss_off, C.c_uint64)
-        self._stats = np.ascontiguousarray(r.stats, dtype=np.float64)
-        d.stats = p(self._stats, C.c_double)
+        d.N, d.M = self.n, self.m
+        if self.ragged is not None:
+            r = self.ragged
+            d.layout = RAGGED
+            d.ones, d.ones_off = p(r.ones, C.c_int32), p(r.ones_off, C.c_uint64)
+            d.twos, d.twos_off = p(r.twos, C.c_int32), p(r.twos_off, C.c_uint64)
+            d.miss, d.miss_off = p(r.miss, C.c_int32), p(r.miss_off, C.c_uint64)
+            self._stats = np.ascontiguousarray(r.stats, dtype=np.float64)
+            d.stats = p(self._stats, C.c_double)
+        else:
+            d.layout = DENSE
+            self._bed = np.ascontiguousarray(self.bed, dtype=np.uint8)
+            self._lut = np.ascontiguousarray(self.lut, dtype=np.float64)
+            d.bed, d.lut = p(self._bed, C.c_uint8), p(self._lut, C.c_double)
         self.data = d
         self.fail = np.ascontiguousarray(self.fail, dtype=np.float64)
         self.sum_failure = np.array([lib().orc_bw_sum_failure(C.byref(d), j, p(self.fail, C.c_double))

@@ -16,6 +16,7 @@ import numpy as np
 import pytest
 from scipy import stats
 
+import bayesrrcmd_amd as brr
 import _oracle as orc
 from _data import make_bw_problem, spec_dosage
 
@@ -230,3 +231,82 @@ def test_bw_chain_recovers_weibull_shape():
     alpha = hist[20:, 1].mean()
     assert abs(alpha - 3.0) < 0.3, alpha
     assert abs(hist[20:, 0].mean() - 4.0) < 0.2
+
+
+# ------------------------------------------------------------- dense layout --
+
+def dense_problem(pr, **kw):
+    lut = np.array([orc.dense_lut(pr.bed[j], pr.n) for j in range(pr.bed.shape[0])])
+    return orc.BayesWProblem(pr.n, pr.m, None, pr.fail, bed=pr.bed, lut=lut, **kw), lut
+
+
+def test_bw_dense_equals_ragged_without_missing():
+    """DenseBayesWKernel and RaggedBayesWKernel are the same model on the same genotypes
+    when no call is missing (the dense standardisation equals the ragged (x - mean) / sd):
+    class likelihoods, classes and draws agree to rounding."""
+    pr = make_bw_problem(1200, 24, seed=41, causal_frac=0.2)
+    wr = orc.BayesWProblem(pr.n, pr.m, pr.ragged, pr.fail)
+    wd, _ = dense_problem(pr)
+    np.testing.assert_allclose(wd.sum_failure, wr.sum_failure, rtol=1e-10, atol=1e-9)
+    for w in (wr, wd):
+        w.init(pr.y)
+        w.refresh_vi()
+    for j in range(pr.m):
+        tr = [orc.OrcBwTrace(), orc.OrcBwTrace()]
+        for w, t in zip((wr, wd), tr):
+            assert orc.lib().orc_bw_process_column(C.byref(w.data), C.byref(w.model), C.byref(w.state), j, 0.7, 3,
+                                                   0, 0, C.byref(t)) == 0
+        np.testing.assert_allclose(np.array(tr[1].ml[:4]), np.array(tr[0].ml[:4]), rtol=1e-9)
+        assert wd.comp[j] == wr.comp[j]
+        assert wd.beta[j] == pytest.approx(wr.beta[j], rel=1e-8, abs=1e-12)
+        np.testing.assert_allclose(wd.eps, wr.eps, rtol=0, atol=1e-10)
+    assert np.count_nonzero(wd.comp) > 0
+
+
+def test_bw_dense_likelihoods_match_brute_force_with_missing():
+    """With missing calls (mean-imputed in the dense column) the dense class likelihoods
+    against the individual-level integrand over the dense column."""
+    from _data import spec_dosage
+    pr = make_bw_problem(900, 8, seed=43, causal_frac=0.25)
+    bed, _ = brr.synth_bed(44, pr.n, pr.m, 0.05, 0.5, 0.05)     # 5% missing calls
+    pr.bed = bed
+    wd, lut = dense_problem(pr)
+    wd.init(pr.y)
+    wd.refresh_vi()
+    st = wd.state
+    d = spec_dosage(bed, pr.n)
+    for j in range(4):
+        cx = lut[j][np.where(d[j] < 0, 3, d[j])]
+        vi = wd.vi.copy()
+        pi = wd.pi.copy()
+        tr = orc.OrcBwTrace()
+        assert orc.lib().orc_bw_process_column(C.byref(wd.data), C.byref(wd.model), C.byref(st), j, 0.5, 5, 0, 0,
+                                               C.byref(tr)) == 0
+        sf = float(np.sum(cx * pr.fail))
+        assert wd.sum_failure[j] == pytest.approx(sf, rel=1e-12, abs=1e-9)
+        _, xs, ws, wc = rule(7)
+        for c, Ck in enumerate(wd.C):
+            sigma = 1.0 / np.sqrt(1 + st.alpha ** 2 * st.sigma_b * Ck * np.sum(vi * cx * cx))
+            sq = np.sqrt(2 * Ck * st.sigma_b)
+            f = lambda s: np.exp(-st.alpha * s * sf * sq + np.sum(vi * (1 - np.exp(-cx * s * sq * st.alpha))) - s * s)
+            gh = sigma * (sum(wi * f(sigma * xi) for xi, wi in zip(xs, ws)) + wc)
+            assert tr.ml[c + 1] == pytest.approx(pi[c + 1] * gh, rel=1e-9)
+        if wd.beta[j] != 0.0:
+            break
+
+
+def test_bw_dense_sweep_residual_identity():
+    pr = make_bw_problem(1500, 40, seed=45, causal_frac=0.1)
+    wd, lut = dense_problem(pr)
+    wd.init(pr.y)
+    assert orc.lib().orc_bw_sample_mu(C.byref(wd.data), C.byref(wd.state), 5, 0, 0) == 0
+    wd.refresh_vi()
+    order, u = philox_order_u(5, 0, pr.m)
+    wd.v[:] = 1.0
+    assert wd.sweep(order, u, 5, 0) == 0
+    from _data import spec_dosage
+    d = spec_dosage(pr.bed, pr.n)
+    x = np.take_along_axis(lut, np.where(d < 0, 3, d), axis=1)
+    np.testing.assert_allclose(wd.eps, pr.y - wd.state.mu - wd.beta @ x, rtol=0, atol=1e-10)
+    np.testing.assert_allclose(wd.vi, np.exp(wd.state.alpha * wd.eps - EULER), rtol=1e-12)
+    assert np.count_nonzero(wd.beta) > 0
