@@ -453,6 +453,13 @@ class HipBayesW:
             _p(a.twos_off, C.c_uint64), _p(a.miss, C.c_int32), _p(a.miss_off, C.c_uint64),
             _p(np.ascontiguousarray(a.stats), C.c_double)), "brr_hip_upload_ragged")
 
+    def upload_dense(self, bed: np.ndarray, lut: np.ndarray):
+        """Dense markers (DenseBayesW): 2-bit columns and their standardised tables."""
+        bed = np.ascontiguousarray(bed, dtype=np.uint8)
+        lut = np.ascontiguousarray(lut, dtype=np.float64)
+        self._ck(lib().brr_hip_upload_dense(self.ctx, _p(bed, C.c_uint8), bed.shape[1], _p(lut, C.c_double)),
+                 "brr_hip_upload_dense")
+
     def set_model(self, mixture_C: np.ndarray, fail: np.ndarray):
         cc = np.ascontiguousarray(mixture_C, dtype=np.float64).reshape(self.K - 1)
         f = np.ascontiguousarray(fail, dtype=np.float64).reshape(self.n)

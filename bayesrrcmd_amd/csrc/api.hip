@@ -289,12 +289,12 @@ const void* batch_kernel_for(uint32_t NT, int layout)
     }
 }
 
-const void* bw_kernel_for(uint32_t R)
+const void* bw_kernel_for(uint32_t R, bool dense)
 {
     switch (R) {
-    case 1: return (const void*)bayesw_sweep_kernel<1>;
-    case 2: return (const void*)bayesw_sweep_kernel<2>;
-    case 4: return (const void*)bayesw_sweep_kernel<4>;
+    case 1: return dense ? (const void*)bayesw_sweep_kernel<1, true> : (const void*)bayesw_sweep_kernel<1, false>;
+    case 2: return dense ? (const void*)bayesw_sweep_kernel<2, true> : (const void*)bayesw_sweep_kernel<2, false>;
+    case 4: return dense ? (const void*)bayesw_sweep_kernel<4, true> : (const void*)bayesw_sweep_kernel<4, false>;
     default: return nullptr;
     }
 }
@@ -312,6 +312,7 @@ BwParams bw_params(brr_hip_ctx* c)
     p.col_words = c->col_words;
     p.col_bytes = c->col_bytes;
     p.stats = c->d_stats;
+    p.lut = c->d_lut;
     p.sumf = c->d_sumf;
     p.fail = c->d_fail;
     p.cls = c->d_cva;
@@ -633,7 +634,8 @@ const char* brr_hip_last_error(const brr_hip_ctx* c)
 int brr_hip_upload_dense(brr_hip_ctx* c, const uint8_t* bed, uint64_t column_stride, const double* lut)
 {
     if (!c || !bed || !lut) return fail(c, BRR_HIP_EINVAL, "null argument");
-    if (c->layout != BRR_HIP_LAYOUT_DENSE_2BIT) return fail(c, BRR_HIP_EINVAL, "context layout is not dense");
+    if (c->layout != BRR_HIP_LAYOUT_DENSE_2BIT && !c->bw)
+        return fail(c, BRR_HIP_EINVAL, "context layout is not dense");
     const uint32_t N = c->cfg.num_inds, M = c->cfg.num_snps;
     const uint64_t col_bytes = ((uint64_t)N + 3) / 4;
     if (column_stride < col_bytes) return fail(c, BRR_HIP_EINVAL, "column_stride < ceil(N/4)");
@@ -648,6 +650,12 @@ int brr_hip_upload_dense(brr_hip_ctx* c, const uint8_t* bed, uint64_t column_str
                            hipMemcpyHostToDevice));
     HIP_TRY(c, hipMemcpy(c->d_lut, lut, sizeof(double) * 4 * M, hipMemcpyHostToDevice));
     c->have_markers = true;
+    if (c->bw) {
+        // a BayesW context takes its layout from the markers it is given
+        c->layout = kLayoutDense;
+        c->cfg.layout = BRR_HIP_LAYOUT_DENSE_2BIT;
+        c->have_model = false;   // sum_failure depends on the columns
+    }
     return BRR_HIP_OK;
 }
 
@@ -656,6 +664,10 @@ int brr_hip_upload_ragged(brr_hip_ctx* c, const int32_t* ones, const uint64_t* o
                           const uint64_t* miss_off, const double* stats)
 {
     if (!c || !ones_off || !twos_off || !miss_off || !stats) return fail(c, BRR_HIP_EINVAL, "null argument");
+    if (c->bw) {
+        c->layout = kLayoutRagged;
+        c->cfg.layout = BRR_HIP_LAYOUT_RAGGED;
+    }
     if (c->layout != BRR_HIP_LAYOUT_RAGGED) return fail(c, BRR_HIP_EINVAL, "context layout is not ragged");
     const uint32_t N = c->cfg.num_inds, M = c->cfg.num_snps;
     const uint64_t n1 = ones_off[M], n2 = twos_off[M], nm = miss_off[M];
@@ -1039,11 +1051,14 @@ int brr_hip_bw_create(const brr_hip_bw_config* cfg, brr_hip_ctx** out)
     c->B = kBwB;
     c->x1_words = (size_t)2 * kBwV * c->P * 2;
     c->x2_words = (size_t)2 * kBwV * 2;
-    int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, bw_kernel_for(R), kThreads, 0) != hipSuccess ||
-        (uint64_t)occ * cus < c->P) {
-        delete c;
-        return BRR_HIP_EDEVICE;
+    for (int dense = 0; dense < 2; ++dense) {
+        int occ = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, bw_kernel_for(R, dense != 0), kThreads, 0) !=
+                hipSuccess ||
+            (uint64_t)occ * cus < c->P) {
+            delete c;
+            return BRR_HIP_EDEVICE;
+        }
     }
     const size_t nscal = 5 + 2 * (size_t)K;
     int rc = BRR_HIP_OK;
@@ -1086,8 +1101,12 @@ int brr_hip_bw_set_model(brr_hip_ctx* c, const double* mixture_C, const double* 
     HIP_TRY(c, hipSetDevice(c->device));
     HIP_TRY(c, hipMemcpy(c->d_cva, mixture_C, sizeof(double) * (K - 1), hipMemcpyHostToDevice));
     HIP_TRY(c, hipMemcpy(c->d_fail, failure, sizeof(double) * N, hipMemcpyHostToDevice));
-    bw_sum_failure_kernel<<<M, kThreads, 0, c->stream>>>(c->d_dense, c->col_words, N, c->d_fail, c->d_stats,
-                                                         c->d_sumf);
+    if (c->layout == kLayoutDense)
+        bw_dense_sum_failure_kernel<<<M, kThreads, 0, c->stream>>>(c->d_dense, c->col_words, N, c->d_fail, c->d_lut,
+                                                                   c->d_sumf);
+    else
+        bw_sum_failure_kernel<<<M, kThreads, 0, c->stream>>>(c->d_dense, c->col_words, N, c->d_fail, c->d_stats,
+                                                             c->d_sumf);
     HIP_TRY(c, hipGetLastError());
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->have_model = true;
@@ -1156,7 +1175,7 @@ int brr_hip_bw_sweep(brr_hip_ctx* c, const uint32_t* order, const double* u, uin
     p.chain = chain;
     p.seed = seed;
     p.order = c->d_order;
-    rc = launch_fn(c, bw_kernel_for(c->R), &p);
+    rc = launch_fn(c, bw_kernel_for(c->R, c->layout == kLayoutDense), &p);
     c->stats.markers = M;
     return rc;
 }
@@ -1189,7 +1208,7 @@ int brr_hip_bw_run_chain(brr_hip_ctx* c, uint64_t seed, uint32_t chain, uint32_t
     p.seed = seed;
     p.order = c->d_order;
     p.hist = c->d_hist;
-    rc = launch_fn(c, bw_kernel_for(c->R), &p);
+    rc = launch_fn(c, bw_kernel_for(c->R, c->layout == kLayoutDense), &p);
     c->stats.markers = total;
     if (rc) return rc;
     if (hist) HIP_TRY(c, hipMemcpy(hist, c->d_hist, sizeof(double) * hsz, hipMemcpyDeviceToHost));
@@ -1220,7 +1239,8 @@ int brr_hip_bw_residual_update(brr_hip_ctx* c, uint32_t marker, double beta, int
     HIP_TRY(c, hipSetDevice(c->device));
     const uint32_t blocks = (c->cfg.num_inds + kThreads - 1) / kThreads;
     bw_residual_kernel<<<blocks, kThreads, 0, c->stream>>>(c->d_dense, c->col_words, c->cfg.num_inds, c->d_stats,
-                                                           marker, beta, sign, c->d_eps);
+                                                           c->layout == kLayoutDense ? c->d_lut : nullptr, marker,
+                                                           beta, sign, c->d_eps);
     HIP_TRY(c, hipGetLastError());
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return BRR_HIP_OK;

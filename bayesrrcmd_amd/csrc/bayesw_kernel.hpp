@@ -27,8 +27,8 @@
 namespace brr {
 
 constexpr int kBwB = 32;                       // slots per block
-constexpr int kBwV = 2 * kBwB + 4;             // values per round: vi_sum, (vi_1, vi_2) per slot,
-                                               // and the shifted (vi_sum, vi_1, vi_2) of a speculated slot
+constexpr int kBwV = 3 * kBwB + 8;             // values per round: vi_sum, the per-code sums of each slot
+                                               // (2 ragged, 3 dense) and the shifted sums of a speculated slot
 constexpr int kArmsN = 100;                    // ARMS envelope points (npoint of bayeswbase.cpp)
 constexpr uint32_t kStreamArms = 4;
 constexpr uint32_t kSiteMu = 0xFFFFFFF0u;
@@ -45,7 +45,8 @@ struct BwParams {
     uint64_t seed;
     const uint32_t* codes;
     uint64_t col_words, col_bytes;
-    const double* stats;          // M x 4 {mean, sd, sqrdZ, Zsum}
+    const double* stats;          // ragged: M x 4 {mean, sd, sqrdZ, Zsum}
+    const double* lut;            // dense: M x 4, the standardised value of dosage 0, 1, 2, missing
     const double* sumf;           // M: calculateSumFailure
     const double* fail;           // N
     const double* cls;            // K - 1 mixture variances C_k
@@ -87,6 +88,7 @@ struct BwShared {
     uint32_t mk[kBwB];
     uint32_t ord[2][kBwB];        // marker ids of the next two blocks, by block parity
     double bold[kBwB], mean[kBwB], sd[kBwB], sumf[kBwB], pu[kBwB];
+    double tab[kBwB][4];          // dense: column value by 2-bit code
     double xv[2][kBwV];           // round totals, by round parity
     double cls[kMaxK], pi[kMaxK], v[kMaxK];
     double gv[kBwB][kMaxK - 1][14];   // weighted integrand terms per slot, slab class and node
@@ -461,6 +463,33 @@ __device__ __forceinline__ double bw_integrand(double s, double alpha, double sq
     return exp(temp);
 }
 
+// Dense layout (densebayeswkernel.cpp:30-42, densebayesw.cpp:25-33). A column
+// holds at most four values c_d (dosage 0, 1, 2, missing), so every sum over
+// individuals regroups into sums over those groups: V_d = sum of vi over group d.
+struct BwGroups {
+    double V[4], c[4];            // by dosage 0, 1, 2, missing
+};
+
+// exponent_sum: sum_i vi_i Cx_i Cx_i
+__device__ __forceinline__ double bw_dense_exp_sum(const BwGroups& g)
+{
+    double e = 0.0;
+#pragma unroll
+    for (int d = 0; d < 4; ++d) e += g.V[d] * g.c[d] * g.c[d];
+    return e;
+}
+
+// integrand_adaptive: exp(-alpha s sf sq + sum_i vi_i (1 - exp(-Cx_i s sq alpha)) - s^2)
+__device__ __forceinline__ double bw_dense_integrand(double s, double alpha, double sq, double sumf,
+                                                     const BwGroups& g)
+{
+    double t = 0.0;
+#pragma unroll
+    for (int d = 0; d < 4; ++d) t += g.V[d] * (1 - exp(-g.c[d] * s * sq * alpha));
+    const double temp = -alpha * s * sumf * sq + t - s * s;
+    return exp(temp);
+}
+
 // Nodes and weights of the n-point adaptive rule in the order the reference
 // sums them (+x1, -x1, +x3, -x3, ...; the sixth node is -x3 for n >= 9), the
 // centre weight last.
@@ -567,10 +596,11 @@ constexpr int kBwPhases = 8;   // block load, round sums, exchange, decisions, b
         }                                                                        \
     } while (0)
 
-template <int R>
+template <int R, bool kDense>
 __global__ void __launch_bounds__(kThreads)
 bayesw_sweep_kernel(BwParams p)
 {
+    constexpr int kPer = kDense ? 3 : 2;   // per-code sums published per slot (codes 2, 0 [, 1])
     __shared__ BwShared<R> L;
     const uint32_t w = blockIdx.x;
     const int tid = threadIdx.x;
@@ -636,7 +666,13 @@ bayesw_sweep_kernel(BwParams p)
     // eps += sign * delta of slot k's column, then vi
     auto apply = [&](int k, double beta, double sign) {
         double dv[4];
-        bw_delta(L.mean[k], L.sd[k], beta, dv);
+        if constexpr (kDense) {
+            // calculateResidualUpdate: Cx * beta (densebayeswkernel.cpp:20-23)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) dv[c] = L.tab[k][c] * beta;
+        } else {
+            bw_delta(L.mean[k], L.sd[k], beta, dv);
+        }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const uint32_t bits = L.codes[k][r][lane >> 1] >> (16 * (lane & 1));
@@ -707,6 +743,7 @@ bayesw_sweep_kernel(BwParams p)
         uint32_t nw[kWords];
         uint32_t nmk = 0, noj = 0;
         double nbold = 0.0, nmean = 0.0, nsd = 0.0, nsumf = 0.0, npu = 0.0;
+        double nt[4] = {0.0, 0.0, 0.0, 0.0};
         auto issue = [&](uint32_t nblk) {
             const uint32_t c0 = nblk * kBwB;
             const int cnb = (int)min((uint32_t)kBwB, p.M - c0);
@@ -715,8 +752,17 @@ bayesw_sweep_kernel(BwParams p)
                 const uint32_t j = ids[tid];
                 nmk = j;
                 nbold = bits_as_double(ld_agent((const uint64_t*)(p.beta + j)));
-                nmean = p.stats[4 * (size_t)j];
-                nsd = p.stats[4 * (size_t)j + 1];
+                if constexpr (kDense) {
+                    // the column value of each 2-bit code (00 -> dosage 2, 01 missing, 10 -> 1, 11 -> 0)
+                    const double* lt = p.lut + 4 * (size_t)j;
+                    nt[0] = lt[2];
+                    nt[1] = lt[3];
+                    nt[2] = lt[1];
+                    nt[3] = lt[0];
+                } else {
+                    nmean = p.stats[4 * (size_t)j];
+                    nsd = p.stats[4 * (size_t)j + 1];
+                }
                 nsumf = p.sumf[j];
                 if (p.mode == kModeChain) {
                     const U4 x = draw_block(p.seed, j, iteration, kStreamMarker, p.chain);
@@ -742,6 +788,8 @@ bayesw_sweep_kernel(BwParams p)
                 L.bold[tid] = nbold;
                 L.mean[tid] = nmean;
                 L.sd[tid] = nsd;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) L.tab[tid][c] = nt[c];
                 L.sumf[tid] = nsumf;
                 L.pu[tid] = npu;
             }
@@ -777,18 +825,18 @@ bayesw_sweep_kernel(BwParams p)
                 const int h = __builtin_amdgcn_readfirstlane(hm ? pos + (int)__ffsll((long long)hm) - 1 : nb);
                 const bool breaker = h == pos;
                 const bool spec = !breaker && h < nb;
-                const int V0 = breaker ? 0 : 1 + 2 * (h - pos);   // first value of the shifted sums
+                const int V0 = breaker ? 0 : 1 + kPer * (h - pos);   // first value of the shifted sums
                 ++rnd;
                 ++n_rounds;
                 int V;
                 if (!breaker) {
                     // vi_sum of the slice and (vi_1, vi_2) of slots pos..h-1: wave q
                     // takes slots pos + q + 4 i, values 2 i (genotype 1) and 2 i + 1
-                    double part[16];
+                    double part[16], part8[8];
 #pragma unroll
                     for (int i = 0; i < 8; ++i) {
                         const int k = pos + wave + kWaves * i;
-                        double s1 = 0.0, s2 = 0.0;
+                        double s1 = 0.0, s2 = 0.0, s3 = 0.0;
                         if (k < h) {
 #pragma unroll
                             for (int r = 0; r < R; ++r) {
@@ -798,17 +846,25 @@ bayesw_sweep_kernel(BwParams p)
                                     const uint32_t c = (bits >> (2 * e)) & 3u;
                                     s1 += (c == 2u) ? vi[r][e] : 0.0;
                                     s2 += (c == 0u) ? vi[r][e] : 0.0;
+                                    if constexpr (kDense) s3 += (c == 1u) ? vi[r][e] : 0.0;
                                 }
                             }
                         }
                         part[2 * i] = s1;
                         part[2 * i + 1] = s2;
+                        part8[i] = s3;
                     }
                     const double tot = transpose_reduce16(part, lane);
                     const int vv = (lane >> 2) & 15;
                     const int k = pos + wave + kWaves * (vv >> 1);
                     if ((lane & 3) == 0 && k < h)
-                        publish_pair(bw_slot1(p, rnd, 1 + 2 * (k - pos) + (vv & 1), w), rnd, tot);
+                        publish_pair(bw_slot1(p, rnd, 1 + kPer * (k - pos) + (vv & 1), w), rnd, tot);
+                    if constexpr (kDense) {
+                        const double t8 = transpose_reduce8(part8, lane);
+                        const int k8 = pos + wave + kWaves * ((lane >> 3) & 7);
+                        if ((lane & 7) == 0 && k8 < h)
+                            publish_pair(bw_slot1(p, rnd, 1 + kPer * (k8 - pos) + 2, w), rnd, t8);
+                    }
                     double sv = 0.0;
 #pragma unroll
                     for (int r = 0; r < R; ++r)
@@ -821,9 +877,14 @@ bayesw_sweep_kernel(BwParams p)
                         // h's sums of vi' = exp(alpha (eps + delta_h) - EulerGamma) (wave 3)
                         if (wave == kWaves - 1) {
                             double dv[4];
-                            bw_delta(L.mean[h], L.sd[h], L.bold[h], dv);
+                            if constexpr (kDense) {
+#pragma unroll
+                                for (int c = 0; c < 4; ++c) dv[c] = L.tab[h][c] * L.bold[h];
+                            } else {
+                                bw_delta(L.mean[h], L.sd[h], L.bold[h], dv);
+                            }
                             const double a = L.alpha;
-                            double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+                            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
 #pragma unroll
                             for (int r = 0; r < R; ++r) {
                                 const uint32_t bits = L.codes[h][r][lane >> 1] >> (16 * (lane & 1));
@@ -836,24 +897,27 @@ bayesw_sweep_kernel(BwParams p)
                                     s0 += x;
                                     s1 += (c == 2u) ? x : 0.0;
                                     s2 += (c == 0u) ? x : 0.0;
+                                    if constexpr (kDense) s3 += (c == 1u) ? x : 0.0;
                                 }
                             }
                             s0 = wave_sum(s0);
                             s1 = wave_sum(s1);
                             s2 = wave_sum(s2);
+                            if constexpr (kDense) s3 = wave_sum(s3);
                             if (lane == 0) {
                                 publish_pair(bw_slot1(p, rnd, V0, w), rnd, s0);
                                 publish_pair(bw_slot1(p, rnd, V0 + 1, w), rnd, s1);
                                 publish_pair(bw_slot1(p, rnd, V0 + 2, w), rnd, s2);
+                                if constexpr (kDense) publish_pair(bw_slot1(p, rnd, V0 + 3, w), rnd, s3);
                             }
                         }
-                        V = V0 + 3;
+                        V = V0 + 1 + kPer;
                     }
                 } else {
                     // a marker in the model: eps += x beta_old, vi re-formed, then its sums
                     apply(pos, L.bold[pos], 1.0);
                     BW_TICK(7);
-                    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+                    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
 #pragma unroll
                     for (int r = 0; r < R; ++r) {
                         const uint32_t bits = L.codes[pos][r][lane >> 1] >> (16 * (lane & 1));
@@ -863,17 +927,20 @@ bayesw_sweep_kernel(BwParams p)
                             s0 += vi[r][e];
                             s1 += (c == 2u) ? vi[r][e] : 0.0;
                             s2 += (c == 0u) ? vi[r][e] : 0.0;
+                            if constexpr (kDense) s3 += (c == 1u) ? vi[r][e] : 0.0;
                         }
                     }
                     s0 = wave_sum(s0);
                     s1 = wave_sum(s1);
                     s2 = wave_sum(s2);
+                    if constexpr (kDense) s3 = wave_sum(s3);
                     if (tid == 0) {
                         publish_pair(bw_slot1(p, rnd, 0, w), rnd, s0);
                         publish_pair(bw_slot1(p, rnd, 1, w), rnd, s1);
                         publish_pair(bw_slot1(p, rnd, 2, w), rnd, s2);
+                        if constexpr (kDense) publish_pair(bw_slot1(p, rnd, 3, w), rnd, s3);
                     }
-                    V = 3;
+                    V = 1 + kPer;
                 }
                 BW_TICK(1);
                 if (!bw_exchange<R>(p, L, rnd, V, w, wave, lane)) goto done;
@@ -894,24 +961,45 @@ bayesw_sweep_kernel(BwParams p)
                         const int sl = t / per, rem = t - sl * per, i = rem / nn, n = rem - i * nn;
                         const int k = pos + sl;
                         const bool sh = spec && k == h;
+                        const int b = sh ? V0 + 1 : 1 + kPer * sl;
                         const double vs = sh ? xv[V0] : xv[0];
-                        const double s1 = sh ? xv[V0 + 1] : xv[1 + 2 * sl], s2 = sh ? xv[V0 + 2] : xv[2 + 2 * sl];
-                        const BwSums v{vs, vs - s1 - s2, s1, s2};
-                        const double mean = L.mean[k], sd = L.sd[k], Ck = L.cls[i];
-                        const double sigma = bw_sigma(L.alpha, L.sigma_b, Ck, mean, sd, v);
+                        const double s1 = xv[b], s2 = xv[b + 1];
+                        const double Ck = L.cls[i];
                         const double sq = sqrt(2 * Ck * L.sigma_b);
-                        L.gv[sl][i][n] = L.ghw[n] * bw_integrand(sigma * L.ghx[n], L.alpha, sq, L.sumf[k], mean / sd,
-                                                                  sd, v);
+                        double f;
+                        if constexpr (kDense) {
+                            const double s3 = xv[b + 2];
+                            const BwGroups g{{vs - s1 - s2 - s3, s1, s2, s3},
+                                             {L.tab[k][3], L.tab[k][2], L.tab[k][0], L.tab[k][1]}};
+                            const double sigma =
+                                1.0 / sqrt(1 + L.alpha * L.alpha * L.sigma_b * Ck * bw_dense_exp_sum(g));
+                            f = bw_dense_integrand(sigma * L.ghx[n], L.alpha, sq, L.sumf[k], g);
+                        } else {
+                            const BwSums v{vs, vs - s1 - s2, s1, s2};
+                            const double mean = L.mean[k], sd = L.sd[k];
+                            const double sigma = bw_sigma(L.alpha, L.sigma_b, Ck, mean, sd, v);
+                            f = bw_integrand(sigma * L.ghx[n], L.alpha, sq, L.sumf[k], mean / sd, sd, v);
+                        }
+                        L.gv[sl][i][n] = L.ghw[n] * f;
                     }
                     __syncthreads();
                     for (int t = tid; t < ns * ncls; t += kThreads) {
                         const int sl = t / ncls, i = t - sl * ncls;
                         const int k = pos + sl;
                         const bool sh = spec && k == h;
+                        const int b = sh ? V0 + 1 : 1 + kPer * sl;
                         const double vs = sh ? xv[V0] : xv[0];
-                        const double s1 = sh ? xv[V0 + 1] : xv[1 + 2 * sl], s2 = sh ? xv[V0 + 2] : xv[2 + 2 * sl];
-                        const BwSums v{vs, vs - s1 - s2, s1, s2};
-                        const double sigma = bw_sigma(L.alpha, L.sigma_b, L.cls[i], L.mean[k], L.sd[k], v);
+                        const double s1 = xv[b], s2 = xv[b + 1];
+                        double sigma;
+                        if constexpr (kDense) {
+                            const double s3 = xv[b + 2];
+                            const BwGroups g{{vs - s1 - s2 - s3, s1, s2, s3},
+                                             {L.tab[k][3], L.tab[k][2], L.tab[k][0], L.tab[k][1]}};
+                            sigma = 1.0 / sqrt(1 + L.alpha * L.alpha * L.sigma_b * L.cls[i] * bw_dense_exp_sum(g));
+                        } else {
+                            const BwSums v{vs, vs - s1 - s2, s1, s2};
+                            sigma = bw_sigma(L.alpha, L.sigma_b, L.cls[i], L.mean[k], L.sd[k], v);
+                        }
                         double temp = L.gv[sl][i][0];
                         for (int n = 1; n < nn; ++n) temp = temp + L.gv[sl][i][n];
                         temp = temp + L.ghwc;
@@ -958,28 +1046,48 @@ bayesw_sweep_kernel(BwParams p)
                             apply(c, L.bold[c], 1.0);
                             BW_TICK(7);
                         }
-                        const int i1 = sh ? V0 + 1 : 1 + 2 * (c - pos);
+                        const int i1 = sh ? V0 + 1 : 1 + kPer * (c - pos);
                         // draw the slot's beta: 0 for class 0, else ARMS on beta_dens
                         double bnew = 0.0;
                         if (sel > 0) {
                             const double s1 = xv[i1];
                             const double s2 = xv[i1 + 1];
                             const double vs = xv[sh ? V0 : 0];
-                            const double v0 = vs - s1 - s2;
                             const double a = L.alpha, sb = L.sigma_b, Ck = L.cls[sel - 1];
-                            const double sf = L.sumf[c], msr = L.mean[c] / L.sd[c], sd = L.sd[c];
+                            const double sf = L.sumf[c];
                             const double bo = L.bold[c];
                             const double safe = 2 * sqrt(sb * Ck);
                             const double xinit[4] = {bo - safe / 10, bo, bo + safe / 20, bo + safe / 10};
-                            const int err = arms_draw<true>(
-                                L.env[wave], xinit, bo - safe, bo + safe, p.seed, L.mk[c], iteration, p.chain,
-                                [&](double x) {
-                                    // beta_dens (sparsebayesw.cpp:26-37)
-                                    return -a * x * sf -
-                                           exp(a * x * msr) * (v0 + s1 * exp(-a * x / sd) + s2 * exp(-2 * a * x / sd)) -
-                                           x * x / (2 * Ck * sb);
-                                },
-                                &bnew);
+                            int err;
+                            if constexpr (kDense) {
+                                const double s3 = xv[i1 + 2];
+                                const BwGroups g{{vs - s1 - s2 - s3, s1, s2, s3},
+                                                 {L.tab[c][3], L.tab[c][2], L.tab[c][0], L.tab[c][1]}};
+                                err = arms_draw<true>(
+                                    L.env[wave], xinit, bo - safe, bo + safe, p.seed, L.mk[c], iteration, p.chain,
+                                    [&](double x) {
+                                        // DenseBayesW beta_dens (densebayesw.cpp:25-33): sum_i exp((eps_i - Cx_i x)
+                                        // alpha - EulerGamma) = sum over groups V_d exp(-Cx_d x alpha)
+                                        double t = 0.0;
+#pragma unroll
+                                        for (int d = 0; d < 4; ++d) t += g.V[d] * exp(-(g.c[d] * x) * a);
+                                        return -a * x * sf - t - x * x / (2 * Ck * sb);
+                                    },
+                                    &bnew);
+                            } else {
+                                const double v0 = vs - s1 - s2;
+                                const double msr = L.mean[c] / L.sd[c], sd = L.sd[c];
+                                err = arms_draw<true>(
+                                    L.env[wave], xinit, bo - safe, bo + safe, p.seed, L.mk[c], iteration, p.chain,
+                                    [&](double x) {
+                                        // beta_dens (sparsebayesw.cpp:26-37)
+                                        return -a * x * sf -
+                                               exp(a * x * msr) *
+                                                   (v0 + s1 * exp(-a * x / sd) + s2 * exp(-2 * a * x / sd)) -
+                                               x * x / (2 * Ck * sb);
+                                    },
+                                    &bnew);
+                            }
                             if (err) {
                                 if (tid == 0) { L.arms_err = err; raise_abort(p, 31); }
                                 __syncthreads();
@@ -1147,6 +1255,27 @@ __global__ void bw_sum_failure_kernel(const uint32_t* codes, uint64_t col_words,
     }
 }
 
+// DenseBayesWKernel::calculateSumFailure (densebayeswkernel.cpp:15-18): sum_i Cx_i f_i.
+__global__ void bw_dense_sum_failure_kernel(const uint32_t* codes, uint64_t col_words, uint32_t N, const double* fail,
+                                            const double* lut, double* sumf)
+{
+    __shared__ double red[kWaves];
+    const uint32_t j = blockIdx.x;
+    const int tid = threadIdx.x;
+    const uint8_t* col = (const uint8_t*)(codes + (size_t)j * col_words);
+    const double* lt = lut + 4 * (size_t)j;
+    const double w[4] = {lt[2], lt[3], lt[1], lt[0]};   // by 2-bit code
+    double s = 0.0;
+    for (uint32_t i = tid; i < N; i += kThreads) {
+        const uint32_t c = (col[i >> 2] >> ((i & 3u) * 2u)) & 3u;
+        s += w[c] * fail[i];
+    }
+    s = wave_sum(s);
+    if ((tid & 63) == 0) red[tid >> 6] = s;
+    __syncthreads();
+    if (tid == 0) sumf[j] = ((red[0] + red[1]) + red[2]) + red[3];
+}
+
 // RaggedBayesWKernel::setVi (raggedbayeswkernel.cpp:10-16) of marker j for
 // vi = exp(alpha eps - EulerGamma): out = {vi_sum, vi_0, vi_1, vi_2}. One workgroup.
 __global__ void bw_set_vi_kernel(const uint32_t* codes, uint64_t col_words, uint32_t N, const double* eps,
@@ -1180,15 +1309,24 @@ __global__ void bw_set_vi_kernel(const uint32_t* codes, uint64_t col_words, uint
 }
 
 // eps += sign * calculateResidualUpdate(beta) of marker j (raggedbayeswkernel.cpp:27-43).
+// Dense layout (lut != null): Cx * beta (densebayeswkernel.cpp:20-23).
 __global__ void bw_residual_kernel(const uint32_t* codes, uint64_t col_words, uint32_t N, const double* stats,
-                                   uint32_t j, double beta, int sign, double* eps)
+                                   const double* lut, uint32_t j, double beta, int sign, double* eps)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     const uint8_t* col = (const uint8_t*)(codes + (size_t)j * col_words);
     const uint32_t c = (col[i >> 2] >> ((i & 3u) * 2u)) & 3u;
     double dv[4];
-    bw_delta(stats[4 * (size_t)j], stats[4 * (size_t)j + 1], beta, dv);
+    if (lut) {
+        const double* lt = lut + 4 * (size_t)j;
+        dv[0] = lt[2] * beta;
+        dv[1] = lt[3] * beta;
+        dv[2] = lt[1] * beta;
+        dv[3] = lt[0] * beta;
+    } else {
+        bw_delta(stats[4 * (size_t)j], stats[4 * (size_t)j + 1], beta, dv);
+    }
     const double d = dv[c];
     eps[i] = (sign > 0) ? eps[i] + d : eps[i] - d;
 }

@@ -173,8 +173,8 @@ __device__ __forceinline__ double wave_sum(double x)
 // every workgroup.
 // One halving stage with compile-time extents, so every array index is static
 // (a runtime index would turn each access into a 16-way select chain).
-template <int kHalf, int kBit>
-__device__ __forceinline__ void transpose_stage(double (&v)[16], int lane)
+template <int kHalf, int kBit, int kN>
+__device__ __forceinline__ void transpose_stage(double (&v)[kN], int lane)
 {
     const bool up = (lane & kBit) != 0;
 #pragma unroll
@@ -195,6 +195,20 @@ __device__ __forceinline__ double transpose_reduce16(double (&v)[16], int lane)
     x += __shfl_xor(x, 2, 64);
     x += __shfl_xor(x, 1, 64);
     return x;   // total of value ((lane >> 2) & 15)
+}
+
+// The same for 8 values: three halving exchanges (xor 32, 16, 8), then a
+// butterfly over xor 4, 2, 1; lanes 8i..8i+7 end with value i's total.
+__device__ __forceinline__ double transpose_reduce8(double (&v)[8], int lane)
+{
+    transpose_stage<4, 32>(v, lane);
+    transpose_stage<2, 16>(v, lane);
+    transpose_stage<1, 8>(v, lane);
+    double x = v[0];
+    x += __shfl_xor(x, 4, 64);
+    x += __shfl_xor(x, 2, 64);
+    x += __shfl_xor(x, 1, 64);
+    return x;   // total of value ((lane >> 3) & 7)
 }
 
 // Value of a 2-bit code: two flat selects on 64-bit halves (no control flow).

@@ -56,6 +56,10 @@ CONFIGS = {
     "c5": dict(workload="BayesW Weibull survival, ragged layout, N=100000 M=50000, single chain",
                n=100_000, m=50_000, layout="ragged", maf=(0.005, 0.5), causal=0.005, h2=0.5, alpha=3.0,
                mu=4.0, censor=0.1, C=(0.001, 0.01, 0.1), quad_points=7, chain=100),
+    "c5-dense": dict(workload="BayesW Weibull survival, dense .bed 2-bit layout (DenseBayesW), N=100000 M=50000, "
+                              "single chain",
+                     n=100_000, m=50_000, layout="dense", maf=(0.005, 0.5), causal=0.005, h2=0.5, alpha=3.0,
+                     mu=4.0, censor=0.1, C=(0.001, 0.01, 0.1), quad_points=7, chain=100),
 }
 
 
@@ -253,9 +257,13 @@ def make_bw_data(cfg, seed, threads):
     logt[cens] += np.log(rng.uniform(0.0, 1.0, cens.size))
     y = logt.astype(np.float32).astype(np.float64)
     t0 = time.time()
+    if cfg["layout"] == "dense":
+        lut = brr.build_dense(bed, n, threads=threads)
+        log(f"dense tables built in {time.time() - t0:.1f}s")
+        return dict(bed=bed, y=y, fail=fail, ragged=None, lut=lut)
     arena = brr.build_ragged(bed, n, threads=threads)
     log(f"ragged lists built in {time.time() - t0:.1f}s ({arena.nbytes() / 1e9:.2f} GB)")
-    return dict(bed=bed, y=y, fail=fail, ragged=arena)
+    return dict(bed=bed, y=y, fail=fail, ragged=arena, lut=None)
 
 
 def cpu_baseline_bw(cfg, data, seed, budget_s):
@@ -267,18 +275,24 @@ def cpu_baseline_bw(cfg, data, seed, budget_s):
     import _oracle as orc
     L = orc.lib(fast=True)
     n = cfg["n"]
-    m_s = min(cfg["m"], 1000)
-    r = data["ragged"]
+    if cfg["layout"] == "dense":
+        # the dense kernel's sums run over every individual per quadrature node and density value
+        m_s = min(cfg["m"], 100)
+        w = orc.BayesWProblem(n, m_s, None, data["fail"], K=4, C=tuple(cfg["C"]), quad_points=cfg["quad_points"],
+                              bed=np.ascontiguousarray(data["bed"][:m_s]), lut=np.ascontiguousarray(data["lut"][:m_s]))
+    else:
+        m_s = min(cfg["m"], 1000)
+        r = data["ragged"]
 
-    class Sub:
-        pass
-    sub = Sub()
-    sub.ones_off = np.ascontiguousarray(r.ones_off[: m_s + 1])
-    sub.twos_off = np.ascontiguousarray(r.twos_off[: m_s + 1])
-    sub.miss_off = np.ascontiguousarray(r.miss_off[: m_s + 1])
-    sub.ones, sub.twos, sub.miss = r.ones, r.twos, r.miss
-    sub.stats = np.ascontiguousarray(r.stats[:m_s])
-    w = orc.BayesWProblem(n, m_s, sub, data["fail"], K=4, C=tuple(cfg["C"]), quad_points=cfg["quad_points"])
+        class Sub:
+            pass
+        sub = Sub()
+        sub.ones_off = np.ascontiguousarray(r.ones_off[: m_s + 1])
+        sub.twos_off = np.ascontiguousarray(r.twos_off[: m_s + 1])
+        sub.miss_off = np.ascontiguousarray(r.miss_off[: m_s + 1])
+        sub.ones, sub.twos, sub.miss = r.ones, r.twos, r.miss
+        sub.stats = np.ascontiguousarray(r.stats[:m_s])
+        w = orc.BayesWProblem(n, m_s, sub, data["fail"], K=4, C=tuple(cfg["C"]), quad_points=cfg["quad_points"])
     y = np.ascontiguousarray(data["y"])
     fail = w.fail
     L.orc_bw_init(C.byref(w.data), C.byref(w.model), C.byref(w.state), orc.p(y, C.c_double),
@@ -302,23 +316,27 @@ def cpu_baseline_bw(cfg, data, seed, budget_s):
         iters += 1
     return dict(value=iters * m_s / t_total, unit="marker-updates/s", cores=1, kind="port",
                 sample=(f"oracle (FP64 restatement of the reference BayesW chain, -O3 -ffast-math "
-                        f"-funroll-loops, 1 thread): first {m_s} markers of the same ragged workload at N={n}, "
-                        f"{iters} chain iterations from init, {t_total:.1f}s"))
+                        f"-funroll-loops, 1 thread): first {m_s} markers of the same {cfg['layout']} workload at "
+                        f"N={n}, {iters} chain iterations from init, {t_total:.1f}s"))
 
 
 def run_bayesw(cfg, args, with_cpu):
     """The BayesW workload on the device: returns the fields of its JSON line."""
     import bayesrrcmd_amd as brr
     n, m = cfg["n"], cfg["m"]
-    steps = args.steps if (args.steps is not None and args.config == "c5") else cfg["chain"]
+    steps = args.steps if (args.steps is not None and args.config in ("c5", "c5-dense")) else cfg["chain"]
     data = make_bw_data(cfg, args.seed, args.threads)
     K = len(cfg["C"]) + 1
     h = brr.HipBayesW(n, m, K, quad_points=cfg["quad_points"], workgroups=args.workgroups)
     t0 = time.time()
-    h.upload_ragged(data["ragged"])
+    if cfg["layout"] == "dense":
+        h.upload_dense(data["bed"], data["lut"])
+        nnz = 0
+    else:
+        h.upload_ragged(data["ragged"])
+        nnz = data["ragged"].ones.size + data["ragged"].twos.size
     h.set_model(np.array(cfg["C"]), data["fail"])
     log(f"bayesw: uploaded in {time.time() - t0:.1f}s")
-    nnz = data["ragged"].ones.size + data["ragged"].twos.size
 
     def fresh_chain():
         eps, sc = brr.bayesw_init(data["y"], data["fail"], m, K)
@@ -367,11 +385,12 @@ def run_bayesw(cfg, args, with_cpu):
         "ms_per_step": wall * 1e3 / steps,
         "dtype": "f64",
         "config": {
-            "workload": cfg["workload"], "N": n, "M": m, "layout": "ragged (held on the device as 2-bit codes)",
+            "workload": cfg["workload"], "N": n, "M": m,
+            "layout": ("dense 2-bit" if cfg["layout"] == "dense" else "ragged (held on the device as 2-bit codes)"),
             "maf": list(cfg["maf"]), "causal_fraction": cfg["causal"], "C": list(cfg["C"]),
             "quad_points": cfg["quad_points"], "censored_fraction": cfg["censor"], "timed_iterations": steps,
             "precision": "fp64", "workgroups": stats["workgroups"], "block_markers": stats["block_markers"],
-            "slice": stats["slice"], "ragged_list_bytes": 4 * nnz,
+            "slice": stats["slice"], "ragged_list_bytes": 4 * nnz if nnz else None,
             "parallelism": "single GPU, persistent cooperative kernel",
             "alpha_posterior_mean": float(hist[burn:, 1].mean()) if steps > 1 else float("nan"),
             "mu_posterior_mean": float(hist[burn:, 0].mean()) if steps > 1 else float("nan"),
@@ -523,7 +542,7 @@ def main():
         log("calibration copy: 1 GiB read + 1 GiB written")
         del a, b
     cfg = CONFIGS[args.config]
-    if args.config in ("c3", "c5"):
+    if args.config in ("c3", "c5", "c5-dense"):
         b = (run_batched if args.config == "c3" else run_bayesw)(cfg, args, not args.no_cpu)
         line = {"metric": METRIC, "value": b["value"], "unit": b["unit"], "n_gpus": 1, "steps": b["steps"],
                 "warmup": args.warmup, "ms_per_step": b["ms_per_step"], "higher_is_better": True,

@@ -193,10 +193,13 @@ int brr_hip_last_phase_ms(const brr_hip_ctx* ctx, double* out, uint32_t n);
 
 /* ---- BayesW: Weibull survival model on ragged markers ---- */
 
-/* A BayesW context holds N x {epsilon, failure indicator}, the ragged columns as
- * 2-bit codes, beta and the mixture component per marker. Markers are uploaded
- * with brr_hip_upload_ragged; brr_hip_set_state / brr_hip_get_state move
- * epsilon, beta and components (acum is not kept: pass NULL). */
+/* A BayesW context holds N x {epsilon, failure indicator}, the columns as 2-bit
+ * codes, beta and the mixture component per marker. Markers are uploaded with
+ * brr_hip_upload_ragged (RaggedBayesWKernel, src/raggedbayeswkernel.cpp) or
+ * brr_hip_upload_dense (DenseBayesWKernel, src/densebayeswkernel.cpp:10-42 with
+ * DenseBayesW::beta_dens, src/densebayesw.cpp:25-33); the last upload fixes the
+ * layout and must be followed by brr_hip_bw_set_model. brr_hip_set_state /
+ * brr_hip_get_state move epsilon, beta and components (acum is not kept: pass NULL). */
 typedef struct {
     uint32_t num_inds;       /* N */
     uint32_t num_snps;       /* M */
@@ -246,6 +249,7 @@ int brr_hip_bw_run_chain(brr_hip_ctx* ctx, uint64_t seed, uint32_t chain,
 /* Per-marker kernel calls on the device epsilon (correctness path).
  * set_vi: RaggedBayesWKernel::setVi of vi = exp(alpha eps - EulerGamma):
  *   out[4] = {vi_sum, vi_0, vi_1, vi_2}.
+ *   (the sums over the individuals with genotype 0 (incl. missing calls), 1 and 2)
  * residual_update: epsilon += sign * calculateResidualUpdate(beta), sign = +1 / -1.
  * sum_failure: calculateSumFailure of every marker (M doubles). */
 int brr_hip_bw_set_vi(brr_hip_ctx* ctx, uint32_t marker, double alpha, double* out4);

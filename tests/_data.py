@@ -68,11 +68,11 @@ class BwProblem:
 
 
 def make_bw_problem(n, m, seed, h2=0.5, causal_frac=0.05, censor=0.1, alpha=3.0, mu=4.0,
-                    maf=(0.05, 0.5)) -> BwProblem:
+                    maf=(0.05, 0.5), miss=0.0) -> BwProblem:
     """Weibull survival data: log T = mu + g + (log E + EulerGamma) / alpha,
     E ~ Exp(1); var(g) / (var(g) + pi^2 / (6 alpha^2)) = h2; a fraction of the
     individuals is right-censored at C ~ U(0, T) (fail = 0, time = C)."""
-    bed, _ = brr.synth_bed(seed, n, m, maf[0], maf[1], 0.0)
+    bed, _ = brr.synth_bed(seed, n, m, maf[0], maf[1], miss)
     ragged = brr.build_ragged(bed, n)
     lut = brr.build_dense(bed, n)
     rng = np.random.default_rng(seed + 1)

@@ -219,3 +219,89 @@ def test_bw_bad_calls():
         assert rc == brr._abi.ESTATE
         with pytest.raises(brr.BrrHipError, match="sign"):
             h.residual_update(0, 0.1, 0)
+
+
+# ------------------------------------------------------------- dense layout --
+
+def dense_setup(pr, K, nq, workgroups, seed):
+    lut = brr.build_dense(pr.bed, pr.n)
+    w = orc.BayesWProblem(pr.n, pr.m, None, pr.fail, K=K, C=classes(K), quad_points=nq, bed=pr.bed, lut=lut)
+    w.init(pr.y)
+    assert orc.lib().orc_bw_sample_mu(orc.C.byref(w.data), orc.C.byref(w.state), seed, 0, 0) == 0
+    w.refresh_vi()
+    h = brr.HipBayesW(pr.n, pr.m, K, quad_points=nq, workgroups=workgroups)
+    h.upload_dense(pr.bed, lut)
+    h.set_model(np.array(classes(K)), pr.fail)
+    return w, h
+
+
+@pytest.mark.parametrize("n,m,K,nq,wg,frac,miss", [
+    (3000, 160, 4, 7, 0, 0.1, 0.0),
+    (2600, 120, 4, 9, 5, 0.1, 0.04),     # missing calls (mean-imputed in the dense column), R = 2
+    (6000, 64, 3, 15, 4, 0.15, 0.02),    # R = 4
+    (1999, 77, 8, 5, 7, 0.2, 0.0),
+])
+def test_bw_dense_sweep_matches_oracle(n, m, K, nq, wg, frac, miss):
+    """DenseBayesWKernel on the device: the O(N) sums of the reference regrouped by
+    the column's four values, against the oracle's element-by-element sums."""
+    seed = 2000 + n + m
+    pr = make_bw_problem(n, m, seed=seed, causal_frac=frac, miss=miss)
+    w, h = dense_setup(pr, K, nq, wg, seed)
+    with h:
+        close(h.sum_failure(), w.sum_failure, rtol=1e-12, what="sum_failure")
+        for it in range(3):
+            push(w, h)
+            order, u = order_u(seed, it, m)
+            w.v[:] = 1.0
+            assert w.sweep(order, u, seed, it) == 0
+            h.sweep(order, u, seed, it)
+            compare(w, h, f"dense sweep {it}")
+        assert np.count_nonzero(w.beta) > 0
+
+
+def test_bw_dense_chain_matches_oracle():
+    seed = 4343
+    n, m, iters = 2500, 100, 3
+    pr = make_bw_problem(n, m, seed=seed, causal_frac=0.1)
+    lut = brr.build_dense(pr.bed, n)
+    w = orc.BayesWProblem(n, m, None, pr.fail, bed=pr.bed, lut=lut)
+    err, hist_o, _ = w.run_gibbs(pr.y, seed, iters)
+    assert err == 0
+    eps0, sc = brr.bayesw_init(pr.y, pr.fail, m, 4)
+    orders = []
+    order = np.arange(m, dtype=np.uint32)
+    for it in range(iters):
+        brr.shuffle(seed, it, 0, order)
+        orders.append(order.copy())
+    with brr.HipBayesW(n, m, 4, quad_points=7) as h:
+        h.upload_dense(pr.bed, lut)
+        h.set_model(np.array(classes(4)), pr.fail)
+        h.set_state(eps0, np.zeros(m), np.zeros(m, dtype=np.int32))
+        h.set_scalars(sc["mu"], sc["alpha"], sc["sigma_b"], sc["d"], sc["pi"], sc["v"])
+        hist = h.run_chain(seed, 0, 0, np.stack(orders))
+        np.testing.assert_array_equal(hist[:, 3], hist_o[:, 3])
+        close(hist, hist_o, rtol=1e-7, what="dense chain history")
+        d = h.get_state()
+        np.testing.assert_array_equal(d["comp"], w.comp)
+        close(d["beta"], w.beta, rtol=1e-7, what="dense chain beta")
+        close(d["eps"], w.eps, rtol=1e-7, what="dense chain eps")
+
+
+def test_bw_dense_per_marker_calls():
+    pr = make_bw_problem(3000, 10, seed=7, causal_frac=0.2)
+    lut = brr.build_dense(pr.bed, pr.n)
+    from _data import spec_dosage
+    d = spec_dosage(pr.bed, pr.n)
+    x = np.take_along_axis(lut, np.where(d < 0, 3, d), axis=1)
+    rng = np.random.default_rng(4)
+    eps = rng.normal(0.0, 0.3, pr.n)
+    with brr.HipBayesW(pr.n, pr.m, 4) as h:
+        h.upload_dense(pr.bed, lut)
+        h.set_model(np.array(classes(4)), pr.fail)
+        close(h.sum_failure(), x @ pr.fail, rtol=1e-12, what="dense sum_failure")
+        h.set_state(eps=eps)
+        e = eps.copy()
+        for j, b, sgn in [(1, 0.4, 1), (6, -0.2, -1)]:
+            e = e + sgn * (x[j] * b)
+            h.residual_update(j, b, sgn)
+        close(h.get_state()["eps"], e, rtol=1e-15, what="dense residual updates")
