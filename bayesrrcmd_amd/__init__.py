@@ -28,7 +28,7 @@ from . import _abi
 from ._abi import LAYOUT_DENSE_2BIT, LAYOUT_RAGGED, STORAGE_AUTO, STORAGE_CODES, STORAGE_LISTS
 
 __all__ = [
-    "BrrHipError", "HipSweep", "HipBayesW", "bayesw_init", "RaggedArena", "read_ragged_sparsebed", "read_ppbed", "SampleWriter", "sample_due", "build_dense", "build_ragged", "synth_bed", "synth_xb",
+    "BrrHipError", "HipSweep", "HipBayesW", "bayesw_init", "bed_file_image", "RaggedArena", "read_ragged_sparsebed", "read_ppbed", "SampleWriter", "sample_due", "build_dense", "build_ragged", "synth_bed", "synth_xb",
     "bayesr_init", "shuffle", "marker_uz", "LAYOUT_DENSE_2BIT", "LAYOUT_RAGGED", "lib",
     "STORAGE_AUTO", "STORAGE_CODES", "STORAGE_LISTS",
 ]
@@ -254,9 +254,43 @@ def marker_uz(seed: int, iteration: int, chain: int, m: int):
     return u, z
 
 
+def bed_file_image(bed: np.ndarray) -> np.ndarray:
+    """A PLINK .bed file's bytes (magic 6c 1b 01, then the SNP-major columns) from M x ceil(N/4) column bytes."""
+    bed = np.ascontiguousarray(bed, dtype=np.uint8)
+    return np.concatenate([np.array([0x6C, 0x1B, 0x01], dtype=np.uint8), bed.reshape(-1)])
+
+
+def _as_file_bytes(bed_file) -> np.ndarray:
+    if isinstance(bed_file, (str, os.PathLike)):
+        return np.fromfile(bed_file, dtype=np.uint8)
+    if isinstance(bed_file, (bytes, bytearray, memoryview)):
+        return np.frombuffer(bed_file, dtype=np.uint8)
+    return np.ascontiguousarray(bed_file, dtype=np.uint8).reshape(-1)
+
+
+class _DeviceMarkers:
+    """Calls shared by the BayesR and BayesW contexts: the device-side .bed decode."""
+
+    def load_bed(self, bed_file, layout: int):
+        """Decode a whole .bed file (path, bytes or uint8 array, magic included) on the device into
+        `layout` markers (PreprocessGraph + the marker builders, src/preprocessgraph.cpp:24-60).
+        Returns the device time of the decode in ms: {"unpack": ..., "tables": ...}."""
+        f = _as_file_bytes(bed_file)
+        self._ck(lib().brr_hip_load_bed(self.ctx, _p(f, C.c_uint8), f.size, int(layout)), "brr_hip_load_bed")
+        ph = np.zeros(12)
+        lib().brr_hip_last_phase_ms(self.ctx, _p(ph, C.c_double), 12)
+        return {"unpack": float(ph[0]), "tables": float(ph[1])}
+
+    def marker_tables(self) -> np.ndarray:
+        """M x 4: dense tables {dosage 0, 1, 2, missing} or ragged statistics {mean, sd, sqrdZ, Zsum}."""
+        out = np.zeros((self.m, 4))
+        self._ck(lib().brr_hip_get_marker_tables(self.ctx, _p(out, C.c_double)), "brr_hip_get_marker_tables")
+        return out
+
+
 # ---------------------------------------------------------- device sweep --
 
-class HipSweep:
+class HipSweep(_DeviceMarkers):
     """Device context: marker arena + chain state on one MI355X (HipSweepGraph)."""
 
     def __init__(self, n: int, m: int, n_components: int, n_groups: int = 1,
@@ -413,7 +447,7 @@ class HipSweep:
 
 # ------------------------------------------------------------ BayesW sweep --
 
-class HipBayesW:
+class HipBayesW(_DeviceMarkers):
     """Device context of the BayesW (Weibull survival) sweep on ragged markers."""
 
     def __init__(self, n: int, m: int, n_components: int, quad_points: int = 7, workgroups: int = 0,

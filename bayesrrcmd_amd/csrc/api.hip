@@ -15,6 +15,7 @@
 #include "sweep_kernel.hpp"
 #include "batch_kernel.hpp"
 #include "bayesw_kernel.hpp"
+#include "bed_kernels.hpp"
 
 using namespace brr;
 
@@ -159,6 +160,15 @@ __global__ void compute_num_kernel(SweepParams p, uint32_t j, double beta_old, d
             const uint32_t v = (col[i >> 2] >> ((i & 3u) * 2u)) & 3u;
             s = fma(wt[v], p.eps[i], s);
         }
+    } else if (!p.ones) {
+        // ragged formulas on the 2-bit codes (no index lists held): genotype 1
+        // adds eps, genotype 2 adds 2 eps
+        const uint8_t* col = (const uint8_t*)(p.dense + (size_t)j * p.col_words);
+        for (uint32_t i = tid; i < p.N; i += kThreads) {
+            const uint32_t v = (col[i >> 2] >> ((i & 3u) * 2u)) & 3u;
+            if (v == 2u) s += p.eps[i];
+            else if (v == 0u) s += 2.0 * p.eps[i];
+        }
     } else {
         for (uint64_t q = p.ones_off[j] + tid; q < p.ones_off[j + 1]; q += kThreads) s += p.eps[p.ones[q]];
         for (uint64_t q = p.twos_off[j] + tid; q < p.twos_off[j + 1]; q += kThreads) s += 2.0 * p.eps[p.twos[q]];
@@ -207,7 +217,14 @@ __global__ void epsilon_change_kernel(SweepParams p, uint32_t j, double beta_old
     const double meanAdjustment = dBeta * mean / sd;
     const double oneAdjustment = dBeta / sd;
     double delta = -meanAdjustment;
-    if (in_list(p.ones + p.ones_off[j], p.ones_off[j + 1] - p.ones_off[j], (int32_t)i)) delta += oneAdjustment;
+    if (!p.ones) {
+        // 2-bit codes: value 2 = genotype 1, 0 = genotype 2, 1 = missing
+        const uint8_t* col = (const uint8_t*)(p.dense + (size_t)j * p.col_words);
+        const uint32_t v = (col[i >> 2] >> ((i & 3u) * 2u)) & 3u;
+        if (v == 2u) delta += oneAdjustment;
+        else if (v == 0u) delta += 2 * oneAdjustment;
+        else if (v == 1u) delta += meanAdjustment;
+    } else if (in_list(p.ones + p.ones_off[j], p.ones_off[j + 1] - p.ones_off[j], (int32_t)i)) delta += oneAdjustment;
     else if (in_list(p.twos + p.twos_off[j], p.twos_off[j + 1] - p.twos_off[j], (int32_t)i)) delta += 2 * oneAdjustment;
     else if (in_list(p.miss + p.miss_off[j], p.miss_off[j + 1] - p.miss_off[j], (int32_t)i)) delta += meanAdjustment;
     p.eps[i] = p.eps[i] + delta;
@@ -731,6 +748,108 @@ int brr_hip_upload_ragged(brr_hip_ctx* c, const int32_t* ones, const uint64_t* o
     }
     c->have_markers = true;
     c->have_model = c->have_model && !c->bw;   // sum_failure depends on the columns
+    return BRR_HIP_OK;
+}
+
+int brr_hip_load_bed(brr_hip_ctx* c, const uint8_t* bed_file, uint64_t file_bytes, int layout)
+{
+    if (!c || !bed_file) return fail(c, BRR_HIP_EINVAL, "null argument");
+    if (layout != BRR_HIP_LAYOUT_DENSE_2BIT && layout != BRR_HIP_LAYOUT_RAGGED)
+        return fail(c, BRR_HIP_EINVAL, "layout must be dense or ragged");
+    if (!c->bw && layout != c->layout) return fail(c, BRR_HIP_EINVAL, "layout differs from the context's");
+    const uint32_t N = c->cfg.num_inds, M = c->cfg.num_snps;
+    const uint64_t cb = ((uint64_t)N + 3) / 4;
+    if (file_bytes < 3 || bed_file[0] != 0x6c || bed_file[1] != 0x1b || bed_file[2] != 0x01)
+        return fail(c, BRR_HIP_EINVAL, "not a SNP-major PLINK .bed file (magic 6c 1b 01)");
+    if (file_bytes != 3 + cb * M)
+        return fail(c, BRR_HIP_EINVAL, "file size " + std::to_string(file_bytes) + " != 3 + M * ceil(N/4) = " +
+                                           std::to_string(3 + cb * M));
+    if (N < 2 || M == 0) return fail(c, BRR_HIP_EINVAL, "need N >= 2 and M >= 1");
+    HIP_TRY(c, hipSetDevice(c->device));
+    const bool dense = layout == BRR_HIP_LAYOUT_DENSE_2BIT;
+    c->have_markers = false;
+    // the packed payload, with 32 zero bytes past its end for the unpacker's
+    // five-word reads
+    const uint64_t payload = cb * M;
+    const uint64_t stage_words = (payload + 32 + 3) / 4;
+    uint32_t* d_stage = nullptr;
+    uint32_t* d_cnt = nullptr;
+    int rc;
+    if ((rc = dalloc(c, &d_stage, stage_words)) || (rc = dalloc(c, &d_cnt, (size_t)4 * M + 1))) {
+        dfree(d_stage);
+        dfree(d_cnt);
+        return rc;
+    }
+    auto cleanup = [&]() { dfree(d_stage); dfree(d_cnt); };
+    auto hip_ok = [&](hipError_t e, const char* what) {
+        if (e == hipSuccess) return true;
+        fail(c, BRR_HIP_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
+        cleanup();
+        return false;
+    };
+    if (!hip_ok(hipMemsetAsync((uint8_t*)d_stage + (payload & ~3ull), 0, stage_words * 4 - (payload & ~3ull),
+                               c->stream), "hipMemsetAsync") ||
+        !hip_ok(hipMemcpyAsync(d_stage, bed_file + 3, payload, hipMemcpyHostToDevice, c->stream), "hipMemcpyAsync"))
+        return BRR_HIP_EDEVICE;
+    // the arena: columns as 2-bit codes plus the per-marker table / statistics
+    c->col_bytes = cb;
+    c->col_words = (cb + 15) / 16 * 4;
+    dfree(c->d_ones); dfree(c->d_twos); dfree(c->d_miss);
+    dfree(c->d_ones_off); dfree(c->d_twos_off); dfree(c->d_miss_off); dfree(c->d_seg);
+    if ((rc = dalloc(c, &c->d_dense, (size_t)c->col_words * M)) ||
+        (rc = dalloc(c, dense ? &c->d_lut : &c->d_stats, (size_t)4 * M))) {
+        cleanup();
+        return rc;
+    }
+    uint32_t* d_bad = d_cnt + 4 * (size_t)M;
+    if (!hip_ok(hipMemsetAsync(d_bad, 0xFF, sizeof(uint32_t), c->stream), "hipMemsetAsync") ||
+        !hip_ok(hipEventRecord(c->ev0, c->stream), "hipEventRecord"))
+        return BRR_HIP_EDEVICE;
+    bed_unpack_kernel<<<M, kBedThreads, 0, c->stream>>>(d_stage, cb, N, c->col_words, c->d_dense, d_cnt,
+                                                        dense ? nullptr : c->d_stats, d_bad);
+    if (!hip_ok(hipGetLastError(), "bed_unpack_kernel") || !hip_ok(hipEventRecord(c->ev1, c->stream), "hipEventRecord") ||
+        !hip_ok(hipStreamSynchronize(c->stream), "bed_unpack_kernel"))
+        return BRR_HIP_EDEVICE;
+    float ms_unpack = 0.f, ms_table = 0.f;
+    (void)hipEventElapsedTime(&ms_unpack, c->ev0, c->ev1);
+    if (dense) {
+        if (!hip_ok(hipEventRecord(c->ev0, c->stream), "hipEventRecord")) return BRR_HIP_EDEVICE;
+        bed_dense_table_kernel<<<(M + kBedThreads - 1) / kBedThreads, kBedThreads, 0, c->stream>>>(
+            c->d_dense, c->col_words, N, M, d_cnt, c->d_lut);
+        if (!hip_ok(hipGetLastError(), "bed_dense_table_kernel") ||
+            !hip_ok(hipEventRecord(c->ev1, c->stream), "hipEventRecord") ||
+            !hip_ok(hipStreamSynchronize(c->stream), "bed_dense_table_kernel"))
+            return BRR_HIP_EDEVICE;
+        (void)hipEventElapsedTime(&ms_table, c->ev0, c->ev1);
+    }
+    uint32_t bad = 0xFFFFFFFFu;
+    if (!hip_ok(hipMemcpy(&bad, d_bad, sizeof(bad), hipMemcpyDeviceToHost), "hipMemcpy")) return BRR_HIP_EDEVICE;
+    cleanup();
+    c->stats = brr_hip_stats{};
+    c->stats.kernel_ms = (double)ms_unpack + (double)ms_table;
+    c->stats.markers = M;
+    std::fill(c->phase_ms, c->phase_ms + kBPhases, 0.0);
+    c->phase_ms[0] = ms_unpack;
+    c->phase_ms[1] = ms_table;
+    if (bad != 0xFFFFFFFFu)
+        return fail(c, BRR_HIP_EINVAL, "marker " + std::to_string(bad) + " has no genotype 1 or 2 calls");
+    c->codes = true;
+    if (c->bw) {
+        c->layout = dense ? kLayoutDense : kLayoutRagged;
+        c->cfg.layout = (uint32_t)layout;
+        c->have_model = false;   // sum_failure depends on the columns
+    }
+    c->have_markers = true;
+    return BRR_HIP_OK;
+}
+
+int brr_hip_get_marker_tables(brr_hip_ctx* c, double* out)
+{
+    if (!c || !out) return fail(c, BRR_HIP_EINVAL, "null argument");
+    if (!c->have_markers) return fail(c, BRR_HIP_ESTATE, "markers not uploaded");
+    const double* src = c->layout == kLayoutDense ? c->d_lut : c->d_stats;
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipMemcpy(out, src, sizeof(double) * 4 * c->cfg.num_snps, hipMemcpyDeviceToHost));
     return BRR_HIP_OK;
 }
 

@@ -43,7 +43,7 @@
 extern "C" {
 #endif
 
-#define BRR_HIP_ABI_VERSION 3
+#define BRR_HIP_ABI_VERSION 4
 
 enum brr_hip_status {
     BRR_HIP_OK = 0,
@@ -140,6 +140,24 @@ int brr_hip_upload_ragged(brr_hip_ctx* ctx,
                           const int32_t* twos, const uint64_t* twos_off,
                           const int32_t* miss, const uint64_t* miss_off,
                           const double* stats);
+
+/* A whole PLINK .bed file image (magic 6c 1b 01, then M SNP-major columns of
+ * ceil(N/4) bytes; file_bytes must be 3 + M * ceil(N/4)) decoded on the device
+ * into the markers of `layout`, in place of the host preprocess pass
+ * (PreprocessGraph, src/preprocessgraph.cpp:24-60, with the builders it runs)
+ * and the upload calls above. Columns are held as 2-bit codes; the per-marker
+ * table (dense: DenseMarkerBuilder::endColumn, src/densemarkerbuilder.cpp:43-60)
+ * or statistics (ragged: SparseMarker::updateStatistics, src/sparsemarker.cpp:8-19)
+ * are computed on the device and equal brr_dense_luts / brr_ragged_fill bit for
+ * bit. No index lists are built. A BayesR context takes its own layout only; a
+ * BayesW context takes either. BRR_HIP_EINVAL for a bad file, or (ragged) a
+ * marker without genotype 1 or 2 calls, as RaggedSparseMarkerBuilder rejects it.
+ * brr_hip_last_stats().kernel_ms and brr_hip_last_phase_ms() give the device
+ * time of the decode (phase 0: unpack and count, phase 1: dense tables). */
+int brr_hip_load_bed(brr_hip_ctx* ctx, const uint8_t* bed_file, uint64_t file_bytes, int layout);
+/* The M x 4 per-marker values held on the device: dense tables {dosage 0, 1, 2,
+ * missing} or ragged statistics {mean, sd, sqrdZ, Zsum}. */
+int brr_hip_get_marker_tables(brr_hip_ctx* ctx, double* out);
 
 /* Group label per marker (Data::G, src/data.cpp:424-453); default all 0. */
 int brr_hip_set_groups(brr_hip_ctx* ctx, const int32_t* group);
