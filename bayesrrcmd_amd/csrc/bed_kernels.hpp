@@ -106,33 +106,23 @@ bed_unpack_kernel(const uint32_t* __restrict__ src, uint64_t cb, uint32_t N, uin
     }
 }
 
-// The running FP64 sum of tab[v] over the values v of one column's individuals
-// 0..N-1, added one at a time in individual order (tab by 2-bit value).
-__device__ __forceinline__ double bed_ordered_sum(const uint4* col, uint32_t N, double acc, double t0, double t1,
-                                                  double t2, double t3)
+// The running FP64 sum of tab(v) over the values v of one column's individuals
+// 0..N-1, added one at a time in individual order. tab(v) = tabl[v * kBedThreads]:
+// the calling thread's entries in LDS, one plane per value so that the lanes of
+// a wave read distinct banks whatever their values.
+__device__ __forceinline__ double bed_ordered_sum(const uint4* col, uint32_t N, const double* tabl)
 {
-    auto add16 = [&](uint32_t x, uint32_t n) {
-        for (uint32_t e = 0; e < n; ++e) {
-            const uint32_t v = (x >> (2u * e)) & 3u;
-            const double lo = (v & 1u) ? t1 : t0;
-            const double hi = (v & 1u) ? t3 : t2;
-            acc = acc + ((v & 2u) ? hi : lo);
-        }
-    };
+    double acc = 0.0;
     const uint32_t nfull = N / 64u;
+    uint4 next = nfull ? col[0] : make_uint4(0u, 0u, 0u, 0u);
     for (uint32_t q = 0; q < nfull; ++q) {
-        const uint4 w = col[q];
+        const uint4 w = next;
+        if (q + 1u < nfull) next = col[q + 1u];
+        const uint32_t xs[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const uint32_t x = k == 0 ? w.x : k == 1 ? w.y : k == 2 ? w.z : w.w;
+        for (int k = 0; k < 4; ++k)
 #pragma unroll
-            for (uint32_t e = 0; e < 16u; ++e) {
-                const uint32_t v = (x >> (2u * e)) & 3u;
-                const double lo = (v & 1u) ? t1 : t0;
-                const double hi = (v & 1u) ? t3 : t2;
-                acc = acc + ((v & 2u) ? hi : lo);
-            }
-        }
+            for (uint32_t e = 0; e < 16u; ++e) acc = acc + tabl[((xs[k] >> (2u * e)) & 3u) * kBedThreads];
     }
     uint32_t rest = N - 64u * nfull;
     if (rest) {
@@ -140,7 +130,7 @@ __device__ __forceinline__ double bed_ordered_sum(const uint4* col, uint32_t N, 
         const uint32_t xs[4] = {w.x, w.y, w.z, w.w};
         for (int k = 0; k < 4 && rest; ++k) {
             const uint32_t n = rest < 16u ? rest : 16u;
-            add16(xs[k], n);
+            for (uint32_t e = 0; e < n; ++e) acc = acc + tabl[((xs[k] >> (2u * e)) & 3u) * kBedThreads];
             rest -= n;
         }
     }
@@ -156,8 +146,10 @@ __global__ void __launch_bounds__(kBedThreads)
 bed_dense_table_kernel(const uint32_t* __restrict__ codes, uint64_t col_words, uint32_t N, uint32_t M,
                        const uint32_t* __restrict__ counts, double* __restrict__ lut)
 {
+    __shared__ double tab[4 * kBedThreads];
     const uint32_t j = blockIdx.x * kBedThreads + threadIdx.x;
     if (j >= M) return;
+    double* tabl = tab + threadIdx.x;                 // this thread's entries, value v at tabl[v * kBedThreads]
     const uint4* col = reinterpret_cast<const uint4*>(codes + (size_t)j * col_words);
     const uint32_t n2 = counts[4 * (size_t)j + 0];   // value 0: dosage 2
     const uint32_t nm = counts[4 * (size_t)j + 1];   // value 1: missing
@@ -165,10 +157,21 @@ bed_dense_table_kernel(const uint32_t* __restrict__ codes, uint64_t col_words, u
     const double numInds = (double)N;
     const double sum = (double)((uint64_t)n1 + 2u * (uint64_t)n2);
     const double mean = sum / (numInds - (double)nm);
-    const double s = nm ? bed_ordered_sum(col, N, 0.0, 2.0, mean, 1.0, 0.0) : sum;
+    double s = sum;                                   // integer terms: the index-order sum is exact
+    if (nm) {
+        tabl[0 * kBedThreads] = 2.0;
+        tabl[1 * kBedThreads] = mean;
+        tabl[2 * kBedThreads] = 1.0;
+        tabl[3 * kBedThreads] = 0.0;
+        s = bed_ordered_sum(col, N, tabl);
+    }
     const double cmean = s / numInds;
     const double d0 = 0.0 - cmean, d1 = 1.0 - cmean, d2 = 2.0 - cmean, dm = mean - cmean;
-    const double sqn = bed_ordered_sum(col, N, 0.0, d2 * d2, dm * dm, d1 * d1, d0 * d0);
+    tabl[0 * kBedThreads] = d2 * d2;
+    tabl[1 * kBedThreads] = dm * dm;
+    tabl[2 * kBedThreads] = d1 * d1;
+    tabl[3 * kBedThreads] = d0 * d0;
+    const double sqn = bed_ordered_sum(col, N, tabl);
     const double sigma = 1.0 / (sqrt(sqn / (numInds - 1.0)));
     double* o = lut + 4 * (size_t)j;
     o[0] = d0 * sigma;

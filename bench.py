@@ -14,13 +14,15 @@ N = 50,000 x M = 50,000, chain length 100, all traits in one persistent launch.
 `--config c5` is BayesW (Weibull survival): N = 100,000, M = 50,000 ragged,
 0.5% causal, log T = 4 + g + (log E + EulerGamma) / 3, 10% right-censored,
 mixture variances 0.001, 0.01, 0.1, 7 Gauss-Hermite points, chain length 100.
+`--config c5-dense` is the same BayesW workload on the dense 2-bit layout.
 
 Prints one JSON line: marker-updates/s (whole job), the roofline of the sweep
 kernel measured with HIP events on its stream, and the CPU reference sweep
 (the FP64 oracle built with the reference's Release flags) timed on a bounded
 sample of the same workload on this host. The default line also carries a
-"batched" object (the c3 workload measured the same way: batched vs plain) and
-a "bayesw" object (the c5 workload).
+"batched" object (the c3 workload measured the same way: batched vs plain), a
+"bayesw" object (the c5 workload) and a "decode" object (the c2 .bed file
+decoded on the device into either layout, brr_hip_load_bed).
 """
 from __future__ import annotations
 
@@ -86,11 +88,13 @@ def make_data(cfg, seed, threads):
     t0 = time.time()
     if cfg["layout"] == "ragged":
         arena = brr.build_ragged(bed, n, threads=threads)
-        log(f"ragged lists built in {time.time() - t0:.1f}s ({arena.nbytes() / 1e9:.2f} GB)")
-        return dict(bed=bed, y=y, ragged=arena, lut=None)
+        dt = time.time() - t0
+        log(f"ragged lists built in {dt:.1f}s ({arena.nbytes() / 1e9:.2f} GB, {threads} threads)")
+        return dict(bed=bed, y=y, ragged=arena, lut=None, build_s=dt)
     lut = brr.build_dense(bed, n, threads=threads)
-    log(f"dense tables built in {time.time() - t0:.1f}s")
-    return dict(bed=bed, y=y, ragged=None, lut=lut)
+    dt = time.time() - t0
+    log(f"dense tables built in {dt:.1f}s ({threads} threads)")
+    return dict(bed=bed, y=y, ragged=None, lut=lut, build_s=dt)
 
 
 def make_batch_data(cfg, seed, threads):
@@ -502,6 +506,58 @@ def run_batched(cfg, args, with_cpu):
     return out
 
 
+def decode_bytes(n, m, layout):
+    """Algorithmic bytes of the unpack kernel: read the packed columns (M ceil(N/4)), write the
+    16-byte-aligned code columns, 16 B of counts and (ragged) 32 B of statistics per marker."""
+    cb = (n + 3) // 4
+    col = (cb + 15) // 16 * 16
+    return float(m) * (cb + col + 16 + (32 if layout == "ragged" else 0))
+
+
+def run_decode(cfg, data, args, repeats=3):
+    """The workload's .bed file decoded on the device (brr_hip_load_bed) in both layouts; the file
+    image is in host memory, its copy to HBM precedes the timed kernels."""
+    import bayesrrcmd_amd as brr
+    n, m = cfg["n"], cfg["m"]
+    f = brr.bed_file_image(data["bed"])
+    out = {"workload": f"{cfg['workload']}: its .bed file ({f.size / 1e9:.2f} GB) decoded on the device",
+           "file_bytes": int(f.size)}
+    for name, layout in (("ragged", brr.LAYOUT_RAGGED), ("dense", brr.LAYOUT_DENSE_2BIT)):
+        h = brr.HipSweep(n, m, 4, 1, layout)
+        unpack, tables, wall = [], [], []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            t = h.load_bed(f, layout)
+            wall.append(time.perf_counter() - t0)
+            unpack.append(t["unpack"])
+            tables.append(t["tables"])
+        got = h.marker_tables()
+        h.close()
+        k = 2000
+        ref = data["ragged"].stats if (name == "ragged" and data.get("ragged") is not None) else None
+        if ref is None:
+            try:
+                ref = (brr.build_ragged(data["bed"][:k], n).stats if name == "ragged"
+                       else brr.build_dense(data["bed"][:k], n))
+            except brr.BrrHipError:            # a marker without genotype 1 / 2 calls: no host statistics
+                ref = np.zeros((0, 4))
+        same = bool(np.array_equal(got[: ref.shape[0]].view(np.uint64), ref.view(np.uint64)))
+        avg = float(np.mean(unpack))
+        nbytes = decode_bytes(n, m, name)
+        achieved = nbytes / (avg * 1e-3) / 1e9
+        out[name] = {
+            "unpack_ms": avg, "tables_ms": float(np.mean(tables)) if name == "dense" else None,
+            "load_bed_wall_s": float(np.mean(wall)),
+            "markers_checked_bit_exact_vs_host_builder": int(ref.shape[0]), "bit_exact": same,
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "brr::bed_unpack_kernel",
+                         "algorithmic_bytes_per_launch": nbytes},
+        }
+        log(f"decode {name}: unpack {avg:.3f} ms ({achieved:.0f} GB/s), tables "
+            f"{np.mean(tables):.3f} ms, load_bed wall {np.mean(wall):.2f} s, bit-exact {same}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -520,6 +576,9 @@ def main():
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-batched", action="store_true", help="skip the batched (c3) leg of the default line")
     ap.add_argument("--no-bayesw", action="store_true", help="skip the BayesW (c5) leg of the default line")
+    ap.add_argument("--no-decode", action="store_true", help="skip the device .bed decode leg of the default line")
+    ap.add_argument("--decode-only", action="store_true",
+                    help="only the decode leg: the c2 genotypes decoded on the device, its JSON object printed")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -542,6 +601,13 @@ def main():
         log("calibration copy: 1 GiB read + 1 GiB written")
         del a, b
     cfg = CONFIGS[args.config]
+    if args.decode_only:
+        t0 = time.time()
+        bed, _ = brr.synth_bed(args.seed, cfg["n"], cfg["m"], cfg["maf"][0], cfg["maf"][1], 0.0,
+                               threads=args.threads)
+        log(f"genotypes {cfg['m']} x {cfg['n']} generated in {time.time() - t0:.1f}s")
+        print(json.dumps({"decode": run_decode(cfg, {"bed": bed}, args)}), flush=True)
+        return
     if args.config in ("c3", "c5", "c5-dense"):
         b = (run_batched if args.config == "c3" else run_bayesw)(cfg, args, not args.no_cpu)
         line = {"metric": METRIC, "value": b["value"], "unit": b["unit"], "n_gpus": 1, "steps": b["steps"],
@@ -652,6 +718,9 @@ def main():
     if not args.no_cpu:
         log("timing the CPU reference sweep ...")
         line["cpu_baseline"] = cpu_baseline(cfg, data, args.seed, args.cpu_budget)
+    if not args.no_decode:
+        line["decode"] = run_decode(cfg, data, args)
+        line["decode"]["host_builder_s"] = data.get("build_s")
     if args.config == "c2" and not args.no_batched:
         del data
         data = None
