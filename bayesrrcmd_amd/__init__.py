@@ -25,11 +25,12 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _abi
-from ._abi import LAYOUT_DENSE_2BIT, LAYOUT_RAGGED, STORAGE_AUTO, STORAGE_CODES, STORAGE_LISTS
+from ._abi import LAYOUT_DENSE_2BIT, LAYOUT_EIGEN_SPARSE, LAYOUT_RAGGED, STORAGE_AUTO, STORAGE_CODES, STORAGE_LISTS
 
 __all__ = [
     "BrrHipError", "HipSweep", "HipBayesW", "bayesw_init", "bed_file_image", "RaggedArena", "read_ragged_sparsebed", "read_ppbed", "SampleWriter", "sample_due", "build_dense", "build_ragged", "synth_bed", "synth_xb",
-    "bayesr_init", "shuffle", "marker_uz", "LAYOUT_DENSE_2BIT", "LAYOUT_RAGGED", "lib",
+    "bayesr_init", "shuffle", "marker_uz", "LAYOUT_DENSE_2BIT", "LAYOUT_RAGGED", "LAYOUT_EIGEN_SPARSE", "lib",
+    "build_eigen", "read_eigen_sparsebed",
     "STORAGE_AUTO", "STORAGE_CODES", "STORAGE_LISTS",
 ]
 
@@ -131,23 +132,52 @@ def build_ragged(bed: np.ndarray, n: int, threads: int | None = None) -> RaggedA
     return RaggedArena(lists[0], offs[0], lists[1], offs[1], lists[2], offs[2], stats)
 
 
-def read_ragged_sparsebed(data: bytes | np.ndarray, index: np.ndarray, n: int, compressed: bool = False,
-                          threads: int | None = None) -> RaggedArena:
-    """A .ragged.sparsebed file's bytes and its index (M x 3 u64) -> RaggedArena."""
+def build_eigen(bed: np.ndarray, n: int, threads: int | None = None) -> RaggedArena:
+    """EigenSparseMarkerBuilder over every column: the ragged lists with the EigenSparse statistics."""
+    bed = np.ascontiguousarray(bed, dtype=np.uint8)
+    m, stride = bed.shape
+    t = threads or _threads()
+    offs = [np.zeros(m + 1, dtype=np.uint64) for _ in range(3)]
+    _check(lib().brr_ragged_count(_p(bed, C.c_uint8), stride, n, m, *(_p(o, C.c_uint64) for o in offs), t),
+           "brr_ragged_count")
+    lists = [np.empty(max(1, int(o[-1])), dtype=np.int32) for o in offs]
+    stats = np.empty((m, 4), dtype=np.float64)
+    _check(lib().brr_eigen_fill(_p(bed, C.c_uint8), stride, n, m, *(_p(o, C.c_uint64) for o in offs),
+                                *(_p(l, C.c_int32) for l in lists), _p(stats, C.c_double), t),
+           "brr_eigen_fill (a column has no nonzero call)")
+    lists = [l[: int(o[-1])] for l, o in zip(lists, offs)]
+    return RaggedArena(lists[0], offs[0], lists[1], offs[1], lists[2], offs[2], stats)
+
+
+def _read_sparse(prefix: str, data, index, n: int, compressed: bool, threads) -> RaggedArena:
     buf = np.frombuffer(data, dtype=np.uint8) if isinstance(data, (bytes, bytearray)) else np.ascontiguousarray(data, np.uint8)
     idx = np.ascontiguousarray(index, dtype=np.uint64).reshape(-1, 3)
     m = idx.shape[0]
     t = threads or _threads()
     offs = [np.zeros(m + 1, dtype=np.uint64) for _ in range(3)]
-    _check(lib().brr_ragged_read_counts(_p(buf, C.c_uint8), buf.size, _p(idx, C.c_uint64), m, int(compressed),
-                                        *(_p(o, C.c_uint64) for o in offs), t), "brr_ragged_read_counts")
+    _check(getattr(lib(), f"{prefix}_read_counts")(_p(buf, C.c_uint8), buf.size, _p(idx, C.c_uint64), m,
+                                                   int(compressed), *(_p(o, C.c_uint64) for o in offs), t),
+           f"{prefix}_read_counts")
     lists = [np.empty(max(1, int(o[-1])), dtype=np.int32) for o in offs]
     stats = np.empty((m, 4), dtype=np.float64)
-    _check(lib().brr_ragged_read_fill(_p(buf, C.c_uint8), buf.size, _p(idx, C.c_uint64), n, m, int(compressed),
-                                      *(_p(o, C.c_uint64) for o in offs), *(_p(l, C.c_int32) for l in lists),
-                                      _p(stats, C.c_double), t), "brr_ragged_read_fill")
+    _check(getattr(lib(), f"{prefix}_read_fill")(_p(buf, C.c_uint8), buf.size, _p(idx, C.c_uint64), n, m,
+                                                 int(compressed), *(_p(o, C.c_uint64) for o in offs),
+                                                 *(_p(l, C.c_int32) for l in lists), _p(stats, C.c_double), t),
+           f"{prefix}_read_fill")
     lists = [l[: int(o[-1])] for l, o in zip(lists, offs)]
     return RaggedArena(lists[0], offs[0], lists[1], offs[1], lists[2], offs[2], stats)
+
+
+def read_ragged_sparsebed(data: bytes | np.ndarray, index: np.ndarray, n: int, compressed: bool = False,
+                          threads: int | None = None) -> RaggedArena:
+    """A .ragged.sparsebed file's bytes and its index (M x 3 u64) -> RaggedArena."""
+    return _read_sparse("brr_ragged", data, index, n, compressed, threads)
+
+
+def read_eigen_sparsebed(data: bytes | np.ndarray, index: np.ndarray, n: int, compressed: bool = False,
+                         threads: int | None = None) -> RaggedArena:
+    """A .eigen.sparsebed file's bytes and its index (M x 3 u64) -> RaggedArena with EigenSparse statistics."""
+    return _read_sparse("brr_eigen", data, index, n, compressed, threads)
 
 
 def read_ppbed(data: bytes | np.ndarray, index: np.ndarray, n: int, compressed: bool = False,

@@ -13,6 +13,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libbrr_hip.so"
 OK, EINVAL, ENOMEM, EDEVICE, ESTATE, ETIMEOUT, ESAMPLER = 0, 1, 2, 3, 4, 5, 6
 LAYOUT_DENSE_2BIT = 1
 LAYOUT_RAGGED = 2
+LAYOUT_EIGEN_SPARSE = 3
 STORAGE_AUTO, STORAGE_LISTS, STORAGE_CODES = 0, 1, 2
 
 P_u8 = C.POINTER(C.c_uint8)
@@ -96,6 +97,12 @@ SIGNATURES = {
     "brr_hip_bw_sum_failure": (C.c_int, [CTX, P_f64]),
     "brr_bayesw_init": (C.c_int, [P_f64, P_f64, C.c_uint32, C.c_uint32, C.c_uint32, P_f64,
                                   C.POINTER(BwScalars)]),
+    "brr_eigen_fill": (C.c_int, [P_u8, C.c_uint64, C.c_uint32, C.c_uint32, P_u64, P_u64, P_u64,
+                                 P_i32, P_i32, P_i32, P_f64, C.c_uint32]),
+    "brr_eigen_read_counts": (C.c_int, [P_u8, C.c_uint64, P_u64, C.c_uint32, C.c_int, P_u64, P_u64, P_u64,
+                                        C.c_uint32]),
+    "brr_eigen_read_fill": (C.c_int, [P_u8, C.c_uint64, P_u64, C.c_uint32, C.c_uint32, C.c_int, P_u64, P_u64,
+                                      P_u64, P_i32, P_i32, P_i32, P_f64, C.c_uint32]),
     "brr_ragged_read_counts": (C.c_int, [P_u8, C.c_uint64, P_u64, C.c_uint32, C.c_int, P_u64, P_u64, P_u64,
                                          C.c_uint32]),
     "brr_ragged_read_fill": (C.c_int, [P_u8, C.c_uint64, P_u64, C.c_uint32, C.c_uint32, C.c_int, P_u64, P_u64,

@@ -164,10 +164,12 @@ __global__ void compute_num_kernel(SweepParams p, uint32_t j, double beta_old, d
         // ragged formulas on the 2-bit codes (no index lists held): genotype 1
         // adds eps, genotype 2 adds 2 eps
         const uint8_t* col = (const uint8_t*)(p.dense + (size_t)j * p.col_words);
+        const double wm = (p.layout == kLayoutEigen) ? p.stats[4 * (size_t)j] : 0.0;   // missing: Eigen's mean
         for (uint32_t i = tid; i < p.N; i += kThreads) {
             const uint32_t v = (col[i >> 2] >> ((i & 3u) * 2u)) & 3u;
             if (v == 2u) s += p.eps[i];
             else if (v == 0u) s += 2.0 * p.eps[i];
+            else if (v == 1u) s += wm * p.eps[i];
         }
     } else {
         for (uint64_t q = p.ones_off[j] + tid; q < p.ones_off[j + 1]; q += kThreads) s += p.eps[p.ones[q]];
@@ -279,6 +281,7 @@ template <int R>
 const void* kernel_r(int layout, bool codes)
 {
     if (layout == kLayoutDense) return (const void*)bayesr_sweep_kernel<R, true, true>;
+    if (layout == kLayoutEigen && !codes) return nullptr;   // Eigen contexts hold codes
     return codes ? (const void*)bayesr_sweep_kernel<R, false, true> : (const void*)bayesr_sweep_kernel<R, false, false>;
 }
 
@@ -534,7 +537,9 @@ int brr_hip_create(const brr_hip_config* cfg, brr_hip_ctx** out)
     if (cfg->num_inds < 2 || cfg->num_snps < 1) return BRR_HIP_EINVAL;
     if (cfg->num_components < 2 || cfg->num_components > (uint32_t)kMaxK) return BRR_HIP_EINVAL;
     if (cfg->num_groups < 1 || cfg->num_groups > (uint32_t)kMaxG) return BRR_HIP_EINVAL;
-    if (cfg->layout != BRR_HIP_LAYOUT_DENSE_2BIT && cfg->layout != BRR_HIP_LAYOUT_RAGGED) return BRR_HIP_EINVAL;
+    if (cfg->layout != BRR_HIP_LAYOUT_DENSE_2BIT && cfg->layout != BRR_HIP_LAYOUT_RAGGED &&
+        cfg->layout != BRR_HIP_LAYOUT_EIGEN_SPARSE)
+        return BRR_HIP_EINVAL;
     if (cfg->block_markers > (uint32_t)kMaxB) return BRR_HIP_EINVAL;
     if (cfg->ragged_storage > 2) return BRR_HIP_EINVAL;
     if (cfg->num_traits > (uint32_t)kMaxT) return BRR_HIP_EINVAL;
@@ -547,7 +552,7 @@ int brr_hip_create(const brr_hip_config* cfg, brr_hip_ctx** out)
     c->cfg = *cfg;
     c->device = (int)cfg->device;
     c->layout = (int)cfg->layout;
-    c->codes = c->layout == kLayoutDense;
+    c->codes = c->layout == kLayoutDense || c->layout == kLayoutEigen;
     if (hipSetDevice(c->device) != hipSuccess) { delete c; return BRR_HIP_EDEVICE; }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, c->device) != hipSuccess) { delete c; return BRR_HIP_EDEVICE; }
@@ -580,7 +585,7 @@ int brr_hip_create(const brr_hip_config* cfg, brr_hip_ctx** out)
         if (R == 3) R = 4;
         c->R = R;
         c->B = cfg->block_markers ? cfg->block_markers : 64u;
-        probe = kernel_for(R, (int)cfg->layout, false);
+        probe = kernel_for(R, (int)cfg->layout, c->layout == kLayoutEigen);
     }
     c->S = S;
     c->P = (N + S - 1) / S;
@@ -685,7 +690,9 @@ int brr_hip_upload_ragged(brr_hip_ctx* c, const int32_t* ones, const uint64_t* o
         c->layout = kLayoutRagged;
         c->cfg.layout = BRR_HIP_LAYOUT_RAGGED;
     }
-    if (c->layout != BRR_HIP_LAYOUT_RAGGED) return fail(c, BRR_HIP_EINVAL, "context layout is not ragged");
+    if (c->layout != BRR_HIP_LAYOUT_RAGGED && c->layout != BRR_HIP_LAYOUT_EIGEN_SPARSE)
+        return fail(c, BRR_HIP_EINVAL, "context layout is not ragged or EigenSparse");
+    const bool eigen = c->layout == kLayoutEigen;
     const uint32_t N = c->cfg.num_inds, M = c->cfg.num_snps;
     const uint64_t n1 = ones_off[M], n2 = twos_off[M], nm = miss_off[M];
     if ((n1 && !ones) || (n2 && !twos) || (nm && !miss)) return fail(c, BRR_HIP_EINVAL, "null list");
@@ -712,7 +719,7 @@ int brr_hip_upload_ragged(brr_hip_ctx* c, const int32_t* ones, const uint64_t* o
         (rc = dalloc(c, &c->d_miss, nm)) || (rc = dalloc(c, &c->d_ones_off, M + 1)) ||
         (rc = dalloc(c, &c->d_twos_off, M + 1)) || (rc = dalloc(c, &c->d_miss_off, M + 1)) ||
         (rc = dalloc(c, &c->d_stats, (size_t)4 * M)) ||
-        (rc = dalloc(c, &c->d_seg, c->bw ? (size_t)1 : (size_t)M * 3 * (c->P + 1))))
+        (rc = dalloc(c, &c->d_seg, (c->bw || eigen) ? (size_t)1 : (size_t)M * 3 * (c->P + 1))))
         return rc;
     if (n1) HIP_TRY(c, hipMemcpy(c->d_ones, ones, sizeof(int32_t) * n1, hipMemcpyHostToDevice));
     if (n2) HIP_TRY(c, hipMemcpy(c->d_twos, twos, sizeof(int32_t) * n2, hipMemcpyHostToDevice));
@@ -721,7 +728,7 @@ int brr_hip_upload_ragged(brr_hip_ctx* c, const int32_t* ones, const uint64_t* o
     HIP_TRY(c, hipMemcpy(c->d_twos_off, twos_off, sizeof(uint64_t) * (M + 1), hipMemcpyHostToDevice));
     HIP_TRY(c, hipMemcpy(c->d_miss_off, miss_off, sizeof(uint64_t) * (M + 1), hipMemcpyHostToDevice));
     HIP_TRY(c, hipMemcpy(c->d_stats, stats, sizeof(double) * 4 * M, hipMemcpyHostToDevice));
-    if (!c->bw) {
+    if (!c->bw && !eigen) {
         const uint64_t total = (uint64_t)M * 3 * (c->P + 1);
         const uint32_t blocks = (uint32_t)((total + 255) / 256);
         build_seg_kernel<<<blocks, 256, 0, c->stream>>>(c->d_ones, c->d_ones_off, c->d_twos, c->d_twos_off,
@@ -732,7 +739,7 @@ int brr_hip_upload_ragged(brr_hip_ctx* c, const int32_t* ones, const uint64_t* o
     const uint64_t list_bytes = 4ull * (n1 + n2 + nm);
     const uint64_t code_bytes = (uint64_t)M * (((uint64_t)N + 3) / 4);
     const uint32_t st = c->cfg.ragged_storage;
-    c->codes = c->T > 0 || c->bw || st == 2 || (st == 0 && code_bytes < list_bytes);
+    c->codes = c->T > 0 || c->bw || eigen || st == 2 || (st == 0 && code_bytes < list_bytes);
     if (c->codes) {
         c->col_bytes = ((uint64_t)N + 3) / 4;
         c->col_words = (c->col_bytes + 15) / 16 * 4;
@@ -742,8 +749,9 @@ int brr_hip_upload_ragged(brr_hip_ctx* c, const int32_t* ones, const uint64_t* o
         HIP_TRY(c, hipGetLastError());
     }
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    if (c->bw) {
-        // the BayesW sweep reads the codes only
+    if (c->bw || eigen) {
+        // the BayesW and EigenSparse paths read the codes only (the per-marker
+        // BayesR kernels too, once the lists are gone)
         dfree(c->d_ones); dfree(c->d_twos); dfree(c->d_miss);
     }
     c->have_markers = true;
@@ -754,8 +762,9 @@ int brr_hip_upload_ragged(brr_hip_ctx* c, const int32_t* ones, const uint64_t* o
 int brr_hip_load_bed(brr_hip_ctx* c, const uint8_t* bed_file, uint64_t file_bytes, int layout)
 {
     if (!c || !bed_file) return fail(c, BRR_HIP_EINVAL, "null argument");
-    if (layout != BRR_HIP_LAYOUT_DENSE_2BIT && layout != BRR_HIP_LAYOUT_RAGGED)
-        return fail(c, BRR_HIP_EINVAL, "layout must be dense or ragged");
+    if (layout != BRR_HIP_LAYOUT_DENSE_2BIT && layout != BRR_HIP_LAYOUT_RAGGED &&
+        (layout != BRR_HIP_LAYOUT_EIGEN_SPARSE || c->bw))
+        return fail(c, BRR_HIP_EINVAL, c->bw ? "layout must be dense or ragged" : "unknown layout");
     if (!c->bw && layout != c->layout) return fail(c, BRR_HIP_EINVAL, "layout differs from the context's");
     const uint32_t N = c->cfg.num_inds, M = c->cfg.num_snps;
     const uint64_t cb = ((uint64_t)N + 3) / 4;
@@ -806,7 +815,8 @@ int brr_hip_load_bed(brr_hip_ctx* c, const uint8_t* bed_file, uint64_t file_byte
         !hip_ok(hipEventRecord(c->ev0, c->stream), "hipEventRecord"))
         return BRR_HIP_EDEVICE;
     bed_unpack_kernel<<<M, kBedThreads, 0, c->stream>>>(d_stage, cb, N, c->col_words, c->d_dense, d_cnt,
-                                                        dense ? nullptr : c->d_stats, d_bad);
+                                                        dense ? nullptr : c->d_stats, d_bad,
+                                                        layout == BRR_HIP_LAYOUT_EIGEN_SPARSE ? 1 : 0);
     if (!hip_ok(hipGetLastError(), "bed_unpack_kernel") || !hip_ok(hipEventRecord(c->ev1, c->stream), "hipEventRecord") ||
         !hip_ok(hipStreamSynchronize(c->stream), "bed_unpack_kernel"))
         return BRR_HIP_EDEVICE;
@@ -832,7 +842,9 @@ int brr_hip_load_bed(brr_hip_ctx* c, const uint8_t* bed_file, uint64_t file_byte
     c->phase_ms[0] = ms_unpack;
     c->phase_ms[1] = ms_table;
     if (bad != 0xFFFFFFFFu)
-        return fail(c, BRR_HIP_EINVAL, "marker " + std::to_string(bad) + " has no genotype 1 or 2 calls");
+        return fail(c, BRR_HIP_EINVAL, "marker " + std::to_string(bad) +
+                                           (layout == BRR_HIP_LAYOUT_EIGEN_SPARSE ? " has no nonzero call"
+                                                                                  : " has no genotype 1 or 2 calls"));
     c->codes = true;
     if (c->bw) {
         c->layout = dense ? kLayoutDense : kLayoutRagged;

@@ -263,7 +263,8 @@ __device__ __forceinline__ void bstore(const BatchParams& p, BBlock& L, const BN
             // residual change per unit beta change (raggedbayesrkernel.cpp:10-30)
             const double mean = x.m0, sd = x.m1;
             L.mstat[k][0] = mean; L.mstat[k][1] = sd; L.mstat[k][2] = x.m2;
-            L.wt[k][0] = 2.0; L.wt[k][1] = 0.0; L.wt[k][2] = 1.0; L.wt[k][3] = 0.0;
+            L.wt[k][0] = 2.0; L.wt[k][2] = 1.0; L.wt[k][3] = 0.0;
+            L.wt[k][1] = (p.layout == kLayoutEigen) ? mean : 0.0;   // EigenBayesRKernel: missing at the mean
             const double meanAdjustment = mean / sd;
             const double oneAdjustment = 1.0 / sd;
             L.ut[k][0] = -meanAdjustment + 2 * oneAdjustment;

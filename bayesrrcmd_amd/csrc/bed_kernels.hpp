@@ -34,11 +34,14 @@ __device__ __forceinline__ uint32_t bed_wave_sum_u32(uint32_t v)
 // codes + j * col_words: the column's cb bytes, then zeros. counts[4j + v] =
 // number of individuals i < N whose value is v (pad bits ignored). When stats is
 // not null, stats[4j..4j+3] = {mean, sd, sqrdZ, Zsum} of SparseMarker and the
-// first marker without genotype 1 or 2 calls lowers *bad.
+// first marker without genotype 1 or 2 calls lowers *bad. eigen = 1: the
+// EigenSparse statistics instead (missing calls imputed at the mean,
+// eigensparsemarkerbuilder.cpp:41-59), and only a marker without any nonzero
+// call (eigensparsemarker.cpp:60-71) lowers *bad.
 __global__ void __launch_bounds__(kBedThreads)
 bed_unpack_kernel(const uint32_t* __restrict__ src, uint64_t cb, uint32_t N, uint64_t col_words,
                   uint32_t* __restrict__ codes, uint32_t* __restrict__ counts, double* __restrict__ stats,
-                  uint32_t* __restrict__ bad)
+                  uint32_t* __restrict__ bad, int eigen)
 {
     __shared__ uint32_t red[3][kBedThreads / 64];
     const uint32_t j = blockIdx.x;
@@ -90,18 +93,22 @@ bed_unpack_kernel(const uint32_t* __restrict__ src, uint64_t cb, uint32_t N, uin
             // SparseMarker::updateStatistics sums the values 1 and 2 in
             // individual order; the sums are integers, so the counts give them
             // exactly. The rest is the same expression in the same order.
-            const uint64_t ones = n[2], twos = n[0];
+            const uint64_t ones = n[2], twos = n[0], nm = n[1];
             const double numInds = (double)N;
-            const double Zsum = (double)(ones + 2u * twos);
-            const double sqrdZ = (double)(ones + 4u * twos);
+            double Zsum = (double)(ones + 2u * twos);
+            double sqrdZ = (double)(ones + 4u * twos);
             const double mean = Zsum / numInds;
+            if (eigen) {
+                sqrdZ += (mean * mean) * (double)nm;
+                Zsum += mean * (double)nm;
+            }
             const double sd = sqrt((sqrdZ - 2.0 * mean * Zsum + numInds * mean * mean) / (numInds - 1.0));
             double* st = stats + 4 * (size_t)j;
             st[0] = mean;
             st[1] = sd;
             st[2] = sqrdZ;
             st[3] = Zsum;
-            if (ones == 0 && twos == 0) atomicMin(bad, j);
+            if (ones == 0 && twos == 0 && (!eigen || nm == 0)) atomicMin(bad, j);
         }
     }
 }

@@ -162,6 +162,48 @@ int brr_ragged_fill(const uint8_t* bed, uint64_t stride, uint32_t N, uint32_t M,
     return BRR_HIP_OK;
 }
 
+int brr_eigen_fill(const uint8_t* bed, uint64_t stride, uint32_t N, uint32_t M, const uint64_t* ones_off,
+                   const uint64_t* twos_off, const uint64_t* miss_off, int32_t* ones, int32_t* twos,
+                   int32_t* miss, double* stats, uint32_t threads)
+{
+    // EigenSparseMarkerBuilder (eigensparsemarkerbuilder.cpp:22-78): the ragged
+    // lists; the statistics of SparseMarker::updateStatistics, then the missing
+    // calls imputed at the mean in sqrdZ and Zsum before the sd.
+    if (!bed || !ones_off || !twos_off || !miss_off || !stats || N < 2 || stride < ((uint64_t)N + 3) / 4)
+        return BRR_HIP_EINVAL;
+    std::vector<uint8_t> invalid(M, 0);
+    parallel_for(M, threads, [&](uint32_t j) {
+        const uint8_t* col = bed + (size_t)j * stride;
+        int32_t* o = ones + ones_off[j];
+        int32_t* t = twos + twos_off[j];
+        int32_t* ms = miss + miss_off[j];
+        uint64_t no = 0, nt = 0, nm = 0;
+        double mean = 0.0, sqrdZ = 0.0, Zsum = 0.0;
+        for (uint32_t i = 0; i < N; ++i) {
+            const unsigned v = code_at(col, i);
+            if (v == 1u) { ms[nm++] = (int32_t)i; continue; }
+            if (v == 3u) continue;
+            const double value = v == 2u ? 1.0 : 2.0;
+            mean += value;
+            sqrdZ += value * value;
+            Zsum += value;
+            if (v == 2u) o[no++] = (int32_t)i;
+            else t[nt++] = (int32_t)i;
+        }
+        const double numInds = (double)N;
+        mean /= numInds;
+        sqrdZ += (mean * mean) * (double)nm;
+        Zsum += mean * (double)nm;
+        const double sd = std::sqrt((sqrdZ - 2.0 * mean * Zsum + numInds * mean * mean) / (numInds - 1.0));
+        double* st = stats + 4 * (size_t)j;
+        st[0] = mean; st[1] = sd; st[2] = sqrdZ; st[3] = Zsum;
+        if (no + nt + nm == 0) invalid[j] = 1;   // EigenSparseMarker::isValid
+    });
+    for (uint32_t j = 0; j < M; ++j)
+        if (invalid[j]) return BRR_HIP_EINVAL;
+    return BRR_HIP_OK;
+}
+
 } // extern "C"
 
 // ---------------------------------------------------------------------------
@@ -222,9 +264,105 @@ bool ragged_layout(const uint8_t* p, uint64_t size, uint64_t n[3], uint64_t at[3
     return off == size;
 }
 
+// EigenSparseMarker layout (eigensparsemarker.cpp:17-58): {mean, sd, sqrdZ,
+// Zsum} f64, an int64 count, count f64 values, count int32 indices. Sets the
+// count and the byte offsets of the values and of the indices.
+bool eigen_layout(const uint8_t* p, uint64_t size, uint64_t* count, uint64_t* vals_at, uint64_t* idx_at)
+{
+    const uint64_t head = 4 * sizeof(double) + sizeof(int64_t);
+    if (size < head) return false;
+    int64_t c = 0;
+    std::memcpy(&c, p + 4 * sizeof(double), sizeof(int64_t));
+    if (c < 0 || (uint64_t)c > (size - head) / (sizeof(double) + sizeof(int32_t))) return false;
+    *count = (uint64_t)c;
+    *vals_at = head;
+    *idx_at = head + (uint64_t)c * sizeof(double);
+    return *idx_at + (uint64_t)c * sizeof(int32_t) == size;
+}
+
+// The class of each nonzero of a serialised EigenSparseMarker: 0 = genotype 1
+// (value 1), 1 = genotype 2 (value 2), 2 = missing (any other value: the
+// builder stores the mean there, eigensparsemarkerbuilder.cpp:67-74).
+inline int eigen_class(double v) { return v == 1.0 ? 0 : v == 2.0 ? 1 : 2; }
+
 } // namespace
 
 extern "C" {
+
+int brr_eigen_read_counts(const uint8_t* data, uint64_t data_bytes, const uint64_t* index, uint32_t M,
+                          int compressed, uint64_t* ones_off, uint64_t* twos_off, uint64_t* miss_off,
+                          uint32_t threads)
+{
+    if (!data || !index || !ones_off || !twos_off || !miss_off || M == 0) return BRR_HIP_EINVAL;
+    std::vector<uint8_t> bad(M, 0);
+    uint64_t* offs[3] = {ones_off, twos_off, miss_off};
+    parallel_for(M, threads, [&](uint32_t j) {
+        std::vector<uint8_t> buf;
+        const uint8_t* p;
+        uint64_t size, count, va, ia;
+        if (!marker_bytes(data, data_bytes, index, j, compressed, buf, &p, &size) ||
+            !eigen_layout(p, size, &count, &va, &ia)) {
+            bad[j] = 1;
+            return;
+        }
+        uint64_t n[3] = {0, 0, 0};
+        for (uint64_t q = 0; q < count; ++q) {
+            double v;
+            std::memcpy(&v, p + va + q * sizeof(double), sizeof(double));
+            ++n[eigen_class(v)];
+        }
+        for (int l = 0; l < 3; ++l) offs[l][j + 1] = n[l];
+    });
+    for (uint32_t j = 0; j < M; ++j)
+        if (bad[j]) return BRR_HIP_EINVAL;
+    for (int l = 0; l < 3; ++l) {
+        offs[l][0] = 0;
+        for (uint32_t j = 0; j < M; ++j) offs[l][j + 1] += offs[l][j];
+    }
+    return BRR_HIP_OK;
+}
+
+int brr_eigen_read_fill(const uint8_t* data, uint64_t data_bytes, const uint64_t* index, uint32_t N, uint32_t M,
+                        int compressed, const uint64_t* ones_off, const uint64_t* twos_off,
+                        const uint64_t* miss_off, int32_t* ones, int32_t* twos, int32_t* miss, double* stats,
+                        uint32_t threads)
+{
+    if (!data || !index || !ones_off || !twos_off || !miss_off || !stats || M == 0) return BRR_HIP_EINVAL;
+    const uint64_t* offs[3] = {ones_off, twos_off, miss_off};
+    int32_t* lists[3] = {ones, twos, miss};
+    for (int l = 0; l < 3; ++l)
+        if (offs[l][M] && !lists[l]) return BRR_HIP_EINVAL;
+    std::vector<uint8_t> bad(M, 0);
+    parallel_for(M, threads, [&](uint32_t j) {
+        std::vector<uint8_t> buf;
+        const uint8_t* p;
+        uint64_t size, count, va, ia;
+        if (!marker_bytes(data, data_bytes, index, j, compressed, buf, &p, &size) ||
+            !eigen_layout(p, size, &count, &va, &ia)) {
+            bad[j] = 1;
+            return;
+        }
+        std::memcpy(stats + 4 * (size_t)j, p, 4 * sizeof(double));
+        uint64_t n[3] = {0, 0, 0};
+        int64_t prev = -1;
+        for (uint64_t q = 0; q < count; ++q) {
+            double v;
+            int32_t idx;
+            std::memcpy(&v, p + va + q * sizeof(double), sizeof(double));
+            std::memcpy(&idx, p + ia + q * sizeof(int32_t), sizeof(int32_t));
+            if (idx <= prev || idx >= (int64_t)N) { bad[j] = 1; return; }
+            prev = idx;
+            const int l = eigen_class(v);
+            if (offs[l][j] + n[l] >= offs[l][j + 1]) { bad[j] = 1; return; }
+            lists[l][offs[l][j] + n[l]++] = idx;
+        }
+        for (int l = 0; l < 3; ++l)
+            if (n[l] != offs[l][j + 1] - offs[l][j]) bad[j] = 1;
+    });
+    for (uint32_t j = 0; j < M; ++j)
+        if (bad[j]) return BRR_HIP_EINVAL;
+    return BRR_HIP_OK;
+}
 
 int brr_ragged_read_counts(const uint8_t* data, uint64_t data_bytes, const uint64_t* index, uint32_t M,
                            int compressed, uint64_t* ones_off, uint64_t* twos_off, uint64_t* miss_off,

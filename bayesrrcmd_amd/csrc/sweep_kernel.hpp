@@ -49,7 +49,9 @@ constexpr uint32_t kMaxP = 256;   // workgroups of a launch (the exchange reduce
 constexpr int kIdxCap = 32;    // prefetched ragged indices per thread per block
 
 enum : int { kModeSweep = 0, kModeChain = 1 };
-enum : int { kLayoutDense = 1, kLayoutRagged = 2 };
+// kLayoutEigen: the ragged formulas with EigenBayesRKernel's dot, whose
+// missing calls count at the marker mean (eigenbayesrkernel.cpp:17-20).
+enum : int { kLayoutDense = 1, kLayoutRagged = 2, kLayoutEigen = 3 };
 constexpr int kPhases = 6;  // block load, dots, exchange, decisions, commit/update, iteration scalars
 
 // ctl words (persist across launches)
@@ -480,7 +482,7 @@ __device__ __forceinline__ void meta_store(const SweepParams& p, Shared<R>& L, i
         L.mstat[t][1] = m.w1;
         L.mstat[t][2] = m.w2;
         L.wt[t][0] = 2.0;   // genotype 2
-        L.wt[t][1] = 0.0;   // missing (not in the ones/twos lists)
+        L.wt[t][1] = (p.layout == kLayoutEigen) ? m.w0 : 0.0;   // missing: the mean (Eigen) or absent
         L.wt[t][2] = 1.0;   // genotype 1
         L.wt[t][3] = 0.0;   // genotype 0
     }

@@ -59,7 +59,10 @@ enum brr_hip_status {
 /* Marker layouts (PreprocessDataType, src/common.h:27-32). */
 enum brr_hip_layout {
     BRR_HIP_LAYOUT_DENSE_2BIT = 1, /* PreprocessDataType::Dense, columns kept as PLINK 2-bit */
-    BRR_HIP_LAYOUT_RAGGED = 2      /* PreprocessDataType::SparseRagged */
+    BRR_HIP_LAYOUT_RAGGED = 2,     /* PreprocessDataType::SparseRagged */
+    BRR_HIP_LAYOUT_EIGEN_SPARSE = 3 /* PreprocessDataType::SparseEigen: EigenBayesRKernel
+                                      (src/eigenbayesrkernel.cpp) on the ragged lists with
+                                      EigenSparse statistics; held as 2-bit codes. BayesR only. */
 };
 
 typedef struct brr_hip_ctx brr_hip_ctx;
@@ -134,7 +137,9 @@ int brr_hip_upload_dense(brr_hip_ctx* ctx, const uint8_t* bed, uint64_t column_s
 
 /* Ragged columns: CSR lists of individuals with genotype 1, 2 and missing,
  * ascending within a column; *_off have M+1 entries. stats: M x 4 doubles
- * {mean, sd, sqrdZ, Zsum} (src/sparsemarker.h:8-11). */
+ * {mean, sd, sqrdZ, Zsum} (src/sparsemarker.h:8-11). On an EigenSparse context
+ * the same lists (the nonzeros of EigenSparseMarker::Zg by value) with the
+ * EigenSparse statistics (brr_eigen_fill / brr_eigen_read_fill). */
 int brr_hip_upload_ragged(brr_hip_ctx* ctx,
                           const int32_t* ones, const uint64_t* ones_off,
                           const int32_t* twos, const uint64_t* twos_off,
@@ -149,9 +154,11 @@ int brr_hip_upload_ragged(brr_hip_ctx* ctx,
  * table (dense: DenseMarkerBuilder::endColumn, src/densemarkerbuilder.cpp:43-60)
  * or statistics (ragged: SparseMarker::updateStatistics, src/sparsemarker.cpp:8-19)
  * are computed on the device and equal brr_dense_luts / brr_ragged_fill bit for
- * bit. No index lists are built. A BayesR context takes its own layout only; a
- * BayesW context takes either. BRR_HIP_EINVAL for a bad file, or (ragged) a
- * marker without genotype 1 or 2 calls, as RaggedSparseMarkerBuilder rejects it.
+ * bit (EigenSparse statistics: brr_eigen_fill). No index lists are built. A
+ * BayesR context takes its own layout only; a BayesW context takes dense or
+ * ragged. BRR_HIP_EINVAL for a bad file, or (ragged) a
+ * marker without genotype 1 or 2 calls, as RaggedSparseMarkerBuilder rejects it
+ * (EigenSparse: a marker without any nonzero call, EigenSparseMarker::isValid).
  * brr_hip_last_stats().kernel_ms and brr_hip_last_phase_ms() give the device
  * time of the decode (phase 0: unpack and count, phase 1: dense tables). */
 int brr_hip_load_bed(brr_hip_ctx* ctx, const uint8_t* bed_file, uint64_t file_bytes, int layout);
@@ -303,6 +310,14 @@ int brr_ragged_fill(const uint8_t* bed, uint64_t column_stride, uint32_t num_ind
                     uint32_t num_snps, const uint64_t* ones_off, const uint64_t* twos_off,
                     const uint64_t* miss_off, int32_t* ones, int32_t* twos, int32_t* miss,
                     double* stats, uint32_t threads);
+/* EigenSparseMarkerBuilder (src/eigensparsemarkerbuilder.cpp:22-78): pass 1 is
+ * brr_ragged_count; pass 2 fills the same lists and the EigenSparse statistics
+ * (missing calls imputed at the mean in sqrdZ and Zsum). BRR_HIP_EINVAL if a
+ * column has no nonzero call (src/eigensparsemarker.cpp:60-71). */
+int brr_eigen_fill(const uint8_t* bed, uint64_t column_stride, uint32_t num_inds,
+                   uint32_t num_snps, const uint64_t* ones_off, const uint64_t* twos_off,
+                   const uint64_t* miss_off, int32_t* ones, int32_t* twos, int32_t* miss,
+                   double* stats, uint32_t threads);
 /* All M dense tables at once (threads host workers). */
 int brr_dense_luts(const uint8_t* bed, uint64_t column_stride, uint32_t num_inds,
                    uint32_t num_snps, double* lut, uint32_t threads);
@@ -320,6 +335,17 @@ int brr_ragged_read_fill(const uint8_t* data, uint64_t data_bytes, const uint64_
                          uint32_t num_inds, uint32_t num_snps, int compressed, const uint64_t* ones_off,
                          const uint64_t* twos_off, const uint64_t* miss_off, int32_t* ones, int32_t* twos,
                          int32_t* miss, double* stats, uint32_t threads);
+/* .eigen.sparsebed (EigenSparseMarker::write, src/eigensparsemarker.cpp:41-58) -> the same
+ * CSR arena (values 1 and 2 give the genotype 1 / 2 lists, any other value is a missing call
+ * imputed at the mean) with the file's statistics, for brr_hip_upload_ragged on an
+ * EigenSparse context. */
+int brr_eigen_read_counts(const uint8_t* data, uint64_t data_bytes, const uint64_t* index,
+                          uint32_t num_snps, int compressed, uint64_t* ones_off, uint64_t* twos_off,
+                          uint64_t* miss_off, uint32_t threads);
+int brr_eigen_read_fill(const uint8_t* data, uint64_t data_bytes, const uint64_t* index,
+                        uint32_t num_inds, uint32_t num_snps, int compressed, const uint64_t* ones_off,
+                        const uint64_t* twos_off, const uint64_t* miss_off, int32_t* ones, int32_t* twos,
+                        int32_t* miss, double* stats, uint32_t threads);
 /* .ppbed (DenseMarker::write, src/densemarker.cpp:34-49, N doubles per column) -> 2-bit codes
  * (M x ceil(N/4) bytes) and tables (M x 4) for brr_hip_upload_dense that reproduce every column
  * value bit for bit; BRR_HIP_EINVAL if a column holds more than 4 distinct values. */

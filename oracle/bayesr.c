@@ -42,6 +42,22 @@ double orc_compute_num(const orc_data* d, uint32_t j, const double* eps,
      * beta_old*(N-1) - mean*epsSum/sd + (sum_ones eps + 2*sum_twos eps)/sd */
     const double mean = d->stats[4 * (size_t)j + 0];
     const double sd = d->stats[4 * (size_t)j + 1];
+    if (d->layout == ORC_EIGEN) {
+        /* eigenbayesrkernel.cpp:17-20: Zg.dot(eps) / sd, Zg's nonzeros in index
+         * order with the values 1, 2 and the mean for a missing call */
+        uint64_t a = d->ones_off[j], b = d->twos_off[j], c = d->miss_off[j];
+        const uint64_t ae = d->ones_off[j + 1], be = d->twos_off[j + 1], ce = d->miss_off[j + 1];
+        double dot = 0.0;
+        while (a < ae || b < be || c < ce) {
+            const int64_t ia = a < ae ? d->ones[a] : INT64_MAX;
+            const int64_t ib = b < be ? d->twos[b] : INT64_MAX;
+            const int64_t ic = c < ce ? d->miss[c] : INT64_MAX;
+            if (ia < ib && ia < ic) { dot += 1.0 * eps[ia]; ++a; }
+            else if (ib < ic) { dot += 2.0 * eps[ib]; ++b; }
+            else { dot += mean * eps[ic]; ++c; }
+        }
+        return beta_old * (N - 1.0) - mean * epsSum / sd + dot / sd;
+    }
     double s1 = 0.0, s2 = 0.0;
     for (uint64_t q = d->ones_off[j]; q < d->ones_off[j + 1]; ++q) s1 += eps[d->ones[q]];
     for (uint64_t q = d->twos_off[j]; q < d->twos_off[j + 1]; ++q) s2 += eps[d->twos[q]];
@@ -63,7 +79,10 @@ void orc_apply_epsilon_change(const orc_data* d, uint32_t j, double* eps,
         }
         return;
     }
-    /* raggedbayesrkernel.cpp:10-30 */
+    /* raggedbayesrkernel.cpp:10-30. EigenBayesRKernel (eigenbayesrkernel.cpp:10-15)
+     * gives the same element values: (dBeta * v) / sd - (dBeta * mean) / sd with
+     * v = 1, 2 (2 dBeta / sd = 2 (dBeta / sd) exactly), 0 for a missing call
+     * (v = mean: the two terms cancel exactly) and -(dBeta * mean) / sd elsewhere. */
     const double mean = d->stats[4 * (size_t)j + 0];
     const double sd = d->stats[4 * (size_t)j + 1];
     const double Zsum = d->stats[4 * (size_t)j + 3];

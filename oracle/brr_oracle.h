@@ -68,7 +68,7 @@ void orc_dirichlet_rng(uint64_t seed, uint32_t iteration, uint32_t first_id, uin
 
 /* ------------------------------------------------------ markers (a1-a4) -- */
 
-enum { ORC_DENSE = 1, ORC_RAGGED = 2 };
+enum { ORC_DENSE = 1, ORC_RAGGED = 2, ORC_EIGEN = 3 };
 
 /* a1: decode one SNP-major PLINK .bed column (ceil(N/4) bytes) into
  * (allele1, allele2) pairs, preprocessgraph.cpp:34-57. a1[i], a2[i] in {0,1}. */
@@ -102,11 +102,30 @@ size_t orc_ragged_serialise(const double stats[4],
                             const int32_t* twos, uint64_t n_twos,
                             const int32_t* miss, uint64_t n_miss, uint8_t* buf);
 
+/* EigenSparseMarkerBuilder (eigensparsemarkerbuilder.cpp:22-78): the ragged
+ * builder's lists, statistics with the missing calls imputed at the mean
+ * (sqrdZ += mean^2 n_miss, Zsum += mean n_miss, then sd). Returns 1 if some
+ * call is nonzero (eigensparsemarker.cpp:60-71), 0 otherwise. */
+int orc_eigen_build(const uint8_t* col, uint32_t N,
+                    int32_t* ones, uint64_t* n_ones,
+                    int32_t* twos, uint64_t* n_twos,
+                    int32_t* miss, uint64_t* n_miss,
+                    double stats[4]);
+
+/* Serialised EigenSparseMarker (eigensparsemarker.cpp:41-58): 4 x f64, an
+ * int64 count, count f64 values (1, 2 or the mean for a missing call), count
+ * int32 indices, in index order. buf may be NULL to query the size. */
+size_t orc_eigen_serialise(const double stats[4],
+                           const int32_t* ones, uint64_t n_ones,
+                           const int32_t* twos, uint64_t n_twos,
+                           const int32_t* miss, uint64_t n_miss, uint8_t* buf);
+
 /* ---------------------------------------------------------- BayesR (a5-a10) */
 
 typedef struct {
     uint32_t N, M;
-    int layout;                 /* ORC_DENSE or ORC_RAGGED */
+    int layout;                 /* ORC_DENSE, ORC_RAGGED or ORC_EIGEN (the ragged
+                                   lists with EigenSparse statistics) */
     /* dense: packed SNP-major columns (no magic), M x ceil(N/4) bytes, and
      * per-marker 4-entry tables (orc_dense_lut) */
     const uint8_t* bed;
@@ -145,7 +164,8 @@ typedef struct {
 } orc_trace;
 
 /* a5: BayesRKernel::computeNum. Dense: densebayesrkernel.cpp:10-15.
- * Ragged: sparsebayesrkernel.cpp:24-27 with raggedbayesrkernel.cpp:32-35. */
+ * Ragged: sparsebayesrkernel.cpp:24-27 with raggedbayesrkernel.cpp:32-35.
+ * Eigen: sparsebayesrkernel.cpp:24-27 with eigenbayesrkernel.cpp:17-20. */
 double orc_compute_num(const orc_data* d, uint32_t j, const double* eps,
                        double beta_old, double epsSum);
 

@@ -162,3 +162,76 @@ size_t orc_ragged_serialise(const double stats[4],
     }
     return off;
 }
+
+int orc_eigen_build(const uint8_t* col, uint32_t N,
+                    int32_t* ones, uint64_t* n_ones,
+                    int32_t* twos, uint64_t* n_twos,
+                    int32_t* miss, uint64_t* n_miss,
+                    double stats[4])
+{
+    /* EigenSparseMarkerBuilder::processAllele (eigensparsemarkerbuilder.cpp:22-39):
+     * the same statistics as the ragged builder (SparseMarker::updateStatistics)
+     * and the same calls kept (1, 2, missing); endColumn (:41-78) then adds the
+     * mean-imputed missing calls to sqrdZ and Zsum before the sd. */
+    double mean = 0.0, sqrdZ = 0.0, Zsum = 0.0;
+    uint64_t no = 0, nt = 0, nm = 0;
+    for (uint32_t i = 0; i < N; ++i) {
+        const uint8_t b = col[i >> 2];
+        const unsigned k = (i & 3u) * 2u;
+        const unsigned allele1 = !((b >> k) & 1u);
+        const unsigned allele2 = !((b >> (k + 1)) & 1u);
+        if (allele1 == 0 && allele2 == 1) {
+            miss[nm++] = (int32_t)i;
+        } else if (allele1 == 1 || allele2 == 1) {
+            const double value = (double)(allele1 + allele2);
+            mean += value;
+            sqrdZ += value * value;
+            Zsum += value;
+            if (allele1 + allele2 == 1) ones[no++] = (int32_t)i;
+            else twos[nt++] = (int32_t)i;
+        }
+    }
+    const double numInds = (double)N;
+    mean /= numInds;
+    sqrdZ += (mean * mean) * (double)nm;
+    Zsum += mean * (double)nm;
+    const double sd = sqrt((sqrdZ - 2.0 * mean * Zsum + numInds * mean * mean) / (numInds - 1.0));
+    stats[0] = mean; stats[1] = sd; stats[2] = sqrdZ; stats[3] = Zsum;
+    *n_ones = no; *n_twos = nt; *n_miss = nm;
+    /* EigenSparseMarker::isValid (eigensparsemarker.cpp:60-71): some nonzero */
+    return (no + nt + nm) > 0 ? 1 : 0;
+}
+
+size_t orc_eigen_serialise(const double stats[4],
+                           const int32_t* ones, uint64_t n_ones,
+                           const int32_t* twos, uint64_t n_twos,
+                           const int32_t* miss, uint64_t n_miss, uint8_t* buf)
+{
+    /* EigenSparseMarker::write (eigensparsemarker.cpp:41-58): the SparseMarker
+     * statistics, an Eigen::Index (int64) count, the values (f64) and the inner
+     * indices (int32) of Zg in index order; a missing call's value is the mean
+     * (eigensparsemarkerbuilder.cpp:67-74). */
+    const uint64_t count = n_ones + n_twos + n_miss;
+    const size_t total = 4 * sizeof(double) + sizeof(int64_t) + count * (sizeof(double) + sizeof(int32_t));
+    if (!buf) return total;
+    size_t off = 0;
+    memcpy(buf + off, stats, 4 * sizeof(double)); off += 4 * sizeof(double);
+    const int64_t c64 = (int64_t)count;
+    memcpy(buf + off, &c64, sizeof(int64_t)); off += sizeof(int64_t);
+    uint8_t* vals = buf + off;
+    uint8_t* idxs = buf + off + count * sizeof(double);
+    uint64_t a = 0, b = 0, m = 0;
+    for (uint64_t q = 0; q < count; ++q) {
+        const int64_t ia = a < n_ones ? ones[a] : INT64_MAX;
+        const int64_t ib = b < n_twos ? twos[b] : INT64_MAX;
+        const int64_t im = m < n_miss ? miss[m] : INT64_MAX;
+        double v;
+        int32_t idx;
+        if (ia < ib && ia < im) { v = 1.0; idx = (int32_t)ia; ++a; }
+        else if (ib < im) { v = 2.0; idx = (int32_t)ib; ++b; }
+        else { v = stats[0]; idx = (int32_t)im; ++m; }
+        memcpy(vals + q * sizeof(double), &v, sizeof(double));
+        memcpy(idxs + q * sizeof(int32_t), &idx, sizeof(int32_t));
+    }
+    return total;
+}

@@ -21,7 +21,7 @@ P_u32 = C.POINTER(C.c_uint32)
 P_u64 = C.POINTER(C.c_uint64)
 P_f64 = C.POINTER(C.c_double)
 
-DENSE, RAGGED = 1, 2
+DENSE, RAGGED, EIGEN = 1, 2, 3
 
 
 class OrcData(C.Structure):
@@ -106,6 +106,9 @@ def lib(fast: bool = False):
         "orc_ragged_build": (C.c_int, [P_u8, C.c_uint32, P_i32, P_u64, P_i32, P_u64, P_i32, P_u64, P_f64]),
         "orc_ragged_serialise": (C.c_size_t, [P_f64, P_i32, C.c_uint64, P_i32, C.c_uint64, P_i32,
                                               C.c_uint64, P_u8]),
+        "orc_eigen_build": (C.c_int, [P_u8, C.c_uint32, P_i32, P_u64, P_i32, P_u64, P_i32, P_u64, P_f64]),
+        "orc_eigen_serialise": (C.c_size_t, [P_f64, P_i32, C.c_uint64, P_i32, C.c_uint64, P_i32,
+                                             C.c_uint64, P_u8]),
         "orc_compute_num": (C.c_double, [C.POINTER(OrcData), C.c_uint32, P_f64, C.c_double, C.c_double]),
         "orc_apply_epsilon_change": (None, [C.POINTER(OrcData), C.c_uint32, P_f64, P_f64, C.c_double,
                                             C.c_double]),
