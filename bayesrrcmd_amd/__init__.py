@@ -375,6 +375,10 @@ class HipSweep(_DeviceMarkers):
             _p(a.twos_off, C.c_uint64), _p(a.miss, C.c_int32), _p(a.miss_off, C.c_uint64),
             _p(np.ascontiguousarray(a.stats), C.c_double)), "brr_hip_upload_ragged")
 
+    def set_async(self, window: int):
+        """ParallelGraph's asynchronous sweep in windows of `window` markers (0: the exact sequential sweep)."""
+        self._ck(lib().brr_hip_set_async(self.ctx, int(window)), "brr_hip_set_async")
+
     def set_groups(self, group: np.ndarray):
         g = np.ascontiguousarray(group, dtype=np.int32)
         self._ck(lib().brr_hip_set_groups(self.ctx, _p(g, C.c_int32)), "brr_hip_set_groups")

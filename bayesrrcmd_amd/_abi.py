@@ -72,6 +72,7 @@ SIGNATURES = {
     "brr_hip_upload_ragged": (C.c_int, [CTX, P_i32, P_u64, P_i32, P_u64, P_i32, P_u64, P_f64]),
     "brr_hip_load_bed": (C.c_int, [CTX, P_u8, C.c_uint64, C.c_int]),
     "brr_hip_get_marker_tables": (C.c_int, [CTX, P_f64]),
+    "brr_hip_set_async": (C.c_int, [CTX, C.c_uint32]),
     "brr_hip_set_groups": (C.c_int, [CTX, P_i32]),
     "brr_hip_set_model": (C.c_int, [CTX, C.POINTER(Model)]),
     "brr_hip_set_state": (C.c_int, [CTX, P_f64, P_f64, P_i32]),

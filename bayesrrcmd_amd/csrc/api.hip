@@ -36,6 +36,8 @@ struct brr_hip_ctx {
     bool bw = false;             // BayesW context (brr_hip_bw_create)
     uint32_t nq = 0;             // BayesW Gauss-Hermite points
     bool codes = false;          // columns held as 2-bit codes on the device
+    uint32_t async = 0;          // asynchronous windows of B markers (brr_hip_set_async)
+    uint32_t B_seq = 0;          // block size of the sequential sweep
     bool have_markers = false;
     bool have_model = false;
     hipStream_t stream = nullptr;
@@ -247,6 +249,7 @@ SweepParams base_params(brr_hip_ctx* c)
     p.B = c->B;
     p.S = c->S;
     p.layout = c->layout;
+    p.async = c->async;
     p.dense = c->d_dense;
     p.col_words = c->col_words;
     p.col_bytes = c->col_bytes;
@@ -585,6 +588,7 @@ int brr_hip_create(const brr_hip_config* cfg, brr_hip_ctx** out)
         if (R == 3) R = 4;
         c->R = R;
         c->B = cfg->block_markers ? cfg->block_markers : 64u;
+        c->B_seq = c->B;
         probe = kernel_for(R, (int)cfg->layout, c->layout == kLayoutEigen);
     }
     c->S = S;
@@ -862,6 +866,16 @@ int brr_hip_get_marker_tables(brr_hip_ctx* c, double* out)
     const double* src = c->layout == kLayoutDense ? c->d_lut : c->d_stats;
     HIP_TRY(c, hipSetDevice(c->device));
     HIP_TRY(c, hipMemcpy(out, src, sizeof(double) * 4 * c->cfg.num_snps, hipMemcpyDeviceToHost));
+    return BRR_HIP_OK;
+}
+
+int brr_hip_set_async(brr_hip_ctx* c, uint32_t window)
+{
+    if (int k = need_kind(c, false)) return k;
+    if (c->T) return fail(c, BRR_HIP_EINVAL, "asynchronous windows need a single-chain context");
+    if (window > (uint32_t)kMaxB) return fail(c, BRR_HIP_EINVAL, "window must be 0 (sequential) or 1..64");
+    c->async = window ? 1u : 0u;
+    c->B = window ? window : c->B_seq;
     return BRR_HIP_OK;
 }
 

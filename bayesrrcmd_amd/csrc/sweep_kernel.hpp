@@ -71,7 +71,8 @@ struct SweepParams {
     uint32_t N, M, K, G;
     uint32_t P, B, S;
     int layout, mode;
-    uint32_t iterations, iter_base, chain, pad0;
+    uint32_t iterations, iter_base, chain;
+    uint32_t async;               // 1: each block is one window of the asynchronous sweep
     uint64_t seed;
     // dense arena: column j at dense + j * col_words (32-bit words)
     const uint32_t* dense;
@@ -137,6 +138,8 @@ struct Shared {
     uint64_t seg_start[3 * kMaxB];
     uint32_t seg_pref[3 * kMaxB + 1];   // exclusive prefix; [3 nb] = total
     double delta[4];
+    double dbeta[kMaxB];                // asynchronous windows: beta_old - beta of each changed slot
+    unsigned long long cmask;           // asynchronous windows: the changed slots
     double mu, sigmaE, epsSum;
     double sigmaG[kMaxG];
     double pi[kMaxG * kMaxK];
@@ -907,7 +910,9 @@ bayesr_sweep_kernel(SweepParams p)
                         changed = (bnew != beta_old);
                     }
                     const unsigned long long cm = __ballot(changed);
-                    const int c = cm ? pos + (int)__ffsll((long long)cm) - 1 : nb;
+                    // sequential sweep: the first change ends the round; asynchronous
+                    // window (pos = 0): every slot decides on the window-start residual
+                    const int c = (cm && !p.async) ? pos + (int)__ffsll((long long)cm) - 1 : nb;
                     const int last = (c < nb) ? c : nb - 1;
                     const bool accepted = k < nb && k <= last;
                     // v counts (integer-valued doubles: exact in any order)
@@ -944,6 +949,27 @@ bayesr_sweep_kernel(SweepParams p)
                             L.epsSum += dBeta * Zsum / sd - dBeta * mean * dN / sd;
                         }
                     }
+                    if (p.async) {
+                        // BayesRBase::processColumnAsync (BayesRBase.cpp:394-491): the window's
+                        // changes are applied afterwards in slot order (updateGlobal, :503-510),
+                        // epsilonSum included (SparseBayesRRG.cpp:94-103)
+                        const double dB = (k < nb) ? L.beta_old[k] - bnew : 0.0;
+                        if (changed) L.dbeta[k] = dB;
+                        if (!dense) {
+                            unsigned long long mset = cm;
+                            while (mset) {
+                                const int b = (int)__ffsll((long long)mset) - 1;
+                                const double db = __shfl(dB, b, 64);
+                                if (lane == 0) {
+                                    const int kk = pos + b;
+                                    const double mean = L.mstat[kk][0], sd = L.mstat[kk][1], Zsum = L.mstat[kk][2];
+                                    L.epsSum += db * Zsum / sd - db * mean * dN / sd;
+                                }
+                                mset &= mset - 1;
+                            }
+                        }
+                        if (lane == 0) L.cmask = cm << pos;
+                    }
                     if (lane == 0) L.change = c;
                 }
                 __syncthreads();
@@ -966,6 +992,39 @@ bayesr_sweep_kernel(SweepParams p)
                         for (int e = 0; e < 8; ++e) {
                             const uint32_t idx2 = lo + (uint32_t)r * kRun + 8u * (uint32_t)lane + (uint32_t)e;
                             if (idx2 < hi) eps[r][e] = eps[r][e] + wt_of((bits >> (2 * e)) & 3u, d0, d1, d2, d3);
+                        }
+                    }
+                }
+                if (p.async) {
+                    unsigned long long mset = L.cmask;
+                    while (mset) {
+                        const int k = (int)__ffsll((long long)mset) - 1;
+                        mset &= mset - 1;
+                        ++n_changes;
+                        const double dBeta = L.dbeta[k];
+                        double d0, d1, d2, d3;
+                        if (dense) {
+                            d0 = dBeta * L.wt[k][0];
+                            d1 = dBeta * L.wt[k][1];
+                            d2 = dBeta * L.wt[k][2];
+                            d3 = dBeta * L.wt[k][3];
+                        } else {
+                            const double mean = L.mstat[k][0], sd = L.mstat[k][1];
+                            const double meanAdjustment = dBeta * mean / sd;
+                            const double oneAdjustment = dBeta / sd;
+                            d3 = -meanAdjustment;
+                            d2 = -meanAdjustment + oneAdjustment;
+                            d0 = -meanAdjustment + 2 * oneAdjustment;
+                            d1 = -meanAdjustment + meanAdjustment;
+                        }
+#pragma unroll
+                        for (int r = 0; r < R; ++r) {
+                            const uint32_t bits = L.codes[k][r][lane >> 1] >> (16 * (lane & 1));
+#pragma unroll
+                            for (int e = 0; e < 8; ++e) {
+                                const uint32_t idx2 = lo + (uint32_t)r * kRun + 8u * (uint32_t)lane + (uint32_t)e;
+                                if (idx2 < hi) eps[r][e] = eps[r][e] + wt_of((bits >> (2 * e)) & 3u, d0, d1, d2, d3);
+                            }
                         }
                     }
                 }

@@ -43,7 +43,7 @@
 extern "C" {
 #endif
 
-#define BRR_HIP_ABI_VERSION 4
+#define BRR_HIP_ABI_VERSION 5
 
 enum brr_hip_status {
     BRR_HIP_OK = 0,
@@ -165,6 +165,18 @@ int brr_hip_load_bed(brr_hip_ctx* ctx, const uint8_t* bed_file, uint64_t file_by
 /* The M x 4 per-marker values held on the device: dense tables {dosage 0, 1, 2,
  * missing} or ragged statistics {mean, sd, sqrdZ, Zsum}. */
 int brr_hip_get_marker_tables(brr_hip_ctx* ctx, double* out);
+
+/* The asynchronous sweep of ParallelGraph (--analysis-type asyncppbayes,
+ * src/parallelgraph.cpp; BayesRBase::processColumnAsync / updateGlobal,
+ * src/BayesRBase.cpp:394-523) in its deterministic schedule: the order is cut
+ * into windows of `window` markers (Options::analysisTokens, 1..64); every
+ * marker of a window computes its num from the residual (and epsilonSum) at the
+ * window start, draws its component and beta, and the window's residual
+ * changes are then applied in order. window = 0 restores the exact sequential
+ * sweep (with the configured block size). Single-chain BayesR contexts only;
+ * applies to brr_hip_sweep and brr_hip_run_chain. Unlike processColumnAsync,
+ * acum is still written per marker. */
+int brr_hip_set_async(brr_hip_ctx* ctx, uint32_t window);
 
 /* Group label per marker (Data::G, src/data.cpp:424-453); default all 0. */
 int brr_hip_set_groups(brr_hip_ctx* ctx, const int32_t* group);

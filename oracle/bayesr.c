@@ -116,11 +116,12 @@ static double inv_sum_exp(const double* logL, int K, int r)
     return 1.0 / s;
 }
 
-void orc_process_column(const orc_data* d, const orc_model* m, orc_state* s,
-                        uint32_t j, double u, double z,
-                        double* num_out, double* acum_out)
+/* The component draw and beta of marker j given its num (BayesRBase.cpp:300-368;
+ * processColumnAsync :420-483 is the same arithmetic). Writes acum, beta, comp,
+ * v and betasqn; returns beta_old and beta_new. */
+static void decide_column(const orc_data* d, const orc_model* m, orc_state* s, uint32_t j, double u, double z,
+                          double num, double* acum_out, double* beta_old_out, double* beta_new_out)
 {
-    /* BayesRBase::processColumn, BayesRBase.cpp:287-392 */
     const int K = m->K;
     const int km1 = K - 1;
     const double NM1 = (double)(d->N - 1u);
@@ -133,8 +134,6 @@ void orc_process_column(const orc_data* d, const orc_model* m, orc_state* s,
     double denom[64], muk[64], logL[64];
     const double sigmaEOverSigmaG = sigmaE / sigmaG;
     for (int k = 0; k < km1; ++k) denom[k] = NM1 + sigmaEOverSigmaG * (1.0 / Sg[k]);
-
-    const double num = orc_compute_num(d, j, s->eps, beta_old, s->epsSum);
 
     muk[0] = 0.0;
     for (int k = 1; k < K; ++k) muk[k] = num / denom[k - 1];
@@ -149,7 +148,6 @@ void orc_process_column(const orc_data* d, const orc_model* m, orc_state* s,
     double acum = guard700(logL, K, 0) ? 0.0 : inv_sum_exp(logL, K, 0);
     s->acum[j] = acum;
     if (acum_out) *acum_out = acum;
-    if (num_out) *num_out = num;
 
     double beta_new = beta_old;
     for (int k = 0; k < K; ++k) {
@@ -170,21 +168,38 @@ void orc_process_column(const orc_data* d, const orc_model* m, orc_state* s,
             if (!guard700(logL, K, k + 1)) acum += inv_sum_exp(logL, K, k + 1);
         }
     }
+    *beta_old_out = beta_old;
+    *beta_new_out = beta_new;
+}
 
+void orc_process_column(const orc_data* d, const orc_model* m, orc_state* s,
+                        uint32_t j, double u, double z,
+                        double* num_out, double* acum_out)
+{
+    /* BayesRBase::processColumn, BayesRBase.cpp:287-392 */
+    const double num = orc_compute_num(d, j, s->eps, s->beta[j], s->epsSum);
+    if (num_out) *num_out = num;
+    double beta_old, beta_new;
+    decide_column(d, m, s, j, u, z, num, acum_out, &beta_old, &beta_new);
     /* BayesRBase.cpp:370-378 */
     const int skip = (beta_old == 0.0 && beta_new == 0.0);
     if (!skip) orc_apply_epsilon_change(d, j, s->eps, &s->epsSum, beta_old, beta_new);
 }
 
-void orc_sweep(const orc_data* d, const orc_model* m, orc_state* s,
-               const uint32_t* order, const double* u, const double* z,
-               orc_trace* trace)
+static void clear_sweep_counts(const orc_model* m, orc_state* s)
 {
     const int K = m->K;
     for (int g = 0; g < m->G; ++g) {
         s->betasqn[g] = 0.0;
         for (int k = 0; k < K; ++k) s->v[(size_t)g * K + k] = 0.0;
     }
+}
+
+void orc_sweep(const orc_data* d, const orc_model* m, orc_state* s,
+               const uint32_t* order, const double* u, const double* z,
+               orc_trace* trace)
+{
+    clear_sweep_counts(m, s);
     for (uint32_t p = 0; p < d->M; ++p) {
         const uint32_t j = order[p];
         double num = 0.0, acum = 0.0;
@@ -196,6 +211,37 @@ void orc_sweep(const orc_data* d, const orc_model* m, orc_state* s,
             if (trace->beta) trace->beta[p] = s->beta[j];
         }
     }
+}
+
+void orc_sweep_async(const orc_data* d, const orc_model* m, orc_state* s,
+                     const uint32_t* order, const double* u, const double* z, uint32_t window)
+{
+    /* ParallelGraph (parallelgraph.cpp) runs up to analysisTokens markers at once;
+     * processColumnAsync (BayesRBase.cpp:394-491) reads epsilon and epsilonSum under
+     * a shared lock, decides, and queues its change; updateGlobal (:503-510,
+     * SparseBayesRRG.cpp:94-103) applies the changes one at a time. This is the
+     * schedule in which every marker of a window reads the window-start state. */
+    clear_sweep_counts(m, s);
+    if (window == 0) window = 1;
+    double* num = (double*)malloc(sizeof(double) * window);
+    double* bold = (double*)malloc(sizeof(double) * window);
+    double* bnew = (double*)malloc(sizeof(double) * window);
+    for (uint32_t p0 = 0; p0 < d->M; p0 += window) {
+        const uint32_t n = (d->M - p0 < window) ? d->M - p0 : window;
+        for (uint32_t q = 0; q < n; ++q) {
+            const uint32_t j = order[p0 + q];
+            num[q] = orc_compute_num(d, j, s->eps, s->beta[j], s->epsSum);
+        }
+        for (uint32_t q = 0; q < n; ++q) {
+            const uint32_t j = order[p0 + q];
+            decide_column(d, m, s, j, u[j], z[j], num[q], NULL, &bold[q], &bnew[q]);
+        }
+        for (uint32_t q = 0; q < n; ++q) {
+            if (bold[q] == 0.0 && bnew[q] == 0.0) continue;
+            orc_apply_epsilon_change(d, order[p0 + q], s->eps, &s->epsSum, bold[q], bnew[q]);
+        }
+    }
+    free(num); free(bold); free(bnew);
 }
 
 void orc_init(const orc_data* d, const orc_model* m, orc_state* s,
@@ -282,7 +328,16 @@ void orc_run_gibbs(const orc_data* d, const orc_model* m, orc_state* s,
                    uint32_t iterations, uint32_t* order, int shuffle_mode,
                    double* hist)
 {
-    /* BayesRBase::runGibbs, BayesRBase.cpp:130-285 (no fixed effects) */
+    orc_run_gibbs_window(d, m, s, y, seed, chain, iterations, order, shuffle_mode, hist, 0);
+}
+
+void orc_run_gibbs_window(const orc_data* d, const orc_model* m, orc_state* s,
+                          const double* y, uint64_t seed, uint32_t chain,
+                          uint32_t iterations, uint32_t* order, int shuffle_mode,
+                          double* hist, uint32_t window)
+{
+    /* BayesRBase::runGibbs, BayesRBase.cpp:130-285 (no fixed effects); window > 0:
+     * the sweep in asynchronous windows (orc_sweep_async) */
     const uint32_t M = d->M;
     const int G = m->G, K = m->K;
     orc_init(d, m, s, y, seed, chain);
@@ -293,7 +348,8 @@ void orc_run_gibbs(const orc_data* d, const orc_model* m, orc_state* s,
         orc_fill_marker_uz(seed, it, chain, M, u, z);
         if (shuffle_mode == 1) orc_shuffle_libc(M, order);
         else orc_shuffle_philox(seed, it, chain, M, order);
-        orc_sweep(d, m, s, order, u, z, NULL);
+        if (window) orc_sweep_async(d, m, s, order, u, z, window);
+        else orc_sweep(d, m, s, order, u, z, NULL);
         double e2 = 0.0;
         for (uint32_t i = 0; i < d->N; ++i) e2 += s->eps[i] * s->eps[i];
         double m0_total = 0.0;

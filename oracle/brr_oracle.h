@@ -187,6 +187,14 @@ void orc_sweep(const orc_data* d, const orc_model* m, orc_state* s,
                const uint32_t* order, const double* u, const double* z,
                orc_trace* trace);
 
+/* (f)-4: the asynchronous sweep of ParallelGraph (parallelgraph.cpp,
+ * BayesRBase.cpp:394-523) in windows of `window` markers: each marker of a
+ * window computes num from the window-start eps / epsSum and decides; the
+ * window's residual changes are then applied in order. window = 1 is the
+ * sequential sweep. */
+void orc_sweep_async(const orc_data* d, const orc_model* m, orc_state* s,
+                     const uint32_t* order, const double* u, const double* z, uint32_t window);
+
 /* a8: BayesRBase::init (BayesRBase.cpp:39-99). y is the phenotype as stored
  * by the reference (already rounded through float, data.cpp:357-359). */
 void orc_init(const orc_data* d, const orc_model* m, orc_state* s,
@@ -209,6 +217,12 @@ void orc_run_gibbs(const orc_data* d, const orc_model* m, orc_state* s,
                    const double* y, uint64_t seed, uint32_t chain,
                    uint32_t iterations, uint32_t* order, int shuffle_mode,
                    double* hist);
+
+/* orc_run_gibbs with the sweep in asynchronous windows (window 0: sequential). */
+void orc_run_gibbs_window(const orc_data* d, const orc_model* m, orc_state* s,
+                          const double* y, uint64_t seed, uint32_t chain,
+                          uint32_t iterations, uint32_t* order, int shuffle_mode,
+                          double* hist, uint32_t window);
 
 /* Scalar-draw ids of the Philox scalar stream. */
 enum { ORC_DRAW_MU = 0, ORC_DRAW_SIGMAE = 1, ORC_DRAW_SIGMAG0 = 2 };

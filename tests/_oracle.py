@@ -124,6 +124,11 @@ def lib(fast: bool = False):
                                   C.c_uint32, C.c_uint32]),
         "orc_run_gibbs": (None, [C.POINTER(OrcData), C.POINTER(OrcModel), C.POINTER(OrcState), P_f64,
                                  C.c_uint64, C.c_uint32, C.c_uint32, P_u32, C.c_int, P_f64]),
+        "orc_run_gibbs_window": (None, [C.POINTER(OrcData), C.POINTER(OrcModel), C.POINTER(OrcState), P_f64,
+                                        C.c_uint64, C.c_uint32, C.c_uint32, P_u32, C.c_int, P_f64,
+                                        C.c_uint32]),
+        "orc_sweep_async": (None, [C.POINTER(OrcData), C.POINTER(OrcModel), C.POINTER(OrcState), P_u32, P_f64,
+                                   P_f64, C.c_uint32]),
         "orc_arms": (C.c_int, [P_f64, C.c_int, C.c_double, C.c_double, LOGDENS, C.c_void_p, C.c_int,
                                C.POINTER(OrcArmsRng), P_f64, C.POINTER(C.c_int)]),
         "orc_bw_gh_rule": (C.c_int, [C.c_int, P_f64, P_f64, P_f64]),
@@ -288,6 +293,13 @@ class OracleProblem:
                         p(u, C.c_double), p(z, C.c_double), C.byref(tr) if tr is not None else None)
         return out
 
+    def sweep_async(self, order, u, z, window):
+        order = np.ascontiguousarray(order, dtype=np.uint32)
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        z = np.ascontiguousarray(z, dtype=np.float64)
+        lib().orc_sweep_async(C.byref(self.data), C.byref(self.model), C.byref(self.state), p(order, C.c_uint32),
+                              p(u, C.c_double), p(z, C.c_double), window)
+
     def compute_num(self, j, beta_old):
         return lib().orc_compute_num(C.byref(self.data), j, p(self.eps, C.c_double), beta_old,
                                      self.state.epsSum)
@@ -305,16 +317,26 @@ class OracleProblem:
         lib().orc_post_sweep(C.byref(self.data), C.byref(self.model), C.byref(self.state), seed, iteration,
                              chain)
 
-    def run_gibbs(self, y, seed, iterations, order=None, shuffle_mode=0, chain=0):
+    def run_gibbs(self, y, seed, iterations, order=None, shuffle_mode=0, chain=0, window=0):
         y = np.ascontiguousarray(y, dtype=np.float64)
         if order is None:
             order = np.arange(self.m, dtype=np.uint32)
         order = np.ascontiguousarray(order, dtype=np.uint32)
         hist = np.zeros((iterations, 4 + self.G))
-        lib().orc_run_gibbs(C.byref(self.data), C.byref(self.model), C.byref(self.state), p(y, C.c_double),
-                            seed, chain, iterations, p(order, C.c_uint32), shuffle_mode,
-                            p(hist, C.c_double))
+        lib().orc_run_gibbs_window(C.byref(self.data), C.byref(self.model), C.byref(self.state), p(y, C.c_double),
+                                   seed, chain, iterations, p(order, C.c_uint32), shuffle_mode,
+                                   p(hist, C.c_double), window)
         return hist, order
+
+    def snapshot(self):
+        """Copies of the chain state arrays and scalars."""
+        return dict(eps=self.eps.copy(), beta=self.beta.copy(), comp=self.comp.copy(), acum=self.acum.copy(),
+                    v=self.v.copy(), betasqn=self.betasqn.copy(), epsSum=self.state.epsSum)
+
+    def restore(self, snap):
+        for k in ("eps", "beta", "comp", "acum", "v", "betasqn"):
+            getattr(self, k)[:] = snap[k]
+        self.state.epsSum = snap["epsSum"]
 
 
 @dataclass
