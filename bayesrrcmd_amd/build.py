@@ -41,21 +41,32 @@ def _run(cmd: list[str]) -> None:
         raise RuntimeError(f"command failed ({res.returncode}): {' '.join(cmd)}")
 
 
+# device translation units: the host API with its small kernels, and one file per
+# kernel family (compiled in parallel)
+HIP_SOURCES = ("api.hip", "kern_sweep_seq.hip", "kern_sweep_async.hip", "kern_batch.hip", "kern_bayesw.hip")
+HOST_SOURCES = ("host_markers.cpp", "host_driver.cpp", "synth.cpp")
+
+
 def build_lib() -> Path:
+    from concurrent.futures import ThreadPoolExecutor
     objdir = PKG / "_objs"
     objdir.mkdir(exist_ok=True)
-    host_objs = []
-    for src in ("host_markers.cpp", "host_driver.cpp", "synth.cpp"):
+    cmds, objs = [], []
+    for src in HOST_SOURCES:
         obj = objdir / (Path(src).stem + ".o")
-        _run(["g++", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-pthread",
-              f"-I{INCLUDE}", f"-I{CSRC}", "-c", str(CSRC / src), "-o", str(obj)])
-        host_objs.append(str(obj))
-    dev_obj = objdir / "api.o"
-    _run([HIPCC, *HIP_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", "-c",
-          str(CSRC / "api.hip"), "-o", str(dev_obj)])
+        cmds.append(["g++", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-pthread",
+                     f"-I{INCLUDE}", f"-I{CSRC}", "-c", str(CSRC / src), "-o", str(obj)])
+        objs.append(str(obj))
+    for src in HIP_SOURCES:
+        obj = objdir / (Path(src).stem + ".o")
+        cmds.append([HIPCC, *HIP_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", "-c", str(CSRC / src), "-o", str(obj)])
+        objs.append(str(obj))
+    workers = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with ThreadPoolExecutor(max_workers=workers) as pool:
+        for f in [pool.submit(_run, c) for c in cmds]:
+            f.result()
     tmp = LIB.with_suffix(".so.tmp")
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", str(dev_obj), *host_objs,
-          "-pthread", "-lz", "-o", str(tmp)])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-pthread", "-lz", "-o", str(tmp)])
     os.replace(tmp, LIB)
     shutil.rmtree(objdir, ignore_errors=True)
     return LIB

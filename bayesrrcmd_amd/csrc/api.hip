@@ -16,6 +16,15 @@
 #include "batch_kernel.hpp"
 #include "bayesw_kernel.hpp"
 #include "bed_kernels.hpp"
+#include "kernels.hpp"
+
+namespace brr {
+// instantiated in kern_sweep_seq.hip, kern_sweep_async.hip, kern_batch.hip, kern_bayesw.hip
+BRR_SWEEP_KERNELS_OF(BRR_SWEEP_DECL, false)
+BRR_SWEEP_KERNELS_OF(BRR_SWEEP_DECL, true)
+BRR_BATCH_KERNELS(BRR_BATCH_DECL)
+BRR_BW_KERNELS(BRR_BW_DECL)
+} // namespace brr
 
 using namespace brr;
 
@@ -280,20 +289,28 @@ SweepParams base_params(brr_hip_ctx* c)
     return p;
 }
 
-template <int R>
-const void* kernel_r(int layout, bool codes)
+template <int R, bool kAsync>
+const void* kernel_ra(int layout, bool codes)
 {
-    if (layout == kLayoutDense) return (const void*)bayesr_sweep_kernel<R, true, true>;
-    if (layout == kLayoutEigen && !codes) return nullptr;   // Eigen contexts hold codes
-    return codes ? (const void*)bayesr_sweep_kernel<R, false, true> : (const void*)bayesr_sweep_kernel<R, false, false>;
+    if (layout == kLayoutDense) return (const void*)bayesr_sweep_kernel<R, kLayoutDense, true, kAsync>;
+    if (layout == kLayoutEigen)   // Eigen contexts hold codes
+        return codes ? (const void*)bayesr_sweep_kernel<R, kLayoutEigen, true, kAsync> : nullptr;
+    return codes ? (const void*)bayesr_sweep_kernel<R, kLayoutRagged, true, kAsync>
+                 : (const void*)bayesr_sweep_kernel<R, kLayoutRagged, false, kAsync>;
 }
 
-const void* kernel_for(uint32_t R, int layout, bool codes)
+template <int R>
+const void* kernel_r(int layout, bool codes, bool async)
+{
+    return async ? kernel_ra<R, true>(layout, codes) : kernel_ra<R, false>(layout, codes);
+}
+
+const void* kernel_for(uint32_t R, int layout, bool codes, bool async = false)
 {
     switch (R) {
-    case 1: return kernel_r<1>(layout, codes);
-    case 2: return kernel_r<2>(layout, codes);
-    case 4: return kernel_r<4>(layout, codes);
+    case 1: return kernel_r<1>(layout, codes, async);
+    case 2: return kernel_r<2>(layout, codes, async);
+    case 4: return kernel_r<4>(layout, codes, async);
     default: return nullptr;
     }
 }
@@ -462,7 +479,7 @@ int launch_fn(brr_hip_ctx* c, const void* fn, void* param)
 
 int launch(brr_hip_ctx* c, SweepParams& p)
 {
-    return launch_fn(c, kernel_for(c->R, c->layout, c->codes), &p);
+    return launch_fn(c, kernel_for(c->R, c->layout, c->codes, c->async != 0), &p);
 }
 
 // Batched contexts: the same launch with the sweep-mode / chain-mode fields of p.

@@ -1224,7 +1224,7 @@ done:
 
 // calculateSumFailure (raggedbayeswkernel.cpp:18-25) from the 2-bit codes: one
 // workgroup per marker; list sums pass through an int as in the reference.
-__global__ void bw_sum_failure_kernel(const uint32_t* codes, uint64_t col_words, uint32_t N, const double* fail,
+static __global__ void bw_sum_failure_kernel(const uint32_t* codes, uint64_t col_words, uint32_t N, const double* fail,
                                       const double* stats, double* sumf)
 {
     __shared__ double r1[kWaves], r2[kWaves], rf[kWaves];
@@ -1256,7 +1256,7 @@ __global__ void bw_sum_failure_kernel(const uint32_t* codes, uint64_t col_words,
 }
 
 // DenseBayesWKernel::calculateSumFailure (densebayeswkernel.cpp:15-18): sum_i Cx_i f_i.
-__global__ void bw_dense_sum_failure_kernel(const uint32_t* codes, uint64_t col_words, uint32_t N, const double* fail,
+static __global__ void bw_dense_sum_failure_kernel(const uint32_t* codes, uint64_t col_words, uint32_t N, const double* fail,
                                             const double* lut, double* sumf)
 {
     __shared__ double red[kWaves];
@@ -1278,7 +1278,7 @@ __global__ void bw_dense_sum_failure_kernel(const uint32_t* codes, uint64_t col_
 
 // RaggedBayesWKernel::setVi (raggedbayeswkernel.cpp:10-16) of marker j for
 // vi = exp(alpha eps - EulerGamma): out = {vi_sum, vi_0, vi_1, vi_2}. One workgroup.
-__global__ void bw_set_vi_kernel(const uint32_t* codes, uint64_t col_words, uint32_t N, const double* eps,
+static __global__ void bw_set_vi_kernel(const uint32_t* codes, uint64_t col_words, uint32_t N, const double* eps,
                                  double alpha, uint32_t j, double* out)
 {
     __shared__ double r0[kWaves], r1[kWaves], r2[kWaves];
@@ -1310,7 +1310,7 @@ __global__ void bw_set_vi_kernel(const uint32_t* codes, uint64_t col_words, uint
 
 // eps += sign * calculateResidualUpdate(beta) of marker j (raggedbayeswkernel.cpp:27-43).
 // Dense layout (lut != null): Cx * beta (densebayeswkernel.cpp:20-23).
-__global__ void bw_residual_kernel(const uint32_t* codes, uint64_t col_words, uint32_t N, const double* stats,
+static __global__ void bw_residual_kernel(const uint32_t* codes, uint64_t col_words, uint32_t N, const double* stats,
                                    const double* lut, uint32_t j, double beta, int sign, double* eps)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -38,7 +38,7 @@ __device__ __forceinline__ uint32_t bed_wave_sum_u32(uint32_t v)
 // EigenSparse statistics instead (missing calls imputed at the mean,
 // eigensparsemarkerbuilder.cpp:41-59), and only a marker without any nonzero
 // call (eigensparsemarker.cpp:60-71) lowers *bad.
-__global__ void __launch_bounds__(kBedThreads)
+static __global__ void __launch_bounds__(kBedThreads)
 bed_unpack_kernel(const uint32_t* __restrict__ src, uint64_t cb, uint32_t N, uint64_t col_words,
                   uint32_t* __restrict__ codes, uint32_t* __restrict__ counts, double* __restrict__ stats,
                   uint32_t* __restrict__ bad, int eigen)
@@ -149,7 +149,7 @@ __device__ __forceinline__ double bed_ordered_sum(const uint4* col, uint32_t N, 
 // missing call. The mean of the non-missing dosages and, without missing calls,
 // the column mean are integer sums (exact from the counts); with missing calls
 // the column sum, and always the squared norm, are summed in individual order.
-__global__ void __launch_bounds__(kBedThreads)
+static __global__ void __launch_bounds__(kBedThreads)
 bed_dense_table_kernel(const uint32_t* __restrict__ codes, uint64_t col_words, uint32_t N, uint32_t M,
                        const uint32_t* __restrict__ counts, double* __restrict__ lut)
 {

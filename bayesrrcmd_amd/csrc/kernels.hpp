@@ -1,0 +1,27 @@
+// kernels.hpp -- every kernel instantiation the library launches.
+//
+// The kern_*.hip files instantiate them (one translation unit per family, so
+// they compile in parallel); api.hip declares them extern and takes their
+// addresses for the launches and occupancy queries.
+#pragma once
+
+// bayesr_sweep_kernel<R, layout, codes, async>
+#define BRR_SWEEP_KERNELS_OF(X, A)                                                                      \
+    X(1, kLayoutDense, true, A) X(1, kLayoutRagged, true, A) X(1, kLayoutRagged, false, A)              \
+    X(1, kLayoutEigen, true, A) X(2, kLayoutDense, true, A) X(2, kLayoutRagged, true, A)               \
+    X(2, kLayoutRagged, false, A) X(2, kLayoutEigen, true, A) X(4, kLayoutDense, true, A)              \
+    X(4, kLayoutRagged, true, A) X(4, kLayoutRagged, false, A) X(4, kLayoutEigen, true, A)
+
+// bayesr_batch_kernel<NT, dense>
+#define BRR_BATCH_KERNELS(X) X(4, true) X(4, false) X(8, true) X(8, false) X(13, true) X(13, false) \
+    X(16, true) X(16, false)
+
+// bayesw_sweep_kernel<R, dense>
+#define BRR_BW_KERNELS(X) X(1, true) X(1, false) X(2, true) X(2, false) X(4, true) X(4, false)
+
+#define BRR_SWEEP_INST(R, L, C, A) template __global__ void bayesr_sweep_kernel<R, L, C, A>(SweepParams);
+#define BRR_SWEEP_DECL(R, L, C, A) extern template __global__ void bayesr_sweep_kernel<R, L, C, A>(SweepParams);
+#define BRR_BATCH_INST(NT, D) template __global__ void bayesr_batch_kernel<NT, D>(BatchParams);
+#define BRR_BATCH_DECL(NT, D) extern template __global__ void bayesr_batch_kernel<NT, D>(BatchParams);
+#define BRR_BW_INST(R, D) template __global__ void bayesw_sweep_kernel<R, D>(BwParams);
+#define BRR_BW_DECL(R, D) extern template __global__ void bayesw_sweep_kernel<R, D>(BwParams);

@@ -137,9 +137,8 @@ struct Shared {
     // ragged: segments of the next block (bounds fetched a block ahead)
     uint64_t seg_start[3 * kMaxB];
     uint32_t seg_pref[3 * kMaxB + 1];   // exclusive prefix; [3 nb] = total
-    double delta[4];
-    double dbeta[kMaxB];                // asynchronous windows: beta_old - beta of each changed slot
-    unsigned long long cmask;           // asynchronous windows: the changed slots
+    double dl[kMaxB][4];                // residual change per 2-bit code of each applied slot
+    unsigned long long cmask;           // the slots whose change is applied this round
     double mu, sigmaE, epsSum;
     double sigmaG[kMaxG];
     double pi[kMaxG * kMaxK];
@@ -467,7 +466,7 @@ __device__ __forceinline__ void meta_issue(const SweepParams& p, uint32_t j, uin
     }
 }
 
-template <int R>
+template <int R, int kLay>
 __device__ __forceinline__ void meta_store(const SweepParams& p, Shared<R>& L, int t, const MetaNext& m)
 {
     L.mk[t] = m.j;
@@ -475,6 +474,9 @@ __device__ __forceinline__ void meta_store(const SweepParams& p, Shared<R>& L, i
     L.beta_old[t] = bits_as_double(m.beta_bits);
     L.uu[t] = m.u;
     L.zz[t] = m.z;
+    // a run-time branch on purpose: with a compile-time one the compiler sinks
+    // the prefetch loads of meta_issue towards this commit, and the block then
+    // waits for their full latency
     if (p.layout == kLayoutDense) {
         L.wt[t][0] = m.w0;
         L.wt[t][1] = m.w1;
@@ -485,7 +487,8 @@ __device__ __forceinline__ void meta_store(const SweepParams& p, Shared<R>& L, i
         L.mstat[t][1] = m.w1;
         L.mstat[t][2] = m.w2;
         L.wt[t][0] = 2.0;   // genotype 2
-        L.wt[t][1] = (p.layout == kLayoutEigen) ? m.w0 : 0.0;   // missing: the mean (Eigen) or absent
+        // missing: absent from the ragged dot, at the marker mean in EigenBayesRKernel's
+        L.wt[t][1] = (kLay == kLayoutEigen) ? m.w0 : 0.0;
         L.wt[t][2] = 1.0;   // genotype 1
         L.wt[t][3] = 0.0;   // genotype 0
     }
@@ -645,9 +648,11 @@ __device__ __forceinline__ void ragged_store(const SweepParams& p, Shared<R>& L,
 
 // -------------------------------------------------------------------- kernel --
 
-// kDenseMath: DenseBayesRRmz formulas (else SparseBayesRRG / ragged formulas).
+// kLay: kLayoutDense (DenseBayesRRmz formulas), kLayoutRagged or kLayoutEigen
+// (SparseBayesRRG formulas with RaggedBayesRKernel's / EigenBayesRKernel's dot).
 // kCodes: columns stored as 2-bit codes (else as ragged index lists).
-template <int R, bool kDenseMath, bool kCodes>
+// kAsync: each block is one window of the asynchronous sweep.
+template <int R, int kLay, bool kCodes, bool kAsync>
 __global__ void __launch_bounds__(kThreads)
 bayesr_sweep_kernel(SweepParams p)
 {
@@ -662,7 +667,7 @@ bayesr_sweep_kernel(SweepParams p)
     const ScalarLayout SL{p.G, p.K};
     const double dN = (double)p.N;
     const double NM1 = (double)(p.N - 1u);
-    constexpr bool dense = kDenseMath;
+    constexpr bool dense = kLay == kLayoutDense;
 
     // eps slice in registers: individual lo + 512 r + 8 lane + e
     double eps[R][8];
@@ -771,7 +776,7 @@ bayesr_sweep_kernel(SweepParams p)
                 idx_issue<R>(p, L, nb0, tid, idx, itm);
                 ragged_store<R>(p, L, nb0, lo, tid, idx, itm);
             }
-            if (tid < nb0) meta_store<R>(p, L, tid, m);
+            if (tid < nb0) meta_store<R, kLay>(p, L, tid, m);
             __syncthreads();
             if (nblocks > 1) {
                 const int nb1 = (int)min(p.B, p.M - p.B);
@@ -912,7 +917,7 @@ bayesr_sweep_kernel(SweepParams p)
                     const unsigned long long cm = __ballot(changed);
                     // sequential sweep: the first change ends the round; asynchronous
                     // window (pos = 0): every slot decides on the window-start residual
-                    const int c = (cm && !p.async) ? pos + (int)__ffsll((long long)cm) - 1 : nb;
+                    const int c = (cm && !kAsync) ? pos + (int)__ffsll((long long)cm) - 1 : nb;
                     const int last = (c < nb) ? c : nb - 1;
                     const bool accepted = k < nb && k <= last;
                     // v counts (integer-valued doubles: exact in any order)
@@ -932,44 +937,43 @@ bayesr_sweep_kernel(SweepParams p)
                         if (sel >= 0) p.comp[j] = sel;
                         p.acum[j] = acum0;
                     }
-                    if (k < nb && k == c) {
-                        const double dBeta = L.beta_old[k] - bnew;
+                    // residual change per 2-bit code of each slot applied below: the first
+                    // change (sequential) or every change of the window (asynchronous:
+                    // BayesRBase::processColumnAsync, BayesRBase.cpp:394-491, with the
+                    // window's updateGlobal calls, :503-510, in slot order)
+                    const bool apply = k < nb && (kAsync ? changed : k == c);
+                    const double dBeta = apply ? L.beta_old[k] - bnew : 0.0;
+                    if (apply) {
                         if (dense) {
 #pragma unroll
-                            for (int q = 0; q < 4; ++q) L.delta[q] = dBeta * L.wt[k][q];
+                            for (int q = 0; q < 4; ++q) L.dl[k][q] = dBeta * L.wt[k][q];
                         } else {
                             // raggedbayesrkernel.cpp:10-30 and sparsebayesrkernel.cpp:29-34
                             const double mean = L.mstat[k][0], sd = L.mstat[k][1], Zsum = L.mstat[k][2];
                             const double meanAdjustment = dBeta * mean / sd;
                             const double oneAdjustment = dBeta / sd;
-                            L.delta[3] = -meanAdjustment;
-                            L.delta[2] = -meanAdjustment + oneAdjustment;
-                            L.delta[0] = -meanAdjustment + 2 * oneAdjustment;
-                            L.delta[1] = -meanAdjustment + meanAdjustment;
-                            L.epsSum += dBeta * Zsum / sd - dBeta * mean * dN / sd;
+                            L.dl[k][3] = -meanAdjustment;
+                            L.dl[k][2] = -meanAdjustment + oneAdjustment;
+                            L.dl[k][0] = -meanAdjustment + 2 * oneAdjustment;
+                            L.dl[k][1] = -meanAdjustment + meanAdjustment;
+                            if (!kAsync) L.epsSum += dBeta * Zsum / sd - dBeta * mean * dN / sd;
                         }
                     }
-                    if (p.async) {
-                        // BayesRBase::processColumnAsync (BayesRBase.cpp:394-491): the window's
-                        // changes are applied afterwards in slot order (updateGlobal, :503-510),
-                        // epsilonSum included (SparseBayesRRG.cpp:94-103)
-                        const double dB = (k < nb) ? L.beta_old[k] - bnew : 0.0;
-                        if (changed) L.dbeta[k] = dB;
-                        if (!dense) {
-                            unsigned long long mset = cm;
-                            while (mset) {
-                                const int b = (int)__ffsll((long long)mset) - 1;
-                                const double db = __shfl(dB, b, 64);
-                                if (lane == 0) {
-                                    const int kk = pos + b;
-                                    const double mean = L.mstat[kk][0], sd = L.mstat[kk][1], Zsum = L.mstat[kk][2];
-                                    L.epsSum += db * Zsum / sd - db * mean * dN / sd;
-                                }
-                                mset &= mset - 1;
+                    if (kAsync && !dense) {
+                        // the window's epsilonSum updates in slot order (SparseBayesRRG.cpp:94-103)
+                        unsigned long long mset = cm;
+                        while (mset) {
+                            const int b = (int)__ffsll((long long)mset) - 1;
+                            const double db = __shfl(dBeta, b, 64);
+                            if (lane == 0) {
+                                const int kk = pos + b;
+                                const double mean = L.mstat[kk][0], sd = L.mstat[kk][1], Zsum = L.mstat[kk][2];
+                                L.epsSum += db * Zsum / sd - db * mean * dN / sd;
                             }
+                            mset &= mset - 1;
                         }
-                        if (lane == 0) L.cmask = cm << pos;
                     }
+                    if (kAsync && lane == 0) L.cmask = cm << pos;
                     if (lane == 0) L.change = c;
                 }
                 __syncthreads();
@@ -982,51 +986,29 @@ bayesr_sweep_kernel(SweepParams p)
                 }
                 const int c = L.change;
                 const int last = (c < nb) ? c : nb - 1;
-                if (c < nb) {
+                // the applied slots: every change of an asynchronous window, else slot c
+                auto apply_slot = [&](int k) {
                     ++n_changes;
-                    const double d0 = L.delta[0], d1 = L.delta[1], d2 = L.delta[2], d3 = L.delta[3];
+                    const double d0 = L.dl[k][0], d1 = L.dl[k][1], d2 = L.dl[k][2], d3 = L.dl[k][3];
 #pragma unroll
                     for (int r = 0; r < R; ++r) {
-                        const uint32_t bits = L.codes[c][r][lane >> 1] >> (16 * (lane & 1));
+                        const uint32_t bits = L.codes[k][r][lane >> 1] >> (16 * (lane & 1));
 #pragma unroll
                         for (int e = 0; e < 8; ++e) {
                             const uint32_t idx2 = lo + (uint32_t)r * kRun + 8u * (uint32_t)lane + (uint32_t)e;
                             if (idx2 < hi) eps[r][e] = eps[r][e] + wt_of((bits >> (2 * e)) & 3u, d0, d1, d2, d3);
                         }
                     }
-                }
-                if (p.async) {
+                };
+                if constexpr (kAsync) {
                     unsigned long long mset = L.cmask;
                     while (mset) {
                         const int k = (int)__ffsll((long long)mset) - 1;
                         mset &= mset - 1;
-                        ++n_changes;
-                        const double dBeta = L.dbeta[k];
-                        double d0, d1, d2, d3;
-                        if (dense) {
-                            d0 = dBeta * L.wt[k][0];
-                            d1 = dBeta * L.wt[k][1];
-                            d2 = dBeta * L.wt[k][2];
-                            d3 = dBeta * L.wt[k][3];
-                        } else {
-                            const double mean = L.mstat[k][0], sd = L.mstat[k][1];
-                            const double meanAdjustment = dBeta * mean / sd;
-                            const double oneAdjustment = dBeta / sd;
-                            d3 = -meanAdjustment;
-                            d2 = -meanAdjustment + oneAdjustment;
-                            d0 = -meanAdjustment + 2 * oneAdjustment;
-                            d1 = -meanAdjustment + meanAdjustment;
-                        }
-#pragma unroll
-                        for (int r = 0; r < R; ++r) {
-                            const uint32_t bits = L.codes[k][r][lane >> 1] >> (16 * (lane & 1));
-#pragma unroll
-                            for (int e = 0; e < 8; ++e) {
-                                const uint32_t idx2 = lo + (uint32_t)r * kRun + 8u * (uint32_t)lane + (uint32_t)e;
-                                if (idx2 < hi) eps[r][e] = eps[r][e] + wt_of((bits >> (2 * e)) & 3u, d0, d1, d2, d3);
-                            }
-                        }
+                        apply_slot(k);
                     }
+                } else if (c < nb) {
+                    apply_slot(c);
                 }
                 pos = last + 1;
                 BRR_TICK(4);
@@ -1040,7 +1022,7 @@ bayesr_sweep_kernel(SweepParams p)
             if (has1) {
                 if constexpr (kCodes) dense_store<R>(L, nb1, tid, words);
                 else ragged_store<R>(p, L, nb1, lo, tid, idx, itm);
-                if (tid < nb1) meta_store<R>(p, L, tid, mnext);
+                if (tid < nb1) meta_store<R, kLay>(p, L, tid, mnext);
                 __syncthreads();
                 if (has2) {
                     if (tid < nb2) L.mk_next[tid] = jnext2;

@@ -135,7 +135,7 @@ def algorithmic_bytes(cfg, data, storage, markers_per_step, changes, eps_elem=8)
     return g_total * (markers_per_step / m) + eps_elem * n * (markers_per_step + changes)
 
 
-def cpu_baseline(cfg, data, seed, budget_s):
+def cpu_baseline(cfg, data, seed, budget_s, window=0):
     """Reference-style sweep on the host: the FP64 oracle built with -O3
     -ffast-math -funroll-loops, one thread, over the first markers of the same
     workload for a chain of a few iterations."""
@@ -175,14 +175,19 @@ def cpu_baseline(cfg, data, seed, budget_s):
         L.orc_update_mu(C.byref(prob.data), C.byref(prob.state), seed, iters, 0)
         L.orc_fill_marker_uz(seed, iters, 0, m_s, orc.p(u, C.c_double), orc.p(z, C.c_double))
         L.orc_shuffle_philox(seed, iters, 0, m_s, orc.p(order, C.c_uint32))
-        L.orc_sweep(C.byref(prob.data), C.byref(prob.model), C.byref(prob.state), orc.p(order, C.c_uint32),
-                    orc.p(u, C.c_double), orc.p(z, C.c_double), None)
+        if window:
+            L.orc_sweep_async(C.byref(prob.data), C.byref(prob.model), C.byref(prob.state),
+                              orc.p(order, C.c_uint32), orc.p(u, C.c_double), orc.p(z, C.c_double), window)
+        else:
+            L.orc_sweep(C.byref(prob.data), C.byref(prob.model), C.byref(prob.state), orc.p(order, C.c_uint32),
+                        orc.p(u, C.c_double), orc.p(z, C.c_double), None)
         L.orc_post_sweep(C.byref(prob.data), C.byref(prob.model), C.byref(prob.state), seed, iters, 0)
         t_total += time.perf_counter() - t0
         iters += 1
         mk += m_s
     return dict(value=mk / t_total, unit="marker-updates/s", cores=1, kind="port",
-                sample=(f"oracle (FP64 restatement of the reference sweep, -O3 -ffast-math "
+                sample=(f"oracle (FP64 restatement of the reference sweep"
+                        f"{f', asynchronous windows of {window}' if window else ''}, -O3 -ffast-math "
                         f"-funroll-loops, 1 thread): first {m_s} markers of the same {cfg['layout']} "
                         f"workload at N={n}, {iters} chain iterations from init, {t_total:.1f}s"))
 
@@ -577,6 +582,9 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched (c3) leg of the default line")
     ap.add_argument("--no-bayesw", action="store_true", help="skip the BayesW (c5) leg of the default line")
     ap.add_argument("--no-decode", action="store_true", help="skip the device .bed decode leg of the default line")
+    ap.add_argument("--async", dest="async_window", type=int, default=0,
+                    help="ParallelGraph's asynchronous sweep in windows of this many markers (1..64; "
+                         "0 = the exact sequential sweep)")
     ap.add_argument("--decode-only", action="store_true",
                     help="only the decode leg: the c2 genotypes decoded on the device, its JSON object printed")
     args = ap.parse_args()
@@ -635,6 +643,8 @@ def main():
     else:
         h.upload_dense(data["bed"], data["lut"])
     h.set_model(S_DEFAULT)
+    if args.async_window:
+        h.set_async(args.async_window)
     log(f"uploaded in {time.time() - t0:.1f}s")
 
     def fresh_chain():
@@ -703,6 +713,8 @@ def main():
             "S": S_DEFAULT.tolist(), "timed_iterations": steps, "precision": "fp64",
             "workgroups": stats["workgroups"], "block_markers": stats["block_markers"],
             "slice": stats["slice"], "parallelism": "single GPU, persistent cooperative kernel",
+            "schedule": (f"asynchronous windows of {args.async_window} markers (ParallelGraph)"
+                         if args.async_window else "sequential (exact)"),
             "h2_posterior_mean": h2,
         },
         "roofline": {
@@ -717,7 +729,7 @@ def main():
     h.close()
     if not args.no_cpu:
         log("timing the CPU reference sweep ...")
-        line["cpu_baseline"] = cpu_baseline(cfg, data, args.seed, args.cpu_budget)
+        line["cpu_baseline"] = cpu_baseline(cfg, data, args.seed, args.cpu_budget, args.async_window)
     if not args.no_decode:
         line["decode"] = run_decode(cfg, data, args)
         line["decode"]["host_builder_s"] = data.get("build_s")
