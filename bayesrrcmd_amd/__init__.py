@@ -375,6 +375,31 @@ class HipSweep(_DeviceMarkers):
             _p(a.twos_off, C.c_uint64), _p(a.miss, C.c_int32), _p(a.miss_off, C.c_uint64),
             _p(np.ascontiguousarray(a.stats), C.c_double)), "brr_hip_upload_ragged")
 
+    def set_fixed(self, X: np.ndarray):
+        """Fixed-effect covariates, (N, F): stored as the reference's float matrix (column-major)."""
+        x = np.asarray(X, dtype=np.float32).reshape(self.n, -1)
+        self.F = x.shape[1]
+        self._fixedX = np.ascontiguousarray(x.T)          # column f contiguous
+        self._ck(lib().brr_hip_set_fixed(self.ctx, _p(self._fixedX, C.c_float), self.F), "brr_hip_set_fixed")
+
+    def set_fixed_state(self, gamma=None, order=None, sigma_f: float = 1.0):
+        g = None if gamma is None else np.ascontiguousarray(gamma, dtype=np.float64).reshape(self.F)
+        o = None if order is None else np.ascontiguousarray(order, dtype=np.uint32).reshape(self.F)
+        self._ck(lib().brr_hip_set_fixed_state(self.ctx, _p(g, C.c_double), _p(o, C.c_uint32), sigma_f),
+                 "brr_hip_set_fixed_state")
+
+    def get_fixed_state(self):
+        g = np.empty(self.F)
+        o = np.empty(self.F, dtype=np.uint32)
+        sf = C.c_double()
+        self._ck(lib().brr_hip_get_fixed_state(self.ctx, _p(g, C.c_double), _p(o, C.c_uint32), C.byref(sf)),
+                 "brr_hip_get_fixed_state")
+        return dict(gamma=g, order=o, sigma_f=sf.value)
+
+    def fixed_effects(self, seed: int, iteration: int, chain: int = 0):
+        """The fixed-effects block alone on the device state (BayesRBase.cpp:212-235)."""
+        self._ck(lib().brr_hip_fixed_effects(self.ctx, seed, iteration, chain), "brr_hip_fixed_effects")
+
     def set_async(self, window: int):
         """ParallelGraph's asynchronous sweep in windows of `window` markers (0: the exact sequential sweep)."""
         self._ck(lib().brr_hip_set_async(self.ctx, int(window)), "brr_hip_set_async")

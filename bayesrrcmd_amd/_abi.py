@@ -73,6 +73,10 @@ SIGNATURES = {
     "brr_hip_load_bed": (C.c_int, [CTX, P_u8, C.c_uint64, C.c_int]),
     "brr_hip_get_marker_tables": (C.c_int, [CTX, P_f64]),
     "brr_hip_set_async": (C.c_int, [CTX, C.c_uint32]),
+    "brr_hip_set_fixed": (C.c_int, [CTX, C.POINTER(C.c_float), C.c_uint32]),
+    "brr_hip_set_fixed_state": (C.c_int, [CTX, P_f64, P_u32, C.c_double]),
+    "brr_hip_get_fixed_state": (C.c_int, [CTX, P_f64, P_u32, C.POINTER(C.c_double)]),
+    "brr_hip_fixed_effects": (C.c_int, [CTX, C.c_uint64, C.c_uint32, C.c_uint32]),
     "brr_hip_set_groups": (C.c_int, [CTX, P_i32]),
     "brr_hip_set_model": (C.c_int, [CTX, C.POINTER(Model)]),
     "brr_hip_set_state": (C.c_int, [CTX, P_f64, P_f64, P_i32]),

@@ -87,6 +87,11 @@ struct brr_hip_ctx {
     uint32_t* d_ctl = nullptr;
     unsigned long long* d_counters = nullptr;
     double* d_tmp = nullptr;
+    // fixed effects: covariates (column-major N x F floats), {sigmaF, gamma[F]}, order
+    uint32_t F = 0;
+    float* d_fixedX = nullptr;
+    double* d_fixed = nullptr;
+    uint32_t* d_forder = nullptr;
     // BayesW
     double* d_fail = nullptr;
     double* d_sumf = nullptr;
@@ -286,6 +291,10 @@ SweepParams base_params(brr_hip_ctx* c)
     p.xchg2 = c->d_xchg2;
     p.ctl = c->d_ctl;
     p.counters = c->d_counters;
+    p.F = c->F;
+    p.fixedX = c->d_fixedX;
+    p.fixed = c->d_fixed;
+    p.forder = c->d_forder;
     return p;
 }
 
@@ -659,6 +668,7 @@ void brr_hip_destroy(brr_hip_ctx* c)
     dfree(c->d_ones_off); dfree(c->d_twos_off); dfree(c->d_miss_off);
     dfree(c->d_seg); dfree(c->d_stats);
     dfree(c->d_group); dfree(c->d_cva);
+    dfree(c->d_fixedX); dfree(c->d_fixed); dfree(c->d_forder);
     dfree(c->d_eps); dfree(c->d_beta); dfree(c->d_comp); dfree(c->d_acum);
     dfree(c->d_scal); dfree(c->d_hist); dfree(c->d_order); dfree(c->d_u); dfree(c->d_z);
     dfree(c->d_xchg1); dfree(c->d_xchg2); dfree(c->d_ctl); dfree(c->d_counters); dfree(c->d_tmp);
@@ -896,6 +906,63 @@ int brr_hip_set_async(brr_hip_ctx* c, uint32_t window)
     return BRR_HIP_OK;
 }
 
+int brr_hip_set_fixed(brr_hip_ctx* c, const float* X, uint32_t num_fixed)
+{
+    if (int k = need_kind(c, false)) return k;
+    if (c->T) return fail(c, BRR_HIP_EINVAL, "fixed effects need a single-chain context");
+    if (num_fixed > (uint32_t)kMaxF) return fail(c, BRR_HIP_EINVAL, "at most 128 fixed-effect covariates");
+    if (num_fixed && !X) return fail(c, BRR_HIP_EINVAL, "null covariates");
+    HIP_TRY(c, hipSetDevice(c->device));
+    const uint32_t N = c->cfg.num_inds;
+    c->F = 0;
+    if (num_fixed == 0) return BRR_HIP_OK;
+    int rc;
+    if ((rc = dalloc(c, &c->d_fixedX, (size_t)N * num_fixed)) || (rc = dalloc(c, &c->d_fixed, 1 + num_fixed)) ||
+        (rc = dalloc(c, &c->d_forder, num_fixed)))
+        return rc;
+    HIP_TRY(c, hipMemcpy(c->d_fixedX, X, sizeof(float) * N * num_fixed, hipMemcpyHostToDevice));
+    // BayesRBase::runGibbs: gamma = 0 and xI = 0..F-1 before the first iteration
+    // (BayesRBase.cpp:170-173); sigmaF = s02F (its value after every iteration)
+    std::vector<double> st(1 + num_fixed, 0.0);
+    st[0] = 1.0;
+    std::vector<uint32_t> ord(num_fixed);
+    for (uint32_t f = 0; f < num_fixed; ++f) ord[f] = f;
+    HIP_TRY(c, hipMemcpy(c->d_fixed, st.data(), sizeof(double) * st.size(), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->d_forder, ord.data(), sizeof(uint32_t) * num_fixed, hipMemcpyHostToDevice));
+    c->F = num_fixed;
+    return BRR_HIP_OK;
+}
+
+int brr_hip_set_fixed_state(brr_hip_ctx* c, const double* gamma, const uint32_t* order, double sigma_f)
+{
+    if (int k = need_kind(c, false)) return k;
+    if (c->F == 0) return fail(c, BRR_HIP_ESTATE, "no fixed effects (brr_hip_set_fixed)");
+    if (!(sigma_f > 0.0)) return fail(c, BRR_HIP_EINVAL, "sigmaF must be > 0");
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (order) {
+        std::vector<uint8_t> seen(c->F, 0);
+        for (uint32_t f = 0; f < c->F; ++f) {
+            if (order[f] >= c->F || seen[order[f]]) return fail(c, BRR_HIP_EINVAL, "order is not a permutation");
+            seen[order[f]] = 1;
+        }
+        HIP_TRY(c, hipMemcpy(c->d_forder, order, sizeof(uint32_t) * c->F, hipMemcpyHostToDevice));
+    }
+    HIP_TRY(c, hipMemcpy(c->d_fixed, &sigma_f, sizeof(double), hipMemcpyHostToDevice));
+    if (gamma) HIP_TRY(c, hipMemcpy(c->d_fixed + 1, gamma, sizeof(double) * c->F, hipMemcpyHostToDevice));
+    return BRR_HIP_OK;
+}
+
+int brr_hip_get_fixed_state(brr_hip_ctx* c, double* gamma, uint32_t* order, double* sigma_f)
+{
+    if (int k = need_kind(c, false)) return k;
+    if (c->F == 0) return fail(c, BRR_HIP_ESTATE, "no fixed effects (brr_hip_set_fixed)");
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (sigma_f) HIP_TRY(c, hipMemcpy(sigma_f, c->d_fixed, sizeof(double), hipMemcpyDeviceToHost));
+    if (gamma) HIP_TRY(c, hipMemcpy(gamma, c->d_fixed + 1, sizeof(double) * c->F, hipMemcpyDeviceToHost));
+    if (order) HIP_TRY(c, hipMemcpy(order, c->d_forder, sizeof(uint32_t) * c->F, hipMemcpyDeviceToHost));
+    return BRR_HIP_OK;
+}
+
 int brr_hip_set_groups(brr_hip_ctx* c, const int32_t* group)
 {
     if (int k = need_kind(c, false)) return k;
@@ -1029,6 +1096,24 @@ int brr_hip_sweep(brr_hip_ctx* c, const uint32_t* order, const double* u, const 
     p.z = c->d_z;
     rc = launch_any(c, p);
     c->stats.markers = (uint64_t)M * c->Tn;
+    return rc;
+}
+
+int brr_hip_fixed_effects(brr_hip_ctx* c, uint64_t seed, uint32_t iteration, uint32_t chain)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if ((rc = need_kind(c, false))) return rc;
+    if (c->F == 0) return fail(c, BRR_HIP_ESTATE, "no fixed effects (brr_hip_set_fixed)");
+    HIP_TRY(c, hipSetDevice(c->device));
+    SweepParams p = base_params(c);
+    p.mode = kModeFixed;
+    p.iterations = 1;
+    p.iter_base = iteration;
+    p.chain = chain;
+    p.seed = seed;
+    rc = launch_any(c, p);
+    c->stats.markers = 0;
     return rc;
 }
 

@@ -36,7 +36,8 @@
 
 namespace brr {
 
-enum : uint32_t { kStreamMarker = 1, kStreamShuffle = 2, kStreamScalar = 3 };
+// kStreamFixed (5): the covariate order of the fixed effects, shuffle rule as above.
+enum : uint32_t { kStreamMarker = 1, kStreamShuffle = 2, kStreamScalar = 3, kStreamFixed = 5 };
 enum : uint32_t { kDrawMu = 0, kDrawSigmaE = 1, kDrawSigmaG0 = 2 };
 constexpr uint32_t kInitIteration = 0xFFFFFFFFu;
 constexpr uint32_t kBoostAttempt = 0xFFFFFFu;

@@ -39,6 +39,7 @@ namespace brr {
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
 constexpr int kMaxB = 64;      // markers per speculation block (one wave of decisions)
+constexpr int kMaxF = 128;   // fixed-effect covariates per context
 constexpr int kMaxK = 8;       // mixture components incl. zero
 constexpr int kMaxG = 32;      // marker groups
 constexpr int kMaxRuns = 4;    // slice <= 512 * kMaxRuns individuals
@@ -48,7 +49,8 @@ constexpr uint64_t kSpinLimit = 1ull << 22;
 constexpr uint32_t kMaxP = 256;   // workgroups of a launch (the exchange reducers read <= 4 per lane)
 constexpr int kIdxCap = 32;    // prefetched ragged indices per thread per block
 
-enum : int { kModeSweep = 0, kModeChain = 1 };
+// kModeFixed: only the fixed-effects block of one iteration (brr_hip_fixed_effects).
+enum : int { kModeSweep = 0, kModeChain = 1, kModeFixed = 2 };
 // kLayoutEigen: the ragged formulas with EigenBayesRKernel's dot, whose
 // missing calls count at the marker mean (eigenbayesrkernel.cpp:17-20).
 enum : int { kLayoutDense = 1, kLayoutRagged = 2, kLayoutEigen = 3 };
@@ -73,6 +75,12 @@ struct SweepParams {
     int layout, mode;
     uint32_t iterations, iter_base, chain;
     uint32_t async;               // 1: each block is one window of the asynchronous sweep
+    // fixed effects (BayesRBase.cpp:212-235): F covariates, column-major N x F
+    // floats (Data::X); state {sigmaF, gamma[F]} and the carried covariate order
+    const float* fixedX;
+    double* fixed;
+    uint32_t* forder;
+    uint32_t F, pad1;
     uint64_t seed;
     // dense arena: column j at dense + j * col_words (32-bit words)
     const uint32_t* dense;
@@ -148,6 +156,10 @@ struct Shared {
     double gc[kMaxG * kMaxK];           // log pi_k - 0.5 log(logLScale S_k + 1)
     double gden[kMaxG * kMaxK];         // denom_k (index k >= 1)
     double gbsd[kMaxG * kMaxK];         // sqrt(sigmaE / denom_k)
+    // fixed effects
+    double gamma[kMaxF];
+    uint32_t forder[kMaxF];
+    double sigmaF;
     int change;
     int abort;
 };
@@ -376,6 +388,69 @@ __device__ __forceinline__ bool allreduce_scalar(const SweepParams& p, Shared<R>
     if (threadIdx.x == 0) publish_pair(slot1(p, rnd, 0, w), rnd, part);
     if (!exchange<R>(p, L, rnd, 0, 1, w, wave, lane)) return false;
     *total = L.dot[0];
+    return true;
+}
+
+// The fixed-effects block of BayesRBase::runGibbs (BayesRBase.cpp:212-235) on
+// this workgroup's slice; every workgroup draws the same gammas. The covariate
+// order (carried across iterations like the reference's xI) takes the
+// libstdc++ random_shuffle rule on its own stream; gamma_f is scalar draw
+// 2 + G + G K + f. epsilonSum is left as is, as in the reference.
+template <int R>
+__device__ __forceinline__ bool fixed_effects(const SweepParams& p, Shared<R>& L, uint32_t& rnd,
+                                              unsigned long long& n_rounds, double (&eps)[R][8],
+                                              uint32_t iteration, uint32_t lo, uint32_t hi, uint32_t w,
+                                              int wave, int lane)
+{
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        for (uint32_t i = 1; i < p.F; ++i) {
+            const U4 x = draw_block(p.seed, i, iteration, kStreamFixed, p.chain);
+            const uint64_t r = ((uint64_t)x.x[0] << 32) | x.x[1];
+            const uint32_t j = (uint32_t)(r % ((uint64_t)i + 1));
+            if (j != i) {
+                const uint32_t t = L.forder[i];
+                L.forder[i] = L.forder[j];
+                L.forder[j] = t;
+            }
+        }
+    }
+    __syncthreads();
+    const double dNm1 = (double)(p.N - 1u);
+    const double sigE_sigF = L.sigmaE / L.sigmaF;
+    const uint32_t draw0 = 2u + p.G + p.G * p.K;
+    for (uint32_t i = 0; i < p.F; ++i) {
+        const uint32_t f = L.forder[i];
+        const float* x = p.fixedX + (size_t)f * p.N;
+        const double gamma_old = L.gamma[f];
+        double xv[R][8];
+        double part = 0.0;
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const uint32_t idx = lo + (uint32_t)r * kRun + 8u * (uint32_t)lane + (uint32_t)e;
+                xv[r][e] = (idx < hi) ? (double)x[idx] : 0.0;
+                part += xv[r][e] * (eps[r][e] + gamma_old * xv[r][e]);
+            }
+        double num;
+        if (!allreduce_scalar<R>(p, L, rnd, part, w, wave, lane, &num)) return false;
+        ++n_rounds;
+        const double denom = dNm1 + sigE_sigF;
+        const double g = norm_rng(p.seed, iteration, draw0 + f, p.chain, num / denom, L.sigmaE / denom);
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const uint32_t idx = lo + (uint32_t)r * kRun + 8u * (uint32_t)lane + (uint32_t)e;
+                if (idx < hi) eps[r][e] = eps[r][e] + (gamma_old - g) * xv[r][e];
+            }
+        __syncthreads();
+        if (tid == 0) L.gamma[f] = g;
+        __syncthreads();
+    }
+    if (tid == 0) L.sigmaF = 1.0;   // s02F (BayesRBase.hpp:90)
+    __syncthreads();
     return true;
 }
 
@@ -694,6 +769,11 @@ bayesr_sweep_kernel(SweepParams p)
         L.v[t] = 0.0;
     }
     for (uint32_t t = tid; t < p.G * (p.K - 1); t += kThreads) L.cva[t] = p.cva[t];
+    if (tid == 0) L.sigmaF = p.F ? p.fixed[0] : 1.0;
+    for (uint32_t t = tid; t < p.F; t += kThreads) {
+        L.gamma[t] = p.fixed[1 + t];
+        L.forder[t] = p.forder[t];
+    }
     uint32_t rnd = __hip_atomic_load(p.ctl + kCtlRound, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     unsigned long long n_rounds = 0, n_changes = 0;
     uint64_t tk[kPhases] = {0, 0, 0, 0, 0, 0};
@@ -705,6 +785,10 @@ bayesr_sweep_kernel(SweepParams p)
     for (uint32_t it = 0; it < iters; ++it) {
         const uint32_t iteration = p.iter_base + it;
         const uint32_t* order = p.order + (size_t)((p.mode == kModeChain) ? it : 0u) * p.M;
+        if (p.mode == kModeFixed) {
+            if (!fixed_effects<R>(p, L, rnd, n_rounds, eps, iteration, lo, hi, w, wave, lane)) goto done;
+            break;
+        }
 
         if (p.mode == kModeChain) {
             // ---- updateMu (DenseBayesRRmz.cpp:36-42 / SparseBayesRRG.cpp:106-113)
@@ -1033,6 +1117,10 @@ bayesr_sweep_kernel(SweepParams p)
             BRR_TICK(0);
         }
 
+        if (p.mode == kModeChain && p.F) {
+            // ---- fixed effects (BayesRBase.cpp:212-235)
+            if (!fixed_effects<R>(p, L, rnd, n_rounds, eps, iteration, lo, hi, w, wave, lane)) goto done;
+        }
         if (p.mode == kModeChain) {
             // ---- sigmaE, sigmaG, pi draws (BayesRBase.cpp:237-247)
             double part = 0.0;
@@ -1106,6 +1194,11 @@ done:
             for (uint32_t t = tid; t < p.G * p.K; t += kThreads) {
                 p.scal[SL.pi() + t] = L.pi[t];
                 p.scal[SL.v() + t] = L.v[t];
+            }
+            if (p.F && tid == 0) p.fixed[0] = L.sigmaF;
+            for (uint32_t t = tid; t < p.F; t += kThreads) {
+                p.fixed[1 + t] = L.gamma[t];
+                p.forder[t] = L.forder[t];
             }
         }
     }

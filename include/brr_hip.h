@@ -43,7 +43,7 @@
 extern "C" {
 #endif
 
-#define BRR_HIP_ABI_VERSION 5
+#define BRR_HIP_ABI_VERSION 6
 
 enum brr_hip_status {
     BRR_HIP_OK = 0,
@@ -165,6 +165,26 @@ int brr_hip_load_bed(brr_hip_ctx* ctx, const uint8_t* bed_file, uint64_t file_by
 /* The M x 4 per-marker values held on the device: dense tables {dosage 0, 1, 2,
  * missing} or ragged statistics {mean, sd, sqrdZ, Zsum}. */
 int brr_hip_get_marker_tables(brr_hip_ctx* ctx, double* out);
+
+/* Fixed effects (--fixed / Options::fixedEffectNumber, BayesRBase.cpp:212-235):
+ * X is Data::X, the N x F float matrix in Eigen's column-major order (column f
+ * = covariate f over the individuals), F <= 128. Resets gamma to 0, the
+ * covariate order to 0..F-1 and sigmaF to s02F = 1 (the reference reads an
+ * uninitialised m_sigmaF in iteration 0 and s02F after it). brr_hip_run_chain
+ * then runs the fixed-effects block after every sweep; the reference's order
+ * is kept: the covariate order (carried across iterations like xI) shuffled by
+ * the random_shuffle rule on its own stream (tag 5), then per covariate f num =
+ * sum_k X(k,f) (eps_k + gamma_old X(k,f)), denom = (N - 1) + sigmaE / sigmaF,
+ * gamma_f ~ N(num / denom, sigmaE / denom) (scalar draw 2 + G + G K + f),
+ * eps += (gamma_old - gamma_f) X(., f); epsilonSum is not updated, as in the
+ * reference. F = 0 removes them. Single-chain BayesR contexts. */
+int brr_hip_set_fixed(brr_hip_ctx* ctx, const float* X, uint32_t num_fixed);
+/* gamma (F), the covariate order (F) and sigmaF; NULL arrays are skipped. */
+int brr_hip_set_fixed_state(brr_hip_ctx* ctx, const double* gamma, const uint32_t* order, double sigma_f);
+int brr_hip_get_fixed_state(brr_hip_ctx* ctx, double* gamma, uint32_t* order, double* sigma_f);
+/* The fixed-effects block alone on the device state, for a host that runs
+ * runGibbs itself and calls brr_hip_sweep for exec (BayesRBase.cpp:212-235). */
+int brr_hip_fixed_effects(brr_hip_ctx* ctx, uint64_t seed, uint32_t iteration, uint32_t chain);
 
 /* The asynchronous sweep of ParallelGraph (--analysis-type asyncppbayes,
  * src/parallelgraph.cpp; BayesRBase::processColumnAsync / updateGlobal,

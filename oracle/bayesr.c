@@ -331,13 +331,47 @@ void orc_run_gibbs(const orc_data* d, const orc_model* m, orc_state* s,
     orc_run_gibbs_window(d, m, s, y, seed, chain, iterations, order, shuffle_mode, hist, 0);
 }
 
+void orc_fixed_effects(const orc_data* d, const orc_model* m, orc_state* s, const float* X, uint32_t F,
+                       double* gamma, uint32_t* forder, double* sigmaF, uint64_t seed, uint32_t iteration,
+                       uint32_t chain)
+{
+    /* BayesRBase.cpp:212-235 */
+    const uint32_t N = d->N;
+    const double dNm1 = (double)(N - 1u);
+    if (F == 0) return;
+    orc_shuffle_philox_stream(seed, iteration, chain, ORC_STREAM_FIXED, F, forder);
+    const double sigE_sigF = s->sigmaE / *sigmaF;
+    for (uint32_t i = 0; i < F; ++i) {
+        const uint32_t f = forder[i];
+        const float* x = X + (size_t)f * N;
+        const double gamma_old = gamma[f];
+        double num_f = 0.0;
+        for (uint32_t k = 0; k < N; ++k) num_f += (double)x[k] * (s->eps[k] + gamma_old * (double)x[k]);
+        const double denom_f = dNm1 + sigE_sigF;
+        gamma[f] = orc_norm_rng(seed, iteration, ORC_DRAW_FIXED0(m->G, m->K) + f, chain, num_f / denom_f,
+                                s->sigmaE / denom_f);
+        for (uint32_t k = 0; k < N; ++k) s->eps[k] = s->eps[k] + (gamma_old - gamma[f]) * (double)x[k];
+    }
+    *sigmaF = 1.0;   /* s02F (BayesRBase.hpp:90) */
+}
+
 void orc_run_gibbs_window(const orc_data* d, const orc_model* m, orc_state* s,
                           const double* y, uint64_t seed, uint32_t chain,
                           uint32_t iterations, uint32_t* order, int shuffle_mode,
                           double* hist, uint32_t window)
 {
-    /* BayesRBase::runGibbs, BayesRBase.cpp:130-285 (no fixed effects); window > 0:
-     * the sweep in asynchronous windows (orc_sweep_async) */
+    orc_run_gibbs_fixed(d, m, s, y, seed, chain, iterations, order, shuffle_mode, hist, window, NULL, 0, NULL,
+                        NULL, NULL);
+}
+
+void orc_run_gibbs_fixed(const orc_data* d, const orc_model* m, orc_state* s,
+                         const double* y, uint64_t seed, uint32_t chain,
+                         uint32_t iterations, uint32_t* order, int shuffle_mode,
+                         double* hist, uint32_t window, const float* X, uint32_t F,
+                         double* gamma, uint32_t* forder, double* sigmaF)
+{
+    /* BayesRBase::runGibbs, BayesRBase.cpp:130-285; window > 0: the sweep in
+     * asynchronous windows (orc_sweep_async); F > 0: the fixed effects after it */
     const uint32_t M = d->M;
     const int G = m->G, K = m->K;
     orc_init(d, m, s, y, seed, chain);
@@ -350,6 +384,7 @@ void orc_run_gibbs_window(const orc_data* d, const orc_model* m, orc_state* s,
         else orc_shuffle_philox(seed, it, chain, M, order);
         if (window) orc_sweep_async(d, m, s, order, u, z, window);
         else orc_sweep(d, m, s, order, u, z, NULL);
+        if (F) orc_fixed_effects(d, m, s, X, F, gamma, forder, sigmaF, seed, it, chain);
         double e2 = 0.0;
         for (uint32_t i = 0; i < d->N; ++i) e2 += s->eps[i] * s->eps[i];
         double m0_total = 0.0;

@@ -29,7 +29,7 @@ extern "C" {
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 
 /* Stream tags (ctr word 2, low 8 bits). */
-enum { ORC_STREAM_MARKER = 1, ORC_STREAM_SHUFFLE = 2, ORC_STREAM_SCALAR = 3 };
+enum { ORC_STREAM_MARKER = 1, ORC_STREAM_SHUFFLE = 2, ORC_STREAM_SCALAR = 3, ORC_STREAM_FIXED = 5 };
 
 /* Deterministic elementary functions built from + - * / only (bit-identical on
  * any IEEE-754 binary64 platform without contraction). */
@@ -49,6 +49,10 @@ void orc_fill_marker_uz(uint64_t seed, uint32_t iteration, uint32_t chain, uint3
  * Philox shuffle stream. */
 void orc_shuffle_philox(uint64_t seed, uint32_t iteration, uint32_t chain, uint32_t M,
                         uint32_t* order);
+/* The same rule with r from the given stream tag (ORC_STREAM_FIXED: the
+ * covariate order of the fixed effects). */
+void orc_shuffle_philox_stream(uint64_t seed, uint32_t iteration, uint32_t chain, uint32_t stream,
+                               uint32_t M, uint32_t* order);
 /* Same rule with r = rand() from the C library (the reference's stock source;
  * the caller seeds it with srand). */
 void orc_shuffle_libc(uint32_t M, uint32_t* order);
@@ -218,6 +222,26 @@ void orc_run_gibbs(const orc_data* d, const orc_model* m, orc_state* s,
                    uint32_t iterations, uint32_t* order, int shuffle_mode,
                    double* hist);
 
+/* Fixed effects of BayesRBase::runGibbs (BayesRBase.cpp:212-235), after the
+ * sweep: the covariate order forder (F, carried across iterations like the
+ * reference's xI) is shuffled on ORC_STREAM_FIXED, then for each covariate f in
+ * that order num = sum_k X(k,f) (eps_k + gamma_old X(k,f)), denom = (N - 1) +
+ * sigmaE / sigmaF, gamma_f ~ N(num / denom, sigmaE / denom) (scalar draw
+ * ORC_DRAW_FIXED0(G, K) + f), eps += (gamma_old - gamma_f) X(., f). X is the
+ * reference's float matrix (Data::X, column-major N x F). epsSum is left as is
+ * (the reference does not update it here). sigmaF becomes s02F = 1. */
+void orc_fixed_effects(const orc_data* d, const orc_model* m, orc_state* s, const float* X, uint32_t F,
+                       double* gamma, uint32_t* forder, double* sigmaF, uint64_t seed, uint32_t iteration,
+                       uint32_t chain);
+
+/* orc_run_gibbs with fixed effects after every sweep (F = 0: none) and the sweep in
+ * asynchronous windows (window 0: sequential). */
+void orc_run_gibbs_fixed(const orc_data* d, const orc_model* m, orc_state* s,
+                         const double* y, uint64_t seed, uint32_t chain,
+                         uint32_t iterations, uint32_t* order, int shuffle_mode,
+                         double* hist, uint32_t window, const float* X, uint32_t F,
+                         double* gamma, uint32_t* forder, double* sigmaF);
+
 /* orc_run_gibbs with the sweep in asynchronous windows (window 0: sequential). */
 void orc_run_gibbs_window(const orc_data* d, const orc_model* m, orc_state* s,
                           const double* y, uint64_t seed, uint32_t chain,
@@ -227,6 +251,7 @@ void orc_run_gibbs_window(const orc_data* d, const orc_model* m, orc_state* s,
 /* Scalar-draw ids of the Philox scalar stream. */
 enum { ORC_DRAW_MU = 0, ORC_DRAW_SIGMAE = 1, ORC_DRAW_SIGMAG0 = 2 };
 #define ORC_DRAW_PI0(G) (2u + (uint32_t)(G))
+#define ORC_DRAW_FIXED0(G, K) (2u + (uint32_t)(G) + (uint32_t)(G) * (uint32_t)(K))
 #define ORC_INIT_ITERATION 0xFFFFFFFFu
 
 /* ------------------------------------------------------ BayesW (a11-a13) -- */

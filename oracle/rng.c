@@ -213,10 +213,16 @@ void orc_fill_marker_uz(uint64_t seed, uint32_t iteration, uint32_t chain, uint3
 void orc_shuffle_philox(uint64_t seed, uint32_t iteration, uint32_t chain, uint32_t M,
                         uint32_t* order)
 {
+    orc_shuffle_philox_stream(seed, iteration, chain, ORC_STREAM_SHUFFLE, M, order);
+}
+
+void orc_shuffle_philox_stream(uint64_t seed, uint32_t iteration, uint32_t chain, uint32_t stream,
+                               uint32_t M, uint32_t* order)
+{
     /* libstdc++ random_shuffle: for i in [1, M): swap(order[i], order[r % (i + 1)]) */
     for (uint32_t i = 1; i < M; ++i) {
         uint32_t x[4];
-        block(seed, i, iteration, ORC_STREAM_SHUFFLE, chain, x);
+        block(seed, i, iteration, stream, chain, x);
         const uint64_t r = ((uint64_t)x[0] << 32) | x[1];
         const uint32_t j = (uint32_t)(r % ((uint64_t)i + 1));
         if (j != i) { const uint32_t t = order[i]; order[i] = order[j]; order[j] = t; }

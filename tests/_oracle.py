@@ -22,6 +22,7 @@ P_u64 = C.POINTER(C.c_uint64)
 P_f64 = C.POINTER(C.c_double)
 
 DENSE, RAGGED, EIGEN = 1, 2, 3
+ORC_STREAM_FIXED = 5
 
 
 class OrcData(C.Structure):
@@ -127,6 +128,12 @@ def lib(fast: bool = False):
         "orc_run_gibbs_window": (None, [C.POINTER(OrcData), C.POINTER(OrcModel), C.POINTER(OrcState), P_f64,
                                         C.c_uint64, C.c_uint32, C.c_uint32, P_u32, C.c_int, P_f64,
                                         C.c_uint32]),
+        "orc_fixed_effects": (None, [C.POINTER(OrcData), C.POINTER(OrcModel), C.POINTER(OrcState),
+                                     C.POINTER(C.c_float), C.c_uint32, P_f64, P_u32, P_f64, C.c_uint64,
+                                     C.c_uint32, C.c_uint32]),
+        "orc_run_gibbs_fixed": (None, [C.POINTER(OrcData), C.POINTER(OrcModel), C.POINTER(OrcState), P_f64,
+                                       C.c_uint64, C.c_uint32, C.c_uint32, P_u32, C.c_int, P_f64,
+                                       C.c_uint32, C.POINTER(C.c_float), C.c_uint32, P_f64, P_u32, P_f64]),
         "orc_sweep_async": (None, [C.POINTER(OrcData), C.POINTER(OrcModel), C.POINTER(OrcState), P_u32, P_f64,
                                    P_f64, C.c_uint32]),
         "orc_arms": (C.c_int, [P_f64, C.c_int, C.c_double, C.c_double, LOGDENS, C.c_void_p, C.c_int,
@@ -326,6 +333,32 @@ class OracleProblem:
         lib().orc_run_gibbs_window(C.byref(self.data), C.byref(self.model), C.byref(self.state), p(y, C.c_double),
                                    seed, chain, iterations, p(order, C.c_uint32), shuffle_mode,
                                    p(hist, C.c_double), window)
+        return hist, order
+
+    def set_fixed(self, X):
+        """Covariates (N, F) as the reference's float matrix, column-major; gamma 0, order 0..F-1, sigmaF 1."""
+        x = np.asarray(X, dtype=np.float32).reshape(self.n, -1)
+        self.F = x.shape[1]
+        self.fixedX = np.ascontiguousarray(x.T)
+        self.gamma = np.zeros(self.F)
+        self.forder = np.arange(self.F, dtype=np.uint32)
+        self.sigmaF = np.array([1.0])
+
+    def fixed_effects(self, seed, iteration, chain=0):
+        lib().orc_fixed_effects(C.byref(self.data), C.byref(self.model), C.byref(self.state),
+                                p(self.fixedX, C.c_float), self.F, p(self.gamma, C.c_double),
+                                p(self.forder, C.c_uint32), p(self.sigmaF, C.c_double), seed, iteration, chain)
+
+    def run_gibbs_fixed(self, y, seed, iterations, order=None, chain=0, window=0):
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        if order is None:
+            order = np.arange(self.m, dtype=np.uint32)
+        order = np.ascontiguousarray(order, dtype=np.uint32)
+        hist = np.zeros((iterations, 4 + self.G))
+        lib().orc_run_gibbs_fixed(C.byref(self.data), C.byref(self.model), C.byref(self.state), p(y, C.c_double),
+                                  seed, chain, iterations, p(order, C.c_uint32), 0, p(hist, C.c_double), window,
+                                  p(self.fixedX, C.c_float), self.F, p(self.gamma, C.c_double),
+                                  p(self.forder, C.c_uint32), p(self.sigmaF, C.c_double))
         return hist, order
 
     def snapshot(self):
