@@ -553,6 +553,26 @@ class HipBayesW(_DeviceMarkers):
         self._ck(lib().brr_hip_upload_dense(self.ctx, _p(bed, C.c_uint8), bed.shape[1], _p(lut, C.c_double)),
                  "brr_hip_upload_dense")
 
+    def set_fixed(self, X: np.ndarray):
+        """Fixed-effect covariates, (N, F): stored as the reference's float matrix (column-major)."""
+        x = np.asarray(X, dtype=np.float32).reshape(self.n, -1)
+        self.F = x.shape[1]
+        self._fixedX = np.ascontiguousarray(x.T)
+        self._ck(lib().brr_hip_bw_set_fixed(self.ctx, _p(self._fixedX, C.c_float), self.F), "brr_hip_bw_set_fixed")
+
+    def set_theta(self, theta):
+        t = np.ascontiguousarray(theta, dtype=np.float64).reshape(self.F)
+        self._ck(lib().brr_hip_bw_set_theta(self.ctx, _p(t, C.c_double)), "brr_hip_bw_set_theta")
+
+    def get_theta(self) -> np.ndarray:
+        t = np.empty(self.F)
+        self._ck(lib().brr_hip_bw_get_theta(self.ctx, _p(t, C.c_double)), "brr_hip_bw_get_theta")
+        return t
+
+    def fixed_effects(self, seed: int, iteration: int, chain: int = 0):
+        """Every sampleTheta of one iteration on the device state."""
+        self._ck(lib().brr_hip_bw_fixed_effects(self.ctx, seed, iteration, chain), "brr_hip_bw_fixed_effects")
+
     def set_model(self, mixture_C: np.ndarray, fail: np.ndarray):
         cc = np.ascontiguousarray(mixture_C, dtype=np.float64).reshape(self.K - 1)
         f = np.ascontiguousarray(fail, dtype=np.float64).reshape(self.n)

@@ -100,6 +100,10 @@ SIGNATURES = {
     "brr_hip_bw_set_vi": (C.c_int, [CTX, C.c_uint32, C.c_double, P_f64]),
     "brr_hip_bw_residual_update": (C.c_int, [CTX, C.c_uint32, C.c_double, C.c_int]),
     "brr_hip_bw_sum_failure": (C.c_int, [CTX, P_f64]),
+    "brr_hip_bw_set_fixed": (C.c_int, [CTX, C.POINTER(C.c_float), C.c_uint32]),
+    "brr_hip_bw_set_theta": (C.c_int, [CTX, P_f64]),
+    "brr_hip_bw_get_theta": (C.c_int, [CTX, P_f64]),
+    "brr_hip_bw_fixed_effects": (C.c_int, [CTX, C.c_uint64, C.c_uint32, C.c_uint32]),
     "brr_bayesw_init": (C.c_int, [P_f64, P_f64, C.c_uint32, C.c_uint32, C.c_uint32, P_f64,
                                   C.POINTER(BwScalars)]),
     "brr_eigen_fill": (C.c_int, [P_u8, C.c_uint64, C.c_uint32, C.c_uint32, P_u64, P_u64, P_u64,

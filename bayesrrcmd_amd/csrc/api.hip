@@ -90,8 +90,9 @@ struct brr_hip_ctx {
     // fixed effects: covariates (column-major N x F floats), {sigmaF, gamma[F]}, order
     uint32_t F = 0;
     float* d_fixedX = nullptr;
-    double* d_fixed = nullptr;
+    double* d_fixed = nullptr;             // BayesW: theta[F]
     uint32_t* d_forder = nullptr;
+    double* d_sumf_fix = nullptr;          // BayesW: sum_k X(k,f) failure_k
     // BayesW
     double* d_fail = nullptr;
     double* d_sumf = nullptr;
@@ -374,6 +375,10 @@ BwParams bw_params(brr_hip_ctx* c)
     p.xchg2 = c->d_xchg2;
     p.ctl = c->d_ctl;
     p.counters = c->d_counters;
+    p.F = c->F;
+    p.fixedX = c->d_fixedX;
+    p.theta = c->d_fixed;
+    p.sumf_fix = c->d_sumf_fix;
     return p;
 }
 
@@ -475,10 +480,11 @@ int launch_fn(brr_hip_ctx* c, const void* fn, void* param)
     c->stats.storage = c->codes ? 2u : 1u;
     if (ctl[kCtlAbort]) {
         reset_exchange(c);
-        if (c->bw && ctl[kCtlAbortWhere] >= 30 && ctl[kCtlAbortWhere] <= 32)
+        if (c->bw && ctl[kCtlAbortWhere] >= 30 && ctl[kCtlAbortWhere] <= 33)
             return fail(c, BRR_HIP_ESAMPLER, std::string("adaptive rejection sampler failed for ") +
                                                  (ctl[kCtlAbortWhere] == 30 ? "mu" :
-                                                  ctl[kCtlAbortWhere] == 31 ? "beta" : "alpha") +
+                                                  ctl[kCtlAbortWhere] == 31 ? "beta" :
+                                                  ctl[kCtlAbortWhere] == 32 ? "alpha" : "a fixed effect") +
                                                  " with error " + std::to_string(counters[9]));
         return fail(c, BRR_HIP_ETIMEOUT, "device exchange wait exceeded its bound (site " +
                                              std::to_string(ctl[kCtlAbortWhere]) + ")");
@@ -668,7 +674,7 @@ void brr_hip_destroy(brr_hip_ctx* c)
     dfree(c->d_ones_off); dfree(c->d_twos_off); dfree(c->d_miss_off);
     dfree(c->d_seg); dfree(c->d_stats);
     dfree(c->d_group); dfree(c->d_cva);
-    dfree(c->d_fixedX); dfree(c->d_fixed); dfree(c->d_forder);
+    dfree(c->d_fixedX); dfree(c->d_fixed); dfree(c->d_forder); dfree(c->d_sumf_fix);
     dfree(c->d_eps); dfree(c->d_beta); dfree(c->d_comp); dfree(c->d_acum);
     dfree(c->d_scal); dfree(c->d_hist); dfree(c->d_order); dfree(c->d_u); dfree(c->d_z);
     dfree(c->d_xchg1); dfree(c->d_xchg2); dfree(c->d_ctl); dfree(c->d_counters); dfree(c->d_tmp);
@@ -1355,8 +1361,59 @@ int brr_hip_bw_set_model(brr_hip_ctx* c, const double* mixture_C, const double* 
         bw_sum_failure_kernel<<<M, kThreads, 0, c->stream>>>(c->d_dense, c->col_words, N, c->d_fail, c->d_stats,
                                                              c->d_sumf);
     HIP_TRY(c, hipGetLastError());
+    if (c->F) {
+        bw_fixed_sumf_kernel<<<c->F, kThreads, 0, c->stream>>>(c->d_fixedX, N, c->d_fail, c->d_sumf_fix);
+        HIP_TRY(c, hipGetLastError());
+    }
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->have_model = true;
+    return BRR_HIP_OK;
+}
+
+int brr_hip_bw_set_fixed(brr_hip_ctx* c, const float* X, uint32_t num_fixed)
+{
+    int rc;
+    if ((rc = need_kind(c, true))) return rc;
+    if (num_fixed > (uint32_t)kMaxF) return fail(c, BRR_HIP_EINVAL, "at most 128 fixed-effect covariates");
+    if (num_fixed && !X) return fail(c, BRR_HIP_EINVAL, "null covariates");
+    HIP_TRY(c, hipSetDevice(c->device));
+    const uint32_t N = c->cfg.num_inds;
+    c->F = 0;
+    if (num_fixed == 0) return BRR_HIP_OK;
+    if ((rc = dalloc(c, &c->d_fixedX, (size_t)N * num_fixed)) || (rc = dalloc(c, &c->d_fixed, num_fixed)) ||
+        (rc = dalloc(c, &c->d_sumf_fix, num_fixed)))
+        return rc;
+    HIP_TRY(c, hipMemcpy(c->d_fixedX, X, sizeof(float) * N * num_fixed, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemset(c->d_fixed, 0, sizeof(double) * num_fixed));   // m_theta.setZero()
+    HIP_TRY(c, hipMemset(c->d_sumf_fix, 0, sizeof(double) * num_fixed));
+    c->F = num_fixed;
+    if (c->have_model) {
+        bw_fixed_sumf_kernel<<<c->F, kThreads, 0, c->stream>>>(c->d_fixedX, N, c->d_fail, c->d_sumf_fix);
+        HIP_TRY(c, hipGetLastError());
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+    }
+    return BRR_HIP_OK;
+}
+
+int brr_hip_bw_set_theta(brr_hip_ctx* c, const double* theta)
+{
+    int rc;
+    if ((rc = need_kind(c, true))) return rc;
+    if (c->F == 0) return fail(c, BRR_HIP_ESTATE, "no fixed effects (brr_hip_bw_set_fixed)");
+    if (!theta) return fail(c, BRR_HIP_EINVAL, "null argument");
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipMemcpy(c->d_fixed, theta, sizeof(double) * c->F, hipMemcpyHostToDevice));
+    return BRR_HIP_OK;
+}
+
+int brr_hip_bw_get_theta(brr_hip_ctx* c, double* theta)
+{
+    int rc;
+    if ((rc = need_kind(c, true))) return rc;
+    if (c->F == 0) return fail(c, BRR_HIP_ESTATE, "no fixed effects (brr_hip_bw_set_fixed)");
+    if (!theta) return fail(c, BRR_HIP_EINVAL, "null argument");
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipMemcpy(theta, c->d_fixed, sizeof(double) * c->F, hipMemcpyDeviceToHost));
     return BRR_HIP_OK;
 }
 
@@ -1424,6 +1481,23 @@ int brr_hip_bw_sweep(brr_hip_ctx* c, const uint32_t* order, const double* u, uin
     p.order = c->d_order;
     rc = launch_fn(c, bw_kernel_for(c->R, c->layout == kLayoutDense), &p);
     c->stats.markers = M;
+    return rc;
+}
+
+int brr_hip_bw_fixed_effects(brr_hip_ctx* c, uint64_t seed, uint32_t iteration, uint32_t chain)
+{
+    int rc;
+    if ((rc = need_kind(c, true)) || (rc = check_ready(c))) return rc;
+    if (c->F == 0) return fail(c, BRR_HIP_ESTATE, "no fixed effects (brr_hip_bw_set_fixed)");
+    HIP_TRY(c, hipSetDevice(c->device));
+    BwParams p = bw_params(c);
+    p.mode = kModeFixed;
+    p.iterations = 1;
+    p.iter_base = iteration;
+    p.chain = chain;
+    p.seed = seed;
+    rc = launch_fn(c, bw_kernel_for(c->R, c->layout == kLayoutDense), &p);
+    c->stats.markers = 0;
     return rc;
 }
 

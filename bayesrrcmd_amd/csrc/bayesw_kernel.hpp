@@ -33,6 +33,7 @@ constexpr int kArmsN = 100;                    // ARMS envelope points (npoint o
 constexpr uint32_t kStreamArms = 4;
 constexpr uint32_t kSiteMu = 0xFFFFFFF0u;
 constexpr uint32_t kSiteAlpha = 0xFFFFFFF1u;
+constexpr uint32_t kSiteTheta0 = 0xFFFFFF00u;   // + covariate index f < 128
 constexpr uint32_t kBwDrawSigmaB = 0;
 constexpr uint32_t kBwDrawPi0 = 1;
 constexpr double kEuler = 0.577215664901532;   // EuMasc (bayeswbase.cpp:26)
@@ -61,6 +62,12 @@ struct BwParams {
     uint64_t* xchg2;              // [2][kBwV][2]
     uint32_t* ctl;
     unsigned long long* counters;
+    // fixed effects (bayeswbase.cpp:572-601): F covariates, column-major N x F
+    // floats (Data::X), theta and sum_k X(k,f) failure_k per covariate
+    const float* fixedX;
+    double* theta;
+    const double* sumf_fix;
+    uint32_t F, pad_f;
 };
 
 enum : int { kBwMu = 0, kBwAlpha = 1, kBwSigmaB = 2, kBwD = 3, kBwBetaSq = 4, kBwPi = 5 };
@@ -84,6 +91,7 @@ struct ArmsEnv {
 
 template <int R>
 struct BwShared {
+    double theta[kMaxF];          // fixed effects
     uint32_t codes[kBwB][R][32];
     uint32_t mk[kBwB];
     uint32_t ord[2][kBwB];        // marker ids of the next two blocks, by block parity
@@ -630,6 +638,7 @@ bayesw_sweep_kernel(BwParams p)
     }
     if (tid < (int)p.nq - 1) bw_gh_nodes((int)p.nq, tid, &L.ghx[tid], &L.ghw[tid]);
     if (tid == 0) L.ghwc = kGhWc[(p.nq - 3) >> 1];
+    for (uint32_t t = tid; t < p.F; t += kThreads) L.theta[t] = p.theta[t];
     if (tid < kMaxK) {
         L.cls[tid] = (tid < K - 1) ? p.cls[tid] : 0.0;
         L.pi[tid] = (tid < K) ? p.scal[kBwPi + tid] : 0.0;
@@ -692,29 +701,71 @@ bayesw_sweep_kernel(BwParams p)
         const uint32_t iteration = p.iter_base + it;
         const uint32_t* order = p.order + (size_t)((p.mode == kModeChain) ? it : 0u) * p.M;
 
-        if (p.mode == kModeChain) {
-            // ---- sampleMu (bayeswbase.cpp:542-569): the density's sum over
-            // individuals factors as exp(-alpha x) sum_i exp(alpha (eps_i + mu) - EulerGamma)
-            double part = 0.0;
+        if (p.mode == kModeChain || p.mode == kModeFixed) {
+            // ---- sampleMu (bayeswbase.cpp:542-569; chain mode only), then sampleTheta
+            // (bayeswbase.cpp:572-601, 783-788) of every covariate in turn, through one
+            // sampler call site: task -1 is mu, task f the fixed effect f.
+            // mu_dens (bayeswbase.cpp:66-77, sigma_mu = 100): the sum over individuals
+            // factors as exp(-alpha x) sum_i exp(alpha (eps_i + mu) - EulerGamma).
+            // theta_dens (bayeswbase.cpp:87-97): one grid sum per density value of the
+            // residual shifted by X_f theta_f.
+            double smu = 0.0;
+            if (p.mode == kModeChain) {
+                double part = 0.0;
 #pragma unroll
-            for (int r = 0; r < R; ++r)
+                for (int r = 0; r < R; ++r)
 #pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    const uint32_t idx = lo + (uint32_t)r * kRun + 8u * (uint32_t)lane + (uint32_t)e;
-                    if (idx < hi) part += exp(L.alpha * (eps[r][e] + L.mu) - kEuler);
+                    for (int e = 0; e < 8; ++e) {
+                        const uint32_t idx = lo + (uint32_t)r * kRun + 8u * (uint32_t)lane + (uint32_t)e;
+                        if (idx < hi) part += exp(L.alpha * (eps[r][e] + L.mu) - kEuler);
+                    }
+                if (!allreduce(part, &smu)) goto done;
+            }
+            for (int task = (p.mode == kModeChain) ? -1 : 0; task < (int)p.F; ++task) {
+                const bool is_mu = task < 0;
+                const float* xf = p.fixedX + (size_t)(is_mu ? 0 : task) * p.N;
+                const double a = L.alpha, d = L.d;
+                const double x0 = is_mu ? L.mu : L.theta[task];
+                const double sf = is_mu ? 0.0 : p.sumf_fix[task];
+                if (!is_mu) {
+                    // params.epsilon = eps + X_f theta_f
+#pragma unroll
+                    for (int r = 0; r < R; ++r)
+#pragma unroll
+                        for (int e = 0; e < 8; ++e) {
+                            const uint32_t idx = lo + (uint32_t)r * kRun + 8u * (uint32_t)lane + (uint32_t)e;
+                            if (idx < hi) eps[r][e] = eps[r][e] + (double)xf[idx] * x0;
+                        }
                 }
-            double smu;
-            if (!allreduce(part, &smu)) goto done;
-            {
-                const double mu = L.mu, a = L.alpha, d = L.d;
-                const double xinit[4] = {0.995 * mu, mu, 1.005 * mu, 1.01 * mu};
-                double xs = mu;
-                // mu_dens (bayeswbase.cpp:66-77), sigma_mu = 100
-                const int err = arms_draw<false>(L.env[wave], xinit, 2.0, 5.0, p.seed, kSiteMu, iteration, p.chain,
-                                          [&](double x) { return -a * x * d - exp(-a * x) * smu - x * x / (2 * 100.0); },
-                                          &xs);
+                double xinit[4];
+                if (is_mu) {
+                    xinit[0] = 0.995 * x0; xinit[1] = x0; xinit[2] = 1.005 * x0; xinit[3] = 1.01 * x0;
+                } else {
+                    xinit[0] = x0 - 0.01; xinit[1] = x0; xinit[2] = x0 + 0.005; xinit[3] = x0 + 0.01;
+                }
+                double xs = x0;
+                bool ok = true;
+                const int err = arms_draw<false>(
+                    L.env[wave], xinit, is_mu ? 2.0 : -2.0, is_mu ? 5.0 : 2.0, p.seed,
+                    is_mu ? kSiteMu : kSiteTheta0 + (uint32_t)task, iteration, p.chain,
+                    [&](double x) {
+                        if (is_mu) return -a * x * d - exp(-a * x) * smu - x * x / (2 * 100.0);
+                        double sum = 0.0;
+#pragma unroll
+                        for (int r = 0; r < R; ++r)
+#pragma unroll
+                            for (int e = 0; e < 8; ++e) {
+                                const uint32_t idx = lo + (uint32_t)r * kRun + 8u * (uint32_t)lane + (uint32_t)e;
+                                if (idx < hi) sum += exp((eps[r][e] - (double)xf[idx] * x) * a - kEuler);
+                            }
+                        double tot = 0.0;
+                        if (!allreduce(sum, &tot)) ok = false;
+                        return -a * x * sf - tot - x * x / (2 * 100.0);
+                    },
+                    &xs);
+                if (!ok) goto done;
                 if (err) {
-                    if (tid == 0) { L.arms_err = err; raise_abort(p, 30); }
+                    if (tid == 0) { L.arms_err = err; raise_abort(p, is_mu ? 30 : 33); }
                     __syncthreads();
                     goto done;
                 }
@@ -723,13 +774,21 @@ bayesw_sweep_kernel(BwParams p)
 #pragma unroll
                     for (int e = 0; e < 8; ++e) {
                         const uint32_t idx = lo + (uint32_t)r * kRun + 8u * (uint32_t)lane + (uint32_t)e;
-                        if (idx < hi) eps[r][e] = (eps[r][e] + mu) - xs;
+                        if (idx < hi)
+                            eps[r][e] = is_mu ? (eps[r][e] + x0) - xs : eps[r][e] - (double)xf[idx] * xs;
                     }
                 __syncthreads();
-                if (tid == 0) L.mu = xs;
-                if (tid < kMaxK) L.v[tid] = (tid < K) ? 1.0 : 0.0;
-                if (tid == 0) L.betasq = 0.0;
+                if (tid == 0) {
+                    if (is_mu) L.mu = xs;
+                    else L.theta[task] = xs;
+                }
+                if (is_mu) {
+                    if (tid < kMaxK) L.v[tid] = (tid < K) ? 1.0 : 0.0;
+                    if (tid == 0) L.betasq = 0.0;
+                }
+                __syncthreads();
             }
+            if (p.mode == kModeFixed) break;
         }
         refresh_vi();
         __syncthreads();
@@ -1199,6 +1258,8 @@ done:
                     if (idx < hi) p.eps[idx] = eps[r][e];
                 }
         }
+        if (w == 0)
+            for (uint32_t t = tid; t < p.F; t += kThreads) p.theta[t] = L.theta[t];
         if (w == 0 && tid == 0) {
             p.scal[kBwMu] = L.mu;
             p.scal[kBwAlpha] = L.alpha;
@@ -1277,6 +1338,22 @@ static __global__ void bw_dense_sum_failure_kernel(const uint32_t* codes, uint64
 }
 
 // RaggedBayesWKernel::setVi (raggedbayeswkernel.cpp:10-16) of marker j for
+// sum_k X(k,f) failure_k of each covariate (BayesWBase::init, bayeswbase.cpp:532-537):
+// one workgroup per covariate.
+static __global__ void bw_fixed_sumf_kernel(const float* X, uint32_t N, const double* fail, double* out)
+{
+    __shared__ double red[kWaves];
+    const uint32_t f = blockIdx.x;
+    const int tid = threadIdx.x;
+    const float* x = X + (size_t)f * N;
+    double s = 0.0;
+    for (uint32_t i = tid; i < N; i += kThreads) s += (double)x[i] * fail[i];
+    s = wave_sum(s);
+    if ((tid & 63) == 0) red[tid >> 6] = s;
+    __syncthreads();
+    if (tid == 0) out[f] = ((red[0] + red[1]) + red[2]) + red[3];
+}
+
 // vi = exp(alpha eps - EulerGamma): out = {vi_sum, vi_0, vi_1, vi_2}. One workgroup.
 static __global__ void bw_set_vi_kernel(const uint32_t* codes, uint64_t col_words, uint32_t N, const double* eps,
                                  double alpha, uint32_t j, double* out)

@@ -302,6 +302,21 @@ int brr_hip_bw_sweep(brr_hip_ctx* ctx, const uint32_t* order, const double* u,
 int brr_hip_bw_run_chain(brr_hip_ctx* ctx, uint64_t seed, uint32_t chain,
                          uint32_t first_iteration, uint32_t iterations,
                          const uint32_t* orders, double* hist);
+/* BayesW fixed effects (Data::X, bayeswbase.cpp:532-537, 572-601, 783-788): X
+ * is the N x F float matrix in Eigen's column-major order, F <= 128; theta
+ * starts at 0 and sum_k X(k,f) failure_k is formed on the device (after
+ * brr_hip_bw_set_model). brr_hip_bw_run_chain then draws theta_0 .. theta_F-1
+ * in turn after mu each iteration: ARMS over theta_dens with the starting points
+ * theta - 0.01, theta, theta + 0.005, theta + 0.01 and bounds (-2, 2), the
+ * residual shifted by X_f theta before and X_f theta_new after (ARMS site
+ * 0xFFFFFF00 + f). F = 0 removes them. */
+int brr_hip_bw_set_fixed(brr_hip_ctx* ctx, const float* X, uint32_t num_fixed);
+int brr_hip_bw_set_theta(brr_hip_ctx* ctx, const double* theta);
+int brr_hip_bw_get_theta(brr_hip_ctx* ctx, double* theta);
+/* The fixed effects alone (every sampleTheta of one iteration) on the device
+ * state, for a host that runs runGibbs itself. */
+int brr_hip_bw_fixed_effects(brr_hip_ctx* ctx, uint64_t seed, uint32_t iteration, uint32_t chain);
+
 
 /* Per-marker kernel calls on the device epsilon (correctness path).
  * set_vi: RaggedBayesWKernel::setVi of vi = exp(alpha eps - EulerGamma):

@@ -754,18 +754,78 @@ void orc_bw_post_sweep(const orc_data* d, const orc_bw_model* m, orc_bw_state* s
     orc_dirichlet_rng(seed, iteration, ORC_BW_DRAW_PI0, chain, m->K, s->v, s->pi);
 }
 
+typedef struct {
+    double alpha, sumf;
+    const double* eps;      /* eps + X_f theta_f */
+    const float* x;
+    uint32_t N;
+} theta_ctx;
+
+/* theta_dens (bayeswbase.cpp:87-97), sigma_mu = 100 */
+static double theta_dens(double x, void* c)
+{
+    const theta_ctx* p = (const theta_ctx*)c;
+    double s = 0.0;
+    for (uint32_t i = 0; i < p->N; ++i) s += exp((p->eps[i] - (double)p->x[i] * x) * p->alpha - BW_EULER);
+    return -p->alpha * x * p->sumf - s - x * x / (2 * 100.0);
+}
+
+void orc_bw_sum_failure_fixed(uint32_t N, const float* X, uint32_t F, const double* fail, double* sumf)
+{
+    /* BayesWBase::init (bayeswbase.cpp:532-537): sum_k X(k,f) failure_k */
+    for (uint32_t f = 0; f < F; ++f) {
+        double t = 0.0;
+        for (uint32_t k = 0; k < N; ++k) t += (double)X[(size_t)f * N + k] * fail[k];
+        sumf[f] = t;
+    }
+}
+
+int orc_bw_sample_theta(const orc_data* d, orc_bw_state* s, const float* X, uint32_t f, double* theta,
+                        const double* sumf, uint64_t seed, uint32_t iteration, uint32_t chain)
+{
+    /* sampleTheta (bayeswbase.cpp:572-601) */
+    const uint32_t N = d->N;
+    const float* x = X + (size_t)f * N;
+    const double th = theta[f];
+    double* eps_adj = (double*)malloc(sizeof(double) * N);
+    for (uint32_t i = 0; i < N; ++i) eps_adj[i] = s->eps[i] + (double)x[i] * th;
+    theta_ctx c = {s->alpha, sumf[f], eps_adj, x, N};
+    const double xinit[4] = {th - 0.01, th, th + 0.005, th + 0.01};
+    const orc_arms_rng r = {seed, ORC_BW_SITE_THETA0 + f, iteration, chain};
+    double xs = th;
+    const int err = orc_arms(xinit, 4, -2.0, 2.0, theta_dens, &c, 100, &r, &xs, NULL);
+    if (!err) {
+        theta[f] = xs;
+        for (uint32_t i = 0; i < N; ++i) s->eps[i] = eps_adj[i] - (double)x[i] * xs;
+    }
+    free(eps_adj);
+    return err;
+}
+
 int orc_bw_run_gibbs(const orc_data* d, const orc_bw_model* m, orc_bw_state* s, const double* y,
                      const double* fail, uint64_t seed, uint32_t chain, uint32_t iterations, uint32_t* order,
                      double* hist)
 {
-    /* BayesWBase::runGibbs (bayeswbase.cpp:737-830), Philox streams */
+    return orc_bw_run_gibbs_fixed(d, m, s, y, fail, seed, chain, iterations, order, hist, NULL, 0, NULL);
+}
+
+int orc_bw_run_gibbs_fixed(const orc_data* d, const orc_bw_model* m, orc_bw_state* s, const double* y,
+                           const double* fail, uint64_t seed, uint32_t chain, uint32_t iterations,
+                           uint32_t* order, double* hist, const float* X, uint32_t F, double* theta)
+{
+    /* BayesWBase::runGibbs (bayeswbase.cpp:737-830), Philox streams; F > 0: the
+     * fixed effects after mu (:783-788) */
     const uint32_t M = d->M;
     orc_bw_init(d, m, s, y, fail);
     double* u = (double*)malloc(sizeof(double) * M);
     double* z = (double*)malloc(sizeof(double) * M);
+    double* sumf = (double*)malloc(sizeof(double) * (F ? F : 1));
+    if (F) orc_bw_sum_failure_fixed(d->N, X, F, fail, sumf);
     int err = 0;
     for (uint32_t it = 0; it < iterations && !err; ++it) {
         if ((err = orc_bw_sample_mu(d, s, seed, it, chain))) break;
+        for (uint32_t f = 0; f < F && !err; ++f) err = orc_bw_sample_theta(d, s, X, f, theta, sumf, seed, it, chain);
+        if (err) break;
         refresh_vi(d->N, s->eps, s->alpha, s->vi);
         orc_shuffle_philox(seed, it, chain, M, order);
         for (int k = 0; k < m->K; ++k) s->v[k] = 1.0;
@@ -782,5 +842,6 @@ int orc_bw_run_gibbs(const orc_data* d, const orc_bw_model* m, orc_bw_state* s, 
     }
     free(u);
     free(z);
+    free(sumf);
     return err;
 }

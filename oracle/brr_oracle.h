@@ -268,6 +268,7 @@ typedef struct {
 } orc_arms_rng;
 #define ORC_BW_SITE_MU 0xFFFFFFF0u
 #define ORC_BW_SITE_ALPHA 0xFFFFFFF1u
+#define ORC_BW_SITE_THETA0 0xFFFFFF00u   /* + covariate index f < 128 */
 /* scalar draws of a BayesW iteration: sigma_b, then the Dirichlet gammas */
 enum { ORC_BW_DRAW_SIGMAB = 0, ORC_BW_DRAW_PI0 = 1 };
 #define ORC_BW_MAXK 8
@@ -325,6 +326,19 @@ int orc_bw_sample_alpha(const orc_data* d, orc_bw_state* s, const double* fail, 
 void orc_bw_post_sweep(const orc_data* d, const orc_bw_model* m, orc_bw_state* s, uint64_t seed,
                        uint32_t iteration, uint32_t chain);
 /* hist: iterations x 5 {mu, alpha, sigma_b, nonzero markers, |beta|^2}. */
+/* BayesW fixed effects: sum_failure_fix (bayeswbase.cpp:532-537) over the
+ * reference's float covariates (column-major N x F), and sampleTheta
+ * (bayeswbase.cpp:572-601) of covariate f: ARMS over theta_dens with the
+ * starting points theta +- {0.01, 0, 0.005, 0.01}, bounds (-2, 2), the
+ * residual shifted by X_f theta_f before and after (ARMS site
+ * ORC_BW_SITE_THETA0 + f). */
+void orc_bw_sum_failure_fixed(uint32_t N, const float* X, uint32_t F, const double* fail, double* sumf);
+int orc_bw_sample_theta(const orc_data* d, orc_bw_state* s, const float* X, uint32_t f, double* theta,
+                        const double* sumf, uint64_t seed, uint32_t iteration, uint32_t chain);
+/* orc_bw_run_gibbs with the fixed effects after mu each iteration (F = 0: none). */
+int orc_bw_run_gibbs_fixed(const orc_data* d, const orc_bw_model* m, orc_bw_state* s, const double* y,
+                           const double* fail, uint64_t seed, uint32_t chain, uint32_t iterations,
+                           uint32_t* order, double* hist, const float* X, uint32_t F, double* theta);
 int orc_bw_run_gibbs(const orc_data* d, const orc_bw_model* m, orc_bw_state* s, const double* y,
                      const double* fail, uint64_t seed, uint32_t chain, uint32_t iterations, uint32_t* order,
                      double* hist);

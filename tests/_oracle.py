@@ -156,6 +156,12 @@ def lib(fast: bool = False):
                                      C.c_uint32, C.c_uint32]),
         "orc_bw_run_gibbs": (C.c_int, [C.POINTER(OrcData), C.POINTER(OrcBwModel), C.POINTER(OrcBwState), P_f64,
                                        P_f64, C.c_uint64, C.c_uint32, C.c_uint32, P_u32, P_f64]),
+        "orc_bw_run_gibbs_fixed": (C.c_int, [C.POINTER(OrcData), C.POINTER(OrcBwModel), C.POINTER(OrcBwState),
+                                             P_f64, P_f64, C.c_uint64, C.c_uint32, C.c_uint32, P_u32, P_f64,
+                                             C.POINTER(C.c_float), C.c_uint32, P_f64]),
+        "orc_bw_sample_theta": (C.c_int, [C.POINTER(OrcData), C.POINTER(OrcBwState), C.POINTER(C.c_float),
+                                          C.c_uint32, P_f64, P_f64, C.c_uint64, C.c_uint32, C.c_uint32]),
+        "orc_bw_sum_failure_fixed": (None, [C.c_uint32, C.POINTER(C.c_float), C.c_uint32, P_f64, P_f64]),
     }
     for n, (r, a) in sig.items():
         f = getattr(L, n)
@@ -432,6 +438,31 @@ class BayesWProblem:
         u = np.ascontiguousarray(u, dtype=np.float64)
         return lib().orc_bw_sweep(C.byref(self.data), C.byref(self.model), C.byref(self.state),
                                   p(order, C.c_uint32), p(u, C.c_double), seed, iteration, chain)
+
+    def set_fixed(self, X):
+        """Covariates (N, F) as the reference's float matrix (column-major); theta 0."""
+        x = np.asarray(X, dtype=np.float32).reshape(self.n, -1)
+        self.F = x.shape[1]
+        self.fixedX = np.ascontiguousarray(x.T)
+        self.theta = np.zeros(self.F)
+        self.sumf_fix = np.zeros(self.F)
+        lib().orc_bw_sum_failure_fixed(self.n, p(self.fixedX, C.c_float), self.F, p(self.fail, C.c_double),
+                                       p(self.sumf_fix, C.c_double))
+
+    def sample_theta(self, f, seed, iteration, chain=0):
+        return lib().orc_bw_sample_theta(C.byref(self.data), C.byref(self.state), p(self.fixedX, C.c_float), f,
+                                         p(self.theta, C.c_double), p(self.sumf_fix, C.c_double), seed,
+                                         iteration, chain)
+
+    def run_gibbs_fixed(self, y, seed, iterations, chain=0):
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        order = np.arange(self.m, dtype=np.uint32)
+        hist = np.zeros((iterations, 5))
+        err = lib().orc_bw_run_gibbs_fixed(C.byref(self.data), C.byref(self.model), C.byref(self.state),
+                                           p(y, C.c_double), p(self.fail, C.c_double), seed, chain, iterations,
+                                           p(order, C.c_uint32), p(hist, C.c_double), p(self.fixedX, C.c_float),
+                                           self.F, p(self.theta, C.c_double))
+        return err, hist, order
 
     def run_gibbs(self, y, seed, iterations, chain=0):
         y = np.ascontiguousarray(y, dtype=np.float64)
