@@ -28,7 +28,7 @@ from . import _abi
 from ._abi import LAYOUT_DENSE_2BIT, LAYOUT_EIGEN_SPARSE, LAYOUT_RAGGED, STORAGE_AUTO, STORAGE_CODES, STORAGE_LISTS
 
 __all__ = [
-    "BrrHipError", "HipSweep", "HipBayesW", "bayesw_init", "bed_file_image", "RaggedArena", "read_ragged_sparsebed", "read_ppbed", "SampleWriter", "sample_due", "build_dense", "build_ragged", "synth_bed", "synth_xb",
+    "BrrHipError", "HipSweep", "HipBayesW", "bayesw_init", "bed_file_image", "read_samples_binary", "RaggedArena", "read_ragged_sparsebed", "read_ppbed", "SampleWriter", "sample_due", "build_dense", "build_ragged", "synth_bed", "synth_xb",
     "bayesr_init", "shuffle", "marker_uz", "LAYOUT_DENSE_2BIT", "LAYOUT_RAGGED", "LAYOUT_EIGEN_SPARSE", "lib",
     "build_eigen", "read_eigen_sparsebed",
     "STORAGE_AUTO", "STORAGE_CODES", "STORAGE_LISTS",
@@ -195,10 +195,12 @@ def read_ppbed(data: bytes | np.ndarray, index: np.ndarray, n: int, compressed: 
 
 
 class SampleWriter:
-    """The reference's CSV sample file (SampleWriter): kind "bayesr" or "bayesw"."""
+    """The reference's sample file (SampleWriter): kind "bayesr" or "bayesw"; CSV, or with binary=True the
+    same rows as raw doubles (read back with read_samples_binary)."""
 
-    def __init__(self, path: str, kind: str, m: int, n: int = 0, groups: int = 1, fixed: int = 0):
-        self.kind = {"bayesr": 0, "bayesw": 1}[kind]
+    def __init__(self, path: str, kind: str, m: int, n: int = 0, groups: int = 1, fixed: int = 0,
+                 binary: bool = False):
+        self.kind = {"bayesr": 0, "bayesw": 1}[kind] + (2 if binary else 0)
         self.m, self.n, self.g, self.f = m, n, groups, fixed
         h = C.c_void_p()
         _check(lib().brr_samples_open(str(path).encode(), self.kind, m, n, groups, fixed, C.byref(h)),
@@ -233,6 +235,19 @@ class SampleWriter:
 
     def __exit__(self, *exc):
         self.close()
+
+
+def read_samples_binary(path: str):
+    """A binary sample file: (header dict {kind, M, N, G, F}, rows as an (iterations, row length) array)."""
+    raw = np.fromfile(path, dtype=np.uint8)
+    if raw.size < 32 or raw[:8].tobytes() != b"BRRSAMP1":
+        raise BrrHipError(f"{path}: not a binary sample file")
+    kind, m, n, g, f, row_len = (int(v) for v in raw[8:32].view(np.uint32))
+    body = raw[32:]
+    if body.size % (8 * row_len):
+        raise BrrHipError(f"{path}: truncated row")
+    rows = body.view(np.float64).reshape(-1, row_len)
+    return dict(kind=kind, M=m, N=n, G=g, F=f), rows
 
 
 def sample_due(iteration: int, burn_in: int, thinning: int) -> bool:

@@ -16,6 +16,7 @@
 #include <cstdlib>
 #include <fstream>
 #include <string>
+#include <vector>
 
 #include "brr_hip.h"
 #include "brr_rng.h"
@@ -116,23 +117,56 @@ int brr_bayesw_init(const double* y, const double* fail, uint32_t N, uint32_t M,
 struct brr_samples {
     std::ofstream out;
     int kind = 0;
+    bool binary = false;
     uint32_t M = 0, N = 0, G = 0, F = 0;
+    std::vector<double> row;
 };
+
+namespace {
+
+// One sample row: the values of the reference's `sample` vector in its order,
+// as a default std::ostream prints them after ", " separators (CSV) or as raw
+// little-endian doubles (binary).
+int emit_row(brr_samples* w)
+{
+    std::ofstream& o = w->out;
+    if (w->binary) {
+        o.write(reinterpret_cast<const char*>(w->row.data()), (std::streamsize)(sizeof(double) * w->row.size()));
+    } else {
+        for (size_t i = 0; i < w->row.size(); ++i) {
+            if (i) o << ", ";
+            o << w->row[i];
+        }
+        o << std::endl;
+    }
+    return o ? BRR_HIP_OK : BRR_HIP_EINVAL;
+}
+
+} // namespace
 
 extern "C" {
 
 int brr_samples_open(const char* path, int kind, uint32_t M, uint32_t N, uint32_t G, uint32_t F, brr_samples** out)
 {
-    if (!path || !out || M == 0 || (kind != BRR_SAMPLES_BAYESR && kind != BRR_SAMPLES_BAYESW)) return BRR_HIP_EINVAL;
-    if (kind == BRR_SAMPLES_BAYESR && (N == 0 || G == 0)) return BRR_HIP_EINVAL;
+    const bool binary = kind == BRR_SAMPLES_BAYESR_BINARY || kind == BRR_SAMPLES_BAYESW_BINARY;
+    const int base = binary ? kind - 2 : kind;
+    if (!path || !out || M == 0 || (base != BRR_SAMPLES_BAYESR && base != BRR_SAMPLES_BAYESW)) return BRR_HIP_EINVAL;
+    if (base == BRR_SAMPLES_BAYESR && (N == 0 || G == 0)) return BRR_HIP_EINVAL;
     *out = nullptr;
     brr_samples* w = new brr_samples();
-    w->kind = kind;
+    w->kind = base;
+    w->binary = binary;
     w->M = M; w->N = N; w->G = G; w->F = F;
-    w->out.open(path);
+    w->out.open(path, binary ? std::ios::out | std::ios::binary : std::ios::out);
     if (!w->out) { delete w; return BRR_HIP_EINVAL; }
     std::ofstream& o = w->out;
-    if (kind == BRR_SAMPLES_BAYESR) {
+    if (binary) {
+        // "BRRSAMP1", then u32 {kind, M, N, G, F, doubles per row}
+        const uint64_t row_len = base == BRR_SAMPLES_BAYESR ? 3ull + 3ull * M + G + F + N : 4ull + 2ull * M + F;
+        const uint32_t head[6] = {(uint32_t)kind, M, N, G, F, (uint32_t)row_len};
+        o.write("BRRSAMP1", 8);
+        o.write(reinterpret_cast<const char*>(head), sizeof(head));
+    } else if (base == BRR_SAMPLES_BAYESR) {
         // SampleWriter::openGroups (samplewriter.cpp:82-118)
         o << "iteration," << "mu,";
         for (uint32_t i = 0; i < M; ++i) o << "beta[" << (i + 1) << "],";
@@ -144,6 +178,7 @@ int brr_samples_open(const char* path, int kind, uint32_t M, uint32_t N, uint32_
         uint32_t i;
         for (i = 0; i < N - 1; ++i) o << "epsilon[" << (i + 1) << "],";
         o << "epsilon[" << (i + 1) << "]";
+        o << std::endl;
     } else {
         // SampleWriter::open_bayesW / open_bayesW_fixed (samplewriter.cpp:34-80)
         o << "iteration," << "alpha," << "mu,";
@@ -151,8 +186,8 @@ int brr_samples_open(const char* path, int kind, uint32_t M, uint32_t N, uint32_
         for (uint32_t i = 0; i < M; ++i) o << "beta[" << (i + 1) << "],";
         for (uint32_t i = 0; i < M; ++i) o << "comp[" << (i + 1) << "],";
         o << "sigma_b";
+        o << std::endl;
     }
-    o << std::endl;
     o.flush();
     *out = w;
     return o ? BRR_HIP_OK : BRR_HIP_EINVAL;
@@ -165,18 +200,18 @@ int brr_samples_write_bayesr(brr_samples* w, uint32_t iteration, double mu, cons
     // BayesRBase.cpp:251: iteration, mu, beta, sigmaE, sigmaG, gamma, components, acum, epsilon
     if (!w || w->kind != BRR_SAMPLES_BAYESR || !beta || !sigmaG || !comp || !acum || !eps || (w->F && !gamma))
         return BRR_HIP_EINVAL;
-    std::ofstream& o = w->out;
-    const char* sep = ", ";
-    o << (double)iteration << sep << mu;
-    for (uint32_t i = 0; i < w->M; ++i) o << sep << beta[i];
-    o << sep << sigmaE;
-    for (uint32_t i = 0; i < w->G; ++i) o << sep << sigmaG[i];
-    for (uint32_t i = 0; i < w->F; ++i) o << sep << gamma[i];
-    for (uint32_t i = 0; i < w->M; ++i) o << sep << (double)comp[i];
-    for (uint32_t i = 0; i < w->M; ++i) o << sep << acum[i];
-    for (uint32_t i = 0; i < w->N; ++i) o << sep << eps[i];
-    o << std::endl;
-    return o ? BRR_HIP_OK : BRR_HIP_EINVAL;
+    std::vector<double>& r = w->row;
+    r.clear();
+    r.push_back((double)iteration);
+    r.push_back(mu);
+    r.insert(r.end(), beta, beta + w->M);
+    r.push_back(sigmaE);
+    r.insert(r.end(), sigmaG, sigmaG + w->G);
+    if (w->F) r.insert(r.end(), gamma, gamma + w->F);
+    for (uint32_t i = 0; i < w->M; ++i) r.push_back((double)comp[i]);
+    r.insert(r.end(), acum, acum + w->M);
+    r.insert(r.end(), eps, eps + w->N);
+    return emit_row(w);
 }
 
 int brr_samples_write_bayesw(brr_samples* w, uint32_t iteration, double alpha, double mu, const double* theta,
@@ -184,15 +219,16 @@ int brr_samples_write_bayesw(brr_samples* w, uint32_t iteration, double alpha, d
 {
     // bayeswbase.cpp:810-815: iteration, alpha, mu, (theta), beta, components, sigma_b
     if (!w || w->kind != BRR_SAMPLES_BAYESW || !beta || !comp || (w->F && !theta)) return BRR_HIP_EINVAL;
-    std::ofstream& o = w->out;
-    const char* sep = ", ";
-    o << (double)iteration << sep << alpha << sep << mu;
-    for (uint32_t i = 0; i < w->F; ++i) o << sep << theta[i];
-    for (uint32_t i = 0; i < w->M; ++i) o << sep << beta[i];
-    for (uint32_t i = 0; i < w->M; ++i) o << sep << (double)comp[i];
-    o << sep << sigma_b;
-    o << std::endl;
-    return o ? BRR_HIP_OK : BRR_HIP_EINVAL;
+    std::vector<double>& r = w->row;
+    r.clear();
+    r.push_back((double)iteration);
+    r.push_back(alpha);
+    r.push_back(mu);
+    if (w->F) r.insert(r.end(), theta, theta + w->F);
+    r.insert(r.end(), beta, beta + w->M);
+    for (uint32_t i = 0; i < w->M; ++i) r.push_back((double)comp[i]);
+    r.push_back(sigma_b);
+    return emit_row(w);
 }
 
 int brr_samples_close(brr_samples* w)

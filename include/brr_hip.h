@@ -404,8 +404,12 @@ int brr_dense_read_ppbed(const uint8_t* data, uint64_t data_bytes, const uint64_
 
 /* The reference's CSV samples: the header of SampleWriter::openGroups (BayesR, kind 0) or
  * open_bayesW[_fixed] (BayesW, kind 1), then one row per kept iteration, values separated by
- * ", " and printed as a default std::ostream prints a double (6 significant digits). */
-enum brr_samples_kind { BRR_SAMPLES_BAYESR = 0, BRR_SAMPLES_BAYESW = 1 };
+ * ", " and printed as a default std::ostream prints a double (6 significant digits).
+ * Kinds 2 and 3 write the same rows as raw little-endian doubles (full precision, no text
+ * for the N residuals of a BayesR row): the 8 bytes "BRRSAMP1", then u32 {kind, M, N, G, F,
+ * doubles per row}, then the rows. */
+enum brr_samples_kind { BRR_SAMPLES_BAYESR = 0, BRR_SAMPLES_BAYESW = 1, BRR_SAMPLES_BAYESR_BINARY = 2,
+                        BRR_SAMPLES_BAYESW_BINARY = 3 };
 typedef struct brr_samples brr_samples;
 int brr_samples_open(const char* path, int kind, uint32_t num_snps, uint32_t num_inds, uint32_t num_groups,
                      uint32_t num_fixed, brr_samples** out);

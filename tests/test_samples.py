@@ -110,3 +110,39 @@ def test_bad_writer_calls(tmp_path):
     with pytest.raises(brr.BrrHipError):
         brr.SampleWriter(tmp_path / "no_such_dir" / "x.csv", "bayesw", 3)
     assert brr.lib().brr_samples_close(None) == brr._abi.EINVAL
+
+
+@pytest.mark.parametrize("kind", ["bayesr", "bayesw"])
+def test_binary_samples_hold_the_csv_rows_exactly(tmp_path, kind):
+    """The binary file carries the same row values as the CSV, at full precision."""
+    rng = np.random.default_rng(9)
+    m, n, g, f = 7, 11, 2, 3
+    paths = {b: tmp_path / f"s_{kind}_{int(b)}" for b in (False, True)}
+    rows = []
+    writers = {b: brr.SampleWriter(str(paths[b]), kind, m, n, g, f, binary=b) for b in (False, True)}
+    for it in (0, 5, 10):
+        beta = rng.normal(size=m)
+        comp = rng.integers(0, 4, m).astype(np.int32)
+        if kind == "bayesr":
+            mu, se, sg, gm = rng.normal(), rng.random(), rng.random(g), rng.normal(size=f)
+            acum, eps = rng.random(m), rng.normal(size=n)
+            row = np.concatenate([[it, mu], beta, [se], sg, gm, comp, acum, eps])
+            for w in writers.values():
+                w.write_bayesr(it, mu, beta, se, sg, comp, acum, eps, gamma=gm)
+        else:
+            al, mu, sb, th = rng.random() + 1, rng.normal(), rng.random(), rng.normal(size=f)
+            row = np.concatenate([[it, al, mu], th, beta, comp, [sb]])
+            for w in writers.values():
+                w.write_bayesw(it, al, mu, beta, comp, sb, theta=th)
+        rows.append(row)
+    for w in writers.values():
+        w.close()
+    head, got = brr.read_samples_binary(str(paths[True]))
+    want = np.array(rows)
+    assert head == dict(kind=2 if kind == "bayesr" else 3, M=m, N=n, G=g, F=f)
+    assert got.shape == want.shape == (3, (3 + 3 * m + g + f + n) if kind == "bayesr" else (4 + 2 * m + f))
+    assert np.array_equal(got.view(np.uint64), want.view(np.uint64))
+    csv = np.array([[float(x) for x in line.split(", ")] for line in paths[False].read_text().splitlines()[1:]])
+    assert np.allclose(csv, want, rtol=1e-5, atol=0)          # the CSV keeps 6 significant digits
+    with pytest.raises(brr.BrrHipError):
+        brr.read_samples_binary(str(paths[False]))
