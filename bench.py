@@ -58,6 +58,11 @@ CONFIGS = {
     "c5": dict(workload="BayesW Weibull survival, ragged layout, N=100000 M=50000, single chain",
                n=100_000, m=50_000, layout="ragged", maf=(0.005, 0.5), causal=0.005, h2=0.5, alpha=3.0,
                mu=4.0, censor=0.1, C=(0.001, 0.01, 0.1), quad_points=7, chain=100),
+    "c4": dict(workload="uneven batch: 32 independent cohorts, N ~ logU(2,000, 200,000), M ~ U(10,000, 80,000), "
+                        "dense 2-bit or ragged per cohort (whichever has fewer bytes, at least one of each), "
+                        "one BayesR chain each, run cohort after cohort",
+               cohorts=32, n=(2000, 200000), m=(10000, 80000), maf=(0.01, 0.5), h2=(0.2, 0.7), causal=0.005,
+               chain=20, layout="mixed"),
     "c5-dense": dict(workload="BayesW Weibull survival, dense .bed 2-bit layout (DenseBayesW), N=100000 M=50000, "
                               "single chain",
                      n=100_000, m=50_000, layout="dense", maf=(0.005, 0.5), causal=0.005, h2=0.5, alpha=3.0,
@@ -511,6 +516,91 @@ def run_batched(cfg, args, with_cpu):
     return out
 
 
+def run_cohorts(cfg, args, with_cpu):
+    """The uneven batch: every cohort's .bed decoded on the device (brr_hip_load_bed) into a context sized to
+    its N, then its chain of `steps` iterations; value = all cohorts' marker-updates / the sum of their chain
+    wall times (data generation and upload excluded)."""
+    import bayesrrcmd_amd as brr
+    steps = args.steps if args.steps is not None else cfg["chain"]
+    rng = np.random.default_rng(args.seed + 4)
+    K = cfg["cohorts"]
+    ns = np.exp(rng.uniform(np.log(cfg["n"][0]), np.log(cfg["n"][1]), K)).astype(int)
+    ms = rng.integers(cfg["m"][0], cfg["m"][1] + 1, K)
+    h2s = rng.uniform(cfg["h2"][0], cfg["h2"][1], K)
+    # bytes of each layout at this MAF range: ragged 4 (n1 + n2) + 32 per marker with E[n1 + n2] / N =
+    # E[2p - p^2] for p ~ U(lo, hi); dense ceil(N/4)
+    lo, hi = cfg["maf"]
+    nnz_frac = (hi ** 2 - lo ** 2) / (hi - lo) - (hi ** 3 - lo ** 3) / (3 * (hi - lo))
+    layouts = ["ragged" if 4 * nnz_frac * n + 32 < (n + 3) // 4 else "dense" for n in ns]
+    for want in ("dense", "ragged"):
+        if want not in layouts:
+            layouts[int(np.argmin(ns)) if want == "ragged" else int(np.argmax(ns))] = want
+    per, wall_total, kernel_total, bytes_total, updates = [], 0.0, 0.0, 0.0, 0
+    cpu = None
+    for c in range(K):
+        n, m, lay = int(ns[c]), int(ms[c]), layouts[c]
+        t0 = time.time()
+        bed, _ = brr.synth_bed(args.seed + 100 + c, n, m, lo, hi, 0.0, threads=args.threads)
+        mc = max(1, int(round(cfg["causal"] * m)))
+        causal = np.sort(rng.choice(m, mc, replace=False))
+        b = rng.normal(0.0, np.sqrt(h2s[c] / mc), mc)
+        g = brr.synth_xb(bed[causal], n, brr.build_dense(bed[causal], n, threads=args.threads), b,
+                         threads=args.threads)
+        y = (g + rng.normal(0.0, np.sqrt(1.0 - h2s[c]), n)).astype(np.float32).astype(np.float64)
+        code = brr.LAYOUT_RAGGED if lay == "ragged" else brr.LAYOUT_DENSE_2BIT
+        h = brr.HipSweep(n, m, 4, 1, code)
+        h.load_bed(brr.bed_file_image(bed), code)
+        h.set_model(S_DEFAULT)
+        setup_s = time.time() - t0
+        eps, sc = brr.bayesr_init(y, 1, 4, args.seed + c)
+        h.set_state(eps, np.zeros(m), np.zeros(m, np.int32))
+        h.set_scalars(sc["mu"], sc["sigmaE"], sc["epsilon_sum"], sc["sigmaG"], sc["pi"])
+        order = np.arange(m, dtype=np.uint32)
+        orders = np.empty((steps, m), dtype=np.uint32)
+        for it in range(steps):
+            brr.shuffle(args.seed + c, it, 0, order)
+            orders[it] = order
+        kms, chg = [], []
+        t1 = time.perf_counter()
+        for it in range(steps):
+            h.run_chain(args.seed + c, 0, it, orders[it:it + 1])
+            s = h.stats()
+            kms.append(s["kernel_ms"])
+            chg.append(s["changes"])
+        wall = time.perf_counter() - t1
+        stats = h.stats()
+        h.close()
+        cfg_c = dict(n=n, m=m, layout="dense")     # the device holds codes in either layout
+        nbytes = sum(algorithmic_bytes(cfg_c, None, 2, m, x) for x in chg)
+        wall_total += wall
+        kernel_total += sum(kms) * 1e-3
+        bytes_total += nbytes
+        updates += m * steps
+        per.append(dict(N=n, M=m, layout=lay, workgroups=stats["workgroups"], ms_per_iteration=wall * 1e3 / steps,
+                        marker_updates_per_s=m * steps / wall, setup_s=round(setup_s, 2)))
+        log(f"cohort {c}: N={n} M={m} {lay}, {stats['workgroups']} workgroups, {wall * 1e3 / steps:.2f} ms/iteration "
+            f"({m * steps / wall / 1e6:.2f} M marker-updates/s), setup {setup_s:.1f}s")
+        if with_cpu and cpu is None and lay == "dense":
+            k = min(m, 4000)                          # the markers the CPU sample uses (dense layout)
+            data = dict(bed=bed[:k], lut=brr.build_dense(bed[:k], n, threads=args.threads), y=y, ragged=None)
+            cpu = cpu_baseline(dict(n=n, m=m, layout="dense", chain=steps), data, args.seed, args.cpu_budget)
+        del bed
+    achieved = bytes_total / kernel_total / 1e9
+    out = dict(value=updates / wall_total, unit="marker-updates/s", steps=steps,
+               ms_per_step=wall_total * 1e3 / steps, dtype="f64",
+               config={"workload": cfg["workload"], "cohorts": K, "timed_iterations_per_cohort": steps,
+                       "precision": "fp64", "parallelism": "single GPU; one persistent cooperative launch per "
+                       "cohort iteration, cohorts in sequence", "per_cohort": per},
+               roofline={"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                         "kernel": "brr::bayesr_sweep_kernel (all cohorts)",
+                         "avg_launch_ms": kernel_total * 1e3 / (steps * K),
+                         "algorithmic_bytes_per_launch": bytes_total / (steps * K)})
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
+    return out
+
+
 def decode_bytes(n, m, layout):
     """Algorithmic bytes of the unpack kernel: read the packed columns (M ceil(N/4)), write the
     16-byte-aligned code columns, 16 B of counts and (ragged) 32 B of statistics per marker."""
@@ -616,13 +706,16 @@ def main():
         log(f"genotypes {cfg['m']} x {cfg['n']} generated in {time.time() - t0:.1f}s")
         print(json.dumps({"decode": run_decode(cfg, {"bed": bed}, args)}), flush=True)
         return
-    if args.config in ("c3", "c5", "c5-dense"):
-        b = (run_batched if args.config == "c3" else run_bayesw)(cfg, args, not args.no_cpu)
+    if args.config in ("c3", "c4", "c5", "c5-dense"):
+        run = {"c3": run_batched, "c4": run_cohorts}.get(args.config, run_bayesw)
+        b = run(cfg, args, not args.no_cpu)
         line = {"metric": METRIC, "value": b["value"], "unit": b["unit"], "n_gpus": 1, "steps": b["steps"],
                 "warmup": args.warmup, "ms_per_step": b["ms_per_step"], "higher_is_better": True,
                 "scaling": "weak", "vs_baseline": None, "dtype": b["dtype"],
                 "data": ("synthetic (Philox-seeded PLINK genotypes, HWE, 64 phenotypes y = X beta + e)"
                          if args.config == "c3" else
+                         "synthetic (Philox-seeded PLINK genotypes, HWE, one phenotype y = X beta + e per cohort)"
+                         if args.config == "c4" else
                          "synthetic (Philox-seeded PLINK genotypes, HWE, Weibull log-times, 10% censored)"),
                 "config": b["config"], "roofline": b["roofline"]}
         if "cpu_baseline" in b:
