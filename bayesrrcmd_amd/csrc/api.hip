@@ -625,8 +625,8 @@ int brr_hip_create(const brr_hip_config* cfg, brr_hip_ctx** out)
     }
     c->S = S;
     c->P = (N + S - 1) / S;
-    c->x1_words = c->T ? (size_t)2 * c->P * kBVmax * 2 : (size_t)2 * kMaxB * c->P * 2;
-    c->x2_words = c->T ? (size_t)2 * kBVmax * 2 : (size_t)2 * kMaxB * 2;
+    c->x1_words = c->T ? (size_t)2 * c->P * kBVmax * 2 : (size_t)2 * kSlots * c->P * 2;
+    c->x2_words = c->T ? (size_t)2 * kBVmax * 2 : (size_t)2 * kSlots * 2;
 
     int occ = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, probe, kThreads, 0) != hipSuccess ||

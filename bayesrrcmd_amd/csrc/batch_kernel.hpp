@@ -163,7 +163,7 @@ __device__ __forceinline__ bool breduce(const BatchParams& p, BShared& L, uint32
         if (v < V) {
             for (uint32_t w2 = (uint32_t)(tid >> 3); w2 < p.P; w2 += 32u) {
                 double x;
-                if (!wait_pair(bslot1(p, rnd, w2, v), rnd, p.ctl, &x)) { ok = false; break; }
+                if (!wait_pair(p.xchg1, bslot1(p, rnd, w2, v), rnd, p.ctl, &x)) { ok = false; break; }
                 acc += x;
             }
         }
@@ -175,7 +175,7 @@ __device__ __forceinline__ bool breduce(const BatchParams& p, BShared& L, uint32
         __syncthreads();
         if (L.abort) return false;
         if (tid < 8 && v < V)
-            publish_pair(bslot2(p, rnd, v), rnd, ((L.red[0][tid] + L.red[1][tid]) + L.red[2][tid]) + L.red[3][tid]);
+            publish_pair(p.xchg2, bslot2(p, rnd, v), rnd, ((L.red[0][tid] + L.red[1][tid]) + L.red[2][tid]) + L.red[3][tid]);
         __syncthreads();
     }
     return true;
@@ -188,7 +188,7 @@ __device__ __forceinline__ bool ballreduce_traits(const BatchParams& p, BShared&
     if (!breduce(p, L, rnd, p.T, w, tid)) return false;
     if (tid < (int)p.T) {
         double x;
-        if (!wait_pair(bslot2(p, rnd, (uint32_t)tid), rnd, p.ctl, &x)) { L.abort = 1; raise_abort(p, 12); }
+        if (!wait_pair(p.xchg2, bslot2(p, rnd, (uint32_t)tid), rnd, p.ctl, &x)) { L.abort = 1; raise_abort(p, 12); }
         else L.tvec[tid] = x;
     }
     __syncthreads();
@@ -478,7 +478,7 @@ bayesr_batch_kernel(BatchParams p)
                 part += __shfl_xor(part, 32, 64);
                 ++rnd;
                 ++n_rounds;
-                if (kq == 0 && tval) publish_pair(bslot1(p, rnd, w, (uint32_t)tl), rnd, part);
+                if (kq == 0 && tval) publish_pair(p.xchg1, bslot1(p, rnd, w, (uint32_t)tl), rnd, part);
                 if (!ballreduce_traits(p, L, rnd, w, tid)) goto done;
             } else {
                 __syncthreads();
@@ -605,9 +605,9 @@ bayesr_batch_kernel(BatchParams p)
 #pragma unroll
                         for (int r = 0; r < 4; ++r) {
                             const int k0 = kq + 4 * r;
-                            if (k0 < nb) publish_pair(bslot1(p, rnd, w, (uint32_t)(k0 * T + tl)), rnd, c0[r]);
+                            if (k0 < nb) publish_pair(p.xchg1, bslot1(p, rnd, w, (uint32_t)(k0 * T + tl)), rnd, c0[r]);
                             if (16 + k0 < nb)
-                                publish_pair(bslot1(p, rnd, w, (uint32_t)((16 + k0) * T + tl)), rnd, c1[r]);
+                                publish_pair(p.xchg1, bslot1(p, rnd, w, (uint32_t)((16 + k0) * T + tl)), rnd, c1[r]);
                         }
                     }
                 }
@@ -639,7 +639,7 @@ bayesr_batch_kernel(BatchParams p)
                         for (int r = 0; r < 4; ++r) {
                             const int i = 16 * (int)mi + kq + 4 * r, k = 16 * (int)mc + tq;
                             if (i < k && k < nb)
-                                publish_pair(bslot1(p, rnd, w, (uint32_t)(CT + k * (k - 1) / 2 + i)), rnd, g[r]);
+                                publish_pair(p.xchg1, bslot1(p, rnd, w, (uint32_t)(CT + k * (k - 1) / 2 + i)), rnd, g[r]);
                         }
                     }
                 }
@@ -671,7 +671,7 @@ bayesr_batch_kernel(BatchParams p)
                 BRR_BTICK(8);
                 for (uint32_t v = (uint32_t)tid; v < V; v += kThreads) {
                     double xv;
-                    if (!wait_pair(bslot2(p, rnd, v), rnd, p.ctl, &xv)) { L.abort = 1; raise_abort(p, 13); break; }
+                    if (!wait_pair(p.xchg2, bslot2(p, rnd, v), rnd, p.ctl, &xv)) { L.abort = 1; raise_abort(p, 13); break; }
                     if (v < CT) {
                         const uint32_t k = v / (uint32_t)T;
                         L.cd[k][v - k * (uint32_t)T] = xv;
@@ -858,7 +858,7 @@ bayesr_batch_kernel(BatchParams p)
             part += __shfl_xor(part, 32, 64);
             ++rnd;
             ++n_rounds;
-            if (kq == 0 && tval) publish_pair(bslot1(p, rnd, w, (uint32_t)tl), rnd, part);
+            if (kq == 0 && tval) publish_pair(p.xchg1, bslot1(p, rnd, w, (uint32_t)tl), rnd, part);
             if (!ballreduce_traits(p, L, rnd, w, tid)) goto done;
             if (tid < T) {
                 const int t = tid;

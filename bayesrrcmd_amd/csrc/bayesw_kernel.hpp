@@ -558,16 +558,16 @@ __device__ __forceinline__ bool bw_exchange(const BwParams& p, BwShared<R>& L, u
         if ((uint32_t)v % p.P != w) continue;
         if ((cnt++ % kWaves) != wave) continue;
         double acc = 0.0;
-        ok = wait_sum_lanes([&](uint32_t w2) { return bw_slot1(p, rnd, v, w2); }, p.P, rnd, p.ctl, lane, &acc);
+        ok = wait_sum_lanes(p.xchg1, [&](uint32_t w2) { return bw_slot1(p, rnd, v, w2); }, p.P, rnd, p.ctl, lane, &acc);
         if (__any(!ok)) { ok = false; break; }
         acc = wave_sum(acc);
-        if (lane == 0) publish_pair(bw_slot2(p, rnd, v), rnd, acc);
+        if (lane == 0) publish_pair(p.xchg2, bw_slot2(p, rnd, v), rnd, acc);
     }
     if (!ok && lane == 0) { L.abort = 1; raise_abort(p, 21); }
     const int tid = wave * 64 + lane;
     if (tid < V) {
         double x;
-        if (!wait_pair(bw_slot2(p, rnd, tid), rnd, p.ctl, &x)) { L.abort = 1; raise_abort(p, 22); }
+        if (!wait_pair(p.xchg2, bw_slot2(p, rnd, tid), rnd, p.ctl, &x)) { L.abort = 1; raise_abort(p, 22); }
         else L.xv[rnd & 1u][tid] = x;
     }
     __syncthreads();
@@ -657,7 +657,7 @@ bayesw_sweep_kernel(BwParams p)
         part = wave_sum(part);
         ++rnd;
         ++n_rounds;
-        if (tid == 0) publish_pair(bw_slot1(p, rnd, 0, w), rnd, part);
+        if (tid == 0) publish_pair(p.xchg1, bw_slot1(p, rnd, 0, w), rnd, part);
         if (!bw_exchange<R>(p, L, rnd, 1, w, wave, lane)) return false;
         *total = L.xv[rnd & 1u][0];
         return true;
@@ -917,12 +917,12 @@ bayesw_sweep_kernel(BwParams p)
                     const int vv = (lane >> 2) & 15;
                     const int k = pos + wave + kWaves * (vv >> 1);
                     if ((lane & 3) == 0 && k < h)
-                        publish_pair(bw_slot1(p, rnd, 1 + kPer * (k - pos) + (vv & 1), w), rnd, tot);
+                        publish_pair(p.xchg1, bw_slot1(p, rnd, 1 + kPer * (k - pos) + (vv & 1), w), rnd, tot);
                     if constexpr (kDense) {
                         const double t8 = transpose_reduce8(part8, lane);
                         const int k8 = pos + wave + kWaves * ((lane >> 3) & 7);
                         if ((lane & 7) == 0 && k8 < h)
-                            publish_pair(bw_slot1(p, rnd, 1 + kPer * (k8 - pos) + 2, w), rnd, t8);
+                            publish_pair(p.xchg1, bw_slot1(p, rnd, 1 + kPer * (k8 - pos) + 2, w), rnd, t8);
                     }
                     double sv = 0.0;
 #pragma unroll
@@ -930,7 +930,7 @@ bayesw_sweep_kernel(BwParams p)
 #pragma unroll
                         for (int e = 0; e < 8; ++e) sv += vi[r][e];
                     sv = wave_sum(sv);
-                    if (tid == 0) publish_pair(bw_slot1(p, rnd, 0, w), rnd, sv);
+                    if (tid == 0) publish_pair(p.xchg1, bw_slot1(p, rnd, 0, w), rnd, sv);
                     V = V0;
                     if (spec) {
                         // h's sums of vi' = exp(alpha (eps + delta_h) - EulerGamma) (wave 3)
@@ -964,10 +964,10 @@ bayesw_sweep_kernel(BwParams p)
                             s2 = wave_sum(s2);
                             if constexpr (kDense) s3 = wave_sum(s3);
                             if (lane == 0) {
-                                publish_pair(bw_slot1(p, rnd, V0, w), rnd, s0);
-                                publish_pair(bw_slot1(p, rnd, V0 + 1, w), rnd, s1);
-                                publish_pair(bw_slot1(p, rnd, V0 + 2, w), rnd, s2);
-                                if constexpr (kDense) publish_pair(bw_slot1(p, rnd, V0 + 3, w), rnd, s3);
+                                publish_pair(p.xchg1, bw_slot1(p, rnd, V0, w), rnd, s0);
+                                publish_pair(p.xchg1, bw_slot1(p, rnd, V0 + 1, w), rnd, s1);
+                                publish_pair(p.xchg1, bw_slot1(p, rnd, V0 + 2, w), rnd, s2);
+                                if constexpr (kDense) publish_pair(p.xchg1, bw_slot1(p, rnd, V0 + 3, w), rnd, s3);
                             }
                         }
                         V = V0 + 1 + kPer;
@@ -994,10 +994,10 @@ bayesw_sweep_kernel(BwParams p)
                     s2 = wave_sum(s2);
                     if constexpr (kDense) s3 = wave_sum(s3);
                     if (tid == 0) {
-                        publish_pair(bw_slot1(p, rnd, 0, w), rnd, s0);
-                        publish_pair(bw_slot1(p, rnd, 1, w), rnd, s1);
-                        publish_pair(bw_slot1(p, rnd, 2, w), rnd, s2);
-                        if constexpr (kDense) publish_pair(bw_slot1(p, rnd, 3, w), rnd, s3);
+                        publish_pair(p.xchg1, bw_slot1(p, rnd, 0, w), rnd, s0);
+                        publish_pair(p.xchg1, bw_slot1(p, rnd, 1, w), rnd, s1);
+                        publish_pair(p.xchg1, bw_slot1(p, rnd, 2, w), rnd, s2);
+                        if constexpr (kDense) publish_pair(p.xchg1, bw_slot1(p, rnd, 3, w), rnd, s3);
                     }
                     V = 1 + kPer;
                 }

@@ -132,6 +132,7 @@ template <int R>
 struct Shared {
     uint32_t codes[kMaxB][R][32];       // 2-bit PLINK codes, lane-run layout
     double dot[kMaxB];
+    double gram[kMaxB];                 // Gram entries against the round's first in-model slot
     // current block's per-marker constants
     double beta_old[kMaxB];
     double wt[kMaxB][4];                // column value per 2-bit code
@@ -160,7 +161,9 @@ struct Shared {
     double gamma[kMaxF];
     uint32_t forder[kMaxF];
     double sigmaF;
-    int change;
+    int change;                         // first applied slot of the round (nb: none)
+    int change2;                        // second applied slot (nb: none)
+    int next;                           // first undecided slot after the round
     int abort;
 };
 
@@ -227,6 +230,11 @@ __device__ __forceinline__ double transpose_reduce8(double (&v)[8], int lane)
     return x;   // total of value ((lane >> 3) & 7)
 }
 
+template <bool B>
+struct BoolC {
+    static constexpr bool value = B;
+};
+
 // Value of a 2-bit code: two flat selects on 64-bit halves (no control flow).
 __device__ __forceinline__ double wt_of(uint32_t v, double w0, double w1, double w2, double w3)
 {
@@ -237,20 +245,41 @@ __device__ __forceinline__ double wt_of(uint32_t v, double w0, double w1, double
     return bits_as_double(b1 ? hi : lo);
 }
 
-__device__ __forceinline__ void publish_pair(uint64_t* s, uint32_t rnd, double val)
+// A tagged pair is two 64-bit words (round tag << 32 | 32 bits of the value),
+// written and read as one 16-byte buffer access with the cache policy of an
+// agent-scope atomic (sc1). Each word carries the tag, so a reader that sees
+// both tags current has both halves of this round's value.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+constexpr int kCpolDevice = 16;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t xbuf(const uint64_t* base)
+{
+    return __builtin_amdgcn_make_buffer_rsrc((void*)base, 0, 0x7FFFFFFF, 0x00020000);
+}
+
+__device__ __forceinline__ void publish_pair(const uint64_t* base, uint64_t* s, uint32_t rnd, double val)
 {
     const uint64_t b = double_as_bits(val);
-    const uint64_t tag = (uint64_t)rnd << 32;
-    st_agent(s, tag | (b >> 32));
-    st_agent(s + 1, tag | (b & 0xFFFFFFFFull));
+    const u32x4 v = {(uint32_t)(b >> 32), rnd, (uint32_t)b, rnd};
+    __builtin_amdgcn_raw_buffer_store_b128(v, xbuf(base), (uint32_t)((const char*)s - (const char*)base), 0,
+                                           kCpolDevice);
+}
+
+__device__ __forceinline__ void ld_pair(const uint64_t* base, const uint64_t* s, uint64_t& h, uint64_t& l)
+{
+    uint32_t off = (uint32_t)((const char*)s - (const char*)base);
+    asm volatile("" : "+v"(off));   // opaque offset: every poll issues a fresh load
+    const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(xbuf(base), off, 0, kCpolDevice);
+    h = ((uint64_t)v.y << 32) | v.x;
+    l = ((uint64_t)v.w << 32) | v.z;
 }
 
 // Returns false on timeout or abort.
-__device__ __forceinline__ bool wait_pair(const uint64_t* s, uint32_t rnd, const uint32_t* ctl,
+__device__ __forceinline__ bool wait_pair(const uint64_t* base, const uint64_t* s, uint32_t rnd, const uint32_t* ctl,
                                           double* out)
 {
-    uint64_t h = ld_agent(s);
-    uint64_t l = ld_agent(s + 1);
+    uint64_t h, l;
+    ld_pair(base, s, h, l);
     uint64_t spins = 0;
     while ((uint32_t)(h >> 32) != rnd || (uint32_t)(l >> 32) != rnd) {
         if (++spins > kSpinLimit) return false;
@@ -258,8 +287,7 @@ __device__ __forceinline__ bool wait_pair(const uint64_t* s, uint32_t rnd, const
             __hip_atomic_load(ctl + kCtlAbort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
             return false;
         __builtin_amdgcn_s_sleep(1);
-        h = ld_agent(s);
-        l = ld_agent(s + 1);
+        ld_pair(base, s, h, l);
     }
     *out = bits_as_double((h << 32) | (l & 0xFFFFFFFFull));
     return true;
@@ -271,7 +299,7 @@ __device__ __forceinline__ bool wait_pair(const uint64_t* s, uint32_t rnd, const
 // once; only pairs whose tag is stale are polled again. `slot(w2)` gives the
 // pair's address. False on timeout or abort.
 template <int kPer, class SlotFn>
-__device__ __forceinline__ bool wait_sum_strided(SlotFn slot, uint32_t first, uint32_t stride, uint32_t P,
+__device__ __forceinline__ bool wait_sum_strided(const uint64_t* base, SlotFn slot, uint32_t first, uint32_t stride, uint32_t P,
                                                  uint32_t rnd, const uint32_t* ctl, double* acc_io)
 {
     uint64_t h[kPer], l[kPer];
@@ -282,9 +310,7 @@ __device__ __forceinline__ bool wait_sum_strided(SlotFn slot, uint32_t first, ui
         h[q] = 0;
         l[q] = 0;
         if (w2 < P) {
-            const uint64_t* s = slot(w2);
-            h[q] = ld_agent(s);
-            l[q] = ld_agent(s + 1);
+            ld_pair(base, slot(w2), h[q], l[q]);
             live |= 1u << q;
         }
     }
@@ -303,11 +329,7 @@ __device__ __forceinline__ bool wait_sum_strided(SlotFn slot, uint32_t first, ui
         __builtin_amdgcn_s_sleep(1);
 #pragma unroll
         for (int q = 0; q < kPer; ++q)
-            if ((stale >> q) & 1u) {
-                const uint64_t* s = slot(first + stride * q);
-                h[q] = ld_agent(s);
-                l[q] = ld_agent(s + 1);
-            }
+            if ((stale >> q) & 1u) ld_pair(base, slot(first + stride * q), h[q], l[q]);
     }
     double acc = *acc_io;
 #pragma unroll
@@ -319,11 +341,11 @@ __device__ __forceinline__ bool wait_sum_strided(SlotFn slot, uint32_t first, ui
 
 // One lane's share of a P-workgroup reduction: workgroups lane, lane + 64, ...
 template <class SlotFn>
-__device__ __forceinline__ bool wait_sum_lanes(SlotFn slot, uint32_t P, uint32_t rnd, const uint32_t* ctl, int lane,
-                                               double* out)
+__device__ __forceinline__ bool wait_sum_lanes(const uint64_t* base, SlotFn slot, uint32_t P, uint32_t rnd,
+                                               const uint32_t* ctl, int lane, double* out)
 {
     *out = 0.0;
-    return wait_sum_strided<kMaxP / 64>(slot, (uint32_t)lane, 64u, P, rnd, ctl, out);
+    return wait_sum_strided<kMaxP / 64>(base, slot, (uint32_t)lane, 64u, P, rnd, ctl, out);
 }
 
 template <typename Params>
@@ -333,44 +355,61 @@ __device__ __forceinline__ void raise_abort(const Params& p, uint32_t where)
     __hip_atomic_store(p.ctl + kCtlAbortWhere, where, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ __forceinline__ uint64_t* slot1(const SweepParams& p, uint32_t rnd, int k, uint32_t w)
+// Exchange slots of a round: k < kMaxB carries slot k's dot, kMaxB + k its
+// Gram entry against the round's first in-model slot.
+constexpr int kSlots = 2 * kMaxB;
+
+__device__ __forceinline__ uint64_t* slot1(const SweepParams& p, uint32_t rnd, int s, uint32_t w)
 {
-    return p.xchg1 + ((((size_t)(rnd & 1u) * kMaxB + (size_t)k) * p.P + w) * 2u);
+    return p.xchg1 + ((((size_t)(rnd & 1u) * kSlots + (size_t)s) * p.P + w) * 2u);
 }
 
-__device__ __forceinline__ uint64_t* slot2(const SweepParams& p, uint32_t rnd, int k)
+__device__ __forceinline__ uint64_t* slot2(const SweepParams& p, uint32_t rnd, int s)
 {
-    return p.xchg2 + (((size_t)(rnd & 1u) * kMaxB + (size_t)k) * 2u);
+    return p.xchg2 + (((size_t)(rnd & 1u) * kSlots + (size_t)s) * 2u);
 }
 
-// Reduce-scatter / all-gather of the partials of slots [pos, nb) published at
-// round rnd. Slot k is reduced by workgroup k % P. Afterwards L.dot[k] holds
-// the same total in every workgroup. Returns false (in all threads) on abort.
+// Reduce-scatter / all-gather of the partials published at round rnd: the dots
+// of slots [pos, nb) and the Gram entries of slots [gpos, nb). Slot k (both of
+// its values) is reduced by workgroup k % P. A Gram value is published before
+// the dot of its slot, so only the dots are polled; the Gram words are read
+// once the dot has arrived. Afterwards L.dot[k] and L.gram[k] hold the same
+// totals in every workgroup. Returns false (in all threads) on abort.
 template <int R>
 __device__ __forceinline__ bool exchange(const SweepParams& p, Shared<R>& L, uint32_t rnd, int pos, int nb,
-                         uint32_t w, int wave, int lane)
+                         uint32_t w, int wave, int lane, int gpos = kMaxB)
 {
     int cnt = 0;
     bool ok = true;
     for (int k = pos; k < nb; ++k) {
         if ((uint32_t)k % p.P != w) continue;
         if ((cnt++ % kWaves) != wave) continue;
-        double acc = 0.0;
-        ok = wait_sum_lanes([&](uint32_t w2) { return slot1(p, rnd, k, w2); }, p.P, rnd, p.ctl, lane, &acc);
+        double acc = 0.0, gacc = 0.0;
+        ok = wait_sum_lanes(p.xchg1, [&](uint32_t w2) { return slot1(p, rnd, k, w2); }, p.P, rnd, p.ctl, lane, &acc);
+        if (ok && k >= gpos)
+            ok = wait_sum_lanes(p.xchg1, [&](uint32_t w2) { return slot1(p, rnd, kMaxB + k, w2); }, p.P, rnd, p.ctl, lane,
+                                &gacc);
         if (__any(!ok)) { ok = false; break; }
         acc = wave_sum(acc);
-        if (lane == 0) publish_pair(slot2(p, rnd, k), rnd, acc);
+        if (k >= gpos) {
+            gacc = wave_sum(gacc);
+            if (lane == 0) publish_pair(p.xchg2, slot2(p, rnd, kMaxB + k), rnd, gacc);
+        }
+        if (lane == 0) publish_pair(p.xchg2, slot2(p, rnd, k), rnd, acc);
     }
     if (!ok) {
         if (lane == 0) { L.abort = 1; raise_abort(p, 1); }
     }
-    if (wave == 0 && pos + lane < nb) {
-        double x;
-        if (!wait_pair(slot2(p, rnd, pos + lane), rnd, p.ctl, &x)) {
+    const int k = pos + lane;
+    if (wave == 0 && k < nb) {
+        double x, g = 0.0;
+        if (!wait_pair(p.xchg2, slot2(p, rnd, k), rnd, p.ctl, &x) ||
+            (k >= gpos && !wait_pair(p.xchg2, slot2(p, rnd, kMaxB + k), rnd, p.ctl, &g))) {
             L.abort = 1;
             raise_abort(p, 2);
         } else {
-            L.dot[pos + lane] = x;
+            L.dot[k] = x;
+            L.gram[k] = g;
         }
     }
     __syncthreads();
@@ -385,7 +424,7 @@ __device__ __forceinline__ bool allreduce_scalar(const SweepParams& p, Shared<R>
 {
     part = wave_sum(part);
     ++rnd;
-    if (threadIdx.x == 0) publish_pair(slot1(p, rnd, 0, w), rnd, part);
+    if (threadIdx.x == 0) publish_pair(p.xchg1, slot1(p, rnd, 0, w), rnd, part);
     if (!exchange<R>(p, L, rnd, 0, 1, w, wave, lane)) return false;
     *total = L.dot[0];
     return true;
@@ -914,31 +953,74 @@ bayesr_sweep_kernel(SweepParams p)
                 // handles slots pos + q + 4 i, reduced together
                 ++rnd;
                 ++n_rounds;
-                {
-                    double part[16];
+                // the round's first slot already in the model (beta_old != 0) changes
+                // for certain, so the sequential sweep also sums its Gram entries
+                // G[k] = sum_i u_h(x_hi) w_k(x_ki) against the later slots k, with
+                // u_h the residual change per unit dBeta: when slot h is the round's
+                // first change, dot_k + dBeta_h G[k] is slot k's dot after that change
+                int h = nb;
+                if constexpr (!kAsync) {
+                    const unsigned long long im =
+                        __ballot(pos + lane < nb && L.beta_old[pos + lane < nb ? pos + lane : 0] != 0.0);
+                    if (im) h = pos + (int)__ffsll((long long)im) - 1;
+                }
+                const int gpos = h + 1 < nb ? h + 1 : kMaxB;
+                auto dots = [&](auto gram) {
+                    constexpr bool kG = decltype(gram)::value;
+                    double uh[kG ? R : 1][8];
+                    if constexpr (kG) {
+                        double u0, u1, u2, u3;
+                        if (dense) {
+                            u0 = L.wt[h][0]; u1 = L.wt[h][1]; u2 = L.wt[h][2]; u3 = L.wt[h][3];
+                        } else {
+                            // raggedbayesrkernel.cpp:10-30 per unit dBeta
+                            const double mean = L.mstat[h][0], sd = L.mstat[h][1];
+                            u0 = (2.0 - mean) / sd; u1 = 0.0; u2 = (1.0 - mean) / sd; u3 = -mean / sd;
+                        }
+#pragma unroll
+                        for (int r = 0; r < R; ++r) {
+                            const uint32_t bits = L.codes[h][r][lane >> 1] >> (16 * (lane & 1));
+#pragma unroll
+                            for (int e = 0; e < 8; ++e) {
+                                const uint32_t idx2 = lo + (uint32_t)r * kRun + 8u * (uint32_t)lane + (uint32_t)e;
+                                uh[r][e] = idx2 < hi ? wt_of((bits >> (2 * e)) & 3u, u0, u1, u2, u3) : 0.0;
+                            }
+                        }
+                    }
+                    double part[16], gpart[kG ? 16 : 1];
 #pragma unroll
                     for (int i = 0; i < 16; ++i) {
                         const int k = pos + wave + kWaves * i;
-                        double s = 0.0;
+                        double s = 0.0, gs = 0.0;
                         if (k < nb) {
                             const double w0 = L.wt[k][0], w1 = L.wt[k][1], w2 = L.wt[k][2], w3 = L.wt[k][3];
 #pragma unroll
                             for (int r = 0; r < R; ++r) {
                                 const uint32_t bits = L.codes[k][r][lane >> 1] >> (16 * (lane & 1));
 #pragma unroll
-                                for (int e = 0; e < 8; ++e)
-                                    s = fma(wt_of((bits >> (2 * e)) & 3u, w0, w1, w2, w3), eps[r][e], s);
+                                for (int e = 0; e < 8; ++e) {
+                                    const double x = wt_of((bits >> (2 * e)) & 3u, w0, w1, w2, w3);
+                                    s = fma(x, eps[r][e], s);
+                                    if constexpr (kG) gs = fma(x, uh[r][e], gs);
+                                }
                             }
                         }
                         part[i] = s;
+                        if constexpr (kG) gpart[i] = gs;
                     }
-                    const double tot = transpose_reduce16(part, lane);
                     const int i = (lane >> 2) & 15;
                     const int k = pos + wave + kWaves * i;
-                    if ((lane & 3) == 0 && k < nb) publish_pair(slot1(p, rnd, k, w), rnd, tot);
-                }
+                    const double tot = transpose_reduce16(part, lane);
+                    if constexpr (kG) {
+                        const double gt = transpose_reduce16(gpart, lane);
+                        if ((lane & 3) == 0 && k >= gpos && k < nb) publish_pair(p.xchg1, slot1(p, rnd, kMaxB + k, w), rnd, gt);
+                    }
+                    if ((lane & 3) == 0 && k < nb) publish_pair(p.xchg1, slot1(p, rnd, k, w), rnd, tot);
+                };
+                if (gpos < nb) dots(BoolC<true>{});
+                else dots(BoolC<false>{});
                 BRR_TICK(1);
-                if (!exchange<R>(p, L, rnd, pos, nb, w, wave, lane)) goto done;
+                if (!exchange<R>(p, L, rnd, pos, nb, w, wave, lane, gpos)) goto done;
                 BRR_TICK(2);
                 if (first_round && has2 && tid < nb2) L.mk_next2[tid] = jnext2;
                 if (first_round && p.dbg_meta && w == 0 && tid < nb) {
@@ -947,118 +1029,142 @@ bayesr_sweep_kernel(SweepParams p)
                     d[4] = L.mstat[tid][0]; d[5] = L.mstat[tid][1]; d[6] = L.mstat[tid][2]; d[7] = L.epsSum;
                 }
 
-                // ---- decisions and commit (wave 0, lane = slot), BayesRBase.cpp:300-378
+                // ---- decisions and commit (wave 0, lane = slot), BayesRBase.cpp:300-378.
+                // Pass 0 decides slots [pos, nb) on the exchanged dots; when its first
+                // change is slot h (sequential sweep, Gram entries present), pass 1
+                // decides slots (h, nb) again on the dots corrected for that change.
                 if (wave == 0) {
                     const int k = pos + lane;
-                    bool changed = false;
-                    double bnew = 0.0, acum0 = 0.0;
-                    int sel = -1, g = 0;
-                    if (k < nb) {
-                        g = L.gr[k];
-                        const double* gc = L.gc + g * K;
-                        const double* gden = L.gden + g * K;
-                        const double* gbsd = L.gbsd + g * K;
-                        const double beta_old = L.beta_old[k];
-                        double num;
-                        if (dense) {
-                            num = L.dot[k] + beta_old * NM1;
-                        } else {
-                            const double mean = L.mstat[k][0], sd = L.mstat[k][1];
-                            num = beta_old * (dN - 1.0) - mean * L.epsSum / sd + L.dot[k] / sd;
-                        }
-                        double logL[kMaxK], muk[kMaxK];
-                        logL[0] = gc[0];
-                        muk[0] = 0.0;
-#pragma unroll
-                        for (int c = 1; c < kMaxK; ++c) {
-                            muk[c] = 0.0;
-                            logL[c] = 0.0;
-                            if (c < K) {
-                                muk[c] = num / gden[c];
-                                logL[c] = gc[c] + 0.5 * (muk[c] * num) / L.sigmaE;
+                    int start = pos;
+                    double eS = L.epsSum;
+                    double dcorr = 0.0;
+                    int c0 = nb, cfin = nb, c1 = nb;
+                    for (int pass = 0; pass < (kAsync ? 1 : 2); ++pass) {
+                        const bool act = k >= start && k < nb;
+                        bool changed = false;
+                        double bnew = 0.0, acum0 = 0.0, ed = 0.0;
+                        int sel = -1, g = 0;
+                        if (act) {
+                            g = L.gr[k];
+                            const double* gc = L.gc + g * K;
+                            const double* gden = L.gden + g * K;
+                            const double* gbsd = L.gbsd + g * K;
+                            const double beta_old = L.beta_old[k];
+                            const double dotk = pass ? L.dot[k] + dcorr * L.gram[k] : L.dot[k];
+                            double num;
+                            if (dense) {
+                                num = dotk + beta_old * NM1;
+                            } else {
+                                const double mean = L.mstat[k][0], sd = L.mstat[k][1];
+                                num = beta_old * (dN - 1.0) - mean * eS / sd + dotk / sd;
                             }
-                        }
-                        double acum = guard700(logL, K, logL[0]) ? 0.0 : inv_sum_exp(logL, K, logL[0]);
-                        acum0 = acum;
-                        const double uu = L.uu[k];
-                        bnew = beta_old;
-                        bool done_sel = false;
+                            double logL[kMaxK], muk[kMaxK];
+                            logL[0] = gc[0];
+                            muk[0] = 0.0;
 #pragma unroll
-                        for (int c = 0; c < kMaxK; ++c) {
-                            if (c < K && !done_sel) {
-                                if (uu <= acum) {
-                                    sel = c;
-                                    bnew = (c == 0) ? 0.0 : L.zz[k] * gbsd[c] + muk[c];
-                                    done_sel = true;
-                                } else if (c + 1 < K) {
-                                    const double ref = logL[c + 1 < kMaxK ? c + 1 : 0];
-                                    if (!guard700(logL, K, ref)) acum += inv_sum_exp(logL, K, ref);
+                            for (int c = 1; c < kMaxK; ++c) {
+                                muk[c] = 0.0;
+                                logL[c] = 0.0;
+                                if (c < K) {
+                                    muk[c] = num / gden[c];
+                                    logL[c] = gc[c] + 0.5 * (muk[c] * num) / L.sigmaE;
                                 }
                             }
-                        }
-                        changed = (bnew != beta_old);
-                    }
-                    const unsigned long long cm = __ballot(changed);
-                    // sequential sweep: the first change ends the round; asynchronous
-                    // window (pos = 0): every slot decides on the window-start residual
-                    const int c = (cm && !kAsync) ? pos + (int)__ffsll((long long)cm) - 1 : nb;
-                    const int last = (c < nb) ? c : nb - 1;
-                    const bool accepted = k < nb && k <= last;
-                    // v counts (integer-valued doubles: exact in any order)
-                    if (accepted && sel >= 0) atomicAdd(&L.v[g * K + sel], 1.0);
-                    // betasqnG += beta^2 in marker order (BayesRBase.cpp:353)
-                    unsigned long long sq = __ballot(accepted && sel > 0);
-                    while (sq) {
-                        const int b = (int)__ffsll((long long)sq) - 1;
-                        const double bb = __shfl(bnew, b, 64);
-                        const int gb = __shfl(g, b, 64);
-                        if (lane == 0) L.betasqn[gb] += bb * bb;
-                        sq &= sq - 1;
-                    }
-                    if (w == 0 && accepted) {
-                        const uint32_t j = L.mk[k];
-                        st_agent((uint64_t*)(p.beta + j), double_as_bits(bnew));
-                        if (sel >= 0) p.comp[j] = sel;
-                        p.acum[j] = acum0;
-                    }
-                    // residual change per 2-bit code of each slot applied below: the first
-                    // change (sequential) or every change of the window (asynchronous:
-                    // BayesRBase::processColumnAsync, BayesRBase.cpp:394-491, with the
-                    // window's updateGlobal calls, :503-510, in slot order)
-                    const bool apply = k < nb && (kAsync ? changed : k == c);
-                    const double dBeta = apply ? L.beta_old[k] - bnew : 0.0;
-                    if (apply) {
-                        if (dense) {
+                            double acum = guard700(logL, K, logL[0]) ? 0.0 : inv_sum_exp(logL, K, logL[0]);
+                            acum0 = acum;
+                            const double uu = L.uu[k];
+                            bnew = beta_old;
+                            bool done_sel = false;
 #pragma unroll
-                            for (int q = 0; q < 4; ++q) L.dl[k][q] = dBeta * L.wt[k][q];
-                        } else {
-                            // raggedbayesrkernel.cpp:10-30 and sparsebayesrkernel.cpp:29-34
-                            const double mean = L.mstat[k][0], sd = L.mstat[k][1], Zsum = L.mstat[k][2];
-                            const double meanAdjustment = dBeta * mean / sd;
-                            const double oneAdjustment = dBeta / sd;
-                            L.dl[k][3] = -meanAdjustment;
-                            L.dl[k][2] = -meanAdjustment + oneAdjustment;
-                            L.dl[k][0] = -meanAdjustment + 2 * oneAdjustment;
-                            L.dl[k][1] = -meanAdjustment + meanAdjustment;
-                            if (!kAsync) L.epsSum += dBeta * Zsum / sd - dBeta * mean * dN / sd;
-                        }
-                    }
-                    if (kAsync && !dense) {
-                        // the window's epsilonSum updates in slot order (SparseBayesRRG.cpp:94-103)
-                        unsigned long long mset = cm;
-                        while (mset) {
-                            const int b = (int)__ffsll((long long)mset) - 1;
-                            const double db = __shfl(dBeta, b, 64);
-                            if (lane == 0) {
-                                const int kk = pos + b;
-                                const double mean = L.mstat[kk][0], sd = L.mstat[kk][1], Zsum = L.mstat[kk][2];
-                                L.epsSum += db * Zsum / sd - db * mean * dN / sd;
+                            for (int c = 0; c < kMaxK; ++c) {
+                                if (c < K && !done_sel) {
+                                    if (uu <= acum) {
+                                        sel = c;
+                                        bnew = (c == 0) ? 0.0 : L.zz[k] * gbsd[c] + muk[c];
+                                        done_sel = true;
+                                    } else if (c + 1 < K) {
+                                        const double ref = logL[c + 1 < kMaxK ? c + 1 : 0];
+                                        if (!guard700(logL, K, ref)) acum += inv_sum_exp(logL, K, ref);
+                                    }
+                                }
                             }
-                            mset &= mset - 1;
+                            changed = (bnew != beta_old);
+                        }
+                        const unsigned long long cm = __ballot(changed);
+                        // sequential sweep: the first change ends the pass; asynchronous
+                        // window (pos = 0): every slot decides on the window-start residual
+                        const int c = (cm && !kAsync) ? pos + (int)__ffsll((long long)cm) - 1 : nb;
+                        const int last = (c < nb) ? c : nb - 1;
+                        const bool accepted = act && k <= last;
+                        // v counts (integer-valued doubles: exact in any order)
+                        if (accepted && sel >= 0) atomicAdd(&L.v[g * K + sel], 1.0);
+                        // betasqnG += beta^2 in marker order (BayesRBase.cpp:353)
+                        unsigned long long sq = __ballot(accepted && sel > 0);
+                        while (sq) {
+                            const int b = (int)__ffsll((long long)sq) - 1;
+                            const double bb = __shfl(bnew, b, 64);
+                            const int gb = __shfl(g, b, 64);
+                            if (lane == 0) L.betasqn[gb] += bb * bb;
+                            sq &= sq - 1;
+                        }
+                        if (w == 0 && accepted) {
+                            const uint32_t j = L.mk[k];
+                            st_agent((uint64_t*)(p.beta + j), double_as_bits(bnew));
+                            if (sel >= 0) p.comp[j] = sel;
+                            p.acum[j] = acum0;
+                        }
+                        // residual change per 2-bit code of each slot applied below: the
+                        // pass's first change (sequential) or every change of the window
+                        // (asynchronous: BayesRBase::processColumnAsync, BayesRBase.cpp:394-491,
+                        // with the window's updateGlobal calls, :503-510, in slot order)
+                        const bool apply = act && (kAsync ? changed : k == c);
+                        const double dBeta = apply ? L.beta_old[k] - bnew : 0.0;
+                        if (apply) {
+                            if (dense) {
+#pragma unroll
+                                for (int q = 0; q < 4; ++q) L.dl[k][q] = dBeta * L.wt[k][q];
+                            } else {
+                                // raggedbayesrkernel.cpp:10-30 and sparsebayesrkernel.cpp:29-34
+                                const double mean = L.mstat[k][0], sd = L.mstat[k][1], Zsum = L.mstat[k][2];
+                                const double meanAdjustment = dBeta * mean / sd;
+                                const double oneAdjustment = dBeta / sd;
+                                L.dl[k][3] = -meanAdjustment;
+                                L.dl[k][2] = -meanAdjustment + oneAdjustment;
+                                L.dl[k][0] = -meanAdjustment + 2 * oneAdjustment;
+                                L.dl[k][1] = -meanAdjustment + meanAdjustment;
+                                ed = dBeta * Zsum / sd - dBeta * mean * dN / sd;
+                            }
+                        }
+                        if constexpr (kAsync) {
+                            if (!dense) {
+                                // the window's epsilonSum updates in slot order (SparseBayesRRG.cpp:94-103)
+                                unsigned long long mset = cm;
+                                while (mset) {
+                                    const int b = (int)__ffsll((long long)mset) - 1;
+                                    eS = eS + __shfl(ed, b, 64);
+                                    mset &= mset - 1;
+                                }
+                            }
+                            if (lane == 0) L.cmask = cm << pos;
+                        } else if (!dense && c < nb) {
+                            eS = eS + __shfl(ed, c - pos, 64);
+                        }
+                        cfin = c;
+                        if (pass == 0) {
+                            c0 = c;
+                            if (c != h || gpos >= nb) break;
+                            dcorr = __shfl(dBeta, c - pos, 64);
+                            start = c + 1;
+                        } else {
+                            c1 = c;
                         }
                     }
-                    if (kAsync && lane == 0) L.cmask = cm << pos;
-                    if (lane == 0) L.change = c;
+                    if (lane == 0) {
+                        L.epsSum = eS;
+                        L.change = c0;
+                        L.change2 = c1;
+                        L.next = cfin < nb ? cfin + 1 : nb;
+                    }
                 }
                 __syncthreads();
                 BRR_TICK(3);
@@ -1068,9 +1174,8 @@ bayesr_sweep_kernel(SweepParams p)
                     // stored once block blk+1's segments have been consumed
                     if (!kCodes && has2 && tid < 3 * nb2) seg_issue(p, L.mk_next2[tid / 3], tid % 3, w, st2, ln2);
                 }
-                const int c = L.change;
-                const int last = (c < nb) ? c : nb - 1;
-                // the applied slots: every change of an asynchronous window, else slot c
+                // the applied slots: every change of an asynchronous window, else the
+                // changes of the round's passes in slot order
                 auto apply_slot = [&](int k) {
                     ++n_changes;
                     const double d0 = L.dl[k][0], d1 = L.dl[k][1], d2 = L.dl[k][2], d3 = L.dl[k][3];
@@ -1091,10 +1196,12 @@ bayesr_sweep_kernel(SweepParams p)
                         mset &= mset - 1;
                         apply_slot(k);
                     }
-                } else if (c < nb) {
-                    apply_slot(c);
+                } else {
+                    const int c0 = L.change, c1 = L.change2;
+                    if (c0 < nb) apply_slot(c0);
+                    if (c1 < nb) apply_slot(c1);
                 }
-                pos = last + 1;
+                pos = L.next;
                 BRR_TICK(4);
             }
             // workgroup 0's beta stores must be complete before later rounds
