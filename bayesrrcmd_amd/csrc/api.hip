@@ -626,7 +626,7 @@ int brr_hip_create(const brr_hip_config* cfg, brr_hip_ctx** out)
     c->S = S;
     c->P = (N + S - 1) / S;
     c->x1_words = c->T ? (size_t)2 * c->P * kBVmax * 2 : (size_t)2 * kSlots * c->P * 2;
-    c->x2_words = c->T ? (size_t)2 * kBVmax * 2 : (size_t)2 * kSlots * 2;
+    c->x2_words = c->T ? kCopies * kBCopyWords : kCopies * kCopyWords;
 
     int occ = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, probe, kThreads, 0) != hipSuccess ||
@@ -1303,7 +1303,7 @@ int brr_hip_bw_create(const brr_hip_bw_config* cfg, brr_hip_ctx** out)
     c->P = (N + S - 1) / S;
     c->B = kBwB;
     c->x1_words = (size_t)2 * kBwV * c->P * 2;
-    c->x2_words = (size_t)2 * kBwV * 2;
+    c->x2_words = kCopies * kBwCopyWords;
     for (int dense = 0; dense < 2; ++dense) {
         int occ = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, bw_kernel_for(R, dense != 0), kThreads, 0) !=

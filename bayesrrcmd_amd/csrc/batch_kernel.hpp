@@ -127,9 +127,12 @@ __device__ __forceinline__ uint64_t* bslot1(const BatchParams& p, uint32_t rnd, 
     return p.xchg1 + ((((size_t)(rnd & 1u) * p.P + w) * kBVmax + v) * 2u);
 }
 
-__device__ __forceinline__ uint64_t* bslot2(const BatchParams& p, uint32_t rnd, uint32_t v)
+// totals in kCopies copies, as the BayesR sweep's (copy w % kCopies is read by workgroup w)
+constexpr size_t kBCopyWords = 2 * (size_t)kBVmax * 2 + 32;
+
+__device__ __forceinline__ uint64_t* bslot2(const BatchParams& p, uint32_t rnd, uint32_t v, uint32_t copy)
 {
-    return p.xchg2 + (((size_t)(rnd & 1u) * kBVmax + v) * 2u);
+    return p.xchg2 + copy * kBCopyWords + (((size_t)(rnd & 1u) * kBVmax + v) * 2u);
 }
 
 // Orders the LDS accesses of one wave across its lanes: the wave's LDS
@@ -174,8 +177,11 @@ __device__ __forceinline__ bool breduce(const BatchParams& p, BShared& L, uint32
         if (lane < 8) L.red[wave][lane] = acc;
         __syncthreads();
         if (L.abort) return false;
-        if (tid < 8 && v < V)
-            publish_pair(p.xchg2, bslot2(p, rnd, v), rnd, ((L.red[0][tid] + L.red[1][tid]) + L.red[2][tid]) + L.red[3][tid]);
+        if (tid < 8 * kCopies && v < V) {
+            const int e = tid & 7;
+            publish_pair(p.xchg2, bslot2(p, rnd, v, (uint32_t)(tid >> 3)), rnd,
+                         ((L.red[0][e] + L.red[1][e]) + L.red[2][e]) + L.red[3][e]);
+        }
         __syncthreads();
     }
     return true;
@@ -188,7 +194,7 @@ __device__ __forceinline__ bool ballreduce_traits(const BatchParams& p, BShared&
     if (!breduce(p, L, rnd, p.T, w, tid)) return false;
     if (tid < (int)p.T) {
         double x;
-        if (!wait_pair(p.xchg2, bslot2(p, rnd, (uint32_t)tid), rnd, p.ctl, &x)) { L.abort = 1; raise_abort(p, 12); }
+        if (!wait_pair(p.xchg2, bslot2(p, rnd, (uint32_t)tid, w % kCopies), rnd, p.ctl, &x)) { L.abort = 1; raise_abort(p, 12); }
         else L.tvec[tid] = x;
     }
     __syncthreads();
@@ -671,7 +677,7 @@ bayesr_batch_kernel(BatchParams p)
                 BRR_BTICK(8);
                 for (uint32_t v = (uint32_t)tid; v < V; v += kThreads) {
                     double xv;
-                    if (!wait_pair(p.xchg2, bslot2(p, rnd, v), rnd, p.ctl, &xv)) { L.abort = 1; raise_abort(p, 13); break; }
+                    if (!wait_pair(p.xchg2, bslot2(p, rnd, v, w % kCopies), rnd, p.ctl, &xv)) { L.abort = 1; raise_abort(p, 13); break; }
                     if (v < CT) {
                         const uint32_t k = v / (uint32_t)T;
                         L.cd[k][v - k * (uint32_t)T] = xv;

@@ -539,9 +539,12 @@ __device__ __forceinline__ uint64_t* bw_slot1(const BwParams& p, uint32_t rnd, i
     return p.xchg1 + ((((size_t)(rnd & 1u) * kBwV + (size_t)v) * p.P + w) * 2u);
 }
 
-__device__ __forceinline__ uint64_t* bw_slot2(const BwParams& p, uint32_t rnd, int v)
+// totals in kCopies copies, as the BayesR sweep's (copy w % kCopies is read by workgroup w)
+constexpr size_t kBwCopyWords = 2 * kBwV * 2 + 32;
+
+__device__ __forceinline__ uint64_t* bw_slot2(const BwParams& p, uint32_t rnd, int v, uint32_t copy)
 {
-    return p.xchg2 + (((size_t)(rnd & 1u) * kBwV + (size_t)v) * 2u);
+    return p.xchg2 + copy * kBwCopyWords + (((size_t)(rnd & 1u) * kBwV + (size_t)v) * 2u);
 }
 
 // Values [0, V) published at round rnd: value v is reduced by workgroup v % P
@@ -561,13 +564,13 @@ __device__ __forceinline__ bool bw_exchange(const BwParams& p, BwShared<R>& L, u
         ok = wait_sum_lanes(p.xchg1, [&](uint32_t w2) { return bw_slot1(p, rnd, v, w2); }, p.P, rnd, p.ctl, lane, &acc);
         if (__any(!ok)) { ok = false; break; }
         acc = wave_sum(acc);
-        if (lane == 0) publish_pair(p.xchg2, bw_slot2(p, rnd, v), rnd, acc);
+        if (lane < kCopies) publish_pair(p.xchg2, bw_slot2(p, rnd, v, (uint32_t)lane), rnd, acc);
     }
     if (!ok && lane == 0) { L.abort = 1; raise_abort(p, 21); }
     const int tid = wave * 64 + lane;
     if (tid < V) {
         double x;
-        if (!wait_pair(p.xchg2, bw_slot2(p, rnd, tid), rnd, p.ctl, &x)) { L.abort = 1; raise_abort(p, 22); }
+        if (!wait_pair(p.xchg2, bw_slot2(p, rnd, tid, w % kCopies), rnd, p.ctl, &x)) { L.abort = 1; raise_abort(p, 22); }
         else L.xv[rnd & 1u][tid] = x;
     }
     __syncthreads();

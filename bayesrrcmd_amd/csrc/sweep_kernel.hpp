@@ -364,9 +364,16 @@ __device__ __forceinline__ uint64_t* slot1(const SweepParams& p, uint32_t rnd, i
     return p.xchg1 + ((((size_t)(rnd & 1u) * kSlots + (size_t)s) * p.P + w) * 2u);
 }
 
-__device__ __forceinline__ uint64_t* slot2(const SweepParams& p, uint32_t rnd, int s)
+// Each slot total is published in kCopies copies (copy c read by the
+// workgroups w with w % kCopies == c, one XCD's worth under the round-robin
+// dispatch), so that no single address is polled by every workgroup. Copies
+// are kCopyWords apart, 256 bytes past a 4 KB multiple.
+constexpr int kCopies = 8;
+constexpr size_t kCopyWords = 2 * kSlots * 2 + 32;
+
+__device__ __forceinline__ uint64_t* slot2(const SweepParams& p, uint32_t rnd, int s, uint32_t copy)
 {
-    return p.xchg2 + (((size_t)(rnd & 1u) * kSlots + (size_t)s) * 2u);
+    return p.xchg2 + copy * kCopyWords + (((size_t)(rnd & 1u) * kSlots + (size_t)s) * 2u);
 }
 
 // Reduce-scatter / all-gather of the partials published at round rnd: the dots
@@ -393,9 +400,9 @@ __device__ __forceinline__ bool exchange(const SweepParams& p, Shared<R>& L, uin
         acc = wave_sum(acc);
         if (k >= gpos) {
             gacc = wave_sum(gacc);
-            if (lane == 0) publish_pair(p.xchg2, slot2(p, rnd, kMaxB + k), rnd, gacc);
+            if (lane < kCopies) publish_pair(p.xchg2, slot2(p, rnd, kMaxB + k, (uint32_t)lane), rnd, gacc);
         }
-        if (lane == 0) publish_pair(p.xchg2, slot2(p, rnd, k), rnd, acc);
+        if (lane < kCopies) publish_pair(p.xchg2, slot2(p, rnd, k, (uint32_t)lane), rnd, acc);
     }
     if (!ok) {
         if (lane == 0) { L.abort = 1; raise_abort(p, 1); }
@@ -403,8 +410,9 @@ __device__ __forceinline__ bool exchange(const SweepParams& p, Shared<R>& L, uin
     const int k = pos + lane;
     if (wave == 0 && k < nb) {
         double x, g = 0.0;
-        if (!wait_pair(p.xchg2, slot2(p, rnd, k), rnd, p.ctl, &x) ||
-            (k >= gpos && !wait_pair(p.xchg2, slot2(p, rnd, kMaxB + k), rnd, p.ctl, &g))) {
+        const uint32_t copy = w % kCopies;
+        if (!wait_pair(p.xchg2, slot2(p, rnd, k, copy), rnd, p.ctl, &x) ||
+            (k >= gpos && !wait_pair(p.xchg2, slot2(p, rnd, kMaxB + k, copy), rnd, p.ctl, &g))) {
             L.abort = 1;
             raise_abort(p, 2);
         } else {
