@@ -841,8 +841,6 @@ bayesr_batch_kernel(BatchParams p)
                 }
                 BRR_BTICK(4);
             }
-            // this workgroup's beta stores complete before later rounds let others read them
-            __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
             BRR_BTICK(10);
 
@@ -852,6 +850,9 @@ bayesr_batch_kernel(BatchParams p)
             __syncthreads();
             BRR_BTICK(0);
         }
+        // other workgroups read this workgroup's beta stores only in the block
+        // loads of the next iteration, which follow its next publish
+        __builtin_amdgcn_s_waitcnt(0);
 
         if (p.mode == kModeChain) {
             // ---- sigmaE, sigmaG, pi draws per trait (BayesRBase.cpp:237-247)

@@ -1177,13 +1177,15 @@ bayesw_sweep_kernel(BwParams p)
                 }
                 __syncthreads();
             }
-            if (w == 0) __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
             if (blk + 1 < nblocks) commit();
             if (blk + 2 < nblocks && tid < kBwB) L.ord[blk & 1u][tid] = noj;   // block blk + 2 has blk's parity
             __syncthreads();
             BW_TICK(0);
         }
+        // other workgroups read workgroup 0's beta stores only in the block loads
+        // of the next iteration, which follow this workgroup's next publish
+        if (w == 0) __builtin_amdgcn_s_waitcnt(0);
 
         if (p.mode == kModeChain) {
             // ---- sampleAlpha (bayeswbase.cpp:707-734): one grid sum per density value

@@ -1212,9 +1212,6 @@ bayesr_sweep_kernel(SweepParams p)
                 pos = L.next;
                 BRR_TICK(4);
             }
-            // workgroup 0's beta stores must be complete before later rounds
-            // let other workgroups read them
-            if (w == 0) __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
 
             // ---- commit the prefetch: block blk+1 becomes current
@@ -1231,6 +1228,9 @@ bayesr_sweep_kernel(SweepParams p)
             }
             BRR_TICK(0);
         }
+        // other workgroups read workgroup 0's beta stores only in the block loads
+        // of the next iteration, which follow this workgroup's next publish
+        if (w == 0) __builtin_amdgcn_s_waitcnt(0);
 
         if (p.mode == kModeChain && p.F) {
             // ---- fixed effects (BayesRBase.cpp:212-235)
