@@ -441,6 +441,34 @@ __global__ void ragged_to_codes_kernel(const int32_t* ones, const uint64_t* ones
     }
 }
 
+// Individuals per workgroup of the single-chain kernels. A lane holds R runs
+// of 8 individuals, so a workgroup costs the same for any slice up to 512 R,
+// while every extra workgroup adds partials to each exchange round: by default
+// the fewest runs per lane (1, 2 or 4) that fit N on min(CUs, kMaxP)
+// workgroups, then the fewest workgroups at that R, with balanced slices (a
+// multiple of 16). An explicit workgroup count spreads N over that many.
+uint32_t lane_run_slice(uint32_t N, uint32_t cus, uint32_t requested)
+{
+    const uint32_t Pmax = std::min(cus, kMaxP);
+    uint32_t P = requested ? std::min(requested, Pmax) : 0u;
+    if (!P) {
+        uint32_t R = (N + kRun * Pmax - 1) / (kRun * Pmax);
+        if (R == 3) R = 4;
+        P = (N + kRun * R - 1) / (kRun * R);
+    }
+    const uint32_t S = (N + P - 1) / P;
+    return (S + kSliceAlign - 1) / kSliceAlign * kSliceAlign;
+}
+
+// The launch shape in the statistics, from context creation on.
+void shape_stats(brr_hip_ctx* c)
+{
+    c->stats.workgroups = c->P;
+    c->stats.block_markers = c->T ? (uint32_t)kBB : c->B;
+    c->stats.slice = c->S;
+    c->stats.storage = c->codes ? 2u : 1u;
+}
+
 int reset_exchange(brr_hip_ctx* c)
 {
     HIP_TRY(c, hipMemsetAsync(c->d_xchg1, 0, sizeof(uint64_t) * c->x1_words, c->stream));
@@ -474,10 +502,7 @@ int launch_fn(brr_hip_ctx* c, const void* fn, void* param)
     c->stats.decision_rounds = counters[8];
     for (int q = 0; q < 6; ++q) c->phase_ms[q] = (double)counters[2 + q] * 1e-5;
     for (int q = 6; q < kBPhases; ++q) c->phase_ms[q] = (double)counters[4 + q] * 1e-5;
-    c->stats.workgroups = c->P;
-    c->stats.block_markers = c->T ? (uint32_t)kBB : c->B;
-    c->stats.slice = c->S;
-    c->stats.storage = c->codes ? 2u : 1u;
+    shape_stats(c);
     if (ctl[kCtlAbort]) {
         reset_exchange(c);
         if (c->bw && ctl[kCtlAbortWhere] >= 30 && ctl[kCtlAbortWhere] <= 33)
@@ -615,6 +640,7 @@ int brr_hip_create(const brr_hip_config* cfg, brr_hip_ctx** out)
         c->B = kBB;
         probe = batch_kernel_for(c->NT, c->layout);
     } else {
+        S = lane_run_slice(N, cus, cfg->workgroups);
         uint32_t R = (S + kRun - 1) / kRun;
         if (R > (uint32_t)kMaxRuns) { delete c; return BRR_HIP_EINVAL; }
         if (R == 3) R = 4;
@@ -625,6 +651,7 @@ int brr_hip_create(const brr_hip_config* cfg, brr_hip_ctx** out)
     }
     c->S = S;
     c->P = (N + S - 1) / S;
+    shape_stats(c);
     c->x1_words = c->T ? (size_t)2 * c->P * kBVmax * 2 : (size_t)2 * kSlots * c->P * 2;
     c->x2_words = c->T ? kCopies * kBCopyWords : kCopies * kCopyWords;
 
@@ -873,6 +900,7 @@ int brr_hip_load_bed(brr_hip_ctx* c, const uint8_t* bed_file, uint64_t file_byte
     if (!hip_ok(hipMemcpy(&bad, d_bad, sizeof(bad), hipMemcpyDeviceToHost), "hipMemcpy")) return BRR_HIP_EDEVICE;
     cleanup();
     c->stats = brr_hip_stats{};
+    shape_stats(c);
     c->stats.kernel_ms = (double)ms_unpack + (double)ms_table;
     c->stats.markers = M;
     std::fill(c->phase_ms, c->phase_ms + kBPhases, 0.0);
@@ -1292,9 +1320,7 @@ int brr_hip_bw_create(const brr_hip_bw_config* cfg, brr_hip_ctx** out)
 
     const uint32_t N = cfg->num_inds, M = cfg->num_snps, K = cfg->num_components;
     const uint32_t cus = (uint32_t)prop.multiProcessorCount;
-    uint32_t Preq = std::min(cfg->workgroups ? std::min(cfg->workgroups, cus) : cus, kMaxP);
-    uint32_t S = (N + Preq - 1) / Preq;
-    S = (S + kSliceAlign - 1) / kSliceAlign * kSliceAlign;
+    const uint32_t S = lane_run_slice(N, cus, cfg->workgroups);
     uint32_t R = (S + kRun - 1) / kRun;
     if (R > (uint32_t)kMaxRuns) { delete c; return BRR_HIP_EINVAL; }
     if (R == 3) R = 4;
@@ -1302,6 +1328,7 @@ int brr_hip_bw_create(const brr_hip_bw_config* cfg, brr_hip_ctx** out)
     c->S = S;
     c->P = (N + S - 1) / S;
     c->B = kBwB;
+    shape_stats(c);
     c->x1_words = (size_t)2 * kBwV * c->P * 2;
     c->x2_words = kCopies * kBwCopyWords;
     for (int dense = 0; dense < 2; ++dense) {
