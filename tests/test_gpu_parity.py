@@ -249,3 +249,25 @@ def test_bad_inputs_raise(gpu_lib):
     with pytest.raises(brr.BrrHipError):
         h.set_groups(np.full(pr.m, 5, np.int32))
     h.close()
+
+
+@pytest.mark.parametrize("n", [300, 3001, 100_000, 200_000])
+def test_default_launch_shape(gpu_lib, n):
+    """Single-chain contexts take the fewest lane runs R, then the fewest workgroups holding N at R runs
+    (slices a multiple of 16, at most 512 R), reported from creation on; an explicit count is honoured."""
+    for make in (lambda **kw: brr.HipSweep(n, 10, 4, 1, brr.LAYOUT_DENSE_2BIT, **kw),
+                 lambda **kw: brr.HipBayesW(n, 10, 4, **kw)):
+        h = make()
+        s = h.stats()
+        P, S = s["workgroups"], s["slice"]
+        R = -(-S // 512)
+        assert S % 16 == 0 and (P - 1) * S < n <= P * S, (P, S)
+        assert P == -(-n // (512 * R)), (P, S)
+        if n <= 100_000:
+            assert R == 1, (P, S)
+        h.close()
+        req = max(7, -(-n // 2048))                  # a lane holds at most 4 runs: 2048 per workgroup
+        h = make(workgroups=req)
+        s = h.stats()
+        assert s["workgroups"] <= req and s["workgroups"] * s["slice"] >= n
+        h.close()
