@@ -157,6 +157,8 @@ struct Shared {
     double gc[kMaxG * kMaxK];           // log pi_k - 0.5 log(logLScale S_k + 1)
     double gden[kMaxG * kMaxK];         // denom_k (index k >= 1)
     double gbsd[kMaxG * kMaxK];         // sqrt(sigmaE / denom_k)
+    double gdg[kMaxG * kMaxK];          // logL_k - logL_0 = gdg_k + num^2 ghk_k
+    double ghk[kMaxG * kMaxK];
     // fixed effects
     double gamma[kMaxF];
     uint32_t forder[kMaxF];
@@ -516,6 +518,8 @@ __device__ __forceinline__ void group_constants(const SweepParams& p, Shared<R>&
             L.gc[t] = log(L.pi[t]);
             L.gden[t] = 0.0;
             L.gbsd[t] = 0.0;
+            L.gdg[t] = 0.0;
+            L.ghk[t] = 0.0;
         } else {
             const double S = L.cva[g * (K - 1) + (k - 1)];
             const double sigmaEOverSigmaG = sigmaE / sigmaG;
@@ -524,6 +528,8 @@ __device__ __forceinline__ void group_constants(const SweepParams& p, Shared<R>&
             L.gden[t] = den;
             L.gc[t] = log(L.pi[t]) - 0.5 * log(logLScale * S + 1.0);
             L.gbsd[t] = __dsqrt_rn(sigmaE / den);
+            L.gdg[t] = L.gc[t] - log(L.pi[g * K]);
+            L.ghk[t] = 0.5 / (den * sigmaE);
         }
     }
 }
@@ -1066,36 +1072,77 @@ bayesr_sweep_kernel(SweepParams p)
                                 const double mean = L.mstat[k][0], sd = L.mstat[k][1];
                                 num = beta_old * (dN - 1.0) - mean * eS / sd + dotk / sd;
                             }
-                            double logL[kMaxK], muk[kMaxK];
-                            logL[0] = gc[0];
-                            muk[0] = 0.0;
+                            // d_c = logL_c - logL_0 = gdg_c + num^2 ghk_c. While every |d_c| <= 350
+                            // no 700 guard can fire and no exponential overflows, and the
+                            // probability terms 1 / sum_i exp(logL_i - logL_c) equal
+                            // exp(d_c) / sum_i exp(d_i): one set of K - 1 exponentials.
+                            // Otherwise the reference's own formulation, guards included.
+                            const double n2 = num * num;
+                            double dd[kMaxK];
+                            dd[0] = 0.0;
+                            bool far = false;
 #pragma unroll
                             for (int c = 1; c < kMaxK; ++c) {
-                                muk[c] = 0.0;
-                                logL[c] = 0.0;
+                                dd[c] = 0.0;
                                 if (c < K) {
-                                    muk[c] = num / gden[c];
-                                    logL[c] = gc[c] + 0.5 * (muk[c] * num) / L.sigmaE;
+                                    dd[c] = L.gdg[g * K + c] + n2 * L.ghk[g * K + c];
+                                    far = far || fabs(dd[c]) > 350.0;
                                 }
                             }
-                            double acum = guard700(logL, K, logL[0]) ? 0.0 : inv_sum_exp(logL, K, logL[0]);
-                            acum0 = acum;
                             const double uu = L.uu[k];
-                            bnew = beta_old;
-                            bool done_sel = false;
+                            if (!far) {
+                                double ee[kMaxK];
+                                double esum = 1.0;
+                                ee[0] = 1.0;
 #pragma unroll
-                            for (int c = 0; c < kMaxK; ++c) {
-                                if (c < K && !done_sel) {
-                                    if (uu <= acum) {
-                                        sel = c;
-                                        bnew = (c == 0) ? 0.0 : L.zz[k] * gbsd[c] + muk[c];
-                                        done_sel = true;
-                                    } else if (c + 1 < K) {
-                                        const double ref = logL[c + 1 < kMaxK ? c + 1 : 0];
-                                        if (!guard700(logL, K, ref)) acum += inv_sum_exp(logL, K, ref);
+                                for (int c = 1; c < kMaxK; ++c) {
+                                    ee[c] = 0.0;
+                                    if (c < K) {
+                                        ee[c] = exp(dd[c]);
+                                        esum += ee[c];
+                                    }
+                                }
+                                const double inv = 1.0 / esum;
+                                double acum = inv;
+                                acum0 = acum;
+#pragma unroll
+                                for (int c = 0; c < kMaxK; ++c) {
+                                    if (c < K && sel < 0) {
+                                        if (uu <= acum) sel = c;
+                                        else if (c + 1 < K) acum += ee[c + 1 < kMaxK ? c + 1 : 0] * inv;
+                                    }
+                                }
+                            } else {
+                                double logL[kMaxK], muk[kMaxK];
+                                logL[0] = gc[0];
+                                muk[0] = 0.0;
+#pragma unroll
+                                for (int c = 1; c < kMaxK; ++c) {
+                                    muk[c] = 0.0;
+                                    logL[c] = 0.0;
+                                    if (c < K) {
+                                        muk[c] = num / gden[c];
+                                        logL[c] = gc[c] + 0.5 * (muk[c] * num) / L.sigmaE;
+                                    }
+                                }
+                                double acum = guard700(logL, K, logL[0]) ? 0.0 : inv_sum_exp(logL, K, logL[0]);
+                                acum0 = acum;
+#pragma unroll
+                                for (int c = 0; c < kMaxK; ++c) {
+                                    if (c < K && sel < 0) {
+                                        if (uu <= acum) {
+                                            sel = c;
+                                        } else if (c + 1 < K) {
+                                            const double ref = logL[c + 1 < kMaxK ? c + 1 : 0];
+                                            if (!guard700(logL, K, ref)) acum += inv_sum_exp(logL, K, ref);
+                                        }
                                     }
                                 }
                             }
+                            // the beta draw of the chosen component (BayesRBase.cpp:343-352)
+                            bnew = beta_old;
+                            if (sel == 0) bnew = 0.0;
+                            else if (sel > 0) bnew = L.zz[k] * gbsd[sel] + num / gden[sel];
                             changed = (bnew != beta_old);
                         }
                         const unsigned long long cm = __ballot(changed);
