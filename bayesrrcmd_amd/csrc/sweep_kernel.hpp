@@ -19,8 +19,12 @@
 // identical bits). Decisions before the first marker whose beta changes are
 // exactly those the one-marker-at-a-time sweep makes, so they are accepted;
 // the changed marker's update is applied to each slice and the remaining
-// markers are re-dotted in the next round. One exchange round per block plus
-// one per beta change replaces one grid barrier per marker.
+// markers are re-dotted in the next round. A marker already in the model
+// changes for certain, so when it is the round's first such marker its Gram
+// entries against the later markers ride along with the dots, and after its
+// change the later markers are decided again within the round. One exchange
+// round per block plus about one per other beta change replaces one grid
+// barrier per marker.
 //
 // The genotype codes and per-marker constants of block b+1 are loaded into
 // registers while block b waits on its exchanges and are written to LDS when
