@@ -142,7 +142,7 @@ struct Shared {
     double wt[kMaxB][4];                // column value per 2-bit code
     double mstat[kMaxB][3];             // ragged: mean, sd, Zsum
     double uu[kMaxB];
-    double zz[kMaxB];
+    uint64_t zw[kMaxB];                // z: its two Philox words (chain), or its bits (sweep)
     uint32_t mk[kMaxB];
     int32_t gr[kMaxB];
     uint32_t mk_next[kMaxB];            // marker ids of the next block
@@ -567,7 +567,8 @@ struct MetaNext {
     int32_t g;
     uint64_t beta_bits;
     double w0, w1, w2, w3;   // dense table by 2-bit code / ragged mean, sd, Zsum in w0..w2
-    double u, z;
+    double u;
+    uint64_t zw;             // z deferred: formed only for a marker drawn into a nonzero class
 };
 
 __device__ __forceinline__ void meta_issue(const SweepParams& p, uint32_t j, uint32_t iteration,
@@ -591,10 +592,12 @@ __device__ __forceinline__ void meta_issue(const SweepParams& p, uint32_t j, uin
         m.w3 = 0.0;
     }
     if (p.mode == kModeChain) {
-        marker_uz(p.seed, iteration, j, p.chain, &m.u, &m.z);
+        const U4 x = draw_block(p.seed, j, iteration, kStreamMarker, p.chain);   // as marker_uz
+        m.u = unit_u53(x.x[0], x.x[1]);
+        m.zw = ((uint64_t)x.x[2] << 32) | x.x[3];
     } else {
         m.u = p.u[j];
-        m.z = p.z[j];
+        m.zw = double_as_bits(p.z[j]);
     }
 }
 
@@ -605,7 +608,7 @@ __device__ __forceinline__ void meta_store(const SweepParams& p, Shared<R>& L, i
     L.gr[t] = m.g;
     L.beta_old[t] = bits_as_double(m.beta_bits);
     L.uu[t] = m.u;
-    L.zz[t] = m.z;
+    L.zw[t] = m.zw;
     // a run-time branch on purpose: with a compile-time one the compiler sinks
     // the prefetch loads of meta_issue towards this commit, and the block then
     // waits for their full latency
@@ -1043,7 +1046,7 @@ bayesr_sweep_kernel(SweepParams p)
                 if (first_round && has2 && tid < nb2) L.mk_next2[tid] = jnext2;
                 if (first_round && p.dbg_meta && w == 0 && tid < nb) {
                     double* d = p.dbg_meta + ((size_t)blk * kMaxB + tid) * 8;
-                    d[0] = L.dot[tid]; d[1] = L.beta_old[tid]; d[2] = L.uu[tid]; d[3] = L.zz[tid];
+                    d[0] = L.dot[tid]; d[1] = L.beta_old[tid]; d[2] = L.uu[tid]; d[3] = bits_as_double(L.zw[tid]);
                     d[4] = L.mstat[tid][0]; d[5] = L.mstat[tid][1]; d[6] = L.mstat[tid][2]; d[7] = L.epsSum;
                 }
 
@@ -1146,7 +1149,12 @@ bayesr_sweep_kernel(SweepParams p)
                             // the beta draw of the chosen component (BayesRBase.cpp:343-352)
                             bnew = beta_old;
                             if (sel == 0) bnew = 0.0;
-                            else if (sel > 0) bnew = L.zz[k] * gbsd[sel] + num / gden[sel];
+                            else if (sel > 0) {
+                                const uint64_t zw = L.zw[k];
+                                const double z = p.mode == kModeChain ? unit_normal((uint32_t)(zw >> 32), (uint32_t)zw)
+                                                                      : bits_as_double(zw);
+                                bnew = z * gbsd[sel] + num / gden[sel];
+                            }
                             changed = (bnew != beta_old);
                         }
                         const unsigned long long cm = __ballot(changed);
