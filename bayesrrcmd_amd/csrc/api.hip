@@ -47,6 +47,7 @@ struct brr_hip_ctx {
     bool codes = false;          // columns held as 2-bit codes on the device
     uint32_t async = 0;          // asynchronous windows of B markers (brr_hip_set_async)
     uint32_t B_seq = 0;          // block size of the sequential sweep
+    double change_rate = 1.0;    // beta changes per marker visit in the last chain launch
     bool have_markers = false;
     bool have_model = false;
     hipStream_t stream = nullptr;
@@ -1188,9 +1189,14 @@ int brr_hip_run_chain(brr_hip_ctx* c, uint64_t seed, uint32_t chain, uint32_t fi
     p.seed = seed;
     p.order = c->d_order;
     p.hist = c->d_hist;
+    // the per-marker normal draw only matters for markers drawn into a nonzero
+    // class: formed at block load while many markers change, else deferred to
+    // the decisions that need it (the same bits either way)
+    p.lazy_z = c->change_rate < 0.05 ? 1u : 0u;
     rc = launch_any(c, p);
     c->stats.markers = total * c->Tn;
     if (rc) return rc;
+    c->change_rate = (double)c->stats.changes / (double)(total * c->Tn);
     if (hist) HIP_TRY(c, hipMemcpy(hist, c->d_hist, sizeof(double) * hsz, hipMemcpyDeviceToHost));
     return BRR_HIP_OK;
 }

@@ -84,7 +84,8 @@ struct SweepParams {
     const float* fixedX;
     double* fixed;
     uint32_t* forder;
-    uint32_t F, pad1;
+    uint32_t F;
+    uint32_t lazy_z;              // chain: form each marker's normal draw only if drawn into a nonzero class
     uint64_t seed;
     // dense arena: column j at dense + j * col_words (32-bit words)
     const uint32_t* dense;
@@ -142,7 +143,7 @@ struct Shared {
     double wt[kMaxB][4];                // column value per 2-bit code
     double mstat[kMaxB][3];             // ragged: mean, sd, Zsum
     double uu[kMaxB];
-    uint64_t zw[kMaxB];                // z: its two Philox words (chain), or its bits (sweep)
+    uint64_t zw[kMaxB];                // z: its two Philox words (chain, lazy_z), or its bits
     uint32_t mk[kMaxB];
     int32_t gr[kMaxB];
     uint32_t mk_next[kMaxB];            // marker ids of the next block
@@ -568,7 +569,7 @@ struct MetaNext {
     uint64_t beta_bits;
     double w0, w1, w2, w3;   // dense table by 2-bit code / ragged mean, sd, Zsum in w0..w2
     double u;
-    uint64_t zw;             // z deferred: formed only for a marker drawn into a nonzero class
+    uint64_t zw;             // z, or with lazy_z its Philox words
 };
 
 __device__ __forceinline__ void meta_issue(const SweepParams& p, uint32_t j, uint32_t iteration,
@@ -594,7 +595,7 @@ __device__ __forceinline__ void meta_issue(const SweepParams& p, uint32_t j, uin
     if (p.mode == kModeChain) {
         const U4 x = draw_block(p.seed, j, iteration, kStreamMarker, p.chain);   // as marker_uz
         m.u = unit_u53(x.x[0], x.x[1]);
-        m.zw = ((uint64_t)x.x[2] << 32) | x.x[3];
+        m.zw = p.lazy_z ? ((uint64_t)x.x[2] << 32) | x.x[3] : double_as_bits(unit_normal(x.x[2], x.x[3]));
     } else {
         m.u = p.u[j];
         m.zw = double_as_bits(p.z[j]);
@@ -1151,8 +1152,9 @@ bayesr_sweep_kernel(SweepParams p)
                             if (sel == 0) bnew = 0.0;
                             else if (sel > 0) {
                                 const uint64_t zw = L.zw[k];
-                                const double z = p.mode == kModeChain ? unit_normal((uint32_t)(zw >> 32), (uint32_t)zw)
-                                                                      : bits_as_double(zw);
+                                const double z = (p.mode == kModeChain && p.lazy_z)
+                                                     ? unit_normal((uint32_t)(zw >> 32), (uint32_t)zw)
+                                                     : bits_as_double(zw);
                                 bnew = z * gbsd[sel] + num / gden[sel];
                             }
                             changed = (bnew != beta_old);
