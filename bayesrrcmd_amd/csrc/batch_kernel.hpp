@@ -87,6 +87,8 @@ struct BShared {
     double cd[kBB][kMaxT];            // dots, corrected dots, then the accepted beta deltas
     double gm[kBB][kBB];              // Gram u_i . w_k, i < k
     double bn[kBB][kMaxT];            // decided beta
+    double acm[kBB];                  // acum of the latest draw of this workgroup's output trait
+                                      // (at most one per block while T <= P)
     int8_t sl[kBB][kMaxT];            // decided component (-1: none selected)
     double uu[kBB][kMaxT];            // uniform of (marker slot, trait), drawn while the exchange runs
     uint64_t zw[kBB][kMaxT];          // its normal: Philox words (chain) or the given z's bits
@@ -706,6 +708,7 @@ bayesr_batch_kernel(BatchParams p)
                 const int tt = 16 * wave + tq;             // trait of this lane's accept work
                 const bool tok = tt < T;
                 const bool mine = tok && ((uint32_t)tt + blk * 7u) % p.P == w;   // output workgroup
+                const bool single_out = p.T <= p.P;   // then a workgroup outputs at most one trait per block
                 uint16_t* plist = L.plist + wave * (kBB * 16);
                 for (bool first = true;; first = false) {
                     const int p0 = tok ? L.pos[tt] : nb;
@@ -738,6 +741,7 @@ bayesr_batch_kernel(BatchParams p)
                                               L.uu[k][t], L.zw[k][t], p.mode == kModeChain);
                         L.bn[k][t] = r.bnew;
                         L.sl[k][t] = (int8_t)r.sel;
+                        if (((uint32_t)t + blk * 7u) % p.P == w) L.acm[k] = r.acum0;
                         if (r.bnew != beta_old) atomicMin(&L.fc[t], k);
                     }
                     ++n_drounds;
@@ -753,8 +757,9 @@ bayesr_batch_kernel(BatchParams p)
                                 if (sel >= 0) atomicAdd(&L.v[sel][tt], 1.0);
                                 if (mine) {
                                     const size_t o = (size_t)B.mk[k] * p.T + (size_t)tt;
-                                    const double ac = bdraw(L, tt, K, bnum<kDenseMath>(L, B, k, tt, dN, NM1),
-                                                            B.bo[k][tt], 2.0, 0ull, false).acum0;
+                                    const double ac = single_out ? L.acm[k]
+                                                                 : bdraw(L, tt, K, bnum<kDenseMath>(L, B, k, tt, dN, NM1),
+                                                                         B.bo[k][tt], 2.0, 0ull, false).acum0;
                                     // agent-scope stores: the same slot may be written by
                                     // another workgroup (another XCD's L2) in a later iteration
                                     st_agent((uint64_t*)(p.beta + o), double_as_bits(L.bn[k][tt]));
