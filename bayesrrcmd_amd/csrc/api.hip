@@ -1117,6 +1117,7 @@ int brr_hip_sweep(brr_hip_ctx* c, const uint32_t* order, const double* u, const 
     }
     HIP_TRY(c, hipSetDevice(c->device));
     if (c->order_cap < M) {
+        c->order_cap = 0;
         if ((rc = dalloc(c, &c->d_order, M))) return rc;
         c->order_cap = M;
     }
@@ -1172,11 +1173,13 @@ int brr_hip_run_chain(brr_hip_ctx* c, uint64_t seed, uint32_t chain, uint32_t fi
     }
     HIP_TRY(c, hipSetDevice(c->device));
     if (c->order_cap < total) {
+        c->order_cap = 0;
         if ((rc = dalloc(c, &c->d_order, total))) return rc;
         c->order_cap = total;
     }
     const size_t hsz = (size_t)iterations * c->Tn * (4 + G);
     if (c->hist_cap < hsz) {
+        c->hist_cap = 0;
         if ((rc = dalloc(c, &c->d_hist, hsz))) return rc;
         c->hist_cap = hsz;
     }
@@ -1231,49 +1234,6 @@ int brr_hip_epsilon_change(brr_hip_ctx* c, uint32_t marker, double beta_old, dou
     HIP_TRY(c, hipGetLastError());
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return BRR_HIP_OK;
-}
-
-// Internal check of the block loaders: one sweep with the code maps of every
-// block dumped as each workgroup first uses them ([block][P][64][R*32] words).
-int brr_hip_debug_block_codes(brr_hip_ctx* c, const uint32_t* order, const double* u, const double* z,
-                              uint32_t* out, uint64_t out_words, double* meta)
-{
-    int rc = check_ready(c);
-    if (rc) return rc;
-    if ((rc = need_kind(c, false))) return rc;
-    if (c->T) return fail(c, BRR_HIP_ESTATE, "per-marker calls need a single-chain context");
-    const uint32_t M = c->cfg.num_snps;
-    const uint64_t nblocks = (M + c->B - 1) / c->B;
-    const uint64_t need = nblocks * c->P * kMaxB * c->R * 32;
-    if (!out || out_words < need) return fail(c, BRR_HIP_EINVAL, "debug buffer too small");
-    uint32_t* d = nullptr;
-    if ((rc = dalloc(c, &d, need))) return rc;
-    HIP_TRY(c, hipMemset(d, 0, need * 4));
-    if (c->order_cap < M) {
-        if ((rc = dalloc(c, &c->d_order, M))) { dfree(d); return rc; }
-        c->order_cap = M;
-    }
-    HIP_TRY(c, hipMemcpy(c->d_order, order, sizeof(uint32_t) * M, hipMemcpyHostToDevice));
-    HIP_TRY(c, hipMemcpy(c->d_u, u, sizeof(double) * M, hipMemcpyHostToDevice));
-    HIP_TRY(c, hipMemcpy(c->d_z, z, sizeof(double) * M, hipMemcpyHostToDevice));
-    SweepParams p = base_params(c);
-    p.mode = kModeSweep;
-    p.iterations = 1;
-    p.order = c->d_order;
-    p.u = c->d_u;
-    p.z = c->d_z;
-    p.dbg_codes = d;
-    double* dm = nullptr;
-    const uint64_t mneed = nblocks * kMaxB * 8;
-    if ((rc = dalloc(c, &dm, mneed))) { dfree(d); return rc; }
-    HIP_TRY(c, hipMemset(dm, 0, mneed * 8));
-    p.dbg_meta = dm;
-    rc = launch(c, p);
-    if (rc == 0) HIP_TRY(c, hipMemcpy(out, d, need * 4, hipMemcpyDeviceToHost));
-    if (rc == 0 && meta) HIP_TRY(c, hipMemcpy(meta, dm, mneed * 8, hipMemcpyDeviceToHost));
-    dfree(d);
-    dfree(dm);
-    return rc;
 }
 
 int brr_hip_last_phase_ms(const brr_hip_ctx* c, double* out, uint32_t n)
@@ -1500,6 +1460,7 @@ int brr_hip_bw_sweep(brr_hip_ctx* c, const uint32_t* order, const double* u, uin
     const uint32_t M = c->cfg.num_snps;
     HIP_TRY(c, hipSetDevice(c->device));
     if (c->order_cap < M) {
+        c->order_cap = 0;
         if ((rc = dalloc(c, &c->d_order, M))) return rc;
         c->order_cap = M;
     }
@@ -1545,11 +1506,13 @@ int brr_hip_bw_run_chain(brr_hip_ctx* c, uint64_t seed, uint32_t chain, uint32_t
     const size_t total = (size_t)iterations * M;
     HIP_TRY(c, hipSetDevice(c->device));
     if (c->order_cap < total) {
+        c->order_cap = 0;
         if ((rc = dalloc(c, &c->d_order, total))) return rc;
         c->order_cap = total;
     }
     const size_t hsz = (size_t)iterations * 5;
     if (c->hist_cap < hsz) {
+        c->hist_cap = 0;
         if ((rc = dalloc(c, &c->d_hist, hsz))) return rc;
         c->hist_cap = hsz;
     }

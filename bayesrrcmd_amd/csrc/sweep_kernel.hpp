@@ -116,8 +116,6 @@ struct SweepParams {
     uint64_t* xchg2;              // [2][kMaxB][2] tagged totals
     uint32_t* ctl;                // persistent control words
     unsigned long long* counters; // [0] rounds, [1] changes, [2..7] phase ticks (100 MHz)
-    uint32_t* dbg_codes;          // optional: [block][P][kMaxB][R*32] code maps as first used
-    double* dbg_meta;             // optional: [block][kMaxB][8] workgroup 0's first-round inputs
 };
 
 // Scalar block (doubles): mu, sigmaE, epsSum, sigmaG[G], pi[G*K], v[G*K], betasqn[G]
@@ -963,11 +961,6 @@ bayesr_sweep_kernel(SweepParams p)
             if (has2 && tid < nb2) jnext2 = order[b0 + 2 * p.B + tid];
             BRR_TICK(0);
 
-            if (p.dbg_codes) {
-                const uint32_t per = (uint32_t)R * 32u;
-                for (uint32_t t = tid; t < (uint32_t)nb * per; t += kThreads)
-                    p.dbg_codes[((size_t)blk * p.P + w) * kMaxB * per + t] = (&L.codes[0][0][0])[t];
-            }
             int pos = 0;
             bool first_round = true;
             while (pos < nb) {
@@ -1045,11 +1038,6 @@ bayesr_sweep_kernel(SweepParams p)
                 if (!exchange<R>(p, L, rnd, pos, nb, w, wave, lane, gpos)) goto done;
                 BRR_TICK(2);
                 if (first_round && has2 && tid < nb2) L.mk_next2[tid] = jnext2;
-                if (first_round && p.dbg_meta && w == 0 && tid < nb) {
-                    double* d = p.dbg_meta + ((size_t)blk * kMaxB + tid) * 8;
-                    d[0] = L.dot[tid]; d[1] = L.beta_old[tid]; d[2] = L.uu[tid]; d[3] = bits_as_double(L.zw[tid]);
-                    d[4] = L.mstat[tid][0]; d[5] = L.mstat[tid][1]; d[6] = L.mstat[tid][2]; d[7] = L.epsSum;
-                }
 
                 // ---- decisions and commit (wave 0, lane = slot), BayesRBase.cpp:300-378.
                 // Pass 0 decides slots [pos, nb) on the exchanged dots; when its first

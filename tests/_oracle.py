@@ -449,6 +449,17 @@ class BayesWProblem:
         lib().orc_bw_sum_failure_fixed(self.n, p(self.fixedX, C.c_float), self.F, p(self.fail, C.c_double),
                                        p(self.sumf_fix, C.c_double))
 
+    def sample_mu(self, seed, iteration, chain=0):
+        return lib().orc_bw_sample_mu(C.byref(self.data), C.byref(self.state), seed, iteration, chain)
+
+    def sample_alpha(self, seed, iteration, chain=0):
+        return lib().orc_bw_sample_alpha(C.byref(self.data), C.byref(self.state), p(self.fail, C.c_double),
+                                         seed, iteration, chain)
+
+    def post_sweep(self, seed, iteration, chain=0):
+        lib().orc_bw_post_sweep(C.byref(self.data), C.byref(self.model), C.byref(self.state), seed, iteration,
+                                chain)
+
     def sample_theta(self, f, seed, iteration, chain=0):
         return lib().orc_bw_sample_theta(C.byref(self.data), C.byref(self.state), p(self.fixedX, C.c_float), f,
                                          p(self.theta, C.c_double), p(self.sumf_fix, C.c_double), seed,
