@@ -25,13 +25,14 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _abi
-from ._abi import LAYOUT_DENSE_2BIT, LAYOUT_EIGEN_SPARSE, LAYOUT_RAGGED, STORAGE_AUTO, STORAGE_CODES, STORAGE_LISTS
+from ._abi import (LAYOUT_DENSE_2BIT, LAYOUT_EIGEN_SPARSE, LAYOUT_RAGGED, PRECISION_FP32_REFRESH, PRECISION_FP64,
+                   STORAGE_AUTO, STORAGE_CODES, STORAGE_LISTS)
 
 __all__ = [
     "BrrHipError", "HipSweep", "HipBayesW", "bayesw_init", "bed_file_image", "read_samples_binary", "RaggedArena", "read_ragged_sparsebed", "read_ppbed", "SampleWriter", "sample_due", "build_dense", "build_ragged", "synth_bed", "synth_xb",
     "bayesr_init", "shuffle", "marker_uz", "LAYOUT_DENSE_2BIT", "LAYOUT_RAGGED", "LAYOUT_EIGEN_SPARSE", "lib",
     "build_eigen", "read_eigen_sparsebed",
-    "STORAGE_AUTO", "STORAGE_CODES", "STORAGE_LISTS",
+    "STORAGE_AUTO", "STORAGE_CODES", "STORAGE_LISTS", "PRECISION_FP64", "PRECISION_FP32_REFRESH",
 ]
 
 
@@ -419,6 +420,21 @@ class HipSweep(_DeviceMarkers):
         """ParallelGraph's asynchronous sweep in windows of `window` markers (0: the exact sequential sweep)."""
         self._ck(lib().brr_hip_set_async(self.ctx, int(window)), "brr_hip_set_async")
 
+    def set_precision(self, precision: int, refresh_markers: int = 1024):
+        """PRECISION_FP64 (default) or PRECISION_FP32_REFRESH: FP32 residual slices and dots with
+        compensated wave reductions, the FP64 residual brought up to date every refresh_markers."""
+        self._ck(lib().brr_hip_set_precision(self.ctx, int(precision), int(refresh_markers)),
+                 "brr_hip_set_precision")
+
+    def record_num(self, enable: bool = True):
+        """Record each marker's num (computeNum) in later sweeps; read with get_num()."""
+        self._ck(lib().brr_hip_record_num(self.ctx, int(bool(enable))), "brr_hip_record_num")
+
+    def get_num(self) -> np.ndarray:
+        out = np.zeros(self.m)
+        self._ck(lib().brr_hip_get_num(self.ctx, _p(out, C.c_double)), "brr_hip_get_num")
+        return out
+
     def set_groups(self, group: np.ndarray):
         g = np.ascontiguousarray(group, dtype=np.int32)
         self._ck(lib().brr_hip_set_groups(self.ctx, _p(g, C.c_int32)), "brr_hip_set_groups")
@@ -503,7 +519,7 @@ class HipSweep(_DeviceMarkers):
     def epsilon_change(self, marker: int, beta_old: float, beta: float):
         self._ck(lib().brr_hip_epsilon_change(self.ctx, marker, beta_old, beta), "brr_hip_epsilon_change")
 
-    PHASES = ("block_load", "dots", "exchange", "decisions", "commit_update", "iteration_scalars")
+    PHASES = ("block_load", "dots", "exchange", "decisions", "commit_update", "fp64_refresh_and_iteration_scalars")
     BATCH_PHASES = ("block_commit", "dot_tiles", "exchange_gather", "decision_draws", "residual_update",
                     "iteration_scalars", "gram_tiles", "uz_draws", "exchange_reduce", "accept_correct",
                     "block_end_barrier")

@@ -15,6 +15,7 @@ LAYOUT_DENSE_2BIT = 1
 LAYOUT_RAGGED = 2
 LAYOUT_EIGEN_SPARSE = 3
 STORAGE_AUTO, STORAGE_LISTS, STORAGE_CODES = 0, 1, 2
+PRECISION_FP64, PRECISION_FP32_REFRESH = 0, 1
 
 P_u8 = C.POINTER(C.c_uint8)
 P_i32 = C.POINTER(C.c_int32)
@@ -73,6 +74,9 @@ SIGNATURES = {
     "brr_hip_load_bed": (C.c_int, [CTX, P_u8, C.c_uint64, C.c_int]),
     "brr_hip_get_marker_tables": (C.c_int, [CTX, P_f64]),
     "brr_hip_set_async": (C.c_int, [CTX, C.c_uint32]),
+    "brr_hip_set_precision": (C.c_int, [CTX, C.c_int, C.c_uint32]),
+    "brr_hip_record_num": (C.c_int, [CTX, C.c_int]),
+    "brr_hip_get_num": (C.c_int, [CTX, P_f64]),
     "brr_hip_set_fixed": (C.c_int, [CTX, C.POINTER(C.c_float), C.c_uint32]),
     "brr_hip_set_fixed_state": (C.c_int, [CTX, P_f64, P_u32, C.c_double]),
     "brr_hip_get_fixed_state": (C.c_int, [CTX, P_f64, P_u32, C.POINTER(C.c_double)]),

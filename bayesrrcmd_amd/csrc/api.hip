@@ -22,6 +22,7 @@ namespace brr {
 // instantiated in kern_sweep_seq.hip, kern_sweep_async.hip, kern_batch.hip, kern_bayesw.hip
 BRR_SWEEP_KERNELS_OF(BRR_SWEEP_DECL, false)
 BRR_SWEEP_KERNELS_OF(BRR_SWEEP_DECL, true)
+BRR_SWEEP_F32_KERNELS(BRR_SWEEP_DECL)
 BRR_BATCH_KERNELS(BRR_BATCH_DECL)
 BRR_BW_KERNELS(BRR_BW_DECL)
 } // namespace brr
@@ -47,6 +48,10 @@ struct brr_hip_ctx {
     bool codes = false;          // columns held as 2-bit codes on the device
     uint32_t async = 0;          // asynchronous windows of B markers (brr_hip_set_async)
     uint32_t B_seq = 0;          // block size of the sequential sweep
+    bool f32 = false;            // FP32 residual with FP64 refresh (brr_hip_set_precision)
+    double* d_num = nullptr;     // per-marker num of the last sweep (brr_hip_record_num)
+    bool record_num = false;
+    uint32_t refresh_markers = 1024;
     double change_rate = 1.0;    // beta changes per marker visit in the last chain launch
     bool have_markers = false;
     bool have_model = false;
@@ -257,6 +262,7 @@ __global__ void epsilon_change_kernel(SweepParams p, uint32_t j, double beta_old
 SweepParams base_params(brr_hip_ctx* c)
 {
     SweepParams p{};
+    p.num_out = c->record_num ? c->d_num : nullptr;
     p.N = c->cfg.num_inds;
     p.M = c->cfg.num_snps;
     p.K = c->cfg.num_components;
@@ -303,25 +309,36 @@ SweepParams base_params(brr_hip_ctx* c)
 template <int R, bool kAsync>
 const void* kernel_ra(int layout, bool codes)
 {
-    if (layout == kLayoutDense) return (const void*)bayesr_sweep_kernel<R, kLayoutDense, true, kAsync>;
+    if (layout == kLayoutDense) return (const void*)bayesr_sweep_kernel<R, kLayoutDense, true, kAsync, false>;
     if (layout == kLayoutEigen)   // Eigen contexts hold codes
-        return codes ? (const void*)bayesr_sweep_kernel<R, kLayoutEigen, true, kAsync> : nullptr;
-    return codes ? (const void*)bayesr_sweep_kernel<R, kLayoutRagged, true, kAsync>
-                 : (const void*)bayesr_sweep_kernel<R, kLayoutRagged, false, kAsync>;
+        return codes ? (const void*)bayesr_sweep_kernel<R, kLayoutEigen, true, kAsync, false> : nullptr;
+    return codes ? (const void*)bayesr_sweep_kernel<R, kLayoutRagged, true, kAsync, false>
+                 : (const void*)bayesr_sweep_kernel<R, kLayoutRagged, false, kAsync, false>;
+}
+
+// FP32 residual kernels: sequential sweeps over 2-bit codes only
+template <int R>
+const void* kernel_f32(int layout, bool codes)
+{
+    if (!codes) return nullptr;
+    if (layout == kLayoutDense) return (const void*)bayesr_sweep_kernel<R, kLayoutDense, true, false, true>;
+    if (layout == kLayoutEigen) return (const void*)bayesr_sweep_kernel<R, kLayoutEigen, true, false, true>;
+    return (const void*)bayesr_sweep_kernel<R, kLayoutRagged, true, false, true>;
 }
 
 template <int R>
-const void* kernel_r(int layout, bool codes, bool async)
+const void* kernel_r(int layout, bool codes, bool async, bool f32)
 {
+    if (f32) return async ? nullptr : kernel_f32<R>(layout, codes);
     return async ? kernel_ra<R, true>(layout, codes) : kernel_ra<R, false>(layout, codes);
 }
 
-const void* kernel_for(uint32_t R, int layout, bool codes, bool async = false)
+const void* kernel_for(uint32_t R, int layout, bool codes, bool async = false, bool f32 = false)
 {
     switch (R) {
-    case 1: return kernel_r<1>(layout, codes, async);
-    case 2: return kernel_r<2>(layout, codes, async);
-    case 4: return kernel_r<4>(layout, codes, async);
+    case 1: return kernel_r<1>(layout, codes, async, f32);
+    case 2: return kernel_r<2>(layout, codes, async, f32);
+    case 4: return kernel_r<4>(layout, codes, async, f32);
     default: return nullptr;
     }
 }
@@ -520,7 +537,11 @@ int launch_fn(brr_hip_ctx* c, const void* fn, void* param)
 
 int launch(brr_hip_ctx* c, SweepParams& p)
 {
-    return launch_fn(c, kernel_for(c->R, c->layout, c->codes, c->async != 0), &p);
+    p.refresh_blocks = std::max(1u, c->refresh_markers / c->B);
+    if (c->f32 && !c->codes)
+        return fail(c, BRR_HIP_ESTATE, "the FP32 residual mode needs the columns held as 2-bit codes "
+                                       "(ragged_storage 0 or 2)");
+    return launch_fn(c, kernel_for(c->R, c->layout, c->codes, c->async != 0, c->f32), &p);
 }
 
 // Batched contexts: the same launch with the sweep-mode / chain-mode fields of p.
@@ -706,7 +727,7 @@ void brr_hip_destroy(brr_hip_ctx* c)
     dfree(c->d_eps); dfree(c->d_beta); dfree(c->d_comp); dfree(c->d_acum);
     dfree(c->d_scal); dfree(c->d_hist); dfree(c->d_order); dfree(c->d_u); dfree(c->d_z);
     dfree(c->d_xchg1); dfree(c->d_xchg2); dfree(c->d_ctl); dfree(c->d_counters); dfree(c->d_tmp);
-    dfree(c->d_fail); dfree(c->d_sumf);
+    dfree(c->d_fail); dfree(c->d_sumf); dfree(c->d_num);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -936,8 +957,52 @@ int brr_hip_set_async(brr_hip_ctx* c, uint32_t window)
     if (int k = need_kind(c, false)) return k;
     if (c->T) return fail(c, BRR_HIP_EINVAL, "asynchronous windows need a single-chain context");
     if (window > (uint32_t)kMaxB) return fail(c, BRR_HIP_EINVAL, "window must be 0 (sequential) or 1..64");
+    if (window && c->f32) return fail(c, BRR_HIP_EINVAL, "the FP32 residual mode runs the sequential sweep only");
     c->async = window ? 1u : 0u;
     c->B = window ? window : c->B_seq;
+    return BRR_HIP_OK;
+}
+
+int brr_hip_set_precision(brr_hip_ctx* c, int precision, uint32_t refresh_markers)
+{
+    if (int k = need_kind(c, false)) return k;
+    if (precision == BRR_HIP_PRECISION_FP64) {
+        c->f32 = false;
+        return BRR_HIP_OK;
+    }
+    if (precision != BRR_HIP_PRECISION_FP32_REFRESH) return fail(c, BRR_HIP_EINVAL, "unknown precision");
+    if (c->T) return fail(c, BRR_HIP_EINVAL, "the FP32 residual mode needs a single-chain context");
+    if (c->async) return fail(c, BRR_HIP_EINVAL, "the FP32 residual mode runs the sequential sweep only");
+    if (refresh_markers < 1 || refresh_markers > (uint32_t)kMaxLog)
+        return fail(c, BRR_HIP_EINVAL, "refresh interval must be 1..1024 markers");
+    c->f32 = true;
+    c->refresh_markers = refresh_markers;
+    return BRR_HIP_OK;
+}
+
+int brr_hip_record_num(brr_hip_ctx* c, int enable)
+{
+    if (int k = need_kind(c, false)) return k;
+    if (c->T) return fail(c, BRR_HIP_EINVAL, "num records need a single-chain context");
+    HIP_TRY(c, hipSetDevice(c->device));
+    c->record_num = false;
+    if (!enable) return BRR_HIP_OK;
+    if (!c->d_num) {
+        int rc = dalloc(c, &c->d_num, c->cfg.num_snps);
+        if (rc) return rc;
+    }
+    HIP_TRY(c, hipMemset(c->d_num, 0, sizeof(double) * c->cfg.num_snps));
+    c->record_num = true;
+    return BRR_HIP_OK;
+}
+
+int brr_hip_get_num(brr_hip_ctx* c, double* out)
+{
+    if (int k = need_kind(c, false)) return k;
+    if (!out) return fail(c, BRR_HIP_EINVAL, "null output");
+    if (!c->d_num) return fail(c, BRR_HIP_ESTATE, "num records are off (brr_hip_record_num)");
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipMemcpy(out, c->d_num, sizeof(double) * c->cfg.num_snps, hipMemcpyDeviceToHost));
     return BRR_HIP_OK;
 }
 

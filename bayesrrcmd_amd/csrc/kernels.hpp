@@ -5,12 +5,22 @@
 // addresses for the launches and occupancy queries.
 #pragma once
 
-// bayesr_sweep_kernel<R, layout, codes, async>
+// bayesr_sweep_kernel<R, layout, codes, async, fp32>: FP64 residual
 #define BRR_SWEEP_KERNELS_OF(X, A)                                                                      \
-    X(1, kLayoutDense, true, A) X(1, kLayoutRagged, true, A) X(1, kLayoutRagged, false, A)              \
-    X(1, kLayoutEigen, true, A) X(2, kLayoutDense, true, A) X(2, kLayoutRagged, true, A)               \
-    X(2, kLayoutRagged, false, A) X(2, kLayoutEigen, true, A) X(4, kLayoutDense, true, A)              \
-    X(4, kLayoutRagged, true, A) X(4, kLayoutRagged, false, A) X(4, kLayoutEigen, true, A)
+    X(1, kLayoutDense, true, A, false) X(1, kLayoutRagged, true, A, false)                              \
+    X(1, kLayoutRagged, false, A, false) X(1, kLayoutEigen, true, A, false)                             \
+    X(2, kLayoutDense, true, A, false) X(2, kLayoutRagged, true, A, false)                              \
+    X(2, kLayoutRagged, false, A, false) X(2, kLayoutEigen, true, A, false)                             \
+    X(4, kLayoutDense, true, A, false) X(4, kLayoutRagged, true, A, false)                              \
+    X(4, kLayoutRagged, false, A, false) X(4, kLayoutEigen, true, A, false)
+
+// FP32 residual with FP64 refresh: sequential sweeps over 2-bit codes
+#define BRR_SWEEP_F32_KERNELS(X)                                                                        \
+    X(1, kLayoutDense, true, false, true) X(1, kLayoutRagged, true, false, true)                        \
+    X(1, kLayoutEigen, true, false, true) X(2, kLayoutDense, true, false, true)                         \
+    X(2, kLayoutRagged, true, false, true) X(2, kLayoutEigen, true, false, true)                        \
+    X(4, kLayoutDense, true, false, true) X(4, kLayoutRagged, true, false, true)                        \
+    X(4, kLayoutEigen, true, false, true)
 
 // bayesr_batch_kernel<NT, dense>
 #define BRR_BATCH_KERNELS(X) X(4, true) X(4, false) X(8, true) X(8, false) X(13, true) X(13, false) \
@@ -19,8 +29,8 @@
 // bayesw_sweep_kernel<R, dense>
 #define BRR_BW_KERNELS(X) X(1, true) X(1, false) X(2, true) X(2, false) X(4, true) X(4, false)
 
-#define BRR_SWEEP_INST(R, L, C, A) template __global__ void bayesr_sweep_kernel<R, L, C, A>(SweepParams);
-#define BRR_SWEEP_DECL(R, L, C, A) extern template __global__ void bayesr_sweep_kernel<R, L, C, A>(SweepParams);
+#define BRR_SWEEP_INST(R, L, C, A, F) template __global__ void bayesr_sweep_kernel<R, L, C, A, F>(SweepParams);
+#define BRR_SWEEP_DECL(R, L, C, A, F) extern template __global__ void bayesr_sweep_kernel<R, L, C, A, F>(SweepParams);
 #define BRR_BATCH_INST(NT, D) template __global__ void bayesr_batch_kernel<NT, D>(BatchParams);
 #define BRR_BATCH_DECL(NT, D) extern template __global__ void bayesr_batch_kernel<NT, D>(BatchParams);
 #define BRR_BW_INST(R, D) template __global__ void bayesw_sweep_kernel<R, D>(BwParams);

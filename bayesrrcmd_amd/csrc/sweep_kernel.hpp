@@ -36,6 +36,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "brr_rng.h"
 
 namespace brr {
@@ -86,6 +88,7 @@ struct SweepParams {
     uint32_t* forder;
     uint32_t F;
     uint32_t lazy_z;              // chain: form each marker's normal draw only if drawn into a nonzero class
+    uint32_t refresh_blocks;      // FP32 residual: FP64 refresh after every this many blocks (>= 1)
     uint64_t seed;
     // dense arena: column j at dense + j * col_words (32-bit words)
     const uint32_t* dense;
@@ -110,6 +113,7 @@ struct SweepParams {
     double* beta;                 // M (agent-coherent words)
     int32_t* comp;                // M
     double* acum;                 // M
+    double* num_out;              // M or null: the num of every marker's decision (computeNum)
     double* scal;                 // scalar block, see ScalarLayout
     double* hist;                 // iterations x (4 + G) or null
     uint64_t* xchg1;              // [2][kMaxB][P][2] tagged partials
@@ -249,6 +253,100 @@ __device__ __forceinline__ double wt_of(uint32_t v, double w0, double w1, double
     const uint64_t hi = b0 ? double_as_bits(w3) : double_as_bits(w2);
     return bits_as_double(b1 ? hi : lo);
 }
+
+// The same for an FP32 column table.
+__device__ __forceinline__ float wt_of_f(uint32_t v, float w0, float w1, float w2, float w3)
+{
+    const bool b0 = (v & 1u) != 0;
+    const bool b1 = (v & 2u) != 0;
+    const float lo = b0 ? w1 : w0;
+    const float hi = b0 ? w3 : w2;
+    return b1 ? hi : lo;
+}
+
+__device__ __forceinline__ float wt_t(uint32_t v, float w0, float w1, float w2, float w3)
+{
+    return wt_of_f(v, w0, w1, w2, w3);
+}
+__device__ __forceinline__ double wt_t(uint32_t v, double w0, double w1, double w2, double w3)
+{
+    return wt_of(v, w0, w1, w2, w3);
+}
+
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// Knuth's TwoSum: s + e == a + b exactly (needs IEEE rounding and no
+// contraction, which the build's -ffp-contract=off guarantees).
+__device__ __forceinline__ void two_sum(float a, float b, float& s, float& e)
+{
+    s = a + b;
+    const float bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+// transpose_reduce16 on FP32 partials with compensated additions: every add of
+// the exchange tree is a TwoSum whose rounding error is carried in a second
+// FP32 word and exchanged alongside, so the wave total is hi + lo, accurate to
+// about 2^-48 of the summed magnitudes instead of 2^-24 (Ogita, Rump & Oishi,
+// "Accurate sum and dot product", SIAM J. Sci. Comput. 26, 2005: Sum2).
+template <int kHalf, int kBit>
+__device__ __forceinline__ void transpose_stage_c(float (&hi)[16], float (&lo)[16], int lane)
+{
+    const bool up = (lane & kBit) != 0;
+#pragma unroll
+    for (int i = 0; i < kHalf; ++i) {
+        const float sh = up ? hi[i] : hi[i + kHalf];
+        const float sl = up ? lo[i] : lo[i + kHalf];
+        const float rh = __shfl_xor(sh, kBit, 64);
+        const float rl = __shfl_xor(sl, kBit, 64);
+        const float kh = up ? hi[i + kHalf] : hi[i];
+        const float kl = up ? lo[i + kHalf] : lo[i];
+        float s, e;
+        two_sum(kh, rh, s, e);
+        hi[i] = s;
+        lo[i] = (kl + rl) + e;
+    }
+}
+
+__device__ __forceinline__ double transpose_reduce16_c(float (&v)[16], int lane)
+{
+    float lo[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) lo[i] = 0.f;
+    transpose_stage_c<8, 32>(v, lo, lane);
+    transpose_stage_c<4, 16>(v, lo, lane);
+    transpose_stage_c<2, 8>(v, lo, lane);
+    transpose_stage_c<1, 4>(v, lo, lane);
+    float h = v[0], l = lo[0];
+#pragma unroll
+    for (int off = 2; off > 0; off >>= 1) {
+        const float rh = __shfl_xor(h, off, 64);
+        const float rl = __shfl_xor(l, off, 64);
+        float s, e;
+        two_sum(h, rh, s, e);
+        h = s;
+        l = (l + rl) + e;
+    }
+    return (double)h + (double)l;   // total of value ((lane >> 2) & 15)
+}
+
+// FP32-residual mode: the residual changes applied since the last FP64
+// refresh, in the order they were applied (marker id, residual change per
+// 2-bit code), and a staging area that hands wave 0's refreshed FP64 slice to
+// the other waves.
+constexpr int kMaxLog = 1024;
+template <bool kF32, int R>
+struct RefreshLog {
+    uint32_t j[kMaxLog];
+    double dl[kMaxLog][4];
+    double ed[R * kRun];
+    int n;
+};
+template <int R>
+struct RefreshLog<false, R> {
+    int n;
+};
 
 // A tagged pair is two 64-bit words (round tag << 32 | 32 bits of the value),
 // written and read as one 16-byte buffer access with the cache policy of an
@@ -786,11 +884,20 @@ __device__ __forceinline__ void ragged_store(const SweepParams& p, Shared<R>& L,
 // (SparseBayesRRG formulas with RaggedBayesRKernel's / EigenBayesRKernel's dot).
 // kCodes: columns stored as 2-bit codes (else as ragged index lists).
 // kAsync: each block is one window of the asynchronous sweep.
-template <int R, int kLay, bool kCodes, bool kAsync>
+// kF32: the residual slice is held in FP32 and the dots run in FP32 with
+// compensated wave reductions; an FP64 copy of the residual (p.eps) is brought
+// up to date every p.refresh_blocks blocks and at every iteration boundary by
+// replaying the logged residual changes in FP64, and the FP32 copy is then
+// re-rounded from it. The decisions, the scalar draws, updateMu, the fixed
+// effects and |eps|^2 are FP64 throughout.
+template <int R, int kLay, bool kCodes, bool kAsync, bool kF32>
 __global__ void __launch_bounds__(kThreads)
 bayesr_sweep_kernel(SweepParams p)
 {
+    static_assert(!kF32 || kCodes, "the FP32 residual mode reads its columns as 2-bit codes");
+    using E = typename std::conditional<kF32, float, double>::type;
     __shared__ Shared<R> L;
+    __shared__ RefreshLog<kF32, R> RL;
     const uint32_t w = blockIdx.x;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -804,14 +911,103 @@ bayesr_sweep_kernel(SweepParams p)
     constexpr bool dense = kLay == kLayoutDense;
 
     // eps slice in registers: individual lo + 512 r + 8 lane + e
-    double eps[R][8];
+    E eps[R][8];
 #pragma unroll
     for (int r = 0; r < R; ++r)
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             const uint32_t idx = lo + (uint32_t)r * kRun + 8u * (uint32_t)lane + (uint32_t)e;
-            eps[r][e] = (idx < hi) ? p.eps[idx] : 0.0;
+            eps[r][e] = (idx < hi) ? (E)p.eps[idx] : (E)0;
         }
+    if (tid == 0) RL.n = 0;
+
+    // Runs body(ed) on the FP64 residual slice ed[R][8] held by every wave.
+    // FP64 mode: ed is the register slice itself. FP32 mode: wave 0 loads the
+    // FP64 residual of the slice, replays the logged changes on it in order
+    // (the same additions, in the same order, as the FP64 kernel makes), and
+    // hands it to the other waves through LDS; after the body wave 0 stores it
+    // back and every wave re-rounds its FP32 slice from it.
+    auto fp64_region = [&](auto&& body) -> bool {
+        if constexpr (!kF32) {
+            return body(eps);
+        } else {
+            __syncthreads();
+            if (RL.n > kMaxLog) {          // cannot happen: refresh_blocks * B <= kMaxLog
+                if (tid == 0) { L.abort = 1; raise_abort(p, 40); }
+                __syncthreads();
+                return false;
+            }
+            double ed[R][8];
+            if (wave == 0) {
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) {
+                        const uint32_t idx = lo + (uint32_t)r * kRun + 8u * (uint32_t)lane + (uint32_t)e;
+                        ed[r][e] = (idx < hi) ? p.eps[idx] : 0.0;
+                    }
+                const int n = RL.n;
+                constexpr int kChunk = 8;
+                for (int n0 = 0; n0 < n; n0 += kChunk) {
+                    uint32_t wd[kChunk][R];
+#pragma unroll
+                    for (int q = 0; q < kChunk; ++q) {
+                        const uint32_t j = (n0 + q < n) ? RL.j[n0 + q] : 0u;
+#pragma unroll
+                        for (int r = 0; r < R; ++r) {
+                            const uint32_t rem = (uint32_t)r * 32u + (uint32_t)(lane >> 1);
+                            const uint64_t byte_off = (uint64_t)(lo >> 2) + 4ull * rem;
+                            wd[q][r] = (n0 + q < n && byte_off < p.col_bytes)
+                                           ? p.dense[(size_t)j * p.col_words + (lo >> 4) + rem] : 0u;
+                        }
+                    }
+#pragma unroll
+                    for (int q = 0; q < kChunk; ++q) {
+                        if (n0 + q >= n) break;
+                        const double d0 = RL.dl[n0 + q][0], d1 = RL.dl[n0 + q][1];
+                        const double d2 = RL.dl[n0 + q][2], d3 = RL.dl[n0 + q][3];
+#pragma unroll
+                        for (int r = 0; r < R; ++r) {
+                            const uint32_t bits = wd[q][r] >> (16 * (lane & 1));
+#pragma unroll
+                            for (int e = 0; e < 8; ++e) {
+                                const uint32_t idx = lo + (uint32_t)r * kRun + 8u * (uint32_t)lane + (uint32_t)e;
+                                if (idx < hi) ed[r][e] = ed[r][e] + wt_of((bits >> (2 * e)) & 3u, d0, d1, d2, d3);
+                            }
+                        }
+                    }
+                }
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) RL.ed[r * kRun + 8 * lane + e] = ed[r][e];
+            }
+            __syncthreads();
+            if (wave != 0) {
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) ed[r][e] = RL.ed[r * kRun + 8 * lane + e];
+            }
+            __syncthreads();
+            if (tid == 0) RL.n = 0;
+            const bool ok = body(ed);
+            if (ok && wave == 0) {
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) {
+                        const uint32_t idx = lo + (uint32_t)r * kRun + 8u * (uint32_t)lane + (uint32_t)e;
+                        if (idx < hi) p.eps[idx] = ed[r][e];
+                    }
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+#pragma unroll
+                for (int e = 0; e < 8; ++e) eps[r][e] = (float)ed[r][e];
+            return ok;
+        }
+    };
 
     if (tid == 0) {
         L.abort = 0;
@@ -845,46 +1041,53 @@ bayesr_sweep_kernel(SweepParams p)
         const uint32_t iteration = p.iter_base + it;
         const uint32_t* order = p.order + (size_t)((p.mode == kModeChain) ? it : 0u) * p.M;
         if (p.mode == kModeFixed) {
-            if (!fixed_effects<R>(p, L, rnd, n_rounds, eps, iteration, lo, hi, w, wave, lane)) goto done;
+            if (!fp64_region([&](double (&ed)[R][8]) {
+                    return fixed_effects<R>(p, L, rnd, n_rounds, ed, iteration, lo, hi, w, wave, lane);
+                }))
+                goto done;
             break;
         }
 
         if (p.mode == kModeChain) {
             // ---- updateMu (DenseBayesRRmz.cpp:36-42 / SparseBayesRRG.cpp:106-113)
             const double old_mu = L.mu;
-#pragma unroll
-            for (int r = 0; r < R; ++r)
-#pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    const uint32_t idx = lo + (uint32_t)r * kRun + 8u * (uint32_t)lane + (uint32_t)e;
-                    if (idx < hi) eps[r][e] = eps[r][e] + old_mu;
-                }
-            double mean_for_mu;
-            if (dense) {
-                double part = 0.0;
+            double new_mu = 0.0;
+            const bool ok = fp64_region([&](double (&ed)[R][8]) {
 #pragma unroll
                 for (int r = 0; r < R; ++r)
 #pragma unroll
-                    for (int e = 0; e < 8; ++e) part += eps[r][e];
-                double total;
-                if (!allreduce_scalar<R>(p, L, rnd, part, w, wave, lane, &total)) goto done;
-                ++n_rounds;
-                mean_for_mu = total / dN;
-            } else {
-                __syncthreads();
-                if (tid == 0) L.epsSum += old_mu * dN;
-                __syncthreads();
-                mean_for_mu = L.epsSum / dN;
-            }
-            const double new_mu = norm_rng(p.seed, iteration, kDrawMu, p.chain, mean_for_mu,
-                                           L.sigmaE / dN);
+                    for (int e = 0; e < 8; ++e) {
+                        const uint32_t idx = lo + (uint32_t)r * kRun + 8u * (uint32_t)lane + (uint32_t)e;
+                        if (idx < hi) ed[r][e] = ed[r][e] + old_mu;
+                    }
+                double mean_for_mu;
+                if (dense) {
+                    double part = 0.0;
 #pragma unroll
-            for (int r = 0; r < R; ++r)
+                    for (int r = 0; r < R; ++r)
 #pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    const uint32_t idx = lo + (uint32_t)r * kRun + 8u * (uint32_t)lane + (uint32_t)e;
-                    if (idx < hi) eps[r][e] = eps[r][e] - new_mu;
+                        for (int e = 0; e < 8; ++e) part += ed[r][e];
+                    double total;
+                    if (!allreduce_scalar<R>(p, L, rnd, part, w, wave, lane, &total)) return false;
+                    ++n_rounds;
+                    mean_for_mu = total / dN;
+                } else {
+                    __syncthreads();
+                    if (tid == 0) L.epsSum += old_mu * dN;
+                    __syncthreads();
+                    mean_for_mu = L.epsSum / dN;
                 }
+                new_mu = norm_rng(p.seed, iteration, kDrawMu, p.chain, mean_for_mu, L.sigmaE / dN);
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) {
+                        const uint32_t idx = lo + (uint32_t)r * kRun + 8u * (uint32_t)lane + (uint32_t)e;
+                        if (idx < hi) ed[r][e] = ed[r][e] - new_mu;
+                    }
+                return true;
+            });
+            if (!ok) goto done;
             __syncthreads();
             if (tid == 0) {
                 L.mu = new_mu;
@@ -982,7 +1185,7 @@ bayesr_sweep_kernel(SweepParams p)
                 const int gpos = h + 1 < nb ? h + 1 : kMaxB;
                 auto dots = [&](auto gram) {
                     constexpr bool kG = decltype(gram)::value;
-                    double uh[kG ? R : 1][8];
+                    E uh[kG ? R : 1][8];
                     if constexpr (kG) {
                         double u0, u1, u2, u3;
                         if (dense) {
@@ -998,25 +1201,25 @@ bayesr_sweep_kernel(SweepParams p)
 #pragma unroll
                             for (int e = 0; e < 8; ++e) {
                                 const uint32_t idx2 = lo + (uint32_t)r * kRun + 8u * (uint32_t)lane + (uint32_t)e;
-                                uh[r][e] = idx2 < hi ? wt_of((bits >> (2 * e)) & 3u, u0, u1, u2, u3) : 0.0;
+                                uh[r][e] = idx2 < hi ? wt_t((bits >> (2 * e)) & 3u, (E)u0, (E)u1, (E)u2, (E)u3) : (E)0;
                             }
                         }
                     }
-                    double part[16], gpart[kG ? 16 : 1];
+                    E part[16], gpart[kG ? 16 : 1];
 #pragma unroll
                     for (int i = 0; i < 16; ++i) {
                         const int k = pos + wave + kWaves * i;
-                        double s = 0.0, gs = 0.0;
+                        E s = 0, gs = 0;
                         if (k < nb) {
-                            const double w0 = L.wt[k][0], w1 = L.wt[k][1], w2 = L.wt[k][2], w3 = L.wt[k][3];
+                            const E w0 = (E)L.wt[k][0], w1 = (E)L.wt[k][1], w2 = (E)L.wt[k][2], w3 = (E)L.wt[k][3];
 #pragma unroll
                             for (int r = 0; r < R; ++r) {
                                 const uint32_t bits = L.codes[k][r][lane >> 1] >> (16 * (lane & 1));
 #pragma unroll
                                 for (int e = 0; e < 8; ++e) {
-                                    const double x = wt_of((bits >> (2 * e)) & 3u, w0, w1, w2, w3);
-                                    s = fma(x, eps[r][e], s);
-                                    if constexpr (kG) gs = fma(x, uh[r][e], gs);
+                                    const E x = wt_t((bits >> (2 * e)) & 3u, w0, w1, w2, w3);
+                                    s = fma_t(x, eps[r][e], s);
+                                    if constexpr (kG) gs = fma_t(x, uh[r][e], gs);
                                 }
                             }
                         }
@@ -1025,9 +1228,13 @@ bayesr_sweep_kernel(SweepParams p)
                     }
                     const int i = (lane >> 2) & 15;
                     const int k = pos + wave + kWaves * i;
-                    const double tot = transpose_reduce16(part, lane);
+                    auto reduce16 = [&](E (&v)[16]) -> double {
+                        if constexpr (kF32) return transpose_reduce16_c(v, lane);
+                        else return transpose_reduce16(v, lane);
+                    };
+                    const double tot = reduce16(part);
                     if constexpr (kG) {
-                        const double gt = transpose_reduce16(gpart, lane);
+                        const double gt = reduce16(gpart);
                         if ((lane & 3) == 0 && k >= gpos && k < nb) publish_pair(p.xchg1, slot1(p, rnd, kMaxB + k, w), rnd, gt);
                     }
                     if ((lane & 3) == 0 && k < nb) publish_pair(p.xchg1, slot1(p, rnd, k, w), rnd, tot);
@@ -1052,7 +1259,7 @@ bayesr_sweep_kernel(SweepParams p)
                     for (int pass = 0; pass < (kAsync ? 1 : 2); ++pass) {
                         const bool act = k >= start && k < nb;
                         bool changed = false;
-                        double bnew = 0.0, acum0 = 0.0, ed = 0.0;
+                        double bnew = 0.0, acum0 = 0.0, ed = 0.0, numk = 0.0;
                         int sel = -1, g = 0;
                         if (act) {
                             g = L.gr[k];
@@ -1068,6 +1275,7 @@ bayesr_sweep_kernel(SweepParams p)
                                 const double mean = L.mstat[k][0], sd = L.mstat[k][1];
                                 num = beta_old * (dN - 1.0) - mean * eS / sd + dotk / sd;
                             }
+                            numk = num;
                             // d_c = logL_c - logL_0 = gdg_c + num^2 ghk_c. While every |d_c| <= 350
                             // no 700 guard can fire and no exponential overflows, and the
                             // probability terms 1 / sum_i exp(logL_i - logL_c) equal
@@ -1169,6 +1377,7 @@ bayesr_sweep_kernel(SweepParams p)
                             st_agent((uint64_t*)(p.beta + j), double_as_bits(bnew));
                             if (sel >= 0) p.comp[j] = sel;
                             p.acum[j] = acum0;
+                            if (p.num_out) p.num_out[j] = numk;
                         }
                         // residual change per 2-bit code of each slot applied below: the
                         // pass's first change (sequential) or every change of the window
@@ -1235,14 +1444,27 @@ bayesr_sweep_kernel(SweepParams p)
                 // changes of the round's passes in slot order
                 auto apply_slot = [&](int k) {
                     ++n_changes;
-                    const double d0 = L.dl[k][0], d1 = L.dl[k][1], d2 = L.dl[k][2], d3 = L.dl[k][3];
+                    const E d0 = (E)L.dl[k][0], d1 = (E)L.dl[k][1], d2 = (E)L.dl[k][2], d3 = (E)L.dl[k][3];
 #pragma unroll
                     for (int r = 0; r < R; ++r) {
                         const uint32_t bits = L.codes[k][r][lane >> 1] >> (16 * (lane & 1));
 #pragma unroll
                         for (int e = 0; e < 8; ++e) {
                             const uint32_t idx2 = lo + (uint32_t)r * kRun + 8u * (uint32_t)lane + (uint32_t)e;
-                            if (idx2 < hi) eps[r][e] = eps[r][e] + wt_of((bits >> (2 * e)) & 3u, d0, d1, d2, d3);
+                            if (idx2 < hi) eps[r][e] = eps[r][e] + wt_t((bits >> (2 * e)) & 3u, d0, d1, d2, d3);
+                        }
+                    }
+                    if constexpr (kF32) {
+                        // the FP64 replay log (a block never holds more than refresh_blocks * B
+                        // <= kMaxLog changes between two refreshes: one per marker at most)
+                        if (tid == 0) {
+                            const int n = RL.n;
+                            if (n < kMaxLog) {
+                                RL.j[n] = L.mk[k];
+#pragma unroll
+                                for (int q = 0; q < 4; ++q) RL.dl[n][q] = L.dl[k][q];
+                            }
+                            RL.n = n + 1;
                         }
                     }
                 };
@@ -1276,25 +1498,39 @@ bayesr_sweep_kernel(SweepParams p)
                 }
             }
             BRR_TICK(0);
+            if constexpr (kF32) {
+                // FP64 refresh of the residual every refresh_blocks blocks and after the
+                // last block (in a chain the end-of-iteration FP64 region does the last)
+                const bool last = blk + 1 == nblocks;
+                const uint32_t rb = p.refresh_blocks ? p.refresh_blocks : 1u;
+                if ((last && p.mode != kModeChain) || (!last && (blk + 1) % rb == 0)) {
+                    if (!fp64_region([](double (&)[R][8]) { return true; })) goto done;
+                    BRR_TICK(5);
+                }
+            }
         }
         // other workgroups read workgroup 0's beta stores only in the block loads
         // of the next iteration, which follow this workgroup's next publish
         if (w == 0) __builtin_amdgcn_s_waitcnt(0);
 
-        if (p.mode == kModeChain && p.F) {
-            // ---- fixed effects (BayesRBase.cpp:212-235)
-            if (!fixed_effects<R>(p, L, rnd, n_rounds, eps, iteration, lo, hi, w, wave, lane)) goto done;
-        }
         if (p.mode == kModeChain) {
+            double e2 = 0.0;
+            const bool ok = fp64_region([&](double (&ed)[R][8]) {
+                // ---- fixed effects (BayesRBase.cpp:212-235)
+                if (p.F && !fixed_effects<R>(p, L, rnd, n_rounds, ed, iteration, lo, hi, w, wave, lane))
+                    return false;
+                // ---- |eps|^2 of the sigmaE draw
+                double part = 0.0;
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) part += ed[r][e] * ed[r][e];
+                if (!allreduce_scalar<R>(p, L, rnd, part, w, wave, lane, &e2)) return false;
+                ++n_rounds;
+                return true;
+            });
+            if (!ok) goto done;
             // ---- sigmaE, sigmaG, pi draws (BayesRBase.cpp:237-247)
-            double part = 0.0;
-#pragma unroll
-            for (int r = 0; r < R; ++r)
-#pragma unroll
-                for (int e = 0; e < 8; ++e) part += eps[r][e] * eps[r][e];
-            double e2;
-            if (!allreduce_scalar<R>(p, L, rnd, part, w, wave, lane, &e2)) goto done;
-            ++n_rounds;
             if (tid == 0) {
                 L.sigmaE = inv_scaled_chisq_rng(p.seed, iteration, kDrawSigmaE, p.chain, p.v0E + p.N,
                                                 (e2 + p.v0E * p.s02E) / (p.v0E + p.N));
@@ -1332,13 +1568,14 @@ bayesr_sweep_kernel(SweepParams p)
 done:
     __syncthreads();
     if (L.abort == 0) {
-        if (wave == 0) {
+        // (FP32 mode: the last FP64 region has stored the residual)
+        if (wave == 0 && !kF32) {
 #pragma unroll
             for (int r = 0; r < R; ++r)
 #pragma unroll
                 for (int e = 0; e < 8; ++e) {
                     const uint32_t idx = lo + (uint32_t)r * kRun + 8u * (uint32_t)lane + (uint32_t)e;
-                    if (idx < hi) p.eps[idx] = eps[r][e];
+                    if (idx < hi) p.eps[idx] = (double)eps[r][e];
                 }
         }
         if (w == 0) {

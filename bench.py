@@ -45,10 +45,10 @@ S_DEFAULT = np.array([0.01, 0.001, 0.0001])
 CONFIGS = {
     "c2": dict(workload="plain BayesR ppbayes, ragged layout, N=100000 M=100000, single chain",
                n=100_000, m=100_000, layout="ragged", maf=(0.005, 0.5), causal=0.005, h2=0.4,
-               chain=200),
+               chain=200, precision="fp32"),
     "c2-dense": dict(workload="plain BayesR ppbayes, dense .bed 2-bit layout, N=100000 M=100000, single chain",
                      n=100_000, m=100_000, layout="dense", maf=(0.005, 0.5), causal=0.005, h2=0.4,
-                     chain=200),
+                     chain=200, precision="fp32"),
     "c1": dict(workload="plain BayesR ppbayes, dense .bed 2-bit, N=5000 M=10000, single chain",
                n=5_000, m=10_000, layout="dense", maf=(0.01, 0.5), causal=0.01, h2=0.5, chain=10),
     "c3": dict(workload="batched traits: 64 BayesR chains (one phenotype each) over one dense .bed 2-bit "
@@ -129,8 +129,9 @@ def make_batch_data(cfg, seed, threads):
 def algorithmic_bytes(cfg, data, storage, markers_per_step, changes, eps_elem=8):
     """Per step: sum_j B_j with B_j = G_j + e N (1 + delta_j): G_j the column
     bytes of the device copy (index lists: 4 (n1 + n2 + nmiss) + 32; 2-bit
-    codes: ceil(N/4)), e N one read of eps (e = 8 bytes in FP64), delta_j = 1
-    when marker j's beta changed (one more eps pass, the update)."""
+    codes: ceil(N/4)), e N one read of eps (e = 4 bytes for the FP32 residual,
+    8 in FP64), delta_j = 1 when marker j's beta changed (one more eps pass, the
+    update). The eps terms are logical passes: the kernel keeps eps in registers."""
     n, m = cfg["n"], cfg["m"]
     if cfg["layout"] == "ragged" and storage == 1:
         r = data["ragged"]
@@ -138,6 +139,79 @@ def algorithmic_bytes(cfg, data, storage, markers_per_step, changes, eps_elem=8)
     else:
         g_total = float(m * ((n + 3) // 4))
     return g_total * (markers_per_step / m) + eps_elem * n * (markers_per_step + changes)
+
+
+def errors_vs_fp64(h, brr, data, m, args, orders, hist32, fresh_chain):
+    """Errors of the FP32-residual chain against the FP64 path of the same
+    device code (whose parity with the FP64 oracle the GPU tests pin):
+    - per marker, from one identical late-chain state (the end of the timed
+      chain): one FP32 and one FP64 sweep over the same order and draws; num
+      (computeNum, whose dot part is x_j . eps) and beta compared marker by marker;
+    - the whole timed chain re-run in FP64 from the same initial state and
+      random stream: sigmaG, sigmaE and h2 = sigmaG / (sigmaG + sigmaE) per
+      iteration, the first iteration whose model size m0 differs (a component
+      flipped and the chains parted), and both chains' posterior means of h2."""
+    t0 = time.time()
+    st = h.get_state()
+    sc = h.get_scalars()
+    order = orders[-1].copy()
+    brr.shuffle(args.seed, len(orders), 0, order)
+    u, z = brr.marker_uz(args.seed, len(orders), 0, m)
+
+    def one_sweep(precision):
+        h.set_precision(precision, args.refresh)
+        h.set_state(st["eps"], st["beta"], st["comp"])
+        h.set_scalars(sc["mu"], sc["sigmaE"], sc["epsilon_sum"], sc["sigmaG"], sc["pi"])
+        h.record_num(True)
+        h.sweep(order, u, z)
+        out = h.get_state()
+        out["num"] = h.get_num()
+        h.record_num(False)
+        return out
+
+    a = one_sweep(brr.PRECISION_FP32_REFRESH)
+    b = one_sweep(brr.PRECISION_FP64)
+    scale = float(np.max(np.abs(b["num"])))
+    dnum = np.abs(a["num"] - b["num"])
+    same = a["comp"] == b["comp"]
+    dbeta = np.abs(a["beta"] - b["beta"])[same]
+    # the chain again in FP64
+    fresh_chain()
+    hist64 = np.array([h.run_chain(args.seed, 0, it, orders[it])[0] for it in range(len(orders))])
+    h.set_precision(brr.PRECISION_FP32_REFRESH, args.refresh)
+    h2a = hist32[:, 4] / (hist32[:, 4] + hist32[:, 1])
+    h2b = hist64[:, 4] / (hist64[:, 4] + hist64[:, 1])
+    parted = np.flatnonzero(hist32[:, 2] != hist64[:, 2])
+    first = int(parted[0]) if parted.size else None
+    upto = first if first is not None else len(orders)
+    burn = min(len(orders) // 2, 50)
+    sem = lambda x: float(np.std(x) / np.sqrt(max(1, len(x))))
+    out = {
+        "reference": "device FP64 path from the same state and random stream (pinned to the FP64 oracle "
+                     "by the GPU parity tests)",
+        "state": f"end of the timed chain (iteration {len(orders)}), one sweep of all {m} markers",
+        "num_max_abs": float(dnum.max()), "num_max_rel_to_max_num": float(dnum.max() / scale),
+        "num_rms": float(np.sqrt(np.mean(dnum ** 2))), "num_scale_max_abs": scale,
+        "components_identical": bool(same.all()), "components_differing": int((~same).sum()),
+        "beta_max_abs": float(dbeta.max()) if dbeta.size else 0.0,
+        "eps_max_abs_after_sweep": float(np.max(np.abs(a["eps"] - b["eps"]))),
+        "chain_iterations": len(orders),
+        "chain_first_parting_iteration": first,
+        "h2_max_abs_before_parting": float(np.max(np.abs(h2a[:upto] - h2b[:upto]))) if upto else None,
+        "sigmaG_max_rel_before_parting": (float(np.max(np.abs(hist32[:upto, 4] - hist64[:upto, 4]) /
+                                                       hist64[:upto, 4])) if upto else None),
+        "h2_abs_err_at_iterations": {str(i): float(abs(h2a[i] - h2b[i]))
+                                     for i in (0, 1, 2, 5, 10, 20, 50, 100, 150, len(orders) - 1) if i < len(orders)},
+        "h2_final_abs": float(abs(h2a[-1] - h2b[-1])),
+        "sigmaG_final_rel": float(abs(hist32[-1, 4] - hist64[-1, 4]) / hist64[-1, 4]),
+        "h2_posterior_mean_fp32": float(np.mean(h2a[burn:])), "h2_posterior_mean_fp64": float(np.mean(h2b[burn:])),
+        "h2_posterior_mean_mc_sem": sem(h2b[burn:]),
+        "seconds": time.time() - t0,
+    }
+    log(f"errors vs fp64: num max {out['num_max_abs']:.3e} (rel {out['num_max_rel_to_max_num']:.2e}), "
+        f"components identical {out['components_identical']}, chain parts at {first}, "
+        f"h2 max err {out['h2_max_abs_before_parting']}")
+    return out
 
 
 def cpu_baseline(cfg, data, seed, budget_s, window=0):
@@ -675,6 +749,12 @@ def main():
     ap.add_argument("--async", dest="async_window", type=int, default=0,
                     help="ParallelGraph's asynchronous sweep in windows of this many markers (1..64; "
                          "0 = the exact sequential sweep)")
+    ap.add_argument("--precision", default=None, choices=["fp32", "fp64"],
+                    help="plain sweep residual arithmetic: fp32 (FP32 residual and dots, compensated wave "
+                         "reductions, FP64 refresh every --refresh markers; the c2 default) or fp64")
+    ap.add_argument("--refresh", type=int, default=1024, help="FP64 residual refresh interval (markers)")
+    ap.add_argument("--no-errors", action="store_true",
+                    help="skip the FP64 comparison run of an fp32 line (errors_vs_fp64)")
     ap.add_argument("--decode-only", action="store_true",
                     help="only the decode leg: the c2 genotypes decoded on the device, its JSON object printed")
     args = ap.parse_args()
@@ -738,7 +818,12 @@ def main():
     h.set_model(S_DEFAULT)
     if args.async_window:
         h.set_async(args.async_window)
-    log(f"uploaded in {time.time() - t0:.1f}s")
+    precision = args.precision or cfg.get("precision", "fp64")
+    if args.async_window:
+        precision = "fp64"
+    if precision == "fp32":
+        h.set_precision(brr.PRECISION_FP32_REFRESH, args.refresh)
+    log(f"uploaded in {time.time() - t0:.1f}s; precision {precision}")
 
     def fresh_chain():
         eps, sc = brr.bayesr_init(data["y"], 1, 4, args.seed)
@@ -775,12 +860,16 @@ def main():
     wall = time.perf_counter() - t_start
     hist = np.array(hist)
     stats = h.stats()
+    errors = None
+    if precision == "fp32" and not args.no_errors:
+        errors = errors_vs_fp64(h, brr, data, m, args, orders, hist, fresh_chain)
 
     total_updates = float(steps) * m
     value = total_updates / wall
     kernel_ms = np.array(kernel_ms)
     avg_ms = float(kernel_ms.mean())
-    bytes_per_launch = np.mean([algorithmic_bytes(cfg, data, stats["storage"], m, c) for c in changes])
+    eps_elem = 4 if precision == "fp32" else 8
+    bytes_per_launch = np.mean([algorithmic_bytes(cfg, data, stats["storage"], m, c, eps_elem) for c in changes])
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     sigmaE, sigmaG = hist[:, 1], hist[:, 4]
     burn = min(steps // 2, 50)
@@ -797,13 +886,15 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if precision == "fp32" else "f64",
         "data": "synthetic (Philox-seeded PLINK genotypes, HWE, y = X beta + e)",
         "config": {
             "workload": cfg["workload"], "N": n, "M": m, "layout": cfg["layout"],
             "device_storage": "2-bit codes" if stats["storage"] == 2 else "index lists",
             "maf": list(cfg["maf"]), "causal_fraction": cfg["causal"], "h2_simulated": cfg["h2"],
-            "S": S_DEFAULT.tolist(), "timed_iterations": steps, "precision": "fp64",
+            "S": S_DEFAULT.tolist(), "timed_iterations": steps,
+            "precision": (f"fp32 residual and dots (compensated wave reductions), fp64 refresh every "
+                          f"{args.refresh} markers, fp64 decisions and draws" if precision == "fp32" else "fp64"),
             "workgroups": stats["workgroups"], "block_markers": stats["block_markers"],
             "slice": stats["slice"], "parallelism": "single GPU, persistent cooperative kernel",
             "schedule": (f"asynchronous windows of {args.async_window} markers (ParallelGraph)"
@@ -819,6 +910,8 @@ def main():
             "phase_ms_per_launch": dict(zip(brr.HipSweep.PHASES, np.mean(phases, axis=0).round(3).tolist())),
         },
     }
+    if errors is not None:
+        line["errors_vs_fp64"] = errors
     h.close()
     if not args.no_cpu:
         log("timing the CPU reference sweep ...")

@@ -43,7 +43,7 @@
 extern "C" {
 #endif
 
-#define BRR_HIP_ABI_VERSION 6
+#define BRR_HIP_ABI_VERSION 7
 
 enum brr_hip_status {
     BRR_HIP_OK = 0,
@@ -165,6 +165,38 @@ int brr_hip_load_bed(brr_hip_ctx* ctx, const uint8_t* bed_file, uint64_t file_by
 /* The M x 4 per-marker values held on the device: dense tables {dosage 0, 1, 2,
  * missing} or ragged statistics {mean, sd, sqrdZ, Zsum}. */
 int brr_hip_get_marker_tables(brr_hip_ctx* ctx, double* out);
+
+/* Arithmetic of the residual in brr_hip_sweep and brr_hip_run_chain. The
+ * reference holds epsilon as an Eigen::VectorXd and updates it in FP64
+ * (BayesRBase.cpp:370-378 with densebayesrkernel.cpp:17-20 /
+ * raggedbayesrkernel.cpp:10-30). BRR_HIP_PRECISION_FP64 (the default) does the
+ * same. BRR_HIP_PRECISION_FP32_REFRESH holds the residual slices in FP32 and
+ * forms the x_j . epsilon dots in FP32 with compensated (TwoSum) wave
+ * reductions; the residual changes are logged and replayed in FP64 on the
+ * context's FP64 residual at least every refresh_markers (1..1024) markers and
+ * at every iteration boundary, after which the FP32 slices are re-rounded from
+ * it. So the residual returned by brr_hip_get_state, updateMu, the fixed
+ * effects and |epsilon|^2 of the sigmaE draw are FP64, as are the decisions and
+ * every scalar draw; only the dots see the FP32 residual. Single-chain
+ * sequential sweeps whose columns are held as 2-bit codes (dense, EigenSparse,
+ * or ragged with ragged_storage 0 or 2). */
+enum brr_hip_precision {
+    BRR_HIP_PRECISION_FP64 = 0,
+    BRR_HIP_PRECISION_FP32_REFRESH = 1
+};
+int brr_hip_set_precision(brr_hip_ctx* ctx, int precision, uint32_t refresh_markers);
+
+/* Per-marker record of num, the value BayesRKernel::computeNum returns and
+ * processColumn decides on (BayesRBase.cpp:300-305, densebayesrkernel.cpp:10-15,
+ * sparsebayesrkernel.cpp:24-27): with enable != 0 every later sweep and chain
+ * launch writes the num of each marker's decision (a chain: of its last
+ * iteration), and brr_hip_get_num copies the M values out. Single-chain BayesR
+ * contexts; the record costs one store per marker. It exists to state the
+ * dot-product error of a precision mode against another run from the same
+ * state: for dense markers num - beta_old (N - 1) is x_j . epsilon, for ragged
+ * ones sd (num - beta_old (N - 1)) + mean epsilonSum is. */
+int brr_hip_record_num(brr_hip_ctx* ctx, int enable);
+int brr_hip_get_num(brr_hip_ctx* ctx, double* out);
 
 /* Fixed effects (--fixed / Options::fixedEffectNumber, BayesRBase.cpp:212-235):
  * X is Data::X, the N x F float matrix in Eigen's column-major order (column f

@@ -36,7 +36,7 @@ def test_every_declared_symbol_is_exported():
 
 
 def test_abi_version():
-    assert brr.lib().brr_hip_abi_version() == 6
+    assert brr.lib().brr_hip_abi_version() == 7
 
 
 def test_create_rejects_bad_config():
