@@ -213,6 +213,58 @@ def test_batch_large_slice_residual_identity(gpu_lib):
     h.close()
 
 
+def test_batch_t64_n50k_against_oracle(gpu_lib):
+    """The batched workload's shape: T = 64 traits, N = 50,000 (the 13-tile kernel,
+    slices of 208 on 241 workgroups, four waves of 16 traits), 3 sweeps; traits 0,
+    21, 42 and 63 (one per wave) against their oracle chains at the batched bars,
+    and every trait's residual equal to y_std - mu - X beta."""
+    pr = make_problem(50_000, 400, seed=211, h2=0.5, causal_frac=0.02)
+    seed, T = 37, 64
+    ys = phenotypes(pr, T, seed=212)
+    eps0 = np.zeros((pr.n, T))
+    h = brr.HipSweep(pr.n, pr.m, 4, 1, brr.LAYOUT_DENSE_2BIT, traits=T)
+    h.upload_dense(pr.bed, pr.lut)
+    h.set_model(S)
+    picked = (0, 21, 42, 63)
+    os_ = {}
+    for t, y in enumerate(ys):
+        e, sc = brr.bayesr_init(y, 1, 4, seed, chain=t)
+        eps0[:, t] = e
+        h.set_scalars(sc["mu"], sc["sigmaE"], sc["epsilon_sum"], sc["sigmaG"], sc["pi"], trait=t)
+        if t in picked:
+            o = orc.OracleProblem(pr.n, pr.m, orc.DENSE, 4, 1, S, bed=pr.bed, lut=pr.lut)
+            o.init(y, seed, chain=t)
+            assert np.array_equal(o.eps, e)
+            os_[t] = o
+    h.set_state(eps0, np.zeros((pr.m, T)), np.zeros((pr.m, T), np.int32))
+    s = h.stats()
+    assert s["slice"] == 208 and s["workgroups"] == 241
+    order = np.arange(pr.m, dtype=np.uint32)
+    for it in range(3):
+        brr.shuffle(seed, it, 0, order)
+        uz = [brr.marker_uz(seed, it, t, pr.m) for t in range(T)]
+        u = np.stack([a for a, _ in uz], axis=1)
+        z = np.stack([b for _, b in uz], axis=1)
+        for t, o in os_.items():
+            o.sweep(order, u[:, t], z[:, t], trace=False)
+        h.sweep(order, u, z)
+        st = h.get_state()
+        for t, o in os_.items():
+            w = f"iteration {it} trait {t}"
+            assert np.array_equal(st["comp"][:, t], o.comp), w
+            close(st["beta"][:, t], o.beta, what=f"{w} beta")
+            close(st["acum"][:, t], o.acum, what=f"{w} acum")
+            close(st["eps"][:, t], o.eps, what=f"{w} eps")
+            sc = h.get_scalars(trait=t)
+            assert np.array_equal(sc["v"].reshape(-1), o.v), f"{w}: v"
+    d = spec_dosage(pr.bed, pr.n)
+    x = np.take_along_axis(pr.lut, np.where(d < 0, 3, d), axis=1)    # (m, n)
+    for t in range(T):
+        mu = h.get_scalars(trait=t)["mu"]
+        close(st["eps"][:, t], eps0[:, t] - mu - st["beta"][:, t] @ x, rtol=1e-9, what=f"trait {t} identity")
+    h.close()
+
+
 def test_batch_bad_configs(gpu_lib):
     pr = make_problem(4000, 20, seed=209)
     with pytest.raises(brr.BrrHipError):

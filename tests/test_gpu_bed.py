@@ -81,6 +81,27 @@ def test_tables_bit_exact(gpu_lib, n, m, miss):
             same_values(h.marker_tables(), host.stats, f"ragged statistics N={n}")
 
 
+def oracle_tables(bed, n, layout):
+    """The oracle's per-marker tables: orc_dense_lut (DenseMarkerBuilder) or the
+    statistics of orc_ragged_build (RaggedSparseMarkerBuilder)."""
+    if layout == brr.LAYOUT_DENSE_2BIT:
+        return np.stack([orc.dense_lut(bed[j], n) for j in range(bed.shape[0])])
+    return np.stack([orc.ragged_build(bed[j], n)[4] for j in range(bed.shape[0])])
+
+
+@pytest.mark.parametrize("n,m,miss", [(37, 40, 0.0), (1001, 97, 0.05), (20003, 33, 0.02), (100003, 24, 0.01)])
+def test_tables_equal_the_oracle_builders(gpu_lib, n, m, miss):
+    """The device decode against the oracle's builders directly (not through the
+    product's host builders): dense tables and ragged statistics bit for bit."""
+    bed, _ = brr.synth_bed(700 + n, n, m, 0.05, 0.5, miss)
+    bed = garble_pad(bed, n, n)
+    f = brr.bed_file_image(bed)
+    for layout in (brr.LAYOUT_DENSE_2BIT, brr.LAYOUT_RAGGED):
+        with brr.HipSweep(n, m, 4, layout=layout) as h:
+            h.load_bed(f, layout)
+            same_values(h.marker_tables(), oracle_tables(bed, n, layout), f"layout {layout} N={n} vs oracle")
+
+
 def test_degenerate_columns(gpu_lib):
     """All missing, all one dosage, a single non-missing call: tables as the host computes them."""
     n, m = 101, 6
@@ -110,6 +131,10 @@ def test_real_fixture(gpu_lib):
     with brr.HipSweep(n, m, 4, layout=brr.LAYOUT_RAGGED) as h:
         h.load_bed(str(path), brr.LAYOUT_RAGGED)
         same_values(h.marker_tables(), brr.build_ragged(cols, n).stats, "uk10k ragged statistics")
+        same_values(h.marker_tables(), oracle_tables(cols, n, brr.LAYOUT_RAGGED), "uk10k ragged vs oracle")
+    with brr.HipSweep(n, m, 4, layout=brr.LAYOUT_DENSE_2BIT) as h:
+        h.load_bed(str(path), brr.LAYOUT_DENSE_2BIT)
+        same_values(h.marker_tables(), oracle_tables(cols, n, brr.LAYOUT_DENSE_2BIT), "uk10k dense vs oracle")
 
 
 def sweep_inputs(m, it, seed=77):
