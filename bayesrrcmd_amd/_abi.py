@@ -76,6 +76,7 @@ SIGNATURES = {
     "brr_hip_set_async": (C.c_int, [CTX, C.c_uint32]),
     "brr_hip_set_precision": (C.c_int, [CTX, C.c_int, C.c_uint32]),
     "brr_hip_record_num": (C.c_int, [CTX, C.c_int]),
+    "brr_hip_sweep_noalloc": (C.c_int, [CTX, C.c_void_p, C.c_void_p, C.c_void_p]),
     "brr_hip_get_num": (C.c_int, [CTX, P_f64]),
     "brr_hip_set_fixed": (C.c_int, [CTX, C.POINTER(C.c_float), C.c_uint32]),
     "brr_hip_set_fixed_state": (C.c_int, [CTX, P_f64, P_u32, C.c_double]),

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -28,6 +29,11 @@ BRR_BW_KERNELS(BRR_BW_DECL)
 } // namespace brr
 
 using namespace brr;
+
+// words of brr_hip_sweep_noalloc's pinned status block: 4 of control words,
+// 16 of counters, 1 of the order flag (a copy of the scratch words from the
+// control words on)
+constexpr int kStatusWords = 4 + 16 + 1;
 
 // device counters of a launch: [0] rounds, [1] changes, [2..7] phase ticks,
 // batched kernel also [8] decision rounds (summed over waves), [10..15] phase ticks 6..11
@@ -50,6 +56,7 @@ struct brr_hip_ctx {
     uint32_t B_seq = 0;          // block size of the sequential sweep
     bool f32 = false;            // FP32 residual with FP64 refresh (brr_hip_set_precision)
     double* d_num = nullptr;     // per-marker num of the last sweep (brr_hip_record_num)
+    uint64_t* h_status = nullptr; // pinned: control words, counters and order flag of brr_hip_sweep_noalloc
     bool record_num = false;
     uint32_t refresh_markers = 1024;
     double change_rate = 1.0;    // beta changes per marker visit in the last chain launch
@@ -565,6 +572,84 @@ int launch_any(brr_hip_ctx* c, SweepParams& p)
     return c->T ? launch_batch(c, p) : launch(c, p);
 }
 
+// ------------------------------------------------- allocation-free sweep --
+// brr_hip_sweep_noalloc keeps its device memory -- the plain sweep kernel's
+// tagged exchange words (partials and totals), its control words and launch
+// counters, and a bitmap that checks the caller's order -- in this file-scope
+// array. Each GPU holds its own copy for the life of the process. Layout in
+// 64-bit words: partials, totals, then from kOffCtl on 4 words of control
+// words, 16 of counters and 1 of the order flag (the same order as the
+// pinned status block), then the bitmap of up to kScratchMaxM markers.
+constexpr size_t kScratchX1 = (size_t)2 * kSlots * kMaxP * 2;
+constexpr size_t kScratchX2 = (size_t)kCopies * kCopyWords;
+constexpr size_t kScratchMaxM = (size_t)1 << 22;
+constexpr size_t kOffX2 = kScratchX1;
+constexpr size_t kOffCtl = (kOffX2 + kScratchX2 + 31) / 32 * 32;
+constexpr size_t kOffCnt = kOffCtl + 4;
+constexpr size_t kOffFlag = kOffCnt + 16;
+constexpr size_t kOffMap = (kOffFlag + 1 + 31) / 32 * 32;
+constexpr size_t kScratchWords = kOffMap + kScratchMaxM / 64;   // 1.53 MiB
+static __device__ __align__(256) uint64_t scratch[kScratchWords];
+
+// The device address of this GPU's copy of `scratch`, looked up once per device
+// (with that device current) by hipGetSymbolAddress; the array's own name is a
+// host-side stand-in and is never passed to a kernel.
+std::mutex g_scratch_mu;
+uint64_t* g_scratch_ptr[64] = {};
+
+__global__ void scratch_zero_kernel(uint64_t* p, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+// Before a launch: the abort words (the round word is kept: tags only grow), the
+// counters, the order flag and the first `map_words` bitmap words.
+__global__ void scratch_reset_kernel(uint64_t* base, size_t map_words)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t* ctl = (uint32_t*)(base + kOffCtl);
+    if (t == 0) {
+        ctl[kCtlAbort] = 0u;
+        ctl[kCtlAbortWhere] = 0u;
+    }
+    if (t < 17) base[kOffCnt + t] = 0;
+    for (size_t i = t; i < map_words; i += (size_t)gridDim.x * blockDim.x) base[kOffMap + i] = 0;
+}
+
+// Marks every marker of the order in the bitmap; flags an id >= M or a repeat.
+__global__ void order_check_kernel(const uint32_t* order, uint32_t M, uint64_t* base)
+{
+    uint32_t* flag = (uint32_t*)(base + kOffFlag);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x) {
+        const uint32_t j = order[i];
+        if (j >= M) {
+            atomicOr(flag, 1u);
+            continue;
+        }
+        const unsigned long long bit = 1ull << (j & 63u);
+        const unsigned long long old = atomicOr((unsigned long long*)(base + kOffMap + (j >> 6)), bit);
+        if (old & bit) atomicOr(flag, 1u);
+    }
+}
+
+int scratch_for(brr_hip_ctx* c, uint64_t** out)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (c->device < 0 || c->device >= 64) return fail(c, BRR_HIP_EINVAL, "device ordinal beyond 63");
+    if (!g_scratch_ptr[c->device]) {
+        void* ptr = nullptr;
+        HIP_TRY(c, hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return fail(c, BRR_HIP_EDEVICE, "hipGetSymbolAddress returned a null scratch address");
+        scratch_zero_kernel<<<256, 256, 0, c->stream>>>((uint64_t*)ptr, kScratchWords);
+        HIP_TRY(c, hipGetLastError());
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        g_scratch_ptr[c->device] = (uint64_t*)ptr;
+    }
+    *out = g_scratch_ptr[c->device];
+    return BRR_HIP_OK;
+}
+
 int check_ready(brr_hip_ctx* c)
 {
     if (!c) return BRR_HIP_EINVAL;
@@ -699,7 +784,8 @@ int brr_hip_create(const brr_hip_config* cfg, brr_hip_ctx** out)
         return rc;
     }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipHostMalloc((void**)&c->h_status, sizeof(uint64_t) * kStatusWords) != hipSuccess) {
         brr_hip_destroy(c);
         return BRR_HIP_EDEVICE;
     }
@@ -731,6 +817,7 @@ void brr_hip_destroy(brr_hip_ctx* c)
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->h_status) (void)hipHostFree(c->h_status);
     delete c;
 }
 
@@ -1198,6 +1285,74 @@ int brr_hip_sweep(brr_hip_ctx* c, const uint32_t* order, const double* u, const 
     rc = launch_any(c, p);
     c->stats.markers = (uint64_t)M * c->Tn;
     return rc;
+}
+
+int brr_hip_sweep_noalloc(brr_hip_ctx* c, const uint32_t* d_order, const double* d_u, const double* d_z)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if ((rc = need_kind(c, false))) return rc;
+    if (c->T) return fail(c, BRR_HIP_EINVAL, "brr_hip_sweep_noalloc: single-chain contexts only");
+    if (!d_order || !d_u || !d_z) return fail(c, BRR_HIP_EINVAL, "null argument");
+    const uint32_t M = c->cfg.num_snps;
+    if ((size_t)M > kScratchMaxM)
+        return fail(c, BRR_HIP_EINVAL, "brr_hip_sweep_noalloc: at most 4194304 markers (its scratch bitmap)");
+    if (c->x1_words > kScratchX1 || c->x2_words > kScratchX2)
+        return fail(c, BRR_HIP_EINVAL, "brr_hip_sweep_noalloc: the exchange does not fit its scratch");
+    if (c->f32 && !c->codes)
+        return fail(c, BRR_HIP_ESTATE, "the FP32 residual mode needs the columns held as 2-bit codes "
+                                       "(ragged_storage 0 or 2)");
+    const void* fn = kernel_for(c->R, c->layout, c->codes, c->async != 0, c->f32);
+    if (!fn) return fail(c, BRR_HIP_EINVAL, "unsupported slice size");
+    HIP_TRY(c, hipSetDevice(c->device));
+    uint64_t* base = nullptr;
+    if ((rc = scratch_for(c, &base))) return rc;
+    const size_t map_words = ((size_t)M + 63) / 64;
+    scratch_reset_kernel<<<64, 256, 0, c->stream>>>(base, map_words);
+    HIP_TRY(c, hipGetLastError());
+    order_check_kernel<<<std::min<uint32_t>((M + 255) / 256, 1024u), 256, 0, c->stream>>>(d_order, M, base);
+    HIP_TRY(c, hipGetLastError());
+    SweepParams p = base_params(c);
+    p.mode = kModeSweep;
+    p.iterations = 1;
+    p.order = d_order;
+    p.u = d_u;
+    p.z = d_z;
+    p.xchg1 = base;
+    p.xchg2 = base + kOffX2;
+    p.ctl = (uint32_t*)(base + kOffCtl);
+    p.counters = (unsigned long long*)(base + kOffCnt);
+    p.gate = (const uint32_t*)(base + kOffFlag);
+    p.refresh_blocks = std::max(1u, c->refresh_markers / c->B);
+    void* args[] = {&p};
+    HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    HIP_TRY(c, hipLaunchCooperativeKernel(fn, dim3(c->P), dim3(kThreads), args, 0, c->stream));
+    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->h_status, base + kOffCtl, sizeof(uint64_t) * kStatusWords, hipMemcpyDeviceToHost,
+                              c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    HIP_TRY(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    const uint32_t* ctl = (const uint32_t*)c->h_status;
+    const unsigned long long* counters = (const unsigned long long*)(c->h_status + 4);
+    const uint32_t flag = (uint32_t)c->h_status[20];
+    if (flag) return fail(c, BRR_HIP_EINVAL, "order is not a permutation of 0..M-1");
+    c->stats.kernel_ms = ms;
+    c->stats.rounds = counters[0];
+    c->stats.changes = counters[1];
+    c->stats.decision_rounds = counters[8];
+    c->stats.markers = M;
+    for (int q = 0; q < 6; ++q) c->phase_ms[q] = (double)counters[2 + q] * 1e-5;
+    shape_stats(c);
+    if (ctl[kCtlAbort]) {
+        // the exchange may hold tags of the aborted rounds: clear it and the round word
+        scratch_zero_kernel<<<256, 256, 0, c->stream>>>(base, kOffCtl + 4);
+        HIP_TRY(c, hipGetLastError());
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        return fail(c, BRR_HIP_ETIMEOUT, "device exchange wait exceeded its bound (site " +
+                                             std::to_string(ctl[kCtlAbortWhere]) + ")");
+    }
+    return BRR_HIP_OK;
 }
 
 int brr_hip_fixed_effects(brr_hip_ctx* c, uint64_t seed, uint32_t iteration, uint32_t chain)

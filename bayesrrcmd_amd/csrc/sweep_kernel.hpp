@@ -114,6 +114,7 @@ struct SweepParams {
     int32_t* comp;                // M
     double* acum;                 // M
     double* num_out;              // M or null: the num of every marker's decision (computeNum)
+    const uint32_t* gate;         // null, or a word that must be 0 for the launch to run (else all exit at once)
     double* scal;                 // scalar block, see ScalarLayout
     double* hist;                 // iterations x (4 + G) or null
     uint64_t* xchg1;              // [2][kMaxB][P][2] tagged partials
@@ -898,6 +899,7 @@ bayesr_sweep_kernel(SweepParams p)
     using E = typename std::conditional<kF32, float, double>::type;
     __shared__ Shared<R> L;
     __shared__ RefreshLog<kF32, R> RL;
+    if (p.gate && __hip_atomic_load(p.gate, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return;
     const uint32_t w = blockIdx.x;
     const int tid = threadIdx.x;
     const int lane = tid & 63;

@@ -252,6 +252,20 @@ int brr_hip_get_trait_scalars(brr_hip_ctx* ctx, uint32_t trait, brr_hip_scalars*
  * updates the device state. */
 int brr_hip_sweep(brr_hip_ctx* ctx, const uint32_t* order, const double* u, const double* z);
 
+/* brr_hip_sweep without any device allocation. No counterpart in the reference:
+ * it is for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). The same sweep kernel on the
+ * same context state, but order / u / z are device pointers (M uint32, M and
+ * M doubles, caller-owned, read only) and the kernel's exchange words, control
+ * words and counters, and a bitmap that checks the order on the device, live
+ * in the library's file-scope device scratch (1.53 MiB per GPU, held until the
+ * process ends). Limits: single-chain BayesR contexts, M <= 4,194,304 markers
+ * (BRR_HIP_EINVAL beyond); an order that is not a permutation is refused with
+ * BRR_HIP_EINVAL before the sweep touches the state. Calls that overlap on one
+ * GPU race on its scratch; calls in sequence (each is synchronous) are safe.
+ * Outputs: the context's state, read as after brr_hip_sweep. */
+int brr_hip_sweep_noalloc(brr_hip_ctx* ctx, const uint32_t* d_order, const double* d_u, const double* d_z);
+
 /* A device-resident chain: `iterations` full runGibbs iterations (updateMu,
  * sweep, sigmaE / sigmaG / pi draws) in one persistent launch, drawing u, z
  * and the scalars from the counter-based stream (seed, chain). orders holds
