@@ -426,6 +426,38 @@ class HipSweep(_DeviceMarkers):
         self._ck(lib().brr_hip_set_precision(self.ctx, int(precision), int(refresh_markers)),
                  "brr_hip_set_precision")
 
+    def kernel_compute_num(self, marker: int, epsilon: np.ndarray, epsilon_sum: float, beta_old: float) -> float:
+        """BayesRKernel::computeNum(epsilon, beta_old) on a host residual (HipBayesRKernel)."""
+        e = np.ascontiguousarray(epsilon, dtype=np.float64)
+        if e.shape != (self.n,):
+            raise ValueError(f"epsilon must have shape ({self.n},)")
+        out = C.c_double()
+        self._ck(lib().brr_hip_kernel_compute_num(self.ctx, int(marker), _p(e, C.c_double), float(epsilon_sum),
+                                                  float(beta_old), C.byref(out)), "brr_hip_kernel_compute_num")
+        return out.value
+
+    def kernel_epsilon_change(self, marker: int, beta_old: float, beta: float):
+        """BayesRKernel::calculateEpsilonChange(beta_old, beta): (residual change, epsilonSum change)."""
+        d = np.zeros(self.n)
+        es = C.c_double()
+        self._ck(lib().brr_hip_kernel_epsilon_change(self.ctx, int(marker), float(beta_old), float(beta),
+                                                     _p(d, C.c_double), C.byref(es)), "brr_hip_kernel_epsilon_change")
+        return d, es.value
+
+    def set_launch(self, workgroups: int = 0, block_markers: int = 0):
+        """Launch shape: workgroups (0: default rule) and markers per block (0: keep)."""
+        self._ck(lib().brr_hip_set_launch(self.ctx, int(workgroups), int(block_markers)), "brr_hip_set_launch")
+
+    def autotune(self, seed: int, iterations: int = 2) -> list:
+        """Times `iterations` chain iterations from the current state per candidate launch
+        shape, keeps the fastest and the state; returns the table of trials."""
+        tab = (_abi.TuneEntry * 32)()
+        n = C.c_uint32()
+        self._ck(lib().brr_hip_autotune(self.ctx, int(seed), int(iterations), tab, 32, C.byref(n)),
+                 "brr_hip_autotune")
+        return [dict(workgroups=e.workgroups, block_markers=e.block_markers, slice=e.slice,
+                     ms_per_iteration=e.ms_per_iteration) for e in tab[: n.value]]
+
     def record_num(self, enable: bool = True):
         """Record each marker's num (computeNum) in later sweeps; read with get_num()."""
         self._ck(lib().brr_hip_record_num(self.ctx, int(bool(enable))), "brr_hip_record_num")

@@ -50,6 +50,11 @@ class Stats(C.Structure):
                 ("decision_rounds", C.c_uint64)]
 
 
+class TuneEntry(C.Structure):
+    _fields_ = [("workgroups", C.c_uint32), ("block_markers", C.c_uint32), ("slice", C.c_uint32),
+                ("ms_per_iteration", C.c_double)]
+
+
 class BwConfig(C.Structure):
     _fields_ = [("num_inds", C.c_uint32), ("num_snps", C.c_uint32), ("num_components", C.c_uint32),
                 ("quad_points", C.c_uint32), ("workgroups", C.c_uint32), ("device", C.c_uint32)]
@@ -76,6 +81,11 @@ SIGNATURES = {
     "brr_hip_set_async": (C.c_int, [CTX, C.c_uint32]),
     "brr_hip_set_precision": (C.c_int, [CTX, C.c_int, C.c_uint32]),
     "brr_hip_record_num": (C.c_int, [CTX, C.c_int]),
+    "brr_hip_set_launch": (C.c_int, [CTX, C.c_uint32, C.c_uint32]),
+    "brr_hip_autotune": (C.c_int, [CTX, C.c_uint64, C.c_uint32, C.POINTER(TuneEntry), C.c_uint32,
+                                   C.POINTER(C.c_uint32)]),
+    "brr_hip_kernel_compute_num": (C.c_int, [CTX, C.c_uint32, P_f64, C.c_double, C.c_double, P_f64]),
+    "brr_hip_kernel_epsilon_change": (C.c_int, [CTX, C.c_uint32, C.c_double, C.c_double, P_f64, P_f64]),
     "brr_hip_sweep_noalloc": (C.c_int, [CTX, C.c_void_p, C.c_void_p, C.c_void_p]),
     "brr_hip_get_num": (C.c_int, [CTX, P_f64]),
     "brr_hip_set_fixed": (C.c_int, [CTX, C.POINTER(C.c_float), C.c_uint32]),

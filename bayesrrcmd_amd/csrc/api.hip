@@ -55,7 +55,10 @@ struct brr_hip_ctx {
     uint32_t async = 0;          // asynchronous windows of B markers (brr_hip_set_async)
     uint32_t B_seq = 0;          // block size of the sequential sweep
     bool f32 = false;            // FP32 residual with FP64 refresh (brr_hip_set_precision)
+    uint32_t cus = 0;            // compute units of the device
+    size_t x1_cap = 0;           // words allocated for the exchange partials
     double* d_num = nullptr;     // per-marker num of the last sweep (brr_hip_record_num)
+    double* d_keps = nullptr;    // N: a host residual / residual change of the per-marker kernel calls
     uint64_t* h_status = nullptr; // pinned: control words, counters and order flag of brr_hip_sweep_noalloc
     bool record_num = false;
     uint32_t refresh_markers = 1024;
@@ -177,7 +180,7 @@ __global__ void build_seg_kernel(const int32_t* ones, const uint64_t* ones_off,
 }
 
 // BayesRKernel::computeNum for one marker on the device epsilon (one block).
-__global__ void compute_num_kernel(SweepParams p, uint32_t j, double beta_old, double* out)
+__global__ void compute_num_kernel(SweepParams p, uint32_t j, double beta_old, const double* eps_sum, double* out)
 {
     __shared__ double red[kWaves];
     const int tid = threadIdx.x;
@@ -215,7 +218,7 @@ __global__ void compute_num_kernel(SweepParams p, uint32_t j, double beta_old, d
             *out = dot + beta_old * (double)(p.N - 1u);
         } else {
             const double mean = p.stats[4 * (size_t)j], sd = p.stats[4 * (size_t)j + 1];
-            const double epsSum = p.scal[2];
+            const double epsSum = *eps_sum;
             *out = beta_old * (dN - 1.0) - mean * epsSum / sd + dot / sd;
         }
     }
@@ -650,6 +653,70 @@ int scratch_for(brr_hip_ctx* c, uint64_t** out)
     return BRR_HIP_OK;
 }
 
+// Launch shape of a BayesR context for `workgroups` requested (0: the default
+// rule of brr_hip_create) and `block` markers per speculation block (0: keep;
+// single-chain contexts): slice, workgroup count, lane runs or 16-row tiles;
+// the exchange partials grow if needed and index-list storage gets its segment
+// table rebuilt for the new slices. On error the context keeps its shape.
+int configure_shape(brr_hip_ctx* c, uint32_t workgroups, uint32_t block)
+{
+    const uint32_t N = c->cfg.num_inds, M = c->cfg.num_snps;
+    const uint32_t Pmax = std::min(c->cus, kMaxP);
+    if (block > (uint32_t)kMaxB) return fail(c, BRR_HIP_EINVAL, "block_markers must be 0..64");
+    uint32_t S, R = 0, nt = 0, NT = 0;
+    const void* probe;
+    if (c->T) {
+        const uint32_t Preq = workgroups ? std::min(workgroups, Pmax) : Pmax;
+        S = (N + Preq - 1) / Preq;
+        S = (S + kSliceAlign - 1) / kSliceAlign * kSliceAlign;
+        nt = S / 16;
+        if (nt > (uint32_t)kMaxNT) return fail(c, BRR_HIP_EINVAL, "batched slices hold at most 256 individuals");
+        NT = nt <= 4 ? 4u : (nt <= 8 ? 8u : (nt <= 13 ? 13u : 16u));
+        probe = batch_kernel_for(NT, c->layout);
+    } else {
+        S = lane_run_slice(N, c->cus, workgroups);
+        R = (S + kRun - 1) / kRun;
+        if (R == 3) R = 4;
+        if (R > (uint32_t)kMaxRuns) return fail(c, BRR_HIP_EINVAL, "slices hold at most 2048 individuals");
+        probe = kernel_for(R, c->layout, c->codes || c->layout == kLayoutEigen, c->async != 0, c->f32);
+    }
+    const uint32_t P = (N + S - 1) / S;
+    int occ = 0;
+    if (!probe || hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, probe, kThreads, 0) != hipSuccess ||
+        (uint64_t)occ * c->cus < P)
+        return fail(c, BRR_HIP_EDEVICE, "the launch shape does not fit the device");
+    const size_t x1 = c->T ? (size_t)2 * P * kBVmax * 2 : (size_t)2 * kSlots * P * 2;
+    int rc;
+    if (x1 > c->x1_cap) {
+        c->x1_cap = 0;
+        if ((rc = dalloc(c, &c->d_xchg1, x1))) return rc;
+        c->x1_cap = x1;
+    }
+    const bool seg = !c->T && !c->bw && !c->codes && c->have_markers && c->d_ones;
+    if (seg) {
+        if ((rc = dalloc(c, &c->d_seg, (size_t)M * 3 * (P + 1)))) return rc;
+        const uint64_t total = (uint64_t)M * 3 * (P + 1);
+        build_seg_kernel<<<(uint32_t)((total + 255) / 256), 256, 0, c->stream>>>(
+            c->d_ones, c->d_ones_off, c->d_twos, c->d_twos_off, c->d_miss, c->d_miss_off, M, P, S, c->d_seg);
+        HIP_TRY(c, hipGetLastError());
+    }
+    c->S = S;
+    c->P = P;
+    c->x1_words = x1;
+    if (c->T) {
+        c->nt = nt;
+        c->NT = NT;
+    } else {
+        c->R = R;
+        if (block) {
+            c->B_seq = block;
+            if (!c->async) c->B = block;
+        }
+    }
+    shape_stats(c);
+    return reset_exchange(c);
+}
+
 int check_ready(brr_hip_ctx* c)
 {
     if (!c) return BRR_HIP_EINVAL;
@@ -730,6 +797,7 @@ int brr_hip_create(const brr_hip_config* cfg, brr_hip_ctx** out)
     // Slice partition: S multiple of 16 individuals, P = ceil(N / S) <= CUs.
     const uint32_t N = cfg->num_inds;
     const uint32_t cus = (uint32_t)prop.multiProcessorCount;
+    c->cus = cus;
     uint32_t Preq = cfg->workgroups ? cfg->workgroups : cus;
     Preq = std::min(std::min(Preq, cus), kMaxP);
     uint32_t S = (N + Preq - 1) / Preq;
@@ -794,6 +862,7 @@ int brr_hip_create(const brr_hip_config* cfg, brr_hip_ctx** out)
     (void)hipMemset(c->d_comp, 0, sizeof(int32_t) * M * Tn);
     (void)hipMemset(c->d_acum, 0, sizeof(double) * M * Tn);
     (void)hipMemset(c->d_scal, 0, sizeof(double) * SL.size() * Tn);
+    c->x1_cap = c->x1_words;
     if (reset_exchange(c) != BRR_HIP_OK) { brr_hip_destroy(c); return BRR_HIP_EDEVICE; }
     *out = c;
     return BRR_HIP_OK;
@@ -813,7 +882,7 @@ void brr_hip_destroy(brr_hip_ctx* c)
     dfree(c->d_eps); dfree(c->d_beta); dfree(c->d_comp); dfree(c->d_acum);
     dfree(c->d_scal); dfree(c->d_hist); dfree(c->d_order); dfree(c->d_u); dfree(c->d_z);
     dfree(c->d_xchg1); dfree(c->d_xchg2); dfree(c->d_ctl); dfree(c->d_counters); dfree(c->d_tmp);
-    dfree(c->d_fail); dfree(c->d_sumf); dfree(c->d_num);
+    dfree(c->d_fail); dfree(c->d_sumf); dfree(c->d_num); dfree(c->d_keps);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1064,6 +1133,105 @@ int brr_hip_set_precision(brr_hip_ctx* c, int precision, uint32_t refresh_marker
         return fail(c, BRR_HIP_EINVAL, "refresh interval must be 1..1024 markers");
     c->f32 = true;
     c->refresh_markers = refresh_markers;
+    return BRR_HIP_OK;
+}
+
+int brr_hip_set_launch(brr_hip_ctx* c, uint32_t workgroups, uint32_t block_markers)
+{
+    if (int k = need_kind(c, false)) return k;
+    HIP_TRY(c, hipSetDevice(c->device));
+    return configure_shape(c, workgroups, block_markers);
+}
+
+int brr_hip_autotune(brr_hip_ctx* c, uint64_t seed, uint32_t iterations, brr_hip_tune_entry* table,
+                     uint32_t table_cap, uint32_t* entries)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if ((rc = need_kind(c, false))) return rc;
+    if (iterations < 1 || iterations > 16) return fail(c, BRR_HIP_EINVAL, "tuning runs 1..16 iterations per shape");
+    HIP_TRY(c, hipSetDevice(c->device));
+    const uint32_t N = c->cfg.num_inds, M = c->cfg.num_snps;
+    const size_t Tn = c->Tn;
+    const size_t sl = ScalarLayout{c->cfg.num_groups, c->cfg.num_components}.size();
+    // candidate shapes: (workgroups, block markers)
+    const uint32_t Pmax = std::min(c->cus, kMaxP);
+    std::vector<std::pair<uint32_t, uint32_t>> cand;
+    auto add = [&](uint32_t P, uint32_t B) {
+        if (P < 1 || P > Pmax) return;
+        for (auto& q : cand)
+            if (q.first == P && q.second == B) return;
+        cand.emplace_back(P, B);
+    };
+    if (c->T) {
+        for (uint32_t S : {256u, 224u, 192u, 160u, 128u}) add((N + S - 1) / S, 0);
+        add(Pmax, 0);
+    } else {
+        const std::vector<uint32_t> blocks = c->async ? std::vector<uint32_t>{0u} : std::vector<uint32_t>{32u, 48u, 64u};
+        for (uint32_t R : {1u, 2u, 4u}) {
+            const uint32_t P = (N + kRun * R - 1) / (kRun * R);
+            for (uint32_t B : blocks) add(P, B);
+        }
+        for (uint32_t B : blocks) add(Pmax, B);
+    }
+    // snapshot of the chain state, restored before every trial and at the end
+    double *s_eps = nullptr, *s_beta = nullptr, *s_acum = nullptr, *s_scal = nullptr, *s_fixed = nullptr;
+    int32_t* s_comp = nullptr;
+    uint32_t* s_forder = nullptr;
+    auto release = [&]() { dfree(s_eps); dfree(s_beta); dfree(s_acum); dfree(s_scal); dfree(s_comp);
+                           dfree(s_fixed); dfree(s_forder); };
+    if ((rc = dalloc(c, &s_eps, N * Tn)) || (rc = dalloc(c, &s_beta, M * Tn)) || (rc = dalloc(c, &s_acum, M * Tn)) ||
+        (rc = dalloc(c, &s_scal, sl * Tn)) || (rc = dalloc(c, &s_comp, M * Tn)) ||
+        (c->F && ((rc = dalloc(c, &s_fixed, 1 + c->F)) || (rc = dalloc(c, &s_forder, c->F))))) {
+        release();
+        return rc;
+    }
+    auto copy_state = [&](bool save) -> int {
+        auto cp = [&](void* a, void* b, size_t bytes) { return hipMemcpyAsync(save ? a : b, save ? b : a, bytes,
+                                                                               hipMemcpyDeviceToDevice, c->stream); };
+        if (cp(s_eps, c->d_eps, sizeof(double) * N * Tn) != hipSuccess ||
+            cp(s_beta, c->d_beta, sizeof(double) * M * Tn) != hipSuccess ||
+            cp(s_acum, c->d_acum, sizeof(double) * M * Tn) != hipSuccess ||
+            cp(s_scal, c->d_scal, sizeof(double) * sl * Tn) != hipSuccess ||
+            cp(s_comp, c->d_comp, sizeof(int32_t) * M * Tn) != hipSuccess ||
+            (c->F && (cp(s_fixed, c->d_fixed, sizeof(double) * (1 + c->F)) != hipSuccess ||
+                      cp(s_forder, c->d_forder, sizeof(uint32_t) * c->F) != hipSuccess)) ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
+            return fail(c, BRR_HIP_EDEVICE, "autotune: state copy failed");
+        return BRR_HIP_OK;
+    };
+    if ((rc = copy_state(true))) { release(); return rc; }
+    std::vector<uint32_t> orders((size_t)iterations * M);
+    std::vector<uint32_t> cur(M);
+    for (uint32_t i = 0; i < M; ++i) cur[i] = i;
+    for (uint32_t it = 0; it < iterations; ++it) {
+        brr_shuffle(seed, it, 0, M, cur.data(), 0);
+        std::copy(cur.begin(), cur.end(), orders.begin() + (size_t)it * M);
+    }
+    const uint32_t P0 = c->P, B0 = c->B_seq;
+    const double rate0 = c->change_rate;
+    double best_ms = 1e300;
+    uint32_t best = 0, n = 0;
+    for (size_t q = 0; q < cand.size(); ++q) {
+        double ms = -1.0;
+        if (configure_shape(c, cand[q].first, cand[q].second) == BRR_HIP_OK && copy_state(false) == BRR_HIP_OK) {
+            c->change_rate = rate0;
+            if (brr_hip_run_chain(c, seed, 0, 0, iterations, orders.data(), nullptr) == BRR_HIP_OK)
+                ms = c->stats.kernel_ms / iterations;
+        }
+        if (ms >= 0.0 && ms < best_ms) { best_ms = ms; best = (uint32_t)q; }
+        if (table && n < table_cap)
+            table[n++] = brr_hip_tune_entry{c->P, c->T ? (uint32_t)kBB : c->B, c->S, ms};
+    }
+    if (entries) *entries = n;
+    rc = best_ms < 1e300 ? configure_shape(c, cand[best].first, cand[best].second)
+                         : configure_shape(c, P0, B0);
+    const int rc2 = copy_state(false);
+    c->change_rate = rate0;
+    release();
+    if (rc) return rc;
+    if (rc2) return rc2;
+    if (best_ms >= 1e300) return fail(c, BRR_HIP_EDEVICE, "autotune: no candidate shape ran");
     return BRR_HIP_OK;
 }
 
@@ -1433,7 +1601,7 @@ int brr_hip_compute_num(brr_hip_ctx* c, uint32_t marker, double beta_old, double
     if (!num || marker >= c->cfg.num_snps) return fail(c, BRR_HIP_EINVAL, "bad marker or null output");
     HIP_TRY(c, hipSetDevice(c->device));
     SweepParams p = base_params(c);
-    compute_num_kernel<<<1, kThreads, 0, c->stream>>>(p, marker, beta_old, c->d_tmp);
+    compute_num_kernel<<<1, kThreads, 0, c->stream>>>(p, marker, beta_old, c->d_scal + 2, c->d_tmp);
     HIP_TRY(c, hipGetLastError());
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     HIP_TRY(c, hipMemcpy(num, c->d_tmp, sizeof(double), hipMemcpyDeviceToHost));
@@ -1453,6 +1621,61 @@ int brr_hip_epsilon_change(brr_hip_ctx* c, uint32_t marker, double beta_old, dou
     epsilon_change_kernel<<<blocks, kThreads, 0, c->stream>>>(p, marker, beta_old, beta);
     HIP_TRY(c, hipGetLastError());
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return BRR_HIP_OK;
+}
+
+int brr_hip_kernel_compute_num(brr_hip_ctx* c, uint32_t marker, const double* epsilon, double epsilon_sum,
+                               double beta_old, double* num)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if ((rc = need_kind(c, false))) return rc;
+    if (c->T) return fail(c, BRR_HIP_ESTATE, "per-marker calls need a single-chain context");
+    if (!epsilon || !num || marker >= c->cfg.num_snps) return fail(c, BRR_HIP_EINVAL, "bad marker or null argument");
+    const uint32_t N = c->cfg.num_inds;
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (!c->d_keps && (rc = dalloc(c, &c->d_keps, N))) return rc;
+    HIP_TRY(c, hipMemcpy(c->d_keps, epsilon, sizeof(double) * N, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->d_tmp + 1, &epsilon_sum, sizeof(double), hipMemcpyHostToDevice));
+    SweepParams p = base_params(c);
+    p.eps = c->d_keps;
+    compute_num_kernel<<<1, kThreads, 0, c->stream>>>(p, marker, beta_old, c->d_tmp + 1, c->d_tmp);
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(num, c->d_tmp, sizeof(double), hipMemcpyDeviceToHost));
+    return BRR_HIP_OK;
+}
+
+int brr_hip_kernel_epsilon_change(brr_hip_ctx* c, uint32_t marker, double beta_old, double beta, double* delta,
+                                  double* epsilon_sum_delta)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if ((rc = need_kind(c, false))) return rc;
+    if (c->T) return fail(c, BRR_HIP_ESTATE, "per-marker calls need a single-chain context");
+    if (!delta || marker >= c->cfg.num_snps) return fail(c, BRR_HIP_EINVAL, "bad marker or null argument");
+    const uint32_t N = c->cfg.num_inds;
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (!c->d_keps && (rc = dalloc(c, &c->d_keps, N))) return rc;
+    HIP_TRY(c, hipMemsetAsync(c->d_keps, 0, sizeof(double) * N, c->stream));
+    SweepParams p = base_params(c);
+    p.eps = c->d_keps;
+    const uint32_t blocks = (N + kThreads - 1) / kThreads;
+    epsilon_change_kernel<<<blocks, kThreads, 0, c->stream>>>(p, marker, beta_old, beta);
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(delta, c->d_keps, sizeof(double) * N, hipMemcpyDeviceToHost));
+    if (epsilon_sum_delta) {
+        // SparseBayesRKernel::computeEpsilonSumUpdate (sparsebayesrkernel.cpp:29-34); 0 for dense markers
+        double d = 0.0;
+        if (c->layout != kLayoutDense) {
+            double st[4];
+            HIP_TRY(c, hipMemcpy(st, c->d_stats + 4 * (size_t)marker, sizeof(st), hipMemcpyDeviceToHost));
+            const double dBeta = beta_old - beta;
+            d = dBeta * st[3] / st[1] - dBeta * st[0] * (double)N / st[1];
+        }
+        *epsilon_sum_delta = d;
+    }
     return BRR_HIP_OK;
 }
 

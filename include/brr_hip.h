@@ -186,6 +186,53 @@ enum brr_hip_precision {
 };
 int brr_hip_set_precision(brr_hip_ctx* ctx, int precision, uint32_t refresh_markers);
 
+/* The per-marker BayesRKernel interface against a host residual, for a
+ * HipBayesRKernel : BayesRKernel (src/bayesrkernel.h:11-15) that the reference's
+ * Analysis::kernelForMarker hands to processColumn. Correctness path: every call
+ * copies the N-vector across PCIe.
+ *   brr_hip_kernel_compute_num   -> computeNum(const VectorXd &epsilon, double beta_old):
+ *       dense markers num = Cx . epsilon + beta_old (N - 1) (densebayesrkernel.cpp:10-15);
+ *       ragged / EigenSparse markers num = beta_old (N - 1) - mean epsilon_sum / sd +
+ *       dot(epsilon) / sd (sparsebayesrkernel.cpp:24-27), epsilon_sum being the kernel's
+ *       epsilonSum (ignored for dense markers).
+ *   brr_hip_kernel_epsilon_change -> calculateEpsilonChange(beta_old, beta): delta (N)
+ *       receives the residual change (densebayesrkernel.cpp:17-20, raggedbayesrkernel.cpp:10-30:
+ *       the caller adds it to epsilon), *epsilon_sum_delta (if not NULL) the epsilonSum
+ *       update of a sparse marker (sparsebayesrkernel.cpp:29-34; 0 for dense markers).
+ * The context's own residual and state are untouched. */
+int brr_hip_kernel_compute_num(brr_hip_ctx* ctx, uint32_t marker, const double* epsilon, double epsilon_sum,
+                               double beta_old, double* num);
+int brr_hip_kernel_epsilon_change(brr_hip_ctx* ctx, uint32_t marker, double beta_old, double beta,
+                                  double* delta, double* epsilon_sum_delta);
+
+/* Launch shape of a BayesR context (single-chain or batched): `workgroups`
+ * persistent workgroups requested (0: the default rule of brr_hip_create; the
+ * individuals are spread over them in slices that are a multiple of 16) and
+ * `block_markers` markers per speculation block (1..64, 0: keep; single-chain
+ * contexts). The reference's own tunables are --thread and the token counts of
+ * its flow graphs (options.cpp; limitsequencegraph.cpp:41-44); these are the
+ * device's. The state and the uploaded markers are kept. */
+int brr_hip_set_launch(brr_hip_ctx* ctx, uint32_t workgroups, uint32_t block_markers);
+
+/* The autotuner: times `iterations` (1..16) chain iterations from the current
+ * state for each candidate launch shape -- lane runs per slice (1, 2, 4) with
+ * the fewest workgroups that hold N, and all compute units, each with 32, 48
+ * or 64 markers per block (batched contexts: slices of 256, 224, 192, 160 and
+ * 128 individuals, and all compute units) -- restoring the state before every
+ * trial, then keeps the fastest shape and the state as it was. `table` (if not
+ * NULL) receives up to table_cap entries {workgroups, block markers, slice,
+ * device ms per iteration; -1 if the shape could not run}. Tune from a state
+ * like the one the chain will run from (the cost of a round depends on how
+ * many markers change). */
+typedef struct {
+    uint32_t workgroups;
+    uint32_t block_markers;
+    uint32_t slice;
+    double ms_per_iteration;
+} brr_hip_tune_entry;
+int brr_hip_autotune(brr_hip_ctx* ctx, uint64_t seed, uint32_t iterations, brr_hip_tune_entry* table,
+                     uint32_t table_cap, uint32_t* entries);
+
 /* Per-marker record of num, the value BayesRKernel::computeNum returns and
  * processColumn decides on (BayesRBase.cpp:300-305, densebayesrkernel.cpp:10-15,
  * sparsebayesrkernel.cpp:24-27): with enable != 0 every later sweep and chain
