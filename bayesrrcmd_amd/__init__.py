@@ -300,6 +300,31 @@ def marker_uz(seed: int, iteration: int, chain: int, m: int):
     return u, z
 
 
+def run_cohorts(sweeps, seed: int, first_iteration: int, orders, chains=None, graph: bool = True):
+    """Independent chains of several HipSweep contexts side by side
+    (brr_hip_run_cohorts): orders[q] is an (iterations, M_q) array of marker
+    orders. graph=False launches one after another instead of through a hipGraph.
+    Returns the per-cohort histories (iterations x (4 + G)) and a report
+    {launches, device_ms, launch_of}."""
+    n = len(sweeps)
+    ctxs = (C.c_void_p * n)(*[h.ctx for h in sweeps])
+    ords = [np.ascontiguousarray(o, dtype=np.uint32) for o in orders]
+    iters = ords[0].shape[0]
+    if any(o.shape != (iters, h.m) for o, h in zip(ords, sweeps)):
+        raise ValueError("orders[q] must have shape (iterations, M_q)")
+    optrs = (_abi.P_u32 * n)(*[_p(o, C.c_uint32) for o in ords])
+    hists = [np.zeros((iters, 4 + h.G)) for h in sweeps]
+    hptrs = (_abi.P_f64 * n)(*[_p(x, C.c_double) for x in hists])
+    ch = np.ascontiguousarray(chains if chains is not None else np.zeros(n), dtype=np.uint32)
+    rep = _abi.CohortReport()
+    rc = lib().brr_hip_run_cohorts(ctxs, n, int(seed), _p(ch, C.c_uint32), int(first_iteration), iters, optrs,
+                                   hptrs, 0 if graph else 1, C.byref(rep))
+    if rc != 0:
+        raise BrrHipError(f"brr_hip_run_cohorts failed with status {rc}: "
+                          f"{lib().brr_hip_last_error(sweeps[0].ctx).decode(errors='replace')}")
+    return hists, dict(launches=rep.launches, device_ms=rep.device_ms, launch_of=list(rep.launch_of[: min(n, 64)]))
+
+
 def bed_file_image(bed: np.ndarray) -> np.ndarray:
     """A PLINK .bed file's bytes (magic 6c 1b 01, then the SNP-major columns) from M x ceil(N/4) column bytes."""
     bed = np.ascontiguousarray(bed, dtype=np.uint8)

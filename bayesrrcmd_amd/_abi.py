@@ -55,6 +55,10 @@ class TuneEntry(C.Structure):
                 ("ms_per_iteration", C.c_double)]
 
 
+class CohortReport(C.Structure):
+    _fields_ = [("launches", C.c_uint32), ("device_ms", C.c_double), ("launch_of", C.c_uint32 * 64)]
+
+
 class BwConfig(C.Structure):
     _fields_ = [("num_inds", C.c_uint32), ("num_snps", C.c_uint32), ("num_components", C.c_uint32),
                 ("quad_points", C.c_uint32), ("workgroups", C.c_uint32), ("device", C.c_uint32)]
@@ -82,6 +86,8 @@ SIGNATURES = {
     "brr_hip_set_precision": (C.c_int, [CTX, C.c_int, C.c_uint32]),
     "brr_hip_record_num": (C.c_int, [CTX, C.c_int]),
     "brr_hip_set_launch": (C.c_int, [CTX, C.c_uint32, C.c_uint32]),
+    "brr_hip_run_cohorts": (C.c_int, [C.POINTER(CTX), C.c_uint32, C.c_uint64, P_u32, C.c_uint32, C.c_uint32,
+                                      C.POINTER(P_u32), C.POINTER(P_f64), C.c_uint32, C.POINTER(CohortReport)]),
     "brr_hip_autotune": (C.c_int, [CTX, C.c_uint64, C.c_uint32, C.POINTER(TuneEntry), C.c_uint32,
                                    C.POINTER(C.c_uint32)]),
     "brr_hip_kernel_compute_num": (C.c_int, [CTX, C.c_uint32, P_f64, C.c_double, C.c_double, P_f64]),

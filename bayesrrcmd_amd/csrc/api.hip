@@ -24,6 +24,7 @@ namespace brr {
 BRR_SWEEP_KERNELS_OF(BRR_SWEEP_DECL, false)
 BRR_SWEEP_KERNELS_OF(BRR_SWEEP_DECL, true)
 BRR_SWEEP_F32_KERNELS(BRR_SWEEP_DECL)
+BRR_COHORT_KERNELS(BRR_COHORT_DECL)
 BRR_BATCH_KERNELS(BRR_BATCH_DECL)
 BRR_BW_KERNELS(BRR_BW_DECL)
 } // namespace brr
@@ -59,6 +60,8 @@ struct brr_hip_ctx {
     size_t x1_cap = 0;           // words allocated for the exchange partials
     double* d_num = nullptr;     // per-marker num of the last sweep (brr_hip_record_num)
     double* d_keps = nullptr;    // N: a host residual / residual change of the per-marker kernel calls
+    SweepParams* d_cohort = nullptr;   // brr_hip_run_cohorts: the cohorts' parameter blocks
+    uint32_t cohort_cap = 0;
     uint64_t* h_status = nullptr; // pinned: control words, counters and order flag of brr_hip_sweep_noalloc
     bool record_num = false;
     uint32_t refresh_markers = 1024;
@@ -351,6 +354,28 @@ const void* kernel_for(uint32_t R, int layout, bool codes, bool async = false, b
     case 4: return kernel_r<4>(layout, codes, async, f32);
     default: return nullptr;
     }
+}
+
+const void* cohort_kernel_for(uint32_t R, bool f32)
+{
+    switch (R) {
+    case 1: return f32 ? (const void*)bayesr_cohort_kernel<1, true> : (const void*)bayesr_cohort_kernel<1, false>;
+    case 2: return f32 ? (const void*)bayesr_cohort_kernel<2, true> : (const void*)bayesr_cohort_kernel<2, false>;
+    default: return nullptr;
+    }
+}
+
+// Before a cohort launch: the abort words and launch counters of every cohort
+// (one workgroup per cohort; a kernel, not a memset, so that it records into a
+// graph as a kernel node).
+__global__ void cohort_reset_kernel(const SweepParams* params)
+{
+    const SweepParams& p = params[blockIdx.x];
+    if (threadIdx.x == 0) {
+        p.ctl[kCtlAbort] = 0u;
+        p.ctl[kCtlAbortWhere] = 0u;
+    }
+    if (threadIdx.x < 16) p.counters[threadIdx.x] = 0ull;
 }
 
 // Batched kernels by register tiles per slice (4, 8, 13 or 16 tiles of 16
@@ -882,7 +907,7 @@ void brr_hip_destroy(brr_hip_ctx* c)
     dfree(c->d_eps); dfree(c->d_beta); dfree(c->d_comp); dfree(c->d_acum);
     dfree(c->d_scal); dfree(c->d_hist); dfree(c->d_order); dfree(c->d_u); dfree(c->d_z);
     dfree(c->d_xchg1); dfree(c->d_xchg2); dfree(c->d_ctl); dfree(c->d_counters); dfree(c->d_tmp);
-    dfree(c->d_fail); dfree(c->d_sumf); dfree(c->d_num); dfree(c->d_keps);
+    dfree(c->d_fail); dfree(c->d_sumf); dfree(c->d_num); dfree(c->d_keps); dfree(c->d_cohort);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1233,6 +1258,199 @@ int brr_hip_autotune(brr_hip_ctx* c, uint64_t seed, uint32_t iterations, brr_hip
     if (rc2) return rc2;
     if (best_ms >= 1e300) return fail(c, BRR_HIP_EDEVICE, "autotune: no candidate shape ran");
     return BRR_HIP_OK;
+}
+
+int brr_hip_run_cohorts(brr_hip_ctx* const* ctxs, uint32_t n, uint64_t seed, const uint32_t* chains,
+                        uint32_t first_iteration, uint32_t iterations, const uint32_t* const* orders,
+                        double* const* hists, uint32_t flags, brr_hip_cohort_report* report)
+{
+    if (!ctxs || n == 0 || !ctxs[0] || !orders) return BRR_HIP_EINVAL;
+    brr_hip_ctx* c0 = ctxs[0];
+    if (iterations == 0) return fail(c0, BRR_HIP_EINVAL, "zero iterations");
+    int rc;
+    for (uint32_t q = 0; q < n; ++q) {
+        brr_hip_ctx* c = ctxs[q];
+        if (!c) return fail(c0, BRR_HIP_EINVAL, "null context");
+        if ((rc = check_ready(c)) || (rc = need_kind(c, false))) return rc;
+        if (c->T || c->async || c->F || !c->codes)
+            return fail(c0, BRR_HIP_EINVAL, "cohort " + std::to_string(q) + ": cohorts are single-chain sequential "
+                                            "contexts without fixed effects, columns held as 2-bit codes");
+        if (c->R > 2)
+            return fail(c0, BRR_HIP_EINVAL, "cohort " + std::to_string(q) + ": cohort launches hold slices of at most "
+                                            "1024 individuals (two lane runs; brr_hip_set_launch)");
+        if (c->device != c0->device || c->f32 != c0->f32)
+            return fail(c0, BRR_HIP_EINVAL, "cohort " + std::to_string(q) + ": cohorts of one call share the device "
+                                            "and the residual precision");
+        if (!orders[q]) return fail(c0, BRR_HIP_EINVAL, "null orders");
+        if ((rc = check_order(c, orders[q], iterations))) return fail(c0, rc, c->err);
+    }
+    HIP_TRY(c0, hipSetDevice(c0->device));
+    const uint32_t Pmax = std::min(c0->cus, kMaxP);
+    // per-cohort parameter blocks of a chain launch (orders and history uploaded)
+    std::vector<SweepParams> params(n);
+    for (uint32_t q = 0; q < n; ++q) {
+        brr_hip_ctx* c = ctxs[q];
+        const uint32_t M = c->cfg.num_snps, G = c->cfg.num_groups;
+        const size_t total = (size_t)iterations * M;
+        if (c->P > Pmax) return fail(c0, BRR_HIP_EINVAL, "a cohort needs more workgroups than the device has");
+        if (c->order_cap < total) {
+            c->order_cap = 0;
+            if ((rc = dalloc(c, &c->d_order, total))) return fail(c0, rc, c->err);
+            c->order_cap = total;
+        }
+        const size_t hsz = (size_t)iterations * (4 + G);
+        if (c->hist_cap < hsz) {
+            c->hist_cap = 0;
+            if ((rc = dalloc(c, &c->d_hist, hsz))) return fail(c0, rc, c->err);
+            c->hist_cap = hsz;
+        }
+        HIP_TRY(c0, hipMemcpy(c->d_order, orders[q], sizeof(uint32_t) * total, hipMemcpyHostToDevice));
+        SweepParams p = base_params(c);
+        p.mode = kModeChain;
+        p.iterations = iterations;
+        p.iter_base = first_iteration;
+        p.chain = chains ? chains[q] : 0u;
+        p.seed = seed;
+        p.order = c->d_order;
+        p.hist = c->d_hist;
+        p.lazy_z = c->change_rate < 0.05 ? 1u : 0u;
+        p.refresh_blocks = std::max(1u, c->refresh_markers / c->B);
+        params[q] = p;
+    }
+    // launches: cohorts by decreasing estimated chain length in exchange rounds, first
+    // fit into launches of at most one workgroup per compute unit and kMaxCohorts
+    // cohorts (so each launch's first cohort sets its length)
+    std::vector<uint32_t> byLen(n);
+    for (uint32_t q = 0; q < n; ++q) byLen[q] = q;
+    // a chain's exchange rounds per iteration are about M / B + its beta changes: the
+    // change rate of the context's last launch estimates the latter
+    auto rounds_est = [&](uint32_t q) {
+        const brr_hip_ctx* c = ctxs[q];
+        return (double)c->cfg.num_snps * (1.0 / (double)c->B + std::min(1.0, c->change_rate));
+    };
+    std::stable_sort(byLen.begin(), byLen.end(), [&](uint32_t a, uint32_t b) { return rounds_est(a) > rounds_est(b); });
+    std::vector<std::vector<uint32_t>> launches;
+    std::vector<uint32_t> used;
+    for (uint32_t q : byLen) {
+        size_t l = 0;
+        while (l < launches.size() && (used[l] + ctxs[q]->P > Pmax || launches[l].size() >= (size_t)kMaxCohorts)) ++l;
+        if (l == launches.size()) { launches.emplace_back(); used.push_back(0); }
+        launches[l].push_back(q);
+        used[l] += ctxs[q]->P;
+    }
+    // parameter blocks in launch order, on the device
+    std::vector<SweepParams> packed;
+    for (auto& L : launches)
+        for (uint32_t q : L) packed.push_back(params[q]);
+    if (c0->cohort_cap < n) {
+        c0->cohort_cap = 0;
+        if ((rc = dalloc(c0, &c0->d_cohort, n))) return rc;
+        c0->cohort_cap = n;
+    }
+    HIP_TRY(c0, hipMemcpy(c0->d_cohort, packed.data(), sizeof(SweepParams) * n, hipMemcpyHostToDevice));
+    // per launch, the counters' reset and the cooperative cohort kernel: recorded into
+    // one graph (or, with BRR_HIP_COHORTS_DIRECT, launched one after another)
+    std::vector<CohortTable> tables(launches.size());
+    std::vector<const void*> fns(launches.size());
+    uint32_t base = 0;
+    for (size_t l = 0; l < launches.size(); ++l) {
+        CohortTable& t = tables[l];
+        t.params = c0->d_cohort + base;
+        t.n = (uint32_t)launches[l].size();
+        uint32_t wg = 0;
+        for (uint32_t i = 0; i < t.n; ++i) { t.first[i] = wg; wg += ctxs[launches[l][i]]->P; }
+        t.first[t.n] = wg;
+        // the launch's lane runs: the most any of its cohorts needs (a slice shorter than the
+        // kernel's runs is masked at its end, as in any context's last workgroup)
+        uint32_t R = 1;
+        for (uint32_t q : launches[l]) R = std::max(R, ctxs[q]->R);
+        fns[l] = cohort_kernel_for(R, c0->f32);
+        int occ = 0;
+        if (!fns[l] || hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fns[l], kThreads, 0) != hipSuccess ||
+            (uint64_t)occ * c0->cus < wg)
+            return fail(c0, BRR_HIP_EDEVICE, "a cohort launch does not fit the device");
+        base += t.n;
+    }
+    auto enqueue = [&]() -> hipError_t {
+        for (size_t l = 0; l < launches.size(); ++l) {
+            CohortTable& t = tables[l];
+            cohort_reset_kernel<<<t.n, 64, 0, c0->stream>>>(t.params);
+            hipError_t e2 = hipGetLastError();
+            if (e2 != hipSuccess) return e2;
+            void* args[] = {&t};
+            e2 = hipLaunchCooperativeKernel(fns[l], dim3(t.first[t.n]), dim3(kThreads), args, 0, c0->stream);
+            if (e2 != hipSuccess) return e2;
+        }
+        return hipSuccess;
+    };
+    hipError_t e = hipSuccess;
+    if (flags & BRR_HIP_COHORTS_DIRECT) {
+        HIP_TRY(c0, hipEventRecord(c0->ev0, c0->stream));
+        e = enqueue();
+        if (e == hipSuccess) e = hipEventRecord(c0->ev1, c0->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c0->stream);
+        if (e != hipSuccess) return fail(c0, BRR_HIP_EDEVICE, std::string("cohort launch: ") + hipGetErrorString(e));
+    } else {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        HIP_TRY(c0, hipStreamBeginCapture(c0->stream, hipStreamCaptureModeThreadLocal));
+        const hipError_t cap = enqueue();
+        const hipError_t endc = hipStreamEndCapture(c0->stream, &graph);
+        if (cap != hipSuccess || endc != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return fail(c0, BRR_HIP_EDEVICE, std::string("graph capture of the cohort launches failed: ") +
+                                                 hipGetErrorString(cap != hipSuccess ? cap : endc));
+        }
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            (void)hipGraphDestroy(graph);
+            return fail(c0, BRR_HIP_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        }
+        HIP_TRY(c0, hipEventRecord(c0->ev0, c0->stream));
+        e = hipGraphLaunch(exec, c0->stream);
+        if (e == hipSuccess) e = hipEventRecord(c0->ev1, c0->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c0->stream);
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess)
+            return fail(c0, BRR_HIP_EDEVICE, std::string("cohort graph launch: ") + hipGetErrorString(e));
+    }
+    float ms = 0.f;
+    HIP_TRY(c0, hipEventElapsedTime(&ms, c0->ev0, c0->ev1));
+    // results per cohort
+    int status = BRR_HIP_OK;
+    for (uint32_t q = 0; q < n; ++q) {
+        brr_hip_ctx* c = ctxs[q];
+        uint32_t ctl[4] = {0, 0, 0, 0};
+        unsigned long long counters[kCounters] = {};
+        HIP_TRY(c0, hipMemcpy(ctl, c->d_ctl, sizeof(ctl), hipMemcpyDeviceToHost));
+        HIP_TRY(c0, hipMemcpy(counters, c->d_counters, sizeof(counters), hipMemcpyDeviceToHost));
+        std::memcpy(c->last_counters, counters, sizeof(counters));
+        c->stats.kernel_ms = ms;
+        c->stats.rounds = counters[0];
+        c->stats.changes = counters[1];
+        c->stats.markers = (uint64_t)iterations * c->cfg.num_snps;
+        for (int k = 0; k < 6; ++k) c->phase_ms[k] = (double)counters[2 + k] * 1e-5;
+        shape_stats(c);
+        c->change_rate = (double)counters[1] / (double)c->stats.markers;
+        if (ctl[kCtlAbort]) {
+            reset_exchange(c);
+            status = fail(c0, BRR_HIP_ETIMEOUT, "cohort " + std::to_string(q) + ": device exchange wait exceeded "
+                                                "its bound (site " + std::to_string(ctl[kCtlAbortWhere]) + ")");
+            continue;
+        }
+        if (hists && hists[q])
+            HIP_TRY(c0, hipMemcpy(hists[q], c->d_hist, sizeof(double) * iterations * (4 + c->cfg.num_groups),
+                                  hipMemcpyDeviceToHost));
+    }
+    if (report) {
+        report->launches = (uint32_t)launches.size();
+        report->device_ms = ms;
+        for (size_t l = 0; l < launches.size(); ++l)
+            for (uint32_t q : launches[l])
+                if (q < 64) report->launch_of[q] = (uint32_t)l;
+    }
+    return status;
 }
 
 int brr_hip_record_num(brr_hip_ctx* c, int enable)

@@ -22,6 +22,10 @@
     X(4, kLayoutDense, true, false, true) X(4, kLayoutRagged, true, false, true)                        \
     X(4, kLayoutEigen, true, false, true)
 
+// bayesr_cohort_kernel<R, fp32>: one or two lane runs per slice (a four-run
+// variant gave chains that differ from run to run and stays out)
+#define BRR_COHORT_KERNELS(X) X(1, false) X(2, false) X(1, true) X(2, true)
+
 // bayesr_batch_kernel<NT, dense>
 #define BRR_BATCH_KERNELS(X) X(4, true) X(4, false) X(8, true) X(8, false) X(13, true) X(13, false) \
     X(16, true) X(16, false)
@@ -31,6 +35,8 @@
 
 #define BRR_SWEEP_INST(R, L, C, A, F) template __global__ void bayesr_sweep_kernel<R, L, C, A, F>(SweepParams);
 #define BRR_SWEEP_DECL(R, L, C, A, F) extern template __global__ void bayesr_sweep_kernel<R, L, C, A, F>(SweepParams);
+#define BRR_COHORT_INST(R, F) template __global__ void bayesr_cohort_kernel<R, F>(CohortTable);
+#define BRR_COHORT_DECL(R, F) extern template __global__ void bayesr_cohort_kernel<R, F>(CohortTable);
 #define BRR_BATCH_INST(NT, D) template __global__ void bayesr_batch_kernel<NT, D>(BatchParams);
 #define BRR_BATCH_DECL(NT, D) extern template __global__ void bayesr_batch_kernel<NT, D>(BatchParams);
 #define BRR_BW_INST(R, D) template __global__ void bayesw_sweep_kernel<R, D>(BwParams);

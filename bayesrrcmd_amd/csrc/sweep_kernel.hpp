@@ -891,16 +891,16 @@ __device__ __forceinline__ void ragged_store(const SweepParams& p, Shared<R>& L,
 // replaying the logged residual changes in FP64, and the FP32 copy is then
 // re-rounded from it. The decisions, the scalar draws, updateMu, the fixed
 // effects and |eps|^2 are FP64 throughout.
+// The sweep of one context by one of its P workgroups (w), with the LDS of
+// the workgroup passed in: the body of bayesr_sweep_kernel and of the cohort
+// kernel below, which runs several contexts side by side in one launch.
 template <int R, int kLay, bool kCodes, bool kAsync, bool kF32>
-__global__ void __launch_bounds__(kThreads)
-bayesr_sweep_kernel(SweepParams p)
+__device__ __forceinline__ void sweep_body(const SweepParams& p, Shared<R>& L, RefreshLog<kF32, R>& RL,
+                                           const uint32_t w)
 {
     static_assert(!kF32 || kCodes, "the FP32 residual mode reads its columns as 2-bit codes");
     using E = typename std::conditional<kF32, float, double>::type;
-    __shared__ Shared<R> L;
-    __shared__ RefreshLog<kF32, R> RL;
     if (p.gate && __hip_atomic_load(p.gate, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return;
-    const uint32_t w = blockIdx.x;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
@@ -1605,6 +1605,47 @@ done:
             }
         }
     }
+}
+
+template <int R, int kLay, bool kCodes, bool kAsync, bool kF32>
+__global__ void __launch_bounds__(kThreads)
+bayesr_sweep_kernel(SweepParams p)
+{
+    __shared__ Shared<R> L;
+    __shared__ RefreshLog<kF32, R> RL;
+    sweep_body<R, kLay, kCodes, kAsync, kF32>(p, L, RL, blockIdx.x);
+}
+
+// ------------------------------------------------------- cohorts in one launch --
+
+// Independent single-chain contexts (cohorts of uneven N and M) in one
+// persistent cooperative launch: cohort c owns workgroups [first[c], first[c+1])
+// and runs its own context's sweep or chain on them -- its own slices,
+// exchange words, state and counters -- so small cohorts, whose rounds cost as
+// much as a large one's, run side by side instead of one after another. Every
+// cohort of a launch has the same lane runs R and residual precision; the
+// layout (dense / ragged / EigenSparse formulas over 2-bit codes) is per cohort.
+constexpr int kMaxCohorts = 64;
+
+struct CohortTable {
+    const SweepParams* params;          // device array, one per cohort
+    uint32_t n;
+    uint32_t first[kMaxCohorts + 1];    // first workgroup of each cohort; first[n] = grid size
+};
+
+template <int R, bool kF32>
+__global__ void __launch_bounds__(kThreads)
+bayesr_cohort_kernel(CohortTable t)
+{
+    __shared__ Shared<R> L;
+    __shared__ RefreshLog<kF32, R> RL;
+    uint32_t c = 0;
+    while (c + 1 < t.n && blockIdx.x >= t.first[c + 1]) ++c;
+    const SweepParams p = t.params[c];
+    const uint32_t w = blockIdx.x - t.first[c];
+    if (p.layout == kLayoutDense) sweep_body<R, kLayoutDense, true, false, kF32>(p, L, RL, w);
+    else if (p.layout == kLayoutEigen) sweep_body<R, kLayoutEigen, true, false, kF32>(p, L, RL, w);
+    else sweep_body<R, kLayoutRagged, true, false, kF32>(p, L, RL, w);
 }
 
 } // namespace brr

@@ -592,8 +592,11 @@ def run_batched(cfg, args, with_cpu):
 
 def run_cohorts(cfg, args, with_cpu):
     """The uneven batch: every cohort's .bed decoded on the device (brr_hip_load_bed) into a context sized to
-    its N, then its chain of `steps` iterations; value = all cohorts' marker-updates / the sum of their chain
-    wall times (data generation and upload excluded)."""
+    its N; then all cohorts' chains of `steps` iterations side by side (brr_hip_run_cohorts: cohorts packed
+    onto disjoint workgroup sets of persistent launches, the launches captured in one hipGraph). The lane
+    runs per slice (the one launch-shape choice the cohorts share) are picked by timing 2 iterations of the
+    whole batch at each feasible value. value = all cohorts' marker-updates / the wall time of the timed
+    call (orders upload, graph capture and launch included; data generation and upload excluded)."""
     import bayesrrcmd_amd as brr
     steps = args.steps if args.steps is not None else cfg["chain"]
     rng = np.random.default_rng(args.seed + 4)
@@ -609,11 +612,11 @@ def run_cohorts(cfg, args, with_cpu):
     for want in ("dense", "ragged"):
         if want not in layouts:
             layouts[int(np.argmin(ns)) if want == "ragged" else int(np.argmax(ns))] = want
-    per, wall_total, kernel_total, bytes_total, updates = [], 0.0, 0.0, 0.0, 0
-    cpu = None
+    precision = args.precision or "fp32"
+    hs, ys, cpu = [], [], None
+    t0 = time.time()
     for c in range(K):
         n, m, lay = int(ns[c]), int(ms[c]), layouts[c]
-        t0 = time.time()
         bed, _ = brr.synth_bed(args.seed + 100 + c, n, m, lo, hi, 0.0, threads=args.threads)
         mc = max(1, int(round(cfg["causal"] * m)))
         causal = np.sort(rng.choice(m, mc, replace=False))
@@ -625,51 +628,92 @@ def run_cohorts(cfg, args, with_cpu):
         h = brr.HipSweep(n, m, 4, 1, code)
         h.load_bed(brr.bed_file_image(bed), code)
         h.set_model(S_DEFAULT)
-        setup_s = time.time() - t0
-        eps, sc = brr.bayesr_init(y, 1, 4, args.seed + c)
-        h.set_state(eps, np.zeros(m), np.zeros(m, np.int32))
-        h.set_scalars(sc["mu"], sc["sigmaE"], sc["epsilon_sum"], sc["sigmaG"], sc["pi"])
-        order = np.arange(m, dtype=np.uint32)
-        orders = np.empty((steps, m), dtype=np.uint32)
-        for it in range(steps):
-            brr.shuffle(args.seed + c, it, 0, order)
-            orders[it] = order
-        kms, chg = [], []
-        t1 = time.perf_counter()
-        for it in range(steps):
-            h.run_chain(args.seed + c, 0, it, orders[it:it + 1])
-            s = h.stats()
-            kms.append(s["kernel_ms"])
-            chg.append(s["changes"])
-        wall = time.perf_counter() - t1
-        stats = h.stats()
-        h.close()
-        cfg_c = dict(n=n, m=m, layout="dense")     # the device holds codes in either layout
-        nbytes = sum(algorithmic_bytes(cfg_c, None, 2, m, x) for x in chg)
-        wall_total += wall
-        kernel_total += sum(kms) * 1e-3
-        bytes_total += nbytes
-        updates += m * steps
-        per.append(dict(N=n, M=m, layout=lay, workgroups=stats["workgroups"], ms_per_iteration=wall * 1e3 / steps,
-                        marker_updates_per_s=m * steps / wall, setup_s=round(setup_s, 2)))
-        log(f"cohort {c}: N={n} M={m} {lay}, {stats['workgroups']} workgroups, {wall * 1e3 / steps:.2f} ms/iteration "
-            f"({m * steps / wall / 1e6:.2f} M marker-updates/s), setup {setup_s:.1f}s")
+        if precision == "fp32":
+            h.set_precision(brr.PRECISION_FP32_REFRESH, args.refresh)
+        hs.append(h)
+        ys.append(y)
         if with_cpu and cpu is None and lay == "dense":
             k = min(m, 4000)                          # the markers the CPU sample uses (dense layout)
             data = dict(bed=bed[:k], lut=brr.build_dense(bed[:k], n, threads=args.threads), y=y, ragged=None)
             cpu = cpu_baseline(dict(n=n, m=m, layout="dense", chain=steps), data, args.seed, args.cpu_budget)
         del bed
-    achieved = bytes_total / kernel_total / 1e9
-    out = dict(value=updates / wall_total, unit="marker-updates/s", steps=steps,
-               ms_per_step=wall_total * 1e3 / steps, dtype="f64",
-               config={"workload": cfg["workload"], "cohorts": K, "timed_iterations_per_cohort": steps,
-                       "precision": "fp64", "parallelism": "single GPU; one persistent cooperative launch per "
-                       "cohort iteration, cohorts in sequence", "per_cohort": per},
+    log(f"{K} cohorts generated and decoded on the device in {time.time() - t0:.1f}s "
+        f"(N {int(ns.min())}..{int(ns.max())}, M {int(ms.min())}..{int(ms.max())}, "
+        f"{layouts.count('dense')} dense / {layouts.count('ragged')} ragged)")
+
+    def orders_for(steps_):
+        out = []
+        for c in range(K):
+            o = np.empty((steps_, int(ms[c])), dtype=np.uint32)
+            cur = np.arange(int(ms[c]), dtype=np.uint32)
+            for it in range(steps_):
+                brr.shuffle(args.seed, it, c, cur)
+                o[it] = cur
+            out.append(o)
+        return out
+
+    def fresh():
+        for c, h in enumerate(hs):
+            eps, sc = brr.bayesr_init(ys[c], 1, 4, args.seed, chain=c)
+            h.set_state(eps, np.zeros(h.m), np.zeros(h.m, np.int32))
+            h.set_scalars(sc["mu"], sc["sigmaE"], sc["epsilon_sum"], sc["sigmaG"], sc["pi"])
+
+    # the autotuned shape: the least lane runs per slice a cohort may use (1 or 2; more runs, fewer
+    # workgroups per cohort, so more cohorts per launch), each value timed on 2 iterations of the batch
+    def shape(rmin):
+        for h in hs:
+            r = max(rmin, -(-h.n // (512 * 256)))
+            h.set_launch(-(-h.n // (512 * r)), 0)
+
+    tune, o2 = [], orders_for(2)
+    for rmin in (1, 2):
+        shape(rmin)
+        fresh()
+        t1 = time.perf_counter()
+        _, rep = brr.run_cohorts(hs, args.seed, 0, o2, chains=list(range(K)))
+        tune.append(dict(min_lane_runs=rmin, launches=rep["launches"], seconds=time.perf_counter() - t1))
+        log(f"tune: >= {rmin} lane runs per slice -> {rep['launches']} launches, "
+            f"{tune[-1]['seconds']:.2f} s for 2 iterations")
+    best = min(tune, key=lambda e: e["seconds"])
+    shape(best["min_lane_runs"])
+    orders = orders_for(steps)
+    fresh()
+    t1 = time.perf_counter()
+    hists, rep = brr.run_cohorts(hs, args.seed, 0, orders, chains=list(range(K)))
+    wall = time.perf_counter() - t1
+    per, bytes_total, updates = [], 0.0, 0
+    for c, h in enumerate(hs):
+        st = h.stats()
+        cfg_c = dict(n=h.n, m=h.m, layout="dense")     # the device holds codes in either layout
+        bytes_total += algorithmic_bytes(cfg_c, None, 2, steps * h.m, st["changes"], 4 if precision == "fp32" else 8)
+        updates += h.m * steps
+        per.append(dict(N=h.n, M=h.m, layout=layouts[c], workgroups=st["workgroups"], launch=rep["launch_of"][c],
+                        rounds=st["rounds"], changes=st["changes"]))
+    for h in hs:
+        h.close()
+    device_s = rep["device_ms"] * 1e-3
+    achieved = bytes_total / device_s / 1e9
+    log(f"c4: {updates / wall / 1e6:.2f} M marker-updates/s, {rep['launches']} launches in one graph, "
+        f"device {rep['device_ms']:.0f} ms, wall {wall * 1e3:.0f} ms")
+    out = dict(value=updates / wall, unit="marker-updates/s", steps=steps,
+               ms_per_step=wall * 1e3 / steps, dtype="f32+f64" if precision == "fp32" else "f64",
+               config={"workload": cfg["workload"].replace("run cohort after cohort",
+                                                           "cohorts side by side in persistent launches"),
+                       "cohorts": K, "timed_iterations_per_cohort": steps,
+                       "precision": (f"fp32 residual, fp64 refresh every {args.refresh} markers"
+                                     if precision == "fp32" else "fp64"),
+                       "parallelism": "single GPU; cohorts packed onto disjoint workgroup sets of persistent "
+                                      "cooperative launches (at most one workgroup per compute unit), the "
+                                      "launches captured in one hipGraph",
+                       "launches": rep["launches"], "min_lane_runs_per_slice": best["min_lane_runs"],
+                       "autotune": tune, "per_cohort": per},
                roofline={"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                         "kernel": "brr::bayesr_sweep_kernel (all cohorts)",
-                         "avg_launch_ms": kernel_total * 1e3 / (steps * K),
-                         "algorithmic_bytes_per_launch": bytes_total / (steps * K)})
+                         "achieved_basis": "logical: B_j = ceil(N/4) + e N (1 + delta_j) per marker-update, e = 4 "
+                                           "(fp32) or 8; the residual passes are served from registers",
+                         "limiter": "latency (exchange rounds)",
+                         "kernel": "brr::bayesr_cohort_kernel (all cohorts, one graph)",
+                         "device_ms": rep["device_ms"]})
     if cpu is not None:
         out["cpu_baseline"] = cpu
     return out

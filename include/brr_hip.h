@@ -233,6 +233,32 @@ typedef struct {
 int brr_hip_autotune(brr_hip_ctx* ctx, uint64_t seed, uint32_t iterations, brr_hip_tune_entry* table,
                      uint32_t table_cap, uint32_t* entries);
 
+/* Independent chains of several contexts -- cohorts of uneven N and M, the
+ * reference's one runBayesRAnalysis per data set (analysisrunner.cpp:123-152)
+ * -- run side by side: `iterations` whole-chain iterations of every context
+ * (as brr_hip_run_chain; orders[q]: iterations x M_q, hists[q]: NULL or
+ * iterations x (4 + G_q), chains[q]: NULL or each context's chain index). The
+ * contexts are packed into persistent cooperative launches, each cohort on a
+ * disjoint set of workgroups (its context's own count) with its own exchange,
+ * at most one workgroup per compute unit and 64 cohorts per launch, longest
+ * chains first (markers x (1 / block + the context's last change rate)); the
+ * launches are captured into one hipGraph and run by one graph launch. All contexts: one device, single-chain sequential
+ * BayesR without fixed effects, columns held as 2-bit codes, one residual
+ * precision, slices of at most 1024 individuals (one or two lane runs; a context
+ * with more is refused with BRR_HIP_EINVAL; brr_hip_set_launch sets a context's);
+ * each launch runs the kernel of the most lane runs any of its cohorts has. report (if not NULL)
+ * gets the number of launches, the device time of the graph and the launch of
+ * each of the first 64 cohorts. */
+typedef struct {
+    uint32_t launches;
+    double device_ms;
+    uint32_t launch_of[64];
+} brr_hip_cohort_report;
+#define BRR_HIP_COHORTS_DIRECT 1u   /* flags: launch one after another instead of through a graph */
+int brr_hip_run_cohorts(brr_hip_ctx* const* ctxs, uint32_t n, uint64_t seed, const uint32_t* chains,
+                        uint32_t first_iteration, uint32_t iterations, const uint32_t* const* orders,
+                        double* const* hists, uint32_t flags, brr_hip_cohort_report* report);
+
 /* Per-marker record of num, the value BayesRKernel::computeNum returns and
  * processColumn decides on (BayesRBase.cpp:300-305, densebayesrkernel.cpp:10-15,
  * sparsebayesrkernel.cpp:24-27): with enable != 0 every later sweep and chain

@@ -123,6 +123,21 @@ def test_sweep_parity_component_counts(gpu_lib, layout, storage, K):
     h.close()
 
 
+@pytest.mark.parametrize("layout,storage", LAYOUTS, ids=LAYOUT_IDS)
+def test_sweep_parity_four_lane_runs(gpu_lib, layout, storage):
+    """Slices of 2,000 individuals (three workgroups for N = 6,000): every lane holds
+    four runs of eight individuals, the widest single-chain shape."""
+    pr = make_problem(6000, 300, seed=119, causal_frac=0.03)
+    o, h = make_pair(pr, layout, workgroups=3, storage=storage)
+    assert h.stats()["slice"] == 2000
+    for it in range(2):
+        order, u, z = sweep_inputs(pr.m, it)
+        o.sweep(order, u, z, trace=False)
+        h.sweep(order, u, z)
+        compare_state(o, h, layout, f"R = 4 iteration {it}")
+    h.close()
+
+
 @pytest.mark.parametrize("n,m", [(37, 5), (513, 1), (1025, 64), (2048, 9)])
 def test_sweep_parity_edge_sizes(gpu_lib, n, m):
     pr = make_problem(n, m, seed=107 + n, causal_frac=0.2)
