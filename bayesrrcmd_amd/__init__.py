@@ -25,14 +25,15 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _abi
-from ._abi import (LAYOUT_DENSE_2BIT, LAYOUT_EIGEN_SPARSE, LAYOUT_RAGGED, PRECISION_FP32_REFRESH, PRECISION_FP64,
+from ._abi import (LAYOUT_DENSE_2BIT, LAYOUT_EIGEN_SPARSE, LAYOUT_RAGGED, PRECISION_FP32, PRECISION_FP32_REFRESH,
+                   PRECISION_FP64,
                    STORAGE_AUTO, STORAGE_CODES, STORAGE_LISTS)
 
 __all__ = [
     "BrrHipError", "HipSweep", "HipBayesW", "bayesw_init", "bed_file_image", "read_samples_binary", "RaggedArena", "read_ragged_sparsebed", "read_ppbed", "SampleWriter", "sample_due", "build_dense", "build_ragged", "synth_bed", "synth_xb",
     "bayesr_init", "shuffle", "marker_uz", "LAYOUT_DENSE_2BIT", "LAYOUT_RAGGED", "LAYOUT_EIGEN_SPARSE", "lib",
     "build_eigen", "read_eigen_sparsebed",
-    "STORAGE_AUTO", "STORAGE_CODES", "STORAGE_LISTS", "PRECISION_FP64", "PRECISION_FP32_REFRESH",
+    "STORAGE_AUTO", "STORAGE_CODES", "STORAGE_LISTS", "PRECISION_FP64", "PRECISION_FP32_REFRESH", "PRECISION_FP32",
 ]
 
 

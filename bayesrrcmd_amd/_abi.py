@@ -15,7 +15,7 @@ LAYOUT_DENSE_2BIT = 1
 LAYOUT_RAGGED = 2
 LAYOUT_EIGEN_SPARSE = 3
 STORAGE_AUTO, STORAGE_LISTS, STORAGE_CODES = 0, 1, 2
-PRECISION_FP64, PRECISION_FP32_REFRESH = 0, 1
+PRECISION_FP64, PRECISION_FP32_REFRESH, PRECISION_FP32 = 0, 1, 2
 
 P_u8 = C.POINTER(C.c_uint8)
 P_i32 = C.POINTER(C.c_int32)

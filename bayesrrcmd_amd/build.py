@@ -44,7 +44,7 @@ def _run(cmd: list[str]) -> None:
 # device translation units: the host API with its small kernels, and one file per
 # kernel family (compiled in parallel)
 HIP_SOURCES = ("api.hip", "kern_sweep_seq.hip", "kern_sweep_async.hip", "kern_sweep_f32.hip", "kern_cohort.hip",
-               "kern_batch.hip", "kern_bayesw.hip")
+               "kern_batch.hip", "kern_batch_f32.hip", "kern_bayesw.hip")
 HOST_SOURCES = ("host_markers.cpp", "host_driver.cpp", "synth.cpp")
 
 

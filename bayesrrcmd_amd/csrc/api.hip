@@ -26,6 +26,7 @@ BRR_SWEEP_KERNELS_OF(BRR_SWEEP_DECL, true)
 BRR_SWEEP_F32_KERNELS(BRR_SWEEP_DECL)
 BRR_COHORT_KERNELS(BRR_COHORT_DECL)
 BRR_BATCH_KERNELS(BRR_BATCH_DECL)
+BRR_BATCH_F32_KERNELS(BRR_BATCH_DECL)
 BRR_BW_KERNELS(BRR_BW_DECL)
 } // namespace brr
 
@@ -378,18 +379,26 @@ __global__ void cohort_reset_kernel(const SweepParams* params)
     if (threadIdx.x < 16) p.counters[threadIdx.x] = 0ull;
 }
 
-// Batched kernels by register tiles per slice (4, 8, 13 or 16 tiles of 16
-// individuals; 13 tiles fit N = 50,000 on 241 workgroups).
-const void* batch_kernel_for(uint32_t NT, int layout)
+// Batched kernels by the 16-individual tiles of a slice: FP64 residual tiles
+// in variants of 4, 8, 13 or 16 tiles (13 fit N = 50,000 on 241 workgroups),
+// FP32 ones in variants of 8, 16, 25 or 32 (25 fit N = 100,000 on 250).
+const void* batch_kernel_for(uint32_t nt, int layout, bool f32)
 {
     const bool d = layout == kLayoutDense;
-    switch (NT) {
-    case 4: return d ? (const void*)bayesr_batch_kernel<4, true> : (const void*)bayesr_batch_kernel<4, false>;
-    case 8: return d ? (const void*)bayesr_batch_kernel<8, true> : (const void*)bayesr_batch_kernel<8, false>;
-    case 13: return d ? (const void*)bayesr_batch_kernel<13, true> : (const void*)bayesr_batch_kernel<13, false>;
-    case 16: return d ? (const void*)bayesr_batch_kernel<16, true> : (const void*)bayesr_batch_kernel<16, false>;
-    default: return nullptr;
+#define BRR_BK(NT, F) (d ? (const void*)bayesr_batch_kernel<NT, true, F> : (const void*)bayesr_batch_kernel<NT, false, F>)
+    if (!f32) {
+        if (nt <= 4) return BRR_BK(4, false);
+        if (nt <= 8) return BRR_BK(8, false);
+        if (nt <= 13) return BRR_BK(13, false);
+        if (nt <= 16) return BRR_BK(16, false);
+        return nullptr;
     }
+    if (nt <= 8) return BRR_BK(8, true);
+    if (nt <= 16) return BRR_BK(16, true);
+    if (nt <= 25) return BRR_BK(25, true);
+    if (nt <= 32) return BRR_BK(32, true);
+    return nullptr;
+#undef BRR_BK
 }
 
 const void* bw_kernel_for(uint32_t R, bool dense)
@@ -592,7 +601,11 @@ int launch_batch(brr_hip_ctx* c, const SweepParams& p)
     b.u = p.u;
     b.z = p.z;
     b.hist = p.hist;
-    return launch_fn(c, batch_kernel_for(c->NT, c->layout), &b);
+    const void* fn = batch_kernel_for(c->nt, c->layout, c->f32);
+    if (!fn)
+        return fail(c, BRR_HIP_ESTATE, "batched slices of more than 256 individuals need the FP32 residual "
+                                       "(brr_hip_set_precision, BRR_HIP_PRECISION_FP32)");
+    return launch_fn(c, fn, &b);
 }
 
 int launch_any(brr_hip_ctx* c, SweepParams& p)
@@ -695,9 +708,9 @@ int configure_shape(brr_hip_ctx* c, uint32_t workgroups, uint32_t block)
         S = (N + Preq - 1) / Preq;
         S = (S + kSliceAlign - 1) / kSliceAlign * kSliceAlign;
         nt = S / 16;
-        if (nt > (uint32_t)kMaxNT) return fail(c, BRR_HIP_EINVAL, "batched slices hold at most 256 individuals");
-        NT = nt <= 4 ? 4u : (nt <= 8 ? 8u : (nt <= 13 ? 13u : 16u));
-        probe = batch_kernel_for(NT, c->layout);
+        if (nt > (uint32_t)kMaxNT) return fail(c, BRR_HIP_EINVAL, "batched slices hold at most 512 individuals");
+        NT = nt;
+        probe = batch_kernel_for(nt, c->layout, c->f32 || nt > (uint32_t)kMaxNT64);
     } else {
         S = lane_run_slice(N, c->cus, workgroups);
         R = (S + kRun - 1) / kRun;
@@ -835,10 +848,10 @@ int brr_hip_create(const brr_hip_config* cfg, brr_hip_ctx** out)
         // ragged columns are held as 2-bit codes
         c->nt = S / 16;
         if (c->nt > (uint32_t)kMaxNT) { delete c; return BRR_HIP_EINVAL; }
-        c->NT = c->nt <= 4 ? 4u : (c->nt <= 8 ? 8u : (c->nt <= 13 ? 13u : 16u));
+        c->NT = c->nt;
         c->codes = true;
         c->B = kBB;
-        probe = batch_kernel_for(c->NT, c->layout);
+        probe = batch_kernel_for(c->nt, c->layout, c->nt > (uint32_t)kMaxNT64);
     } else {
         S = lane_run_slice(N, cus, cfg->workgroups);
         uint32_t R = (S + kRun - 1) / kRun;
@@ -1151,8 +1164,15 @@ int brr_hip_set_precision(brr_hip_ctx* c, int precision, uint32_t refresh_marker
         c->f32 = false;
         return BRR_HIP_OK;
     }
+    if (precision == BRR_HIP_PRECISION_FP32) {
+        if (!c->T) return fail(c, BRR_HIP_EINVAL, "BRR_HIP_PRECISION_FP32 is the batched contexts' mode "
+                                                  "(single chains: BRR_HIP_PRECISION_FP32_REFRESH)");
+        c->f32 = true;
+        return BRR_HIP_OK;
+    }
     if (precision != BRR_HIP_PRECISION_FP32_REFRESH) return fail(c, BRR_HIP_EINVAL, "unknown precision");
-    if (c->T) return fail(c, BRR_HIP_EINVAL, "the FP32 residual mode needs a single-chain context");
+    if (c->T) return fail(c, BRR_HIP_EINVAL, "the FP32-with-refresh mode needs a single-chain context "
+                                             "(batched contexts: BRR_HIP_PRECISION_FP32)");
     if (c->async) return fail(c, BRR_HIP_EINVAL, "the FP32 residual mode runs the sequential sweep only");
     if (refresh_markers < 1 || refresh_markers > (uint32_t)kMaxLog)
         return fail(c, BRR_HIP_EINVAL, "refresh interval must be 1..1024 markers");

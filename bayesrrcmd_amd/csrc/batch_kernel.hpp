@@ -27,13 +27,16 @@
 // So one exchange per block of 32 markers serves all T traits.
 #pragma once
 
+#include <type_traits>
+
 #include "sweep_kernel.hpp"
 
 namespace brr {
 
 constexpr int kBB = 32;                      // markers per block (two 16-marker tiles)
 constexpr int kMaxT = 64;                    // traits per launch
-constexpr int kMaxNT = 16;                   // 16-individual tiles per workgroup slice
+constexpr int kMaxNT = 32;                   // 16-individual tiles per workgroup slice (FP32 residual)
+constexpr int kMaxNT64 = 16;                 // ... with the FP64 residual
 constexpr int kBRow = 4 * kMaxNT + 4;        // LDS bytes per code row (padded: rows fall in distinct banks)
 constexpr int kBGram = kBB * (kBB - 1) / 2;  // strictly-upper Gram entries of a block
 constexpr uint32_t kBVmax = (uint32_t)(kBB * kMaxT + kBGram);   // values exchanged per block
@@ -43,6 +46,7 @@ constexpr uint32_t kBVmax = (uint32_t)(kBB * kMaxT + kBGram);   // values exchan
 constexpr int kBPhases = 12;
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
 
 struct BatchParams {
     uint32_t N, M, K, T;
@@ -150,6 +154,23 @@ __device__ __forceinline__ dbl4 mfma16x16x4(double a, double b, dbl4 c)
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+__device__ __forceinline__ flt4 mfma16x16x4(float a, float b, flt4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// Row of a 16 x 16 matrix-core accumulator held in entry r of lane (kq, tq)
+// (column tq): the f64 instruction interleaves rows over the lane groups
+// (4 r + kq), the f32 one blocks them (4 kq + r), as measured on the device.
+template <bool kF32>
+__device__ __forceinline__ int acc_row(int r, int kq)
+{
+    return kF32 ? 4 * kq + r : 4 * r + kq;
+}
+
+__device__ __forceinline__ dbl4 to_dbl4(flt4 x) { return dbl4{(double)x[0], (double)x[1], (double)x[2], (double)x[3]}; }
+__device__ __forceinline__ dbl4 to_dbl4(dbl4 x) { return x; }
+
 // Reduce-scatter of the V values published at round rnd. Chunks of 8
 // consecutive values belong to workgroup (chunk % P); thread (g, e) of the
 // owner sums partial (w2, 8 chunk + e) for w2 = g, g + 32, ... in order, the
@@ -207,8 +228,10 @@ __device__ __forceinline__ bool ballreduce_traits(const BatchParams& p, BShared&
 
 // Everything of the next block that can be fetched before it is needed, in
 // registers: code words, per-marker tables and beta of (marker, trait) pairs.
+constexpr int kBWords = kBB * kMaxNT / kThreads;   // code words of a block per thread
+
 struct BNext {
-    uint32_t words[2];          // kBB * kMaxNT words over 256 threads
+    uint32_t words[kBWords];    // kBB * kMaxNT words over 256 threads
     double m0, m1, m2, m3;      // dense: table by dosage; ragged: mean, sd, Zsum
     double b[8];                // beta of pair u * 256 + tid = (k = e >> 6, t = e & 63)
 };
@@ -218,7 +241,7 @@ __device__ __forceinline__ void bissue(const BatchParams& p, const uint32_t* ids
 {
     const uint32_t per = p.nt;   // 32-bit words (16 individuals each) per marker row
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
+    for (int u = 0; u < kBWords; ++u) {
         const uint32_t t = (uint32_t)u * kThreads + (uint32_t)tid;
         x.words[u] = 0u;
         if (t < (uint32_t)nb * per) {
@@ -252,7 +275,7 @@ __device__ __forceinline__ void bstore(const BatchParams& p, BBlock& L, const BN
 {
     const uint32_t per = p.nt;
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
+    for (int u = 0; u < kBWords; ++u) {
         const uint32_t t = (uint32_t)u * kThreads + (uint32_t)tid;
         if (t < (uint32_t)nb * per) *(uint32_t*)&L.codes[t / per][4u * (t % per)] = x.words[u];
     }
@@ -397,10 +420,17 @@ __device__ __forceinline__ BDraw bdraw(const BShared& L, int t, int K, double nu
 
 // -------------------------------------------------------------------- kernel --
 
-template <int NT, bool kDenseMath>
+// kF32: the residual tiles are FP32 on the f32 matrix-core instruction (up to
+// 32 tiles, 512 individuals, per workgroup instead of 16); every 16-individual
+// tile's FP32 dot and Gram products are accumulated across tiles in FP64, and
+// the decisions, exchange and scalar draws stay FP64. The residual is not
+// refreshed: its FP32 rounding accumulates over the chain.
+template <int NT, bool kDenseMath, bool kF32>
 __global__ void __launch_bounds__(kThreads)
 bayesr_batch_kernel(BatchParams p)
 {
+    using ET = typename std::conditional<kF32, float, double>::type;
+    using ET4 = typename std::conditional<kF32, flt4, dbl4>::type;
     __shared__ BShared L;
     const uint32_t w = blockIdx.x;
     const int tid = threadIdx.x;
@@ -422,18 +452,19 @@ bayesr_batch_kernel(BatchParams p)
     constexpr bool dense = kDenseMath;
 
     // residuals in the matrix-core accumulator layout (lane = 16 * (row % 4) + column,
-    // entry r = row / 4): E[s][r] = eps of individual lo + 16 s + 4 r + kq, trait tl.
+    // entry r = row / 4): E[s][r] = eps of individual lo + 16 s + 4 r + kq, trait tl
+    // (FP32 tiles: lo + 16 s + 4 kq + r, the f32 instruction's blocked rows).
     // As the B operand of step (s, r) the same registers hold individuals
     // 16 s + 4 r + 0..3 (k index kq), so a tile feeds the dots without shuffles.
-    dbl4 E[NT];
+    ET4 E[NT];
 #pragma unroll
     for (int s = 0; s < NT; ++s)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const uint32_t i = 16u * (uint32_t)s + 4u * (uint32_t)r + (uint32_t)kq;
+            const uint32_t i = 16u * (uint32_t)s + (uint32_t)acc_row<kF32>(r, kq);
             double e = 0.0;
             if (s < nt && i < nloc && tval) e = p.eps[(size_t)(lo + i) * T + tl];
-            E[s][r] = e;
+            E[s][r] = (ET)e;
         }
 
     if (tid < kMaxT) {
@@ -473,15 +504,15 @@ bayesr_batch_kernel(BatchParams p)
             for (int s = 0; s < NT; ++s)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const uint32_t i = 16u * (uint32_t)s + 4u * (uint32_t)r + (uint32_t)kq;
-                    if (s < nt && i < nloc && tval) E[s][r] = E[s][r] + om;
+                    const uint32_t i = 16u * (uint32_t)s + (uint32_t)acc_row<kF32>(r, kq);
+                    if (s < nt && i < nloc && tval) E[s][r] = (ET)((double)E[s][r] + om);
                 }
             if (dense) {
                 double part = 0.0;
 #pragma unroll
                 for (int s = 0; s < NT; ++s)
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) part += E[s][r];
+                    for (int r = 0; r < 4; ++r) part += (double)E[s][r];
                 part += __shfl_xor(part, 16, 64);
                 part += __shfl_xor(part, 32, 64);
                 ++rnd;
@@ -504,8 +535,8 @@ bayesr_batch_kernel(BatchParams p)
             for (int s = 0; s < NT; ++s)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const uint32_t i = 16u * (uint32_t)s + 4u * (uint32_t)r + (uint32_t)kq;
-                    if (s < nt && i < nloc && tval) E[s][r] = E[s][r] - nm;
+                    const uint32_t i = 16u * (uint32_t)s + (uint32_t)acc_row<kF32>(r, kq);
+                    if (s < nt && i < nloc && tval) E[s][r] = (ET)((double)E[s][r] - nm);
                 }
             __syncthreads();
             if (tid < T) {
@@ -582,37 +613,69 @@ bayesr_batch_kernel(BatchParams p)
             // registers, so each matrix-core step decodes its operand with a few
             // VALU selects instead of dependent LDS reads.
             {
-                uint32_t rw0[NT], rw1[NT];
+                // (FP32 tiles: up to 32 of them, so their code words are read from LDS
+                // tile by tile instead of being held in registers)
+                constexpr int NR = kF32 ? 1 : NT;
+                uint32_t rw0[NR], rw1[NR];
+                if constexpr (!kF32) {
 #pragma unroll
-                for (int s = 0; s < NT; ++s) {
-                    rw0[s] = (s < nt) ? *(const uint32_t*)&B.codes[tq][4 * s] : 0u;
-                    rw1[s] = (s < nt) ? *(const uint32_t*)&B.codes[16 + tq][4 * s] : 0u;
+                    for (int s = 0; s < NT; ++s) {
+                        rw0[s] = (s < nt) ? *(const uint32_t*)&B.codes[tq][4 * s] : 0u;
+                        rw1[s] = (s < nt) ? *(const uint32_t*)&B.codes[16 + tq][4 * s] : 0u;
+                    }
                 }
                 const double w00 = B.wt[tq][0], w01 = B.wt[tq][1], w02 = B.wt[tq][2], w03 = B.wt[tq][3];
                 const double w10 = B.wt[16 + tq][0], w11 = B.wt[16 + tq][1], w12 = B.wt[16 + tq][2],
                              w13 = B.wt[16 + tq][3];
                 if (wave < (int)p.nq) {
-                    // two accumulator chains per marker tile (even / odd residual tiles)
-                    dbl4 acc0[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
-                    dbl4 acc1[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
-                    // straight-line code (no per-step branches, which would force the
-                    // accumulators out of the matrix-core registers at each join):
-                    // tiles past the slice hold zero residuals, slots past the block
-                    // zero weights
+                    dbl4 c0, c1;
+                    if constexpr (!kF32) {
+                        // two accumulator chains per marker tile (even / odd residual tiles)
+                        dbl4 acc0[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+                        dbl4 acc1[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+                        // straight-line code (no per-step branches, which would force the
+                        // accumulators out of the matrix-core registers at each join):
+                        // tiles past the slice hold zero residuals, slots past the block
+                        // zero weights
 #pragma unroll
-                    for (int s = 0; s < NT; ++s) {
+                        for (int s = 0; s < NT; ++s) {
 #pragma unroll
-                        for (int r = 0; r < 4; ++r) {
-                            const uint32_t sh = 8u * (uint32_t)r + 2u * (uint32_t)kq;
-                            acc0[s & 1] = mfma16x16x4(wt_of((rw0[s] >> sh) & 3u, w00, w01, w02, w03), E[s][r], acc0[s & 1]);
-                            acc1[s & 1] = mfma16x16x4(wt_of((rw1[s] >> sh) & 3u, w10, w11, w12, w13), E[s][r], acc1[s & 1]);
+                            for (int r = 0; r < 4; ++r) {
+                                const uint32_t sh = 8u * (uint32_t)r + 2u * (uint32_t)kq;
+                                acc0[s & 1] = mfma16x16x4(wt_of((rw0[s < NR ? s : 0] >> sh) & 3u, w00, w01, w02, w03), E[s][r],
+                                                          acc0[s & 1]);
+                                acc1[s & 1] = mfma16x16x4(wt_of((rw1[s < NR ? s : 0] >> sh) & 3u, w10, w11, w12, w13), E[s][r],
+                                                          acc1[s & 1]);
+                            }
+                        }
+                        c0 = acc0[0] + acc0[1];
+                        c1 = acc1[0] + acc1[1];
+                    } else {
+                        // per residual tile an FP32 product of 16 individuals, the tiles
+                        // summed in FP64; entry r of the B operand is individual 4 kq + r
+                        const float f00 = (float)w00, f01 = (float)w01, f02 = (float)w02, f03 = (float)w03;
+                        const float f10 = (float)w10, f11 = (float)w11, f12 = (float)w12, f13 = (float)w13;
+                        c0 = dbl4{0.0, 0.0, 0.0, 0.0};
+                        c1 = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                        for (int s = 0; s < NT; ++s) {
+                            const uint32_t a0 = (s < nt) ? *(const uint32_t*)&B.codes[tq][4 * s] : 0u;
+                            const uint32_t a1 = (s < nt) ? *(const uint32_t*)&B.codes[16 + tq][4 * s] : 0u;
+                            flt4 t0 = {0.f, 0.f, 0.f, 0.f}, t1 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) {
+                                const uint32_t sh = 8u * (uint32_t)kq + 2u * (uint32_t)r;
+                                t0 = mfma16x16x4(wt_of_f((a0 >> sh) & 3u, f00, f01, f02, f03), (float)E[s][r], t0);
+                                t1 = mfma16x16x4(wt_of_f((a1 >> sh) & 3u, f10, f11, f12, f13), (float)E[s][r], t1);
+                            }
+                            c0 += to_dbl4(t0);
+                            c1 += to_dbl4(t1);
                         }
                     }
-                    const dbl4 c0 = acc0[0] + acc0[1], c1 = acc1[0] + acc1[1];
                     if (tval) {
 #pragma unroll
                         for (int r = 0; r < 4; ++r) {
-                            const int k0 = kq + 4 * r;
+                            const int k0 = acc_row<kF32>(r, kq);
                             if (k0 < nb) publish_pair(p.xchg1, bslot1(p, rnd, w, (uint32_t)(k0 * T + tl)), rnd, c0[r]);
                             if (16 + k0 < nb)
                                 publish_pair(p.xchg1, bslot1(p, rnd, w, (uint32_t)((16 + k0) * T + tl)), rnd, c1[r]);
@@ -628,24 +691,47 @@ bayesr_batch_kernel(BatchParams p)
                         const double u0 = B.ut[ra][0], u1 = B.ut[ra][1], u2 = B.ut[ra][2], u3 = B.ut[ra][3];
                         const double b0 = mc ? w10 : w00, b1 = mc ? w11 : w01, b2 = mc ? w12 : w02,
                                      b3 = mc ? w13 : w03;
-                        dbl4 gg[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+                        dbl4 g;
+                        if constexpr (!kF32) {
+                            dbl4 gg[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
 #pragma unroll
-                        for (int s = 0; s < NT; ++s) {
-                            const uint32_t rowa = mi ? rw1[s] : rw0[s];
-                            const uint32_t rowb = mc ? rw1[s] : rw0[s];
+                            for (int s = 0; s < NT; ++s) {
+                                const uint32_t rowa = mi ? rw1[s < NR ? s : 0] : rw0[s < NR ? s : 0];
+                                const uint32_t rowb = mc ? rw1[s < NR ? s : 0] : rw0[s < NR ? s : 0];
 #pragma unroll
-                            for (int r = 0; r < 4; ++r) {
-                                const uint32_t sh = 8u * (uint32_t)r + 2u * (uint32_t)kq;
-                                const uint32_t i = 16u * (uint32_t)s + 4u * (uint32_t)r + (uint32_t)kq;
-                                const double av = (i < nloc) ? wt_of((rowa >> sh) & 3u, u0, u1, u2, u3) : 0.0;
-                                gg[r & 1] = mfma16x16x4(av, wt_of((rowb >> sh) & 3u, b0, b1, b2, b3), gg[r & 1]);
+                                for (int r = 0; r < 4; ++r) {
+                                    const uint32_t sh = 8u * (uint32_t)r + 2u * (uint32_t)kq;
+                                    const uint32_t i = 16u * (uint32_t)s + 4u * (uint32_t)r + (uint32_t)kq;
+                                    const double av = (i < nloc) ? wt_of((rowa >> sh) & 3u, u0, u1, u2, u3) : 0.0;
+                                    gg[r & 1] = mfma16x16x4(av, wt_of((rowb >> sh) & 3u, b0, b1, b2, b3), gg[r & 1]);
+                                }
+                            }
+                            g = gg[0] + gg[1];
+                        } else {
+                            // per tile an FP32 product, the tiles summed in FP64
+                            const float fu0 = (float)u0, fu1 = (float)u1, fu2 = (float)u2, fu3 = (float)u3;
+                            const float fb0 = (float)b0, fb1 = (float)b1, fb2 = (float)b2, fb3 = (float)b3;
+                            g = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                            for (int s = 0; s < NT; ++s) {
+                                const uint32_t ra0 = (s < nt) ? *(const uint32_t*)&B.codes[16 * (int)mi + tq][4 * s] : 0u;
+                                const uint32_t rb0 = (s < nt) ? *(const uint32_t*)&B.codes[16 * (int)mc + tq][4 * s] : 0u;
+                                const uint32_t rowa = ra0, rowb = rb0;
+                                flt4 gt = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                                for (int r = 0; r < 4; ++r) {
+                                    const uint32_t sh = 8u * (uint32_t)r + 2u * (uint32_t)kq;
+                                    const uint32_t i = 16u * (uint32_t)s + 4u * (uint32_t)r + (uint32_t)kq;
+                                    const float av = (i < nloc) ? wt_of_f((rowa >> sh) & 3u, fu0, fu1, fu2, fu3) : 0.f;
+                                    gt = mfma16x16x4(av, wt_of_f((rowb >> sh) & 3u, fb0, fb1, fb2, fb3), gt);
+                                }
+                                g += to_dbl4(gt);
                             }
                         }
-                        const dbl4 g = gg[0] + gg[1];
                         const int CT = nb * T;
 #pragma unroll
                         for (int r = 0; r < 4; ++r) {
-                            const int i = 16 * (int)mi + kq + 4 * r, k = 16 * (int)mc + tq;
+                            const int i = 16 * (int)mi + acc_row<kF32>(r, kq), k = 16 * (int)mc + tq;
                             if (i < k && k < nb)
                                 publish_pair(p.xchg1, bslot1(p, rnd, w, (uint32_t)(CT + k * (k - 1) / 2 + i)), rnd, g[r]);
                         }
@@ -811,9 +897,9 @@ bayesr_batch_kernel(BatchParams p)
                     constexpr int HS = (NT + 1) / 2;
 #pragma unroll
                     for (int h = 0; h < 2; ++h) {
-                        dbl4 acc[HS];
+                        ET4 acc[HS];
 #pragma unroll
-                        for (int i2 = 0; i2 < HS; ++i2) acc[i2] = dbl4{0.0, 0.0, 0.0, 0.0};
+                        for (int i2 = 0; i2 < HS; ++i2) acc[i2] = ET4{0, 0, 0, 0};
                         for (uint32_t m = fmask; m; m &= m - 1u) {
                             const int km = 4 * __builtin_ctz(m) + kq;
                             const bool kv = km < nb;
@@ -823,16 +909,17 @@ bayesr_batch_kernel(BatchParams p)
                                 const int s2 = h * HS + i2;
                                 row[i2] = (s2 < NT && s2 < nt && kv) ? *(const uint32_t*)&B.codes[km][4 * s2] : 0u;
                             }
-                            const double u0 = B.ut[km][0], u1 = B.ut[km][1], u2 = B.ut[km][2], u3 = B.ut[km][3];
-                            const double bv = kv ? L.cd[km][16 * wave + tq] : 0.0;
+                            const ET u0 = (ET)B.ut[km][0], u1 = (ET)B.ut[km][1], u2 = (ET)B.ut[km][2],
+                                     u3 = (ET)B.ut[km][3];
+                            const ET bv = kv ? (ET)L.cd[km][16 * wave + tq] : (ET)0;
                             // straight-line: rows past the slice are masked to zero by i < nloc
 #pragma unroll
                             for (int i2 = 0; i2 < HS; ++i2) {
                                 const int s2 = h * HS + i2;
                                 if (s2 < NT) {
                                     const uint32_t i = 16u * (uint32_t)s2 + (uint32_t)tq;
-                                    const double av =
-                                        (i < nloc) ? wt_of((row[i2] >> (2u * (uint32_t)tq)) & 3u, u0, u1, u2, u3) : 0.0;
+                                    const ET av =
+                                        (i < nloc) ? wt_t((row[i2] >> (2u * (uint32_t)tq)) & 3u, u0, u1, u2, u3) : (ET)0;
                                     acc[i2] = mfma16x16x4(av, bv, acc[i2]);
                                 }
                             }
@@ -865,7 +952,10 @@ bayesr_batch_kernel(BatchParams p)
 #pragma unroll
             for (int s = 0; s < NT; ++s)
 #pragma unroll
-                for (int r = 0; r < 4; ++r) part += E[s][r] * E[s][r];
+                for (int r = 0; r < 4; ++r) {
+                    const double e = (double)E[s][r];
+                    part += e * e;
+                }
             part += __shfl_xor(part, 16, 64);
             part += __shfl_xor(part, 32, 64);
             ++rnd;
@@ -917,8 +1007,8 @@ done:
         for (int s = 0; s < NT; ++s)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const uint32_t i = 16u * (uint32_t)s + 4u * (uint32_t)r + (uint32_t)kq;
-                if (s < nt && i < nloc && tval) p.eps[(size_t)(lo + i) * T + tl] = E[s][r];
+                const uint32_t i = 16u * (uint32_t)s + (uint32_t)acc_row<kF32>(r, kq);
+                if (s < nt && i < nloc && tval) p.eps[(size_t)(lo + i) * T + tl] = (double)E[s][r];
             }
         if (w == 0) {
             if (tid < T) {

@@ -26,9 +26,12 @@
 // variant gave chains that differ from run to run and stays out)
 #define BRR_COHORT_KERNELS(X) X(1, false) X(2, false) X(1, true) X(2, true)
 
-// bayesr_batch_kernel<NT, dense>
-#define BRR_BATCH_KERNELS(X) X(4, true) X(4, false) X(8, true) X(8, false) X(13, true) X(13, false) \
-    X(16, true) X(16, false)
+// bayesr_batch_kernel<NT, dense, fp32>: FP64 residual tiles (up to 16 per slice)
+#define BRR_BATCH_KERNELS(X) X(4, true, false) X(4, false, false) X(8, true, false) X(8, false, false) \
+    X(13, true, false) X(13, false, false) X(16, true, false) X(16, false, false)
+// FP32 residual tiles (up to 32 per slice)
+#define BRR_BATCH_F32_KERNELS(X) X(8, true, true) X(8, false, true) X(16, true, true) X(16, false, true) \
+    X(25, true, true) X(25, false, true) X(32, true, true) X(32, false, true)
 
 // bayesw_sweep_kernel<R, dense>
 #define BRR_BW_KERNELS(X) X(1, true) X(1, false) X(2, true) X(2, false) X(4, true) X(4, false)
@@ -37,7 +40,7 @@
 #define BRR_SWEEP_DECL(R, L, C, A, F) extern template __global__ void bayesr_sweep_kernel<R, L, C, A, F>(SweepParams);
 #define BRR_COHORT_INST(R, F) template __global__ void bayesr_cohort_kernel<R, F>(CohortTable);
 #define BRR_COHORT_DECL(R, F) extern template __global__ void bayesr_cohort_kernel<R, F>(CohortTable);
-#define BRR_BATCH_INST(NT, D) template __global__ void bayesr_batch_kernel<NT, D>(BatchParams);
-#define BRR_BATCH_DECL(NT, D) extern template __global__ void bayesr_batch_kernel<NT, D>(BatchParams);
+#define BRR_BATCH_INST(NT, D, F) template __global__ void bayesr_batch_kernel<NT, D, F>(BatchParams);
+#define BRR_BATCH_DECL(NT, D, F) extern template __global__ void bayesr_batch_kernel<NT, D, F>(BatchParams);
 #define BRR_BW_INST(R, D) template __global__ void bayesw_sweep_kernel<R, D>(BwParams);
 #define BRR_BW_DECL(R, D) extern template __global__ void bayesw_sweep_kernel<R, D>(BwParams);

@@ -54,7 +54,11 @@ CONFIGS = {
     "c3": dict(workload="batched traits: 64 BayesR chains (one phenotype each) over one dense .bed 2-bit "
                         "matrix, N=50000 M=50000, shared marker order",
                n=50_000, m=50_000, layout="dense", maf=(0.01, 0.5), traits=64, causal=(0.001, 0.02),
-               h2=(0.1, 0.8), chain=100),
+               h2=(0.1, 0.8), chain=100, precision="fp32"),
+    "c3-n100k": dict(workload="batched traits at the plain workload's size: 64 BayesR chains over one dense "
+                              ".bed 2-bit matrix, N=100000 M=100000, shared marker order",
+                     n=100_000, m=100_000, layout="dense", maf=(0.005, 0.5), traits=64, causal=(0.001, 0.02),
+                     h2=(0.1, 0.8), chain=20, precision="fp32"),
     "c5": dict(workload="BayesW Weibull survival, ragged layout, N=100000 M=50000, single chain",
                n=100_000, m=50_000, layout="ragged", maf=(0.005, 0.5), causal=0.005, h2=0.5, alpha=3.0,
                mu=4.0, censor=0.1, C=(0.001, 0.01, 0.1), quad_points=7, chain=100),
@@ -504,13 +508,16 @@ def run_batched(cfg, args, with_cpu):
     """The batched workload on the device: returns the fields of its JSON line."""
     import bayesrrcmd_amd as brr
     n, m, T = cfg["n"], cfg["m"], cfg["traits"]
-    steps = args.steps if (args.steps is not None and args.config == "c3") else cfg["chain"]
+    steps = args.steps if (args.steps is not None and args.config.startswith("c3")) else cfg["chain"]
+    precision = args.precision if args.config.startswith("c3") and args.precision else cfg.get("precision", "fp64")
     data = make_batch_data(cfg, args.seed, args.threads)
     h = brr.HipSweep(n, m, 4, 1, brr.LAYOUT_DENSE_2BIT, workgroups=args.workgroups, traits=T)
     t0 = time.time()
     h.upload_dense(data["bed"], data["lut"])
     h.set_model(S_DEFAULT)
-    log(f"batched: uploaded in {time.time() - t0:.1f}s")
+    if precision == "fp32":
+        h.set_precision(brr.PRECISION_FP32)
+    log(f"batched N={n}: uploaded in {time.time() - t0:.1f}s; precision {precision}")
 
     def fresh_chain(seed):
         eps = np.empty((n, T))
@@ -548,10 +555,11 @@ def run_batched(cfg, args, with_cpu):
     stats = h.stats()
     hist = np.array(hist)                                  # (steps, T, 5)
     avg_ms = float(np.mean(kernel_ms))
-    # B_j = G_j + 8 N T + 8 N (changes of marker j over the traits): the column
-    # once for all traits, one FP64 eps read per trait, one more per change
+    # B_j = G_j + e N T + e N (changes of marker j over the traits): the column
+    # once for all traits, one eps read per trait (e = 4 bytes FP32, 8 FP64), one more per change
     g_col = float((n + 3) // 4)
-    bytes_per_launch = float(np.mean([m * g_col + 8.0 * n * (T * m + c) for c in changes]))
+    e_b = 4.0 if precision == "fp32" else 8.0
+    bytes_per_launch = float(np.mean([m * g_col + e_b * n * (T * m + c) for c in changes]))
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     # matrix-core work per launch: dots 2 N T per marker, Gram 2 N per pair of a 32-marker block
     flops = 2.0 * n * T * m + 2.0 * n * (m / 32.0) * (32 * 31 / 2)
@@ -563,11 +571,13 @@ def run_batched(cfg, args, with_cpu):
         "unit": "marker-updates/s",
         "steps": steps,
         "ms_per_step": wall * 1e3 / steps,
-        "dtype": "f64",
+        "dtype": "f32+f64" if precision == "fp32" else "f64",
         "config": {
             "workload": cfg["workload"], "N": n, "M": m, "traits": T, "layout": "dense 2-bit",
             "maf": list(cfg["maf"]), "causal_fraction_range": list(cfg["causal"]), "h2_range": list(cfg["h2"]),
-            "S": S_DEFAULT.tolist(), "timed_iterations": steps, "precision": "fp64",
+            "S": S_DEFAULT.tolist(), "timed_iterations": steps,
+            "precision": ("fp32 residual tiles (f32 matrix cores, per-tile products summed in fp64), fp64 "
+                          "decisions and draws, no refresh" if precision == "fp32" else "fp64"),
             "workgroups": stats["workgroups"], "block_markers": stats["block_markers"], "slice": stats["slice"],
             "parallelism": "single GPU, persistent cooperative kernel, all traits in one launch",
             "h2_abs_err_mean": float(np.mean(np.abs(h2_post - data["h2"]))),
@@ -577,7 +587,8 @@ def run_batched(cfg, args, with_cpu):
             "frac": achieved / HBM_PEAK_GBS, "traffic": None,
             "kernel": "brr::bayesr_batch_kernel (one launch per iteration)",
             "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": bytes_per_launch,
-            "mfma_fp64_tflops": flops / (avg_ms * 1e-3) / 1e12, "mfma_fp64_peak_tflops": 78.6,
+            "mfma_tflops": flops / (avg_ms * 1e-3) / 1e12,
+            "mfma_peak_tflops": 157.3 if precision == "fp32" else 78.6,
             "rounds_per_launch": float(np.mean(rounds)), "changes_per_launch": float(np.mean(changes)),
             "phase_ms_per_launch": dict(zip(brr.HipSweep.BATCH_PHASES, np.mean(phases, axis=0).round(3).tolist())),
         },
@@ -830,14 +841,14 @@ def main():
         log(f"genotypes {cfg['m']} x {cfg['n']} generated in {time.time() - t0:.1f}s")
         print(json.dumps({"decode": run_decode(cfg, {"bed": bed}, args)}), flush=True)
         return
-    if args.config in ("c3", "c4", "c5", "c5-dense"):
-        run = {"c3": run_batched, "c4": run_cohorts}.get(args.config, run_bayesw)
+    if args.config in ("c3", "c3-n100k", "c4", "c5", "c5-dense"):
+        run = {"c3": run_batched, "c3-n100k": run_batched, "c4": run_cohorts}.get(args.config, run_bayesw)
         b = run(cfg, args, not args.no_cpu)
         line = {"metric": METRIC, "value": b["value"], "unit": b["unit"], "n_gpus": 1, "steps": b["steps"],
                 "warmup": args.warmup, "ms_per_step": b["ms_per_step"], "higher_is_better": True,
                 "scaling": "weak", "vs_baseline": None, "dtype": b["dtype"],
                 "data": ("synthetic (Philox-seeded PLINK genotypes, HWE, 64 phenotypes y = X beta + e)"
-                         if args.config == "c3" else
+                         if args.config.startswith("c3") else
                          "synthetic (Philox-seeded PLINK genotypes, HWE, one phenotype y = X beta + e per cohort)"
                          if args.config == "c4" else
                          "synthetic (Philox-seeded PLINK genotypes, HWE, Weibull log-times, 10% censored)"),
@@ -967,6 +978,7 @@ def main():
         del data
         data = None
         line["batched"] = run_batched(CONFIGS["c3"], args, not args.no_cpu)
+        line["batched_n100k"] = run_batched(CONFIGS["c3-n100k"], args, False)
     if args.config == "c2" and not args.no_bayesw:
         data = None
         line["bayesw"] = run_bayesw(CONFIGS["c5"], args, not args.no_cpu)

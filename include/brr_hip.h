@@ -82,7 +82,8 @@ typedef struct {
     uint32_t num_traits;     /* 0 = one chain. 1..64 = T chains (traits, one phenotype
                                 each) over the same genotype columns and marker order,
                                 swept together; needs num_groups == 1 and at most
-                                16 * 16 individuals per workgroup. The per-individual and
+                                256 individuals per workgroup (512 with the FP32
+                                residual, BRR_HIP_PRECISION_FP32). The per-individual and
                                 per-marker arrays of the state calls are then N x T and
                                 M x T, trait fastest. */
 } brr_hip_config;
@@ -182,7 +183,13 @@ int brr_hip_get_marker_tables(brr_hip_ctx* ctx, double* out);
  * or ragged with ragged_storage 0 or 2). */
 enum brr_hip_precision {
     BRR_HIP_PRECISION_FP64 = 0,
-    BRR_HIP_PRECISION_FP32_REFRESH = 1
+    BRR_HIP_PRECISION_FP32_REFRESH = 1,
+    /* Batched contexts: the residual tiles of every trait in FP32 on the f32
+     * matrix-core instruction (slices of up to 512 individuals instead of 256,
+     * so N up to 131,072 on 256 workgroups); each 16-individual tile's FP32 dot
+     * and Gram products are summed over the tiles in FP64, decisions and draws
+     * are FP64; the residual is not refreshed, its FP32 rounding accumulates. */
+    BRR_HIP_PRECISION_FP32 = 2
 };
 int brr_hip_set_precision(brr_hip_ctx* ctx, int precision, uint32_t refresh_markers);
 

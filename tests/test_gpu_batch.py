@@ -271,8 +271,8 @@ def test_batch_bad_configs(gpu_lib):
         brr.HipSweep(pr.n, pr.m, 4, 1, brr.LAYOUT_DENSE_2BIT, traits=65)
     with pytest.raises(brr.BrrHipError):
         brr.HipSweep(pr.n, pr.m, 4, 2, brr.LAYOUT_DENSE_2BIT, traits=4)
-    with pytest.raises(brr.BrrHipError):   # 4000 / 8 = 500 individuals per workgroup > 256
-        brr.HipSweep(pr.n, pr.m, 4, 1, brr.LAYOUT_DENSE_2BIT, workgroups=8, traits=4)
+    with pytest.raises(brr.BrrHipError):   # 4000 / 7 -> 576 individuals per workgroup > 512
+        brr.HipSweep(pr.n, pr.m, 4, 1, brr.LAYOUT_DENSE_2BIT, workgroups=7, traits=4)
     h = brr.HipSweep(pr.n, pr.m, 4, 1, brr.LAYOUT_DENSE_2BIT, traits=4)
     h.upload_dense(pr.bed, pr.lut)
     h.set_model(S)
