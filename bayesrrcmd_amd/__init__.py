@@ -20,6 +20,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import sys
 from dataclasses import dataclass
 
 import numpy as np
@@ -326,6 +327,14 @@ def run_cohorts(sweeps, seed: int, first_iteration: int, orders, chains=None, gr
     return hists, dict(launches=rep.launches, device_ms=rep.device_ms, launch_of=list(rep.launch_of[: min(n, 64)]))
 
 
+def calibration_copy(nbytes: int = 1 << 30, device: int = 0) -> float:
+    """One device-to-device copy of nbytes by a kernel of this library (a reference
+    for HBM counters); returns its device milliseconds."""
+    ms = C.c_double()
+    _check(lib().brr_hip_calibration_copy(int(device), int(nbytes), C.byref(ms)), "brr_hip_calibration_copy")
+    return ms.value
+
+
 def bed_file_image(bed: np.ndarray) -> np.ndarray:
     """A PLINK .bed file's bytes (magic 6c 1b 01, then the SNP-major columns) from M x ceil(N/4) column bytes."""
     bed = np.ascontiguousarray(bed, dtype=np.uint8)
@@ -390,6 +399,10 @@ class HipSweep(_DeviceMarkers):
             self.ctx = None
 
     def __del__(self):
+        # no HIP calls while the interpreter shuts down: a profiler or the runtime
+        # may already be torn down by then
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -483,6 +496,12 @@ class HipSweep(_DeviceMarkers):
                  "brr_hip_autotune")
         return [dict(workgroups=e.workgroups, block_markers=e.block_markers, slice=e.slice,
                      ms_per_iteration=e.ms_per_iteration) for e in tab[: n.value]]
+
+    def probe_exchange(self, rounds: int = 2000) -> float:
+        """Device microseconds per exchange round of a full block with no other work."""
+        out = C.c_double()
+        self._ck(lib().brr_hip_probe_exchange(self.ctx, int(rounds), C.byref(out)), "brr_hip_probe_exchange")
+        return out.value
 
     def record_num(self, enable: bool = True):
         """Record each marker's num (computeNum) in later sweeps; read with get_num()."""
@@ -615,6 +634,10 @@ class HipBayesW(_DeviceMarkers):
             self.ctx = None
 
     def __del__(self):
+        # no HIP calls while the interpreter shuts down: a profiler or the runtime
+        # may already be torn down by then
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:

@@ -85,6 +85,8 @@ SIGNATURES = {
     "brr_hip_set_async": (C.c_int, [CTX, C.c_uint32]),
     "brr_hip_set_precision": (C.c_int, [CTX, C.c_int, C.c_uint32]),
     "brr_hip_record_num": (C.c_int, [CTX, C.c_int]),
+    "brr_hip_probe_exchange": (C.c_int, [CTX, C.c_uint32, P_f64]),
+    "brr_hip_calibration_copy": (C.c_int, [C.c_uint32, C.c_uint64, P_f64]),
     "brr_hip_set_launch": (C.c_int, [CTX, C.c_uint32, C.c_uint32]),
     "brr_hip_run_cohorts": (C.c_int, [C.POINTER(CTX), C.c_uint32, C.c_uint64, P_u32, C.c_uint32, C.c_uint32,
                                       C.POINTER(P_u32), C.POINTER(P_f64), C.c_uint32, C.POINTER(CohortReport)]),

@@ -1473,6 +1473,60 @@ int brr_hip_run_cohorts(brr_hip_ctx* const* ctxs, uint32_t n, uint64_t seed, con
     return status;
 }
 
+int brr_hip_probe_exchange(brr_hip_ctx* c, uint32_t rounds, double* us_per_round)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if ((rc = need_kind(c, false))) return rc;
+    if (c->T) return fail(c, BRR_HIP_EINVAL, "the exchange probe runs on single-chain contexts");
+    if (!us_per_round || rounds < 1 || rounds > (1u << 20)) return fail(c, BRR_HIP_EINVAL, "1..1048576 rounds");
+    HIP_TRY(c, hipSetDevice(c->device));
+    SweepParams p = base_params(c);
+    p.mode = kModeProbe;
+    p.iterations = rounds;
+    const brr_hip_stats keep = c->stats;
+    rc = launch(c, p);
+    const double ms = c->stats.kernel_ms;
+    c->stats = keep;
+    if (rc) return rc;
+    *us_per_round = ms * 1e3 / rounds;
+    return BRR_HIP_OK;
+}
+
+__global__ void calibration_copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+int brr_hip_calibration_copy(uint32_t device, uint64_t bytes, double* ms)
+{
+    if (!ms || bytes < 16 || bytes % 16) return BRR_HIP_EINVAL;
+    if (hipSetDevice((int)device) != hipSuccess) return BRR_HIP_EDEVICE;
+    uint4 *a = nullptr, *b = nullptr;
+    if (hipMalloc((void**)&a, bytes) != hipSuccess) return BRR_HIP_ENOMEM;
+    if (hipMalloc((void**)&b, bytes) != hipSuccess) { (void)hipFree(a); return BRR_HIP_ENOMEM; }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = BRR_HIP_OK;
+    if (hipMemset(a, 1, bytes) != hipSuccess || hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        rc = BRR_HIP_EDEVICE;
+    float t = 0.f;
+    if (!rc) {
+        (void)hipEventRecord(e0, nullptr);
+        calibration_copy_kernel<<<1024, 256>>>(a, b, bytes / 16);
+        (void)hipEventRecord(e1, nullptr);
+        if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&t, e0, e1) != hipSuccess)
+            rc = BRR_HIP_EDEVICE;
+    }
+    *ms = t;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return rc;
+}
+
 int brr_hip_record_num(brr_hip_ctx* c, int enable)
 {
     if (int k = need_kind(c, false)) return k;

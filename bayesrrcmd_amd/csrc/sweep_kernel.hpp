@@ -56,7 +56,9 @@ constexpr uint32_t kMaxP = 256;   // workgroups of a launch (the exchange reduce
 constexpr int kIdxCap = 32;    // prefetched ragged indices per thread per block
 
 // kModeFixed: only the fixed-effects block of one iteration (brr_hip_fixed_effects).
-enum : int { kModeSweep = 0, kModeChain = 1, kModeFixed = 2 };
+// kModeProbe: `iterations` exchange rounds of a full block with nothing else
+// (brr_hip_probe_exchange: the latency floor of a round at this launch shape).
+enum : int { kModeSweep = 0, kModeChain = 1, kModeFixed = 2, kModeProbe = 3 };
 // kLayoutEigen: the ragged formulas with EigenBayesRKernel's dot, whose
 // missing calls count at the marker mean (eigenbayesrkernel.cpp:17-20).
 enum : int { kLayoutDense = 1, kLayoutRagged = 2, kLayoutEigen = 3 };
@@ -450,6 +452,13 @@ __device__ __forceinline__ bool wait_sum_lanes(const uint64_t* base, SlotFn slot
 {
     *out = 0.0;
     return wait_sum_strided<kMaxP / 64>(base, slot, (uint32_t)lane, 64u, P, rnd, ctl, out);
+}
+
+// The slot whose partial dot lane `lane` of wave `wave` publishes in a round
+// starting at slot 0 (the transposed reduction leaves value (lane >> 2) & 15).
+__device__ __forceinline__ int pos0_slot(int wave, int lane)
+{
+    return wave + kWaves * ((lane >> 2) & 15);
 }
 
 template <typename Params>
@@ -1036,6 +1045,26 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, Shared<R>& L, R
     uint64_t tk[kPhases] = {0, 0, 0, 0, 0, 0};
     uint64_t t_last = __builtin_amdgcn_s_memrealtime();
     __syncthreads();
+
+    if (p.mode == kModeProbe) {
+        // every slot of a block publishes its (zero) partial dot, then the
+        // reduce-scatter / all-gather of a sweep round; no state is touched
+        const int nb = (int)min(p.B, p.M);
+        for (uint32_t q = 0; q < p.iterations; ++q) {
+            ++rnd;
+            ++n_rounds;
+            const int k = pos0_slot(wave, lane);
+            if ((lane & 3) == 0 && k < nb) publish_pair(p.xchg1, slot1(p, rnd, k, w), rnd, 0.0);
+            if (!exchange<R>(p, L, rnd, 0, nb, w, wave, lane)) return;   // abort raised: nothing to store
+        }
+        BRR_TICK(2);
+        if (w == 0 && tid == 0) {
+            __hip_atomic_store(p.ctl + kCtlRound, rnd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            p.counters[0] = n_rounds;
+            for (int q = 0; q < kPhases; ++q) p.counters[2 + q] = tk[q];
+        }
+        return;
+    }
 
     const uint32_t iters = (p.mode == kModeChain) ? p.iterations : 1u;
     const uint32_t nblocks = (p.M + p.B - 1) / p.B;

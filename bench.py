@@ -808,6 +808,7 @@ def main():
                     help="plain sweep residual arithmetic: fp32 (FP32 residual and dots, compensated wave "
                          "reductions, FP64 refresh every --refresh markers; the c2 default) or fp64")
     ap.add_argument("--refresh", type=int, default=1024, help="FP64 residual refresh interval (markers)")
+    ap.add_argument("--no-probe", action="store_true", help="skip the exchange-latency probe of the headline line")
     ap.add_argument("--no-errors", action="store_true",
                     help="skip the FP64 comparison run of an fp32 line (errors_vs_fp64)")
     ap.add_argument("--decode-only", action="store_true",
@@ -824,15 +825,8 @@ def main():
     if args.calibrate:
         # a plain device copy of known size: FETCH_SIZE / WRITE_SIZE of this
         # kernel give the counter scale for the sweep's HBM traffic
-        import torch
-        a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
-        b = torch.empty_like(a)
-        a.fill_(1.0)
-        torch.cuda.synchronize()
-        b.copy_(a)
-        torch.cuda.synchronize()
-        log("calibration copy: 1 GiB read + 1 GiB written")
-        del a, b
+        ms = brr.calibration_copy(1 << 30)
+        log(f"calibration copy: 1 GiB read + 1 GiB written in {ms:.3f} ms ({2.0 * (1 << 30) / ms / 1e6:.0f} GB/s)")
     cfg = CONFIGS[args.config]
     if args.decode_only:
         t0 = time.time()
@@ -918,6 +912,8 @@ def main():
     errors = None
     if precision == "fp32" and not args.no_errors:
         errors = errors_vs_fp64(h, brr, data, m, args, orders, hist, fresh_chain)
+    # the latency floor of a round: exchange rounds of a full block with no other work
+    probe_us = h.probe_exchange(2000) if not (args.async_window or args.no_probe) else None
 
     total_updates = float(steps) * m
     value = total_updates / wall
@@ -959,8 +955,24 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "traffic_note": ("HBM bytes per launch are measured by separate rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE "
+                             "runs of this benchmark (profiles/bayesr_sweep_c2_ragged_N100k_M100k_hbm_counters.md); "
+                             "a counter run cannot share this process"),
+            "achieved_basis": ("logical: sum over the launch's markers of B_j = G_j + e N (1 + delta_j), G_j the "
+                               "column's device bytes, e = 4 (fp32 residual) or 8, delta_j = 1 when beta_j changed; "
+                               "the residual passes are served from registers, so HBM moves only the columns and "
+                               "the exchange words"),
             "kernel": "brr::bayesr_sweep_kernel (one launch per iteration)",
             "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": bytes_per_launch,
+            "column_bytes_per_launch": float(m) * ((n + 3) // 4) if stats["storage"] == 2 else None,
+            "column_GBps": (float(m) * ((n + 3) // 4) / (avg_ms * 1e-3) / 1e9) if stats["storage"] == 2 else None,
+            "limiter": {
+                "kind": "latency: exchange rounds (one per block of markers plus one per other beta change)",
+                "rounds_per_launch": float(np.mean(rounds)),
+                "exchange_us_per_round_probe": probe_us,
+                "exchange_floor_ms": (float(np.mean(rounds)) * probe_us * 1e-3) if probe_us else None,
+                "floor_fraction_of_launch": (float(np.mean(rounds)) * probe_us * 1e-3 / avg_ms) if probe_us else None,
+            },
             "rounds_per_launch": float(np.mean(rounds)), "changes_per_launch": float(np.mean(changes)),
             "phase_ms_per_launch": dict(zip(brr.HipSweep.PHASES, np.mean(phases, axis=0).round(3).tolist())),
         },

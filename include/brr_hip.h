@@ -266,6 +266,18 @@ int brr_hip_run_cohorts(brr_hip_ctx* const* ctxs, uint32_t n, uint64_t seed, con
                         uint32_t first_iteration, uint32_t iterations, const uint32_t* const* orders,
                         double* const* hists, uint32_t flags, brr_hip_cohort_report* report);
 
+/* Measurement helpers (no counterpart in the reference):
+ * brr_hip_probe_exchange runs `rounds` exchange rounds of a full speculation
+ * block on the context's launch shape with no other work -- every workgroup
+ * publishes its partials, the reduce-scatter and the all-gather -- and gives the
+ * device time per round: the latency floor of the sweep's rounds (single-chain
+ * BayesR contexts; the state is untouched).
+ * brr_hip_calibration_copy copies `bytes` (a multiple of 16) between two device
+ * buffers with one kernel and gives its time: a copy of known size whose HBM
+ * counters scale the sweep's. */
+int brr_hip_probe_exchange(brr_hip_ctx* ctx, uint32_t rounds, double* us_per_round);
+int brr_hip_calibration_copy(uint32_t device, uint64_t bytes, double* ms);
+
 /* Per-marker record of num, the value BayesRKernel::computeNum returns and
  * processColumn decides on (BayesRBase.cpp:300-305, densebayesrkernel.cpp:10-15,
  * sparsebayesrkernel.cpp:24-27): with enable != 0 every later sweep and chain
