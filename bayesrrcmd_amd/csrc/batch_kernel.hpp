@@ -791,9 +791,16 @@ bayesr_batch_kernel(BatchParams p)
             // change, the dots of its later slots are corrected by delta * Gram
             // row (BayesRBase.cpp:370-378) and those slots are drawn again.
             if (wave < (int)p.nq) {
+                // the traits whose outputs of this block this workgroup writes: t with
+                // (t + 7 blk) % P == w (one division per block, not one per trait)
+                unsigned long long own = 0;
+                {
+                    const uint32_t sh = (blk * 7u) % p.P;
+                    for (uint32_t t = w >= sh ? w - sh : w + p.P - sh; t < (uint32_t)T; t += p.P) own |= 1ull << t;
+                }
                 const int tt = 16 * wave + tq;             // trait of this lane's accept work
                 const bool tok = tt < T;
-                const bool mine = tok && ((uint32_t)tt + blk * 7u) % p.P == w;   // output workgroup
+                const bool mine = tok && ((own >> tt) & 1ull);   // output workgroup
                 const bool single_out = p.T <= p.P;   // then a workgroup outputs at most one trait per block
                 uint16_t* plist = L.plist + wave * (kBB * 16);
                 for (bool first = true;; first = false) {
@@ -827,7 +834,7 @@ bayesr_batch_kernel(BatchParams p)
                                               L.uu[k][t], L.zw[k][t], p.mode == kModeChain);
                         L.bn[k][t] = r.bnew;
                         L.sl[k][t] = (int8_t)r.sel;
-                        if (((uint32_t)t + blk * 7u) % p.P == w) L.acm[k] = r.acum0;
+                        if ((own >> t) & 1ull) L.acm[k] = r.acum0;
                         if (r.bnew != beta_old) atomicMin(&L.fc[t], k);
                     }
                     ++n_drounds;

@@ -555,11 +555,11 @@ template <int R>
 __device__ __forceinline__ bool bw_exchange(const BwParams& p, BwShared<R>& L, uint32_t rnd, int V, uint32_t w,
                                             int wave, int lane)
 {
-    int cnt = 0;
     bool ok = true;
-    for (int v = 0; v < V; ++v) {
-        if ((uint32_t)v % p.P != w) continue;
-        if ((cnt++ % kWaves) != wave) continue;
+    // values w, w + P, ...: the c-th reduced by wave c % kWaves
+    int c = 0;
+    for (int v = (int)w; v < V; v += (int)p.P, ++c) {
+        if ((c % kWaves) != wave) continue;
         double acc = 0.0;
         ok = wait_sum_lanes(p.xchg1, [&](uint32_t w2) { return bw_slot1(p, rnd, v, w2); }, p.P, rnd, p.ctl, lane, &acc);
         if (__any(!ok)) { ok = false; break; }

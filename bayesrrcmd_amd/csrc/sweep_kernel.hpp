@@ -157,6 +157,13 @@ struct Shared {
     uint64_t seg_start[3 * kMaxB];
     uint32_t seg_pref[3 * kMaxB + 1];   // exclusive prefix; [3 nb] = total
     double dl[kMaxB][4];                // residual change per 2-bit code of each applied slot
+    // column values of the current block by pairs of codes: entry n of slot k is
+    // (wt[k][n & 3], wt[k][n >> 2]), the values of two adjacent individuals whose
+    // codes form the 4-bit field n; FP64, or FP32 in the FP32-residual mode
+    union alignas(16) {
+        double d[kMaxB][16][2];
+        float f[kMaxB][16][2];
+    } pt;
     unsigned long long cmask;           // the slots whose change is applied this round
     double mu, sigmaE, epsSum;
     double sigmaG[kMaxG];
@@ -278,6 +285,9 @@ __device__ __forceinline__ double wt_t(uint32_t v, double w0, double w1, double 
 
 __device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+typedef float flt2 __attribute__((ext_vector_type(2)));
 
 // Knuth's TwoSum: s + e == a + b exactly (needs IEEE rounding and no
 // contraction, which the build's -ffp-contract=off guarantees).
@@ -499,11 +509,14 @@ template <int R>
 __device__ __forceinline__ bool exchange(const SweepParams& p, Shared<R>& L, uint32_t rnd, int pos, int nb,
                          uint32_t w, int wave, int lane, int gpos = kMaxB)
 {
-    int cnt = 0;
     bool ok = true;
-    for (int k = pos; k < nb; ++k) {
-        if ((uint32_t)k % p.P != w) continue;
-        if ((cnt++ % kWaves) != wave) continue;
+    // the slots k in [pos, nb) with k % P == w in increasing order, the c-th of
+    // them reduced by wave c % kWaves (one division per round: a division per
+    // slot, a long scalar sequence, cost more than the rest of the exchange)
+    const uint32_t r0 = (uint32_t)pos % p.P;
+    int c = 0;
+    for (int k = pos + (int)(w >= r0 ? w - r0 : w + p.P - r0); k < nb; k += (int)p.P, ++c) {
+        if ((c % kWaves) != wave) continue;
         double acc = 0.0, gacc = 0.0;
         ok = wait_sum_lanes(p.xchg1, [&](uint32_t w2) { return slot1(p, rnd, k, w2); }, p.P, rnd, p.ctl, lane, &acc);
         if (ok && k >= gpos)
@@ -734,6 +747,36 @@ __device__ __forceinline__ void meta_store(const SweepParams& p, Shared<R>& L, i
         L.wt[t][2] = 1.0;   // genotype 1
         L.wt[t][3] = 0.0;   // genotype 0
     }
+}
+
+// The pair table of the current block's slots [0, nb) from L.wt (all threads;
+// the caller synchronises before the table is read).
+template <bool kF32, int R>
+__device__ __forceinline__ void pair_table_store(Shared<R>& L, int nb, int tid)
+{
+    for (int t = tid; t < nb * 16; t += kThreads) {
+        const int k = t >> 4, n = t & 15;
+        const double a = L.wt[k][n & 3], b = L.wt[k][n >> 2];
+        if constexpr (kF32) {
+            L.pt.f[k][n][0] = (float)a;
+            L.pt.f[k][n][1] = (float)b;
+        } else {
+            L.pt.d[k][n][0] = a;
+            L.pt.d[k][n][1] = b;
+        }
+    }
+}
+
+// The column values of the two individuals whose codes form field n, slot k.
+template <int R>
+__device__ __forceinline__ dbl2 pair_at(const Shared<R>& L, int k, uint32_t n, double)
+{
+    return *reinterpret_cast<const dbl2*>(&L.pt.d[k][n][0]);
+}
+template <int R>
+__device__ __forceinline__ flt2 pair_at(const Shared<R>& L, int k, uint32_t n, float)
+{
+    return *reinterpret_cast<const flt2*>(&L.pt.f[k][n][0]);
 }
 
 // Dense codes of the next block: word t of the flattened [nb][R][32] array.
@@ -1155,6 +1198,8 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, Shared<R>& L, R
             }
             if (tid < nb0) meta_store<R, kLay>(p, L, tid, m);
             __syncthreads();
+            pair_table_store<kF32, R>(L, nb0, tid);
+            __syncthreads();
             if (nblocks > 1) {
                 const int nb1 = (int)min(p.B, p.M - p.B);
                 if (tid < nb1) L.mk_next[tid] = order[p.B + tid];
@@ -1236,26 +1281,55 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, Shared<R>& L, R
                             }
                         }
                     }
+                    // the column values come from the pair table, two per LDS load; slots
+                    // go in groups of four whose loads are issued together (a group past
+                    // nb is skipped, a slot past nb within a group reads slot nb - 1 and
+                    // contributes 0). Per slot the products are summed in individual
+                    // order.
                     E part[16], gpart[kG ? 16 : 1];
 #pragma unroll
-                    for (int i = 0; i < 16; ++i) {
-                        const int k = pos + wave + kWaves * i;
-                        E s = 0, gs = 0;
-                        if (k < nb) {
-                            const E w0 = (E)L.wt[k][0], w1 = (E)L.wt[k][1], w2 = (E)L.wt[k][2], w3 = (E)L.wt[k][3];
+                    for (int g4 = 0; g4 < 4; ++g4) {
+                        if (pos + wave + kWaves * 4 * g4 < nb) {
+                            int kc[4];
+                            uint32_t bits[4][R];
 #pragma unroll
-                            for (int r = 0; r < R; ++r) {
-                                const uint32_t bits = L.codes[k][r][lane >> 1] >> (16 * (lane & 1));
+                            for (int j = 0; j < 4; ++j) {
+                                const int k = pos + wave + kWaves * (4 * g4 + j);
+                                kc[j] = k < nb ? k : nb - 1;
 #pragma unroll
-                                for (int e = 0; e < 8; ++e) {
-                                    const E x = wt_t((bits >> (2 * e)) & 3u, w0, w1, w2, w3);
-                                    s = fma_t(x, eps[r][e], s);
-                                    if constexpr (kG) gs = fma_t(x, uh[r][e], gs);
+                                for (int r = 0; r < R; ++r)
+                                    bits[j][r] = L.codes[kc[j]][r][lane >> 1] >> (16 * (lane & 1));
+                            }
+#pragma unroll
+                            for (int j = 0; j < 4; ++j) {
+                                const int i = 4 * g4 + j;
+                                E s = 0, gs = 0;
+#pragma unroll
+                                for (int r = 0; r < R; ++r) {
+                                    decltype(pair_at<R>(L, 0, 0u, E{})) x[4];
+#pragma unroll
+                                    for (int q = 0; q < 4; ++q) x[q] = pair_at<R>(L, kc[j], (bits[j][r] >> (4 * q)) & 15u, E{});
+#pragma unroll
+                                    for (int q = 0; q < 4; ++q) {
+                                        s = fma_t(x[q].x, eps[r][2 * q], s);
+                                        s = fma_t(x[q].y, eps[r][2 * q + 1], s);
+                                        if constexpr (kG) {
+                                            gs = fma_t(x[q].x, uh[r][2 * q], gs);
+                                            gs = fma_t(x[q].y, uh[r][2 * q + 1], gs);
+                                        }
+                                    }
                                 }
+                                const bool live = pos + wave + kWaves * i < nb;
+                                part[i] = live ? s : (E)0;
+                                if constexpr (kG) gpart[i] = live ? gs : (E)0;
+                            }
+                        } else {
+#pragma unroll
+                            for (int i = 4 * g4; i < 4 * g4 + 4; ++i) {
+                                part[i] = 0;
+                                if constexpr (kG) gpart[i] = 0;
                             }
                         }
-                        part[i] = s;
-                        if constexpr (kG) gpart[i] = gs;
                     }
                     const int i = (lane >> 2) & 15;
                     const int k = pos + wave + kWaves * i;
@@ -1522,11 +1596,13 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, Shared<R>& L, R
                 else ragged_store<R>(p, L, nb1, lo, tid, idx, itm);
                 if (tid < nb1) meta_store<R, kLay>(p, L, tid, mnext);
                 __syncthreads();
+                pair_table_store<kF32, R>(L, nb1, tid);
                 if (has2) {
                     if (tid < nb2) L.mk_next[tid] = jnext2;
                     if constexpr (!kCodes) seg_store_prefix<R>(L, nb2, tid, wave, lane, st2, ln2);
                     else __syncthreads();
                 }
+                if (!kCodes || !has2) __syncthreads();
             }
             BRR_TICK(0);
             if constexpr (kF32) {
