@@ -308,6 +308,8 @@ SweepParams base_params(brr_hip_ctx* c)
     p.beta = c->d_beta;
     p.comp = c->d_comp;
     p.acum = c->d_acum;
+    p.u = c->d_u;          // read by every block load (M values; the chain draws its own)
+    p.z = c->d_z;
     p.scal = c->d_scal;
     p.xchg1 = c->d_xchg1;
     p.xchg2 = c->d_xchg2;

@@ -688,16 +688,33 @@ struct MetaNext {
     uint64_t beta_bits;
     double w0, w1, w2, w3;   // dense table by 2-bit code / ragged mean, sd, Zsum in w0..w2
     double u;
-    uint64_t zw;             // z, or with lazy_z its Philox words
+    uint64_t zw;             // z, or with lazy_z its Philox words (chain mode)
+    double u_in, z_in;       // the given u and z (sweep mode)
 };
 
+// The chain's draws are formed before any load is issued, the given u and z
+// are loaded in every mode (p.u and p.z always point at M values), and the
+// column table is chosen at compile time: a load in a branch not taken still
+// counts as outstanding for the registers it would have written, and a
+// temporary placed in one of them made every block start wait for all of its
+// prefetch loads.
+template <int kLay>
 __device__ __forceinline__ void meta_issue(const SweepParams& p, uint32_t j, uint32_t iteration,
                                            MetaNext& m)
 {
     m.j = j;
+    m.u = 0.0;
+    m.zw = 0;
+    if (p.mode == kModeChain) {
+        const U4 x = draw_block(p.seed, j, iteration, kStreamMarker, p.chain);   // as marker_uz
+        m.u = unit_u53(x.x[0], x.x[1]);
+        m.zw = p.lazy_z ? ((uint64_t)x.x[2] << 32) | x.x[3] : double_as_bits(unit_normal(x.x[2], x.x[3]));
+    }
+    m.u_in = p.u[j];
+    m.z_in = p.z[j];
     m.g = p.group ? p.group[j] : 0;
     m.beta_bits = ld_agent((const uint64_t*)(p.beta + j));
-    if (p.layout == kLayoutDense) {
+    if constexpr (kLay == kLayoutDense) {
         const double* lt = p.lut + 4 * (size_t)j;
         // 2-bit value v -> dosage: 0->2, 1->missing, 2->1, 3->0
         m.w0 = lt[2];
@@ -711,14 +728,6 @@ __device__ __forceinline__ void meta_issue(const SweepParams& p, uint32_t j, uin
         m.w2 = st[3];
         m.w3 = 0.0;
     }
-    if (p.mode == kModeChain) {
-        const U4 x = draw_block(p.seed, j, iteration, kStreamMarker, p.chain);   // as marker_uz
-        m.u = unit_u53(x.x[0], x.x[1]);
-        m.zw = p.lazy_z ? ((uint64_t)x.x[2] << 32) | x.x[3] : double_as_bits(unit_normal(x.x[2], x.x[3]));
-    } else {
-        m.u = p.u[j];
-        m.zw = double_as_bits(p.z[j]);
-    }
 }
 
 template <int R, int kLay>
@@ -727,8 +736,9 @@ __device__ __forceinline__ void meta_store(const SweepParams& p, Shared<R>& L, i
     L.mk[t] = m.j;
     L.gr[t] = m.g;
     L.beta_old[t] = bits_as_double(m.beta_bits);
-    L.uu[t] = m.u;
-    L.zw[t] = m.zw;
+    const bool chain = p.mode == kModeChain;
+    L.uu[t] = chain ? m.u : m.u_in;
+    L.zw[t] = chain ? m.zw : double_as_bits(m.z_in);
     // a run-time branch on purpose: with a compile-time one the compiler sinks
     // the prefetch loads of meta_issue towards this commit, and the block then
     // waits for their full latency
@@ -1181,7 +1191,7 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, Shared<R>& L, R
             if (tid < nb0) L.mk_next[tid] = order[tid];
             __syncthreads();
             MetaNext m{};
-            if (tid < nb0) meta_issue(p, L.mk_next[tid], iteration, m);
+            if (tid < nb0) meta_issue<kLay>(p, L.mk_next[tid], iteration, m);
             if constexpr (kCodes) {
                 uint32_t words[8 * R];
                 dense_issue<R>(p, L, nb0, lo, tid, words);
@@ -1233,7 +1243,7 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, Shared<R>& L, R
             int32_t idx[kCodes ? 1 : kIdxCap];
             uint32_t itm[kCodes ? 1 : kIdxCap / 4];
             if (has1) {
-                if (tid < nb1) meta_issue(p, L.mk_next[tid], iteration, mnext);
+                if (tid < nb1) meta_issue<kLay>(p, L.mk_next[tid], iteration, mnext);
                 if constexpr (kCodes) dense_issue<R>(p, L, nb1, lo, tid, words);
                 else idx_issue<R>(p, L, nb1, tid, idx, itm);
             }
