@@ -1335,7 +1335,6 @@ int brr_hip_run_cohorts(brr_hip_ctx* const* ctxs, uint32_t n, uint64_t seed, con
         p.seed = seed;
         p.order = c->d_order;
         p.hist = c->d_hist;
-        p.lazy_z = c->change_rate < 0.05 ? 1u : 0u;
         p.refresh_blocks = std::max(1u, c->refresh_markers / c->B);
         params[q] = p;
     }
@@ -1874,10 +1873,6 @@ int brr_hip_run_chain(brr_hip_ctx* c, uint64_t seed, uint32_t chain, uint32_t fi
     p.seed = seed;
     p.order = c->d_order;
     p.hist = c->d_hist;
-    // the per-marker normal draw only matters for markers drawn into a nonzero
-    // class: formed at block load while many markers change, else deferred to
-    // the decisions that need it (the same bits either way)
-    p.lazy_z = c->change_rate < 0.05 ? 1u : 0u;
     rc = launch_any(c, p);
     c->stats.markers = total * c->Tn;
     if (rc) return rc;

@@ -89,7 +89,6 @@ struct SweepParams {
     double* fixed;
     uint32_t* forder;
     uint32_t F;
-    uint32_t lazy_z;              // chain: form each marker's normal draw only if drawn into a nonzero class
     uint32_t refresh_blocks;      // FP32 residual: FP64 refresh after every this many blocks (>= 1)
     uint64_t seed;
     // dense arena: column j at dense + j * col_words (32-bit words)
@@ -148,7 +147,10 @@ struct Shared {
     double wt[kMaxB][4];                // column value per 2-bit code
     double mstat[kMaxB][3];             // ragged: mean, sd, Zsum
     double uu[kMaxB];
-    uint64_t zw[kMaxB];                // z: its two Philox words (chain, lazy_z), or its bits
+    uint64_t zw[kMaxB];                // z (bits)
+    // chain: the next block's u and z, formed by wave 1 while wave 0 decides
+    double uu_next[kMaxB];
+    uint64_t zw_next[kMaxB];
     uint32_t mk[kMaxB];
     int32_t gr[kMaxB];
     uint32_t mk_next[kMaxB];            // marker ids of the next block
@@ -688,28 +690,34 @@ struct MetaNext {
     uint64_t beta_bits;
     double w0, w1, w2, w3;   // dense table by 2-bit code / ragged mean, sd, Zsum in w0..w2
     double u;
-    uint64_t zw;             // z, or with lazy_z its Philox words (chain mode)
+    uint64_t zw;             // z (chain mode, block loaded in place)
     double u_in, z_in;       // the given u and z (sweep mode)
 };
 
-// The chain's draws are formed before any load is issued, the given u and z
-// are loaded in every mode (p.u and p.z always point at M values), and the
-// column table is chosen at compile time: a load in a branch not taken still
-// counts as outstanding for the registers it would have written, and a
-// temporary placed in one of them made every block start wait for all of its
-// prefetch loads.
+// A marker's u and z in a chain (as marker_uz).
+__device__ __forceinline__ void chain_draws(const SweepParams& p, uint32_t j, uint32_t iteration, double& u,
+                                            uint64_t& zw)
+{
+    const U4 x = draw_block(p.seed, j, iteration, kStreamMarker, p.chain);
+    u = unit_u53(x.x[0], x.x[1]);
+    zw = double_as_bits(unit_normal(x.x[2], x.x[3]));
+}
+
+// With `draws` (a block loaded in place) the chain's draws are formed here,
+// before any load is issued; a prefetched block's draws come from wave 1
+// (L.uu_next / L.zw_next). The given u and z are loaded in every mode (p.u and
+// p.z always point at M values), and the column table is chosen at compile
+// time: a load in a branch not taken still counts as outstanding for the
+// registers it would have written, and a temporary placed in one of them made
+// every block start wait for all of its prefetch loads.
 template <int kLay>
-__device__ __forceinline__ void meta_issue(const SweepParams& p, uint32_t j, uint32_t iteration,
+__device__ __forceinline__ void meta_issue(const SweepParams& p, uint32_t j, uint32_t iteration, bool draws,
                                            MetaNext& m)
 {
     m.j = j;
     m.u = 0.0;
     m.zw = 0;
-    if (p.mode == kModeChain) {
-        const U4 x = draw_block(p.seed, j, iteration, kStreamMarker, p.chain);   // as marker_uz
-        m.u = unit_u53(x.x[0], x.x[1]);
-        m.zw = p.lazy_z ? ((uint64_t)x.x[2] << 32) | x.x[3] : double_as_bits(unit_normal(x.x[2], x.x[3]));
-    }
+    if (draws && p.mode == kModeChain) chain_draws(p, j, iteration, m.u, m.zw);
     m.u_in = p.u[j];
     m.z_in = p.z[j];
     m.g = p.group ? p.group[j] : 0;
@@ -731,14 +739,22 @@ __device__ __forceinline__ void meta_issue(const SweepParams& p, uint32_t j, uin
 }
 
 template <int R, int kLay>
-__device__ __forceinline__ void meta_store(const SweepParams& p, Shared<R>& L, int t, const MetaNext& m)
+__device__ __forceinline__ void meta_store(const SweepParams& p, Shared<R>& L, int t, const MetaNext& m,
+                                           bool draws)
 {
     L.mk[t] = m.j;
     L.gr[t] = m.g;
     L.beta_old[t] = bits_as_double(m.beta_bits);
-    const bool chain = p.mode == kModeChain;
-    L.uu[t] = chain ? m.u : m.u_in;
-    L.zw[t] = chain ? m.zw : double_as_bits(m.z_in);
+    if (p.mode != kModeChain) {
+        L.uu[t] = m.u_in;
+        L.zw[t] = double_as_bits(m.z_in);
+    } else if (draws) {
+        L.uu[t] = m.u;
+        L.zw[t] = m.zw;
+    } else {
+        L.uu[t] = L.uu_next[t];
+        L.zw[t] = L.zw_next[t];
+    }
     // a run-time branch on purpose: with a compile-time one the compiler sinks
     // the prefetch loads of meta_issue towards this commit, and the block then
     // waits for their full latency
@@ -1191,7 +1207,7 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, Shared<R>& L, R
             if (tid < nb0) L.mk_next[tid] = order[tid];
             __syncthreads();
             MetaNext m{};
-            if (tid < nb0) meta_issue<kLay>(p, L.mk_next[tid], iteration, m);
+            if (tid < nb0) meta_issue<kLay>(p, L.mk_next[tid], iteration, true, m);
             if constexpr (kCodes) {
                 uint32_t words[8 * R];
                 dense_issue<R>(p, L, nb0, lo, tid, words);
@@ -1206,7 +1222,7 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, Shared<R>& L, R
                 idx_issue<R>(p, L, nb0, tid, idx, itm);
                 ragged_store<R>(p, L, nb0, lo, tid, idx, itm);
             }
-            if (tid < nb0) meta_store<R, kLay>(p, L, tid, m);
+            if (tid < nb0) meta_store<R, kLay>(p, L, tid, m, true);
             __syncthreads();
             pair_table_store<kF32, R>(L, nb0, tid);
             __syncthreads();
@@ -1243,7 +1259,7 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, Shared<R>& L, R
             int32_t idx[kCodes ? 1 : kIdxCap];
             uint32_t itm[kCodes ? 1 : kIdxCap / 4];
             if (has1) {
-                if (tid < nb1) meta_issue<kLay>(p, L.mk_next[tid], iteration, mnext);
+                if (tid < nb1) meta_issue<kLay>(p, L.mk_next[tid], iteration, false, mnext);
                 if constexpr (kCodes) dense_issue<R>(p, L, nb1, lo, tid, words);
                 else idx_issue<R>(p, L, nb1, tid, idx, itm);
             }
@@ -1462,11 +1478,7 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, Shared<R>& L, R
                             bnew = beta_old;
                             if (sel == 0) bnew = 0.0;
                             else if (sel > 0) {
-                                const uint64_t zw = L.zw[k];
-                                const double z = (p.mode == kModeChain && p.lazy_z)
-                                                     ? unit_normal((uint32_t)(zw >> 32), (uint32_t)zw)
-                                                     : bits_as_double(zw);
-                                bnew = z * gbsd[sel] + num / gden[sel];
+                                bnew = bits_as_double(L.zw[k]) * gbsd[sel] + num / gden[sel];
                             }
                             changed = (bnew != beta_old);
                         }
@@ -1476,8 +1488,18 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, Shared<R>& L, R
                         const int c = (cm && !kAsync) ? pos + (int)__ffsll((long long)cm) - 1 : nb;
                         const int last = (c < nb) ? c : nb - 1;
                         const bool accepted = act && k <= last;
-                        // v counts (integer-valued doubles: exact in any order)
-                        if (accepted && sel >= 0) atomicAdd(&L.v[g * K + sel], 1.0);
+                        // v counts (integer-valued doubles: exact in any order), one add per
+                        // distinct (group, component) of the accepted slots
+                        {
+                            const int key = (accepted && sel >= 0) ? g * K + sel : -1;
+                            unsigned long long todo = __ballot(key >= 0);
+                            while (todo) {
+                                const int kb = __shfl(key, (int)__ffsll((long long)todo) - 1, 64);
+                                const unsigned long long same = __ballot(key == kb);
+                                if (lane == 0) L.v[kb] += (double)__popcll(same);
+                                todo &= ~same;
+                            }
+                        }
                         // betasqnG += beta^2 in marker order (BayesRBase.cpp:353)
                         unsigned long long sq = __ballot(accepted && sel > 0);
                         while (sq) {
@@ -1546,6 +1568,15 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, Shared<R>& L, R
                         L.change2 = c1;
                         L.next = cfin < nb ? cfin + 1 : nb;
                     }
+                } else if (wave == 1 && first_round && has1 && p.mode == kModeChain) {
+                    // the next block's u and z, while wave 0 decides (stored at its commit)
+                    if (lane < nb1) {
+                        double u;
+                        uint64_t zw;
+                        chain_draws(p, L.mk_next[lane], iteration, u, zw);
+                        L.uu_next[lane] = u;
+                        L.zw_next[lane] = zw;
+                    }
                 }
                 __syncthreads();
                 BRR_TICK(3);
@@ -1604,7 +1635,7 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, Shared<R>& L, R
             if (has1) {
                 if constexpr (kCodes) dense_store<R>(L, nb1, tid, words);
                 else ragged_store<R>(p, L, nb1, lo, tid, idx, itm);
-                if (tid < nb1) meta_store<R, kLay>(p, L, tid, mnext);
+                if (tid < nb1) meta_store<R, kLay>(p, L, tid, mnext, false);
                 __syncthreads();
                 pair_table_store<kF32, R>(L, nb1, tid);
                 if (has2) {
