@@ -1080,13 +1080,14 @@ bayesw_sweep_kernel(BwParams p)
                         const unsigned long long sm = __ballot(stop);
                         const int c = sm ? pos + (int)__ffsll((long long)sm) - 1 : nb;
                         const int cs = __shfl(sel, (c - pos) & 63, 64);
-                        // slots before c keep beta = 0 in class 0
-                        if (k < c && active) {
-                            atomicAdd(&L.v[0], 1.0);
-                            if (w == 0) {
-                                st_agent((uint64_t*)(p.beta + L.mk[k]), double_as_bits(0.0));
-                                p.comp[L.mk[k]] = 0;
-                            }
+                        // slots before c keep beta = 0 in class 0; slot k's outputs are
+                        // written by workgroup k % P
+                        const bool keep0 = k < c && active;
+                        const unsigned long long nk = __ballot(keep0);
+                        if (lane == 0 && nk) L.v[0] += (double)__popcll(nk);
+                        if (keep0 && ((uint32_t)k < p.P ? (uint32_t)k == w : (uint32_t)k % p.P == w)) {
+                            st_agent((uint64_t*)(p.beta + L.mk[k]), double_as_bits(0.0));
+                            __hip_atomic_store(p.comp + L.mk[k], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         }
                         if (lane == 0) {
                             L.stop = c;
@@ -1167,9 +1168,9 @@ bayesw_sweep_kernel(BwParams p)
                         if (tid == 0) {
                             L.v[sel] += 1.0;
                             L.betasq += bnew * bnew;
-                            if (w == 0) {
+                            if ((uint32_t)c < p.P ? (uint32_t)c == w : (uint32_t)c % p.P == w) {
                                 st_agent((uint64_t*)(p.beta + L.mk[c]), double_as_bits(bnew));
-                                p.comp[L.mk[c]] = sel;
+                                __hip_atomic_store(p.comp + L.mk[c], sel, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                             }
                         }
                         pos = c + 1;
@@ -1183,9 +1184,9 @@ bayesw_sweep_kernel(BwParams p)
             __syncthreads();
             BW_TICK(0);
         }
-        // other workgroups read workgroup 0's beta stores only in the block loads
-        // of the next iteration, which follow this workgroup's next publish
-        if (w == 0) __builtin_amdgcn_s_waitcnt(0);
+        // other workgroups read this workgroup's beta stores only in the block
+        // loads of the next iteration, which follow this workgroup's next publish
+        __builtin_amdgcn_s_waitcnt(0);
 
         if (p.mode == kModeChain) {
             // ---- sampleAlpha (bayeswbase.cpp:707-734): one grid sum per density value

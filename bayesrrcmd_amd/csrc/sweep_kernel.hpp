@@ -1509,12 +1509,17 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, Shared<R>& L, R
                             if (lane == 0) L.betasqn[gb] += bb * bb;
                             sq &= sq - 1;
                         }
-                        if (w == 0 && accepted) {
+                        // slot k's outputs are written by workgroup k % P (its reducer), so no
+                        // workgroup holds a round's worth of stores before its next publish
+                        const bool writer = (uint32_t)k < p.P ? (uint32_t)k == w : (uint32_t)k % p.P == w;
+                        if (writer && accepted) {
                             const uint32_t j = L.mk[k];
                             st_agent((uint64_t*)(p.beta + j), double_as_bits(bnew));
-                            if (sel >= 0) p.comp[j] = sel;
-                            p.acum[j] = acum0;
-                            if (p.num_out) p.num_out[j] = numk;
+                            // agent-scope stores: a marker's writer changes between iterations
+                            // and the XCD L2s are not coherent with each other
+                            if (sel >= 0) __hip_atomic_store(p.comp + j, sel, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            st_agent((uint64_t*)(p.acum + j), double_as_bits(acum0));
+                            if (p.num_out) st_agent((uint64_t*)(p.num_out + j), double_as_bits(numk));
                         }
                         // residual change per 2-bit code of each slot applied below: the
                         // pass's first change (sequential) or every change of the window
@@ -1657,9 +1662,9 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, Shared<R>& L, R
                 }
             }
         }
-        // other workgroups read workgroup 0's beta stores only in the block loads
-        // of the next iteration, which follow this workgroup's next publish
-        if (w == 0) __builtin_amdgcn_s_waitcnt(0);
+        // other workgroups read this workgroup's beta stores only in the block
+        // loads of the next iteration, which follow this workgroup's next publish
+        __builtin_amdgcn_s_waitcnt(0);
 
         if (p.mode == kModeChain) {
             double e2 = 0.0;
