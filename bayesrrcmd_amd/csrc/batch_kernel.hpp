@@ -184,15 +184,13 @@ __device__ __forceinline__ bool breduce(const BatchParams& p, BShared& L, uint32
     const uint32_t nch = (V + 7u) / 8u;
     for (uint32_t c = w; c < nch; c += p.P) {
         const uint32_t v = 8u * c + (uint32_t)(tid & 7);
+        // all of a thread's partials are loaded at once and only stale ones polled
+        // again; summed in w2 order as before
         double acc = 0.0;
         bool ok = true;
-        if (v < V) {
-            for (uint32_t w2 = (uint32_t)(tid >> 3); w2 < p.P; w2 += 32u) {
-                double x;
-                if (!wait_pair(p.xchg1, bslot1(p, rnd, w2, v), rnd, p.ctl, &x)) { ok = false; break; }
-                acc += x;
-            }
-        }
+        if (v < V)
+            ok = wait_sum_strided<kMaxP / 32>(p.xchg1, [&](uint32_t w2) { return bslot1(p, rnd, w2, v); },
+                                              (uint32_t)(tid >> 3), 32u, p.P, rnd, p.ctl, &acc);
         if (!ok) { L.abort = 1; raise_abort(p, 11); }
         acc += __shfl_xor(acc, 8, 64);
         acc += __shfl_xor(acc, 16, 64);
@@ -763,19 +761,63 @@ bayesr_batch_kernel(BatchParams p)
                 const uint32_t V = CT + (uint32_t)(nb * (nb - 1) / 2);
                 if (!breduce(p, L, rnd, V, w, tid)) goto done;
                 BRR_BTICK(8);
-                for (uint32_t v = (uint32_t)tid; v < V; v += kThreads) {
-                    double xv;
-                    if (!wait_pair(p.xchg2, bslot2(p, rnd, v, w % kCopies), rnd, p.ctl, &xv)) { L.abort = 1; raise_abort(p, 13); break; }
-                    if (v < CT) {
-                        const uint32_t k = v / (uint32_t)T;
-                        L.cd[k][v - k * (uint32_t)T] = xv;
-                    } else {
-                        const uint32_t e = v - CT;
-                        uint32_t k = (uint32_t)((1.0f + sqrtf(1.0f + 8.0f * (float)e)) * 0.5f);
-                        while (k > 1 && k * (k - 1) / 2 > e) --k;
-                        while ((k + 1) * k / 2 <= e) ++k;
-                        L.gm[e - k * (k - 1) / 2][k] = xv;
+                // totals v = tid + 256 i, four loads in flight per thread (one at a time
+                // costs a full memory round trip per value)
+                constexpr int kG = 4;
+                const float invT = 1.0f / (float)T;
+                for (uint32_t v0 = (uint32_t)tid; v0 < V; v0 += kG * kThreads) {
+                    uint64_t hh[kG], ll[kG];
+                    uint32_t live = 0;
+#pragma unroll
+                    for (int q = 0; q < kG; ++q) {
+                        const uint32_t v = v0 + (uint32_t)q * kThreads;
+                        hh[q] = 0;
+                        ll[q] = 0;
+                        if (v < V) {
+                            ld_pair(p.xchg2, bslot2(p, rnd, v, w % kCopies), hh[q], ll[q]);
+                            live |= 1u << q;
+                        }
                     }
+                    uint64_t spins = 0;
+                    for (;;) {
+                        uint32_t stale = 0;
+#pragma unroll
+                        for (int q = 0; q < kG; ++q)
+                            if (((live >> q) & 1u) && ((uint32_t)(hh[q] >> 32) != rnd || (uint32_t)(ll[q] >> 32) != rnd))
+                                stale |= 1u << q;
+                        if (!stale) break;
+                        if (++spins > kSpinLimit ||
+                            ((spins & 255u) == 0 && __hip_atomic_load(p.ctl + kCtlAbort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+                            L.abort = 1;
+                            raise_abort(p, 13);
+                            live = 0;
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+                        for (int q = 0; q < kG; ++q)
+                            if ((stale >> q) & 1u) ld_pair(p.xchg2, bslot2(p, rnd, v0 + (uint32_t)q * kThreads, w % kCopies), hh[q], ll[q]);
+                    }
+#pragma unroll
+                    for (int q = 0; q < kG; ++q) {
+                        if (!((live >> q) & 1u)) continue;
+                        const uint32_t v = v0 + (uint32_t)q * kThreads;
+                        const double xv = bits_as_double((hh[q] << 32) | (ll[q] & 0xFFFFFFFFull));
+                        if (v < CT) {
+                            // k = v / T without a division: a float estimate, corrected
+                            uint32_t k = (uint32_t)((float)v * invT);
+                            if (k * (uint32_t)T > v) --k;
+                            else if ((k + 1) * (uint32_t)T <= v) ++k;
+                            L.cd[k][v - k * (uint32_t)T] = xv;
+                        } else {
+                            const uint32_t e = v - CT;
+                            uint32_t k = (uint32_t)((1.0f + sqrtf(1.0f + 8.0f * (float)e)) * 0.5f);
+                            while (k > 1 && k * (k - 1) / 2 > e) --k;
+                            while ((k + 1) * k / 2 <= e) ++k;
+                            L.gm[e - k * (k - 1) / 2][k] = xv;
+                        }
+                    }
+                    if (!live && L.abort) break;
                 }
                 if (tid < kMaxT) { L.pos[tid] = (tid < T) ? 0 : nb; L.fc[tid] = nb; }
                 if (tid < kBB / 4) L.flag[tid] = 0u;
