@@ -181,7 +181,11 @@ def errors_vs_fp64(h, brr, data, m, args, orders, hist32, fresh_chain):
     dbeta = np.abs(a["beta"] - b["beta"])[same]
     # the chain again in FP64
     fresh_chain()
-    hist64 = np.array([h.run_chain(args.seed, 0, it, orders[it])[0] for it in range(len(orders))])
+    rows, k_ms = [], 0.0
+    for it in range(len(orders)):
+        rows.append(h.run_chain(args.seed, 0, it, orders[it])[0])
+        k_ms += h.stats()["kernel_ms"]
+    hist64 = np.array(rows)
     h.set_precision(brr.PRECISION_FP32_REFRESH, args.refresh)
     h2a = hist32[:, 4] / (hist32[:, 4] + hist32[:, 1])
     h2b = hist64[:, 4] / (hist64[:, 4] + hist64[:, 1])
@@ -210,6 +214,7 @@ def errors_vs_fp64(h, brr, data, m, args, orders, hist32, fresh_chain):
         "sigmaG_final_rel": float(abs(hist32[-1, 4] - hist64[-1, 4]) / hist64[-1, 4]),
         "h2_posterior_mean_fp32": float(np.mean(h2a[burn:])), "h2_posterior_mean_fp64": float(np.mean(h2b[burn:])),
         "h2_posterior_mean_mc_sem": sem(h2b[burn:]),
+        "fp64_chain_device_marker_updates_per_s": m * len(orders) / (k_ms * 1e-3) if k_ms > 0 else None,
         "seconds": time.time() - t0,
     }
     log(f"errors vs fp64: num max {out['num_max_abs']:.3e} (rel {out['num_max_rel_to_max_num']:.2e}), "
