@@ -324,14 +324,15 @@ __device__ __forceinline__ double bnum(const BShared& L, const BBlock& B, int k,
 }
 
 // logL and mu of every component (BayesRBase.cpp:309-333)
-__device__ __forceinline__ void blogl(const BShared& L, int t, int K, double num, double (&logL)[kMaxK],
-                                      double (&muk)[kMaxK])
+template <int kK>
+__device__ __forceinline__ void blogl(const BShared& L, int t, int K, double num, double (&logL)[kK],
+                                      double (&muk)[kK])
 {
     const double sigmaE = L.sigmaE[t];
     logL[0] = L.gc[0][t];
     muk[0] = 0.0;
 #pragma unroll
-    for (int c = 1; c < kMaxK; ++c) {
+    for (int c = 1; c < kK; ++c) {
         muk[c] = 0.0;
         logL[c] = 0.0;
         if (c < K) {
@@ -352,15 +353,16 @@ struct BDraw {
     int sel;
 };
 
-__device__ __forceinline__ BDraw bdraw(const BShared& L, int t, int K, double num, double beta_old, double u,
+template <int kK>
+__device__ __forceinline__ BDraw bdraw_k(const BShared& L, int t, int K, double num, double beta_old, double u,
                                        uint64_t zw, bool zwords)
 {
     const double n2 = num * num;
-    double d[kMaxK];
+    double d[kK];
     d[0] = 0.0;
     bool far = false;
 #pragma unroll
-    for (int c = 1; c < kMaxK; ++c) {
+    for (int c = 1; c < kK; ++c) {
         d[c] = 0.0;
         if (c < K) {
             d[c] = L.dg[c][t] + n2 * L.hk[c][t];
@@ -369,11 +371,11 @@ __device__ __forceinline__ BDraw bdraw(const BShared& L, int t, int K, double nu
     }
     BDraw r{beta_old, 0.0, -1};
     if (!far) {
-        double e[kMaxK];
+        double e[kK];
         double S = 1.0;
         e[0] = 1.0;
 #pragma unroll
-        for (int c = 1; c < kMaxK; ++c) {
+        for (int c = 1; c < kK; ++c) {
             e[c] = 0.0;
             if (c < K) {
                 e[c] = exp(d[c]);
@@ -384,15 +386,15 @@ __device__ __forceinline__ BDraw bdraw(const BShared& L, int t, int K, double nu
         double acum = invS;
         r.acum0 = acum;
 #pragma unroll
-        for (int c = 0; c < kMaxK; ++c) {
+        for (int c = 0; c < kK; ++c) {
             if (c < K && r.sel < 0) {
                 if (u <= acum) r.sel = c;
-                else if (c + 1 < kMaxK && c + 1 < K) acum += e[c + 1 < kMaxK ? c + 1 : 0] * invS;
+                else if (c + 1 < kK && c + 1 < K) acum += e[c + 1 < kK ? c + 1 : 0] * invS;
             }
         }
     } else {
-        double logL[kMaxK], muk[kMaxK];
-        blogl(L, t, K, num, logL, muk);
+        double logL[kK], muk[kK];
+        blogl<kK>(L, t, K, num, logL, muk);
         double acum = guard700(logL, K, logL[0]) ? 0.0 : inv_sum_exp(logL, K, logL[0]);
         r.acum0 = acum;
 #pragma unroll 1
@@ -401,7 +403,7 @@ __device__ __forceinline__ BDraw bdraw(const BShared& L, int t, int K, double nu
             if (c + 1 < K) {
                 double ref = 0.0;
 #pragma unroll
-                for (int q = 1; q < kMaxK; ++q)
+                for (int q = 1; q < kK; ++q)
                     if (q == c + 1) ref = logL[q];
                 if (!guard700(logL, K, ref)) acum += inv_sum_exp(logL, K, ref);
             }
@@ -414,6 +416,14 @@ __device__ __forceinline__ BDraw bdraw(const BShared& L, int t, int K, double nu
         r.bnew = z * L.gbsd[r.sel][t] + num / L.gden[r.sel][t];
     }
     return r;
+}
+
+// The loops unrolled to 4 components when K <= 4 (the usual mixture), else to kMaxK.
+__device__ __forceinline__ BDraw bdraw(const BShared& L, int t, int K, double num, double beta_old, double u,
+                                       uint64_t zw, bool zwords)
+{
+    return K <= 4 ? bdraw_k<4>(L, t, K, num, beta_old, u, zw, zwords)
+                  : bdraw_k<kMaxK>(L, t, K, num, beta_old, u, zw, zwords);
 }
 
 // -------------------------------------------------------------------- kernel --

@@ -255,6 +255,10 @@ template <bool B>
 struct BoolC {
     static constexpr bool value = B;
 };
+template <int N>
+struct IntC {
+    static constexpr int value = N;
+};
 
 // Value of a 2-bit code: two flat selects on 64-bit halves (no control flow).
 __device__ __forceinline__ double wt_of(uint32_t v, double w0, double w1, double w2, double w3)
@@ -661,22 +665,24 @@ __device__ __forceinline__ void group_constants(const SweepParams& p, Shared<R>&
 }
 
 // any_{i=1..K-1} |logL[i] - logL[r]| > 700 (BayesRBase.cpp:335, 360); loops run
-// to kMaxK with a predicate so the arrays stay in registers
-__device__ __forceinline__ bool guard700(const double (&logL)[kMaxK], int K, double ref)
+// to the array size N >= K with a predicate so the arrays stay in registers
+template <int N>
+__device__ __forceinline__ bool guard700(const double (&logL)[N], int K, double ref)
 {
     bool g = false;
 #pragma unroll
-    for (int i = 1; i < kMaxK; ++i)
+    for (int i = 1; i < N; ++i)
         if (i < K) g = g || (fabs(logL[i] - ref) > 700.0);
     return g;
 }
 
 // 1 / sum_k exp(logL[k] - ref) in index order (BayesRBase.cpp:338, 363)
-__device__ __forceinline__ double inv_sum_exp(const double (&logL)[kMaxK], int K, double ref)
+template <int N>
+__device__ __forceinline__ double inv_sum_exp(const double (&logL)[N], int K, double ref)
 {
     double s = 0.0;
 #pragma unroll
-    for (int k = 0; k < kMaxK; ++k)
+    for (int k = 0; k < N; ++k)
         if (k < K) s += exp(logL[k] - ref);
     return 1.0 / s;
 }
@@ -1392,7 +1398,10 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, Shared<R>& L, R
                         bool changed = false;
                         double bnew = 0.0, acum0 = 0.0, ed = 0.0, numk = 0.0;
                         int sel = -1, g = 0;
-                        if (act) {
+                        // the decision, with its per-component loops unrolled to 4 components
+                        // when K <= 4 (the usual mixture) and to kMaxK otherwise
+                        auto decide = [&](auto kc) {
+                            constexpr int kK = decltype(kc)::value;
                             g = L.gr[k];
                             const double* gc = L.gc + g * K;
                             const double* gden = L.gden + g * K;
@@ -1413,11 +1422,11 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, Shared<R>& L, R
                             // exp(d_c) / sum_i exp(d_i): one set of K - 1 exponentials.
                             // Otherwise the reference's own formulation, guards included.
                             const double n2 = num * num;
-                            double dd[kMaxK];
+                            double dd[kK];
                             dd[0] = 0.0;
                             bool far = false;
 #pragma unroll
-                            for (int c = 1; c < kMaxK; ++c) {
+                            for (int c = 1; c < kK; ++c) {
                                 dd[c] = 0.0;
                                 if (c < K) {
                                     dd[c] = L.gdg[g * K + c] + n2 * L.ghk[g * K + c];
@@ -1426,11 +1435,11 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, Shared<R>& L, R
                             }
                             const double uu = L.uu[k];
                             if (!far) {
-                                double ee[kMaxK];
+                                double ee[kK];
                                 double esum = 1.0;
                                 ee[0] = 1.0;
 #pragma unroll
-                                for (int c = 1; c < kMaxK; ++c) {
+                                for (int c = 1; c < kK; ++c) {
                                     ee[c] = 0.0;
                                     if (c < K) {
                                         ee[c] = exp(dd[c]);
@@ -1441,18 +1450,18 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, Shared<R>& L, R
                                 double acum = inv;
                                 acum0 = acum;
 #pragma unroll
-                                for (int c = 0; c < kMaxK; ++c) {
+                                for (int c = 0; c < kK; ++c) {
                                     if (c < K && sel < 0) {
                                         if (uu <= acum) sel = c;
-                                        else if (c + 1 < K) acum += ee[c + 1 < kMaxK ? c + 1 : 0] * inv;
+                                        else if (c + 1 < K) acum += ee[c + 1 < kK ? c + 1 : 0] * inv;
                                     }
                                 }
                             } else {
-                                double logL[kMaxK], muk[kMaxK];
+                                double logL[kK], muk[kK];
                                 logL[0] = gc[0];
                                 muk[0] = 0.0;
 #pragma unroll
-                                for (int c = 1; c < kMaxK; ++c) {
+                                for (int c = 1; c < kK; ++c) {
                                     muk[c] = 0.0;
                                     logL[c] = 0.0;
                                     if (c < K) {
@@ -1463,12 +1472,12 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, Shared<R>& L, R
                                 double acum = guard700(logL, K, logL[0]) ? 0.0 : inv_sum_exp(logL, K, logL[0]);
                                 acum0 = acum;
 #pragma unroll
-                                for (int c = 0; c < kMaxK; ++c) {
+                                for (int c = 0; c < kK; ++c) {
                                     if (c < K && sel < 0) {
                                         if (uu <= acum) {
                                             sel = c;
                                         } else if (c + 1 < K) {
-                                            const double ref = logL[c + 1 < kMaxK ? c + 1 : 0];
+                                            const double ref = logL[c + 1 < kK ? c + 1 : 0];
                                             if (!guard700(logL, K, ref)) acum += inv_sum_exp(logL, K, ref);
                                         }
                                     }
@@ -1481,6 +1490,10 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, Shared<R>& L, R
                                 bnew = bits_as_double(L.zw[k]) * gbsd[sel] + num / gden[sel];
                             }
                             changed = (bnew != beta_old);
+                        };
+                        if (act) {
+                            if (K <= 4) decide(IntC<4>{});
+                            else decide(IntC<kMaxK>{});
                         }
                         const unsigned long long cm = __ballot(changed);
                         // sequential sweep: the first change ends the pass; asynchronous
