@@ -816,29 +816,41 @@ template <int R>
 __device__ __forceinline__ void dense_issue(const SweepParams& p, const Shared<R>& L, int nb, uint32_t lo,
                                             int tid, uint32_t (&words)[8 * R])
 {
-    const uint32_t per = (uint32_t)R * 32u;
-    const uint32_t total = (uint32_t)nb * per;
+    // word t = u8 * 256 + tid is word rem = tid % (32 R) of slot k = t / (32 R): the
+    // same word of every slot for one thread (256 is a multiple of 32 R). The marker
+    // ids are read first and the loads issued without branches (an invalid word
+    // reads word 0; dense_store drops it), so the LDS reads and the loads go out
+    // together and nothing waits for a load before the block's commit.
+    constexpr uint32_t per = (uint32_t)R * 32u;
+    constexpr uint32_t kStep = kThreads / per;
+    const uint32_t rem = (uint32_t)tid % per;
+    const uint32_t k0 = (uint32_t)tid / per;
+    const bool inb = (uint64_t)(lo >> 2) + 4ull * rem < p.col_bytes;
+    const size_t off = (size_t)(lo >> 4) + rem;
+    uint32_t jj[8 * R];
 #pragma unroll
     for (int u8 = 0; u8 < 8 * R; ++u8) {
-        const uint32_t t = (uint32_t)u8 * kThreads + (uint32_t)tid;
-        words[u8] = 0u;
-        if (t < total) {
-            const uint32_t k = t / per, rem = t % per;
-            const uint32_t j = L.mk_next[k];
-            const uint64_t byte_off = (uint64_t)(lo >> 2) + 4ull * rem;
-            if (byte_off < p.col_bytes) words[u8] = p.dense[(size_t)j * p.col_words + (lo >> 4) + rem];
-        }
+        const uint32_t k = k0 + (uint32_t)u8 * kStep;
+        jj[u8] = L.mk_next[k < (uint32_t)nb ? k : 0u];
+    }
+#pragma unroll
+    for (int u8 = 0; u8 < 8 * R; ++u8) {
+        const bool ok = k0 + (uint32_t)u8 * kStep < (uint32_t)nb && inb;
+        words[u8] = p.dense[ok ? (size_t)jj[u8] * p.col_words + off : 0];
     }
 }
 
 template <int R>
-__device__ __forceinline__ void dense_store(Shared<R>& L, int nb, int tid, const uint32_t (&words)[8 * R])
+__device__ __forceinline__ void dense_store(const SweepParams& p, Shared<R>& L, int nb, uint32_t lo, int tid,
+                                            const uint32_t (&words)[8 * R])
 {
     const uint32_t total = (uint32_t)nb * (uint32_t)R * 32u;
+    const uint32_t rem = (uint32_t)tid % ((uint32_t)R * 32u);
+    const bool inb = (uint64_t)(lo >> 2) + 4ull * rem < p.col_bytes;   // past the column: zero
 #pragma unroll
     for (int u8 = 0; u8 < 8 * R; ++u8) {
         const uint32_t t = (uint32_t)u8 * kThreads + (uint32_t)tid;
-        if (t < total) (&L.codes[0][0][0])[t] = words[u8];
+        if (t < total) (&L.codes[0][0][0])[t] = inb ? words[u8] : 0u;
     }
 }
 
@@ -1217,7 +1229,7 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, Shared<R>& L, R
             if constexpr (kCodes) {
                 uint32_t words[8 * R];
                 dense_issue<R>(p, L, nb0, lo, tid, words);
-                dense_store<R>(L, nb0, tid, words);
+                dense_store<R>(p, L, nb0, lo, tid, words);
             } else {
                 uint64_t st = 0;
                 uint32_t ln = 0;
@@ -1257,8 +1269,11 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, Shared<R>& L, R
 
             // prefetch issue: everything of block blk+1 whose address is known
             // (ids in L.mk_next, ragged segments in L.seg_*), and the ids of blk+2
+            // The ids of blk+2 are loaded first and without a branch (a zeroed register
+            // that a load may still be writing would make the block wait for every
+            // load issued before it, the prefetch included); only tid < nb2 use them.
+            const uint32_t jnext2 = order[has2 && tid < nb2 ? b0 + 2 * p.B + tid : 0u];
             MetaNext mnext{};
-            uint32_t jnext2 = 0;
             uint64_t st2 = 0;
             uint32_t ln2 = 0;
             uint32_t words[kCodes ? 8 * R : 1];
@@ -1269,7 +1284,6 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, Shared<R>& L, R
                 if constexpr (kCodes) dense_issue<R>(p, L, nb1, lo, tid, words);
                 else idx_issue<R>(p, L, nb1, tid, idx, itm);
             }
-            if (has2 && tid < nb2) jnext2 = order[b0 + 2 * p.B + tid];
             BRR_TICK(0);
 
             int pos = 0;
@@ -1651,7 +1665,7 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, Shared<R>& L, R
 
             // ---- commit the prefetch: block blk+1 becomes current
             if (has1) {
-                if constexpr (kCodes) dense_store<R>(L, nb1, tid, words);
+                if constexpr (kCodes) dense_store<R>(p, L, nb1, lo, tid, words);
                 else ragged_store<R>(p, L, nb1, lo, tid, idx, itm);
                 if (tid < nb1) meta_store<R, kLay>(p, L, tid, mnext, false);
                 __syncthreads();
