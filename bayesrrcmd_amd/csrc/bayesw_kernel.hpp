@@ -97,6 +97,11 @@ struct BwShared {
     uint32_t ord[2][kBwB];        // marker ids of the next two blocks, by block parity
     double bold[kBwB], mean[kBwB], sd[kBwB], sumf[kBwB], pu[kBwB];
     double tab[kBwB][4];          // dense: column value by 2-bit code
+    // per-code indicators of two adjacent individuals whose codes form the 4-bit
+    // field n: ind[n][q] = ([code_lo == cq], [code_hi == cq]) for cq = 2, 0, 1
+    // (genotype 1, genotype 2, missing), so a slot's per-code sums of vi are FMAs
+    // with 0 / 1 (the same additions as a select, in the same order)
+    alignas(16) double ind[16][3][2];
     double xv[2][kBwV];           // round totals, by round parity
     double cls[kMaxK], pi[kMaxK], v[kMaxK];
     double gv[kBwB][kMaxK - 1][14];   // weighted integrand terms per slot, slab class and node
@@ -630,6 +635,12 @@ bayesw_sweep_kernel(BwParams p)
             eps[r][e] = (idx < hi) ? p.eps[idx] : 0.0;
             vi[r][e] = 0.0;
         }
+    if (tid < 16 * 3 * 2) {
+        const int n = tid / 6, q = (tid / 2) % 3, half = tid & 1;
+        const uint32_t code = half ? ((uint32_t)n >> 2) : ((uint32_t)n & 3u);
+        const uint32_t want = q == 0 ? 2u : (q == 1 ? 0u : 1u);
+        L.ind[n][q][half] = code == want ? 1.0 : 0.0;
+    }
     if (tid == 0) {
         L.abort = 0;
         L.arms_err = 0;
@@ -894,27 +905,58 @@ bayesw_sweep_kernel(BwParams p)
                 if (!breaker) {
                     // vi_sum of the slice and (vi_1, vi_2) of slots pos..h-1: wave q
                     // takes slots pos + q + 4 i, values 2 i (genotype 1) and 2 i + 1
+                    // slots in groups of four whose code words are read together (a group
+                    // at or past h is skipped; a slot at or past h within a group reads
+                    // slot pos and contributes 0); the indicator pairs come from L.ind
                     double part[16], part8[8];
 #pragma unroll
-                    for (int i = 0; i < 8; ++i) {
-                        const int k = pos + wave + kWaves * i;
-                        double s1 = 0.0, s2 = 0.0, s3 = 0.0;
-                        if (k < h) {
+                    for (int g4 = 0; g4 < 2; ++g4) {
+                        if (pos + wave + kWaves * 4 * g4 < h) {
+                            int kc[4];
+                            uint32_t bits[4][R];
 #pragma unroll
-                            for (int r = 0; r < R; ++r) {
-                                const uint32_t bits = L.codes[k][r][lane >> 1] >> (16 * (lane & 1));
+                            for (int j = 0; j < 4; ++j) {
+                                const int k = pos + wave + kWaves * (4 * g4 + j);
+                                kc[j] = k < h ? k : pos;
 #pragma unroll
-                                for (int e = 0; e < 8; ++e) {
-                                    const uint32_t c = (bits >> (2 * e)) & 3u;
-                                    s1 += (c == 2u) ? vi[r][e] : 0.0;
-                                    s2 += (c == 0u) ? vi[r][e] : 0.0;
-                                    if constexpr (kDense) s3 += (c == 1u) ? vi[r][e] : 0.0;
+                                for (int r = 0; r < R; ++r)
+                                    bits[j][r] = L.codes[kc[j]][r][lane >> 1] >> (16 * (lane & 1));
+                            }
+#pragma unroll
+                            for (int j = 0; j < 4; ++j) {
+                                const int i = 4 * g4 + j;
+                                double s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll
+                                for (int r = 0; r < R; ++r) {
+#pragma unroll
+                                    for (int q = 0; q < 4; ++q) {
+                                        const uint32_t n = (bits[j][r] >> (4 * q)) & 15u;
+                                        const dbl2 a = *reinterpret_cast<const dbl2*>(&L.ind[n][0][0]);
+                                        const dbl2 b = *reinterpret_cast<const dbl2*>(&L.ind[n][1][0]);
+                                        s1 = __builtin_fma(a.x, vi[r][2 * q], s1);
+                                        s1 = __builtin_fma(a.y, vi[r][2 * q + 1], s1);
+                                        s2 = __builtin_fma(b.x, vi[r][2 * q], s2);
+                                        s2 = __builtin_fma(b.y, vi[r][2 * q + 1], s2);
+                                        if constexpr (kDense) {
+                                            const dbl2 m = *reinterpret_cast<const dbl2*>(&L.ind[n][2][0]);
+                                            s3 = __builtin_fma(m.x, vi[r][2 * q], s3);
+                                            s3 = __builtin_fma(m.y, vi[r][2 * q + 1], s3);
+                                        }
+                                    }
                                 }
+                                const bool live = pos + wave + kWaves * i < h;
+                                part[2 * i] = live ? s1 : 0.0;
+                                part[2 * i + 1] = live ? s2 : 0.0;
+                                part8[i] = live ? s3 : 0.0;
+                            }
+                        } else {
+#pragma unroll
+                            for (int i = 4 * g4; i < 4 * g4 + 4; ++i) {
+                                part[2 * i] = 0.0;
+                                part[2 * i + 1] = 0.0;
+                                part8[i] = 0.0;
                             }
                         }
-                        part[2 * i] = s1;
-                        part[2 * i + 1] = s2;
-                        part8[i] = s3;
                     }
                     const double tot = transpose_reduce16(part, lane);
                     const int vv = (lane >> 2) & 15;

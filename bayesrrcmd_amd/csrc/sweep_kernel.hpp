@@ -541,15 +541,42 @@ __device__ __forceinline__ bool exchange(const SweepParams& p, Shared<R>& L, uin
     }
     const int k = pos + lane;
     if (wave == 0 && k < nb) {
-        double x, g = 0.0;
+        // the slot's dot and Gram totals are loaded together and only stale ones
+        // polled again (one round trip when both have arrived)
         const uint32_t copy = w % kCopies;
-        if (!wait_pair(p.xchg2, slot2(p, rnd, k, copy), rnd, p.ctl, &x) ||
-            (k >= gpos && !wait_pair(p.xchg2, slot2(p, rnd, kMaxB + k, copy), rnd, p.ctl, &g))) {
+        const uint32_t need = 1u | (k >= gpos ? 2u : 0u);
+        uint64_t hh[2], ll[2];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            hh[q] = 0;
+            ll[q] = 0;
+            if ((need >> q) & 1u) ld_pair(p.xchg2, slot2(p, rnd, q * kMaxB + k, copy), hh[q], ll[q]);
+        }
+        uint64_t spins = 0;
+        bool ok2 = true;
+        for (;;) {
+            uint32_t stale = 0;
+#pragma unroll
+            for (int q = 0; q < 2; ++q)
+                if (((need >> q) & 1u) && ((uint32_t)(hh[q] >> 32) != rnd || (uint32_t)(ll[q] >> 32) != rnd))
+                    stale |= 1u << q;
+            if (!stale) break;
+            if (++spins > kSpinLimit || ((spins & 255u) == 0 &&
+                                         __hip_atomic_load(p.ctl + kCtlAbort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+                ok2 = false;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+            for (int q = 0; q < 2; ++q)
+                if ((stale >> q) & 1u) ld_pair(p.xchg2, slot2(p, rnd, q * kMaxB + k, copy), hh[q], ll[q]);
+        }
+        if (!ok2) {
             L.abort = 1;
             raise_abort(p, 2);
         } else {
-            L.dot[k] = x;
-            L.gram[k] = g;
+            L.dot[k] = bits_as_double((hh[0] << 32) | (ll[0] & 0xFFFFFFFFull));
+            L.gram[k] = (need & 2u) ? bits_as_double((hh[1] << 32) | (ll[1] & 0xFFFFFFFFull)) : 0.0;
         }
     }
     __syncthreads();
