@@ -130,12 +130,14 @@ def make_batch_data(cfg, seed, threads):
     return dict(bed=bed, lut=lut, y=y, h2=h2, frac=frac)
 
 
-def algorithmic_bytes(cfg, data, storage, markers_per_step, changes, eps_elem=8):
-    """Per step: sum_j B_j with B_j = G_j + e N (1 + delta_j): G_j the column
+def algorithmic_bytes(cfg, data, storage, markers_per_step, changes, eps_elem=4):
+    """Per step: sum_j B_j with B_j = G_j + 4 N (1 + delta_j): G_j the column
     bytes of the device copy (index lists: 4 (n1 + n2 + nmiss) + 32; 2-bit
-    codes: ceil(N/4)), e N one read of eps (e = 4 bytes for the FP32 residual,
-    8 in FP64), delta_j = 1 when marker j's beta changed (one more eps pass, the
-    update). The eps terms are logical passes: the kernel keeps eps in registers."""
+    codes: ceil(N/4)), 4 N one read of an FP32 residual, delta_j = 1 when marker
+    j's beta changed (one more residual pass, the update). The figure is fixed
+    whatever precision the kernel computes in, so the FP32 and FP64 paths are
+    priced alike. The residual terms are logical passes: the kernel keeps the
+    residual in registers."""
     n, m = cfg["n"], cfg["m"]
     if cfg["layout"] == "ragged" and storage == 1:
         r = data["ragged"]
@@ -469,10 +471,11 @@ def run_bayesw(cfg, args, with_cpu):
     stats = h.stats()
     hist = np.array(hist)
     avg_ms = float(np.mean(kernel_ms))
-    # B_j = G_j + 8 N + 24 N delta_j (FP64): the 2-bit column, one vi read, and
-    # on a change an eps read, an eps write and a vi write
+    # B_j = G_j + 4 N + 12 N delta_j: the 2-bit column, one FP32 vi read, and on
+    # a change an eps read, an eps write and a vi write (FP32 words, whatever
+    # precision the kernel computes in)
     g_col = float((n + 3) // 4)
-    bytes_per_launch = float(np.mean([m * g_col + 8.0 * n * m + 24.0 * n * c for c in changes]))
+    bytes_per_launch = float(np.mean([m * g_col + 4.0 * n * m + 12.0 * n * c for c in changes]))
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     burn = min(steps // 2, 30)
     out = {
@@ -496,6 +499,10 @@ def run_bayesw(cfg, args, with_cpu):
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "achieved_basis": ("logical: B_j = G_j + 4 N + 12 N delta_j per marker-update (the 2-bit column, "
+                               "an FP32 vi pass, and on a change an eps read, an eps write and a vi write, "
+                               "whatever precision the kernel computes in); eps and vi live in registers"),
+            "limiter": "latency: exchange rounds, class likelihoods and ARMS draws",
             "kernel": "brr::bayesw_sweep_kernel (one launch per iteration)",
             "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": bytes_per_launch,
             "rounds_per_launch": float(np.mean(rounds)), "changes_per_launch": float(np.mean(changes)),
@@ -560,11 +567,10 @@ def run_batched(cfg, args, with_cpu):
     stats = h.stats()
     hist = np.array(hist)                                  # (steps, T, 5)
     avg_ms = float(np.mean(kernel_ms))
-    # B_j = G_j + e N T + e N (changes of marker j over the traits): the column
-    # once for all traits, one eps read per trait (e = 4 bytes FP32, 8 FP64), one more per change
+    # B_j = G_j / T + 4 N (1 + delta_j) per (marker, trait): the column once for
+    # all traits, one FP32 residual read per trait, one more per change
     g_col = float((n + 3) // 4)
-    e_b = 4.0 if precision == "fp32" else 8.0
-    bytes_per_launch = float(np.mean([m * g_col + e_b * n * (T * m + c) for c in changes]))
+    bytes_per_launch = float(np.mean([m * g_col + 4.0 * n * (T * m + c) for c in changes]))
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     # matrix-core work per launch: dots 2 N T per marker, Gram 2 N per pair of a 32-marker block
     flops = 2.0 * n * T * m + 2.0 * n * (m / 32.0) * (32 * 31 / 2)
@@ -590,6 +596,10 @@ def run_batched(cfg, args, with_cpu):
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "achieved_basis": ("logical: B_j = G_j / T + 4 N (1 + delta_j) per (marker, trait) update (the "
+                               "column once for all traits, an FP32 residual pass per trait, one more per change); "
+                               "the residual tiles live in registers"),
+            "limiter": "latency: decision rounds and exchange rounds",
             "kernel": "brr::bayesr_batch_kernel (one launch per iteration)",
             "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": bytes_per_launch,
             "mfma_tflops": flops / (avg_ms * 1e-3) / 1e12,
@@ -701,7 +711,7 @@ def run_cohorts(cfg, args, with_cpu):
     for c, h in enumerate(hs):
         st = h.stats()
         cfg_c = dict(n=h.n, m=h.m, layout="dense")     # the device holds codes in either layout
-        bytes_total += algorithmic_bytes(cfg_c, None, 2, steps * h.m, st["changes"], 4 if precision == "fp32" else 8)
+        bytes_total += algorithmic_bytes(cfg_c, None, 2, steps * h.m, st["changes"])
         updates += h.m * steps
         per.append(dict(N=h.n, M=h.m, layout=layouts[c], workgroups=st["workgroups"], launch=rep["launch_of"][c],
                         rounds=st["rounds"], changes=st["changes"]))
@@ -725,8 +735,9 @@ def run_cohorts(cfg, args, with_cpu):
                        "autotune": tune, "per_cohort": per},
                roofline={"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                         "achieved_basis": "logical: B_j = ceil(N/4) + e N (1 + delta_j) per marker-update, e = 4 "
-                                           "(fp32) or 8; the residual passes are served from registers",
+                         "achieved_basis": "logical: B_j = ceil(N/4) + 4 N (1 + delta_j) per marker-update (an FP32 "
+                                           "residual pass, whatever the precision); the residual passes are served "
+                                           "from registers",
                          "limiter": "latency (exchange rounds)",
                          "kernel": "brr::bayesr_cohort_kernel (all cohorts, one graph)",
                          "device_ms": rep["device_ms"]})
@@ -924,8 +935,7 @@ def main():
     value = total_updates / wall
     kernel_ms = np.array(kernel_ms)
     avg_ms = float(kernel_ms.mean())
-    eps_elem = 4 if precision == "fp32" else 8
-    bytes_per_launch = np.mean([algorithmic_bytes(cfg, data, stats["storage"], m, c, eps_elem) for c in changes])
+    bytes_per_launch = np.mean([algorithmic_bytes(cfg, data, stats["storage"], m, c) for c in changes])
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     sigmaE, sigmaG = hist[:, 1], hist[:, 4]
     burn = min(steps // 2, 50)
@@ -963,10 +973,10 @@ def main():
             "traffic_note": ("HBM bytes per launch are measured by separate rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE "
                              "runs of this benchmark (profiles/bayesr_sweep_c2_ragged_N100k_M100k_hbm_counters.md); "
                              "a counter run cannot share this process"),
-            "achieved_basis": ("logical: sum over the launch's markers of B_j = G_j + e N (1 + delta_j), G_j the "
-                               "column's device bytes, e = 4 (fp32 residual) or 8, delta_j = 1 when beta_j changed; "
-                               "the residual passes are served from registers, so HBM moves only the columns and "
-                               "the exchange words"),
+            "achieved_basis": ("logical: sum over the launch's markers of B_j = G_j + 4 N (1 + delta_j), G_j the "
+                               "column's device bytes, 4 N one FP32 residual pass (whatever precision the kernel "
+                               "computes in), delta_j = 1 when beta_j changed; the residual passes are served from "
+                               "registers, so HBM moves only the columns and the exchange words"),
             "kernel": "brr::bayesr_sweep_kernel (one launch per iteration)",
             "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": bytes_per_launch,
             "column_bytes_per_launch": float(m) * ((n + 3) // 4) if stats["storage"] == 2 else None,
