@@ -825,6 +825,7 @@ def main():
                          "reductions, FP64 refresh every --refresh markers; the c2 default) or fp64")
     ap.add_argument("--refresh", type=int, default=1024, help="FP64 residual refresh interval (markers)")
     ap.add_argument("--no-probe", action="store_true", help="skip the exchange-latency probe of the headline line")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape of the headline line")
     ap.add_argument("--no-errors", action="store_true",
                     help="skip the FP64 comparison run of an fp32 line (errors_vs_fp64)")
     ap.add_argument("--decode-only", action="store_true",
@@ -901,6 +902,15 @@ def main():
     for it in range(args.warmup):
         brr.shuffle(args.seed + 1, it, 0, order)
         h.run_chain(args.seed + 1, 0, it, order)
+    # the launch shape: the autotuner times each candidate (workgroups x markers per
+    # block) on one iteration from the warm-up chain's state and keeps the fastest
+    tune = None
+    if not (args.workgroups or args.block or args.async_window or args.no_autotune):
+        t0 = time.time()
+        tune = h.autotune(args.seed + 2, 1)
+        st = h.stats()
+        log(f"autotune: {len(tune)} shapes in {time.time() - t0:.1f}s -> {st['workgroups']} workgroups, "
+            f"{st['block_markers']}-marker blocks")
     order = np.arange(m, dtype=np.uint32)
     fresh_chain()
     orders = np.empty((steps, m), dtype=np.uint32)
@@ -966,6 +976,8 @@ def main():
             "schedule": (f"asynchronous windows of {args.async_window} markers (ParallelGraph)"
                          if args.async_window else "sequential (exact)"),
             "h2_posterior_mean": h2,
+            "autotune": ([{k: (round(v, 3) if isinstance(v, float) else v) for k, v in e.items()} for e in tune]
+                         if tune else None),
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
