@@ -941,8 +941,13 @@ bayesr_batch_kernel(BatchParams p)
                 }
                 // betasqnG += beta^2 in marker order (BayesRBase.cpp:353)
                 if (kq == 0 && tok) {
-                    for (int k = 0; k < nb; ++k)
-                        if (L.sl[k][tt] > 0) L.betasqn[tt] += L.bn[k][tt] * L.bn[k][tt];
+                    double bs = L.betasqn[tt];
+#pragma unroll 8
+                    for (int k = 0; k < nb; ++k) {
+                        const double b = L.bn[k][tt];
+                        if (L.sl[k][tt] > 0) bs += b * b;
+                    }
+                    L.betasqn[tt] = bs;
                 }
                 // ---- 4. eps_t += sum_k delta_{k,t} u_k on the matrix cores, for the
                 // 4-marker steps holding a change of this wave's traits
