@@ -147,6 +147,18 @@ def algorithmic_bytes(cfg, data, storage, markers_per_step, changes, eps_elem=4)
     return g_total * (markers_per_step / m) + eps_elem * n * (markers_per_step + changes)
 
 
+def chain_orders(seed, m, count):
+    """The marker orders of iterations 0 .. count-1 of one chain (each iteration
+    reshuffles the previous order, as the reference's runGibbs does)."""
+    import bayesrrcmd_amd as brr
+    order = np.arange(m, dtype=np.uint32)
+    orders = np.empty((count, m), dtype=np.uint32)
+    for it in range(count):
+        brr.shuffle(seed, it, 0, order)
+        orders[it] = order
+    return orders
+
+
 def errors_vs_fp64(h, brr, data, m, args, orders, hist32, fresh_chain):
     """Errors of the FP32-residual chain against the FP64 path of the same
     device code (whose parity with the FP64 oracle the GPU tests pin):
@@ -181,12 +193,13 @@ def errors_vs_fp64(h, brr, data, m, args, orders, hist32, fresh_chain):
     dnum = np.abs(a["num"] - b["num"])
     same = a["comp"] == b["comp"]
     dbeta = np.abs(a["beta"] - b["beta"])[same]
-    # the chain again in FP64
+    # the chain again in FP64 (its device rate over the timed iterations, after the warm-up ones)
     fresh_chain()
     rows, k_ms = [], 0.0
     for it in range(len(orders)):
         rows.append(h.run_chain(args.seed, 0, it, orders[it])[0])
-        k_ms += h.stats()["kernel_ms"]
+        if it >= args.warmup:
+            k_ms += h.stats()["kernel_ms"]
     hist64 = np.array(rows)
     h.set_precision(brr.PRECISION_FP32_REFRESH, args.refresh)
     h2a = hist32[:, 4] / (hist32[:, 4] + hist32[:, 1])
@@ -216,7 +229,8 @@ def errors_vs_fp64(h, brr, data, m, args, orders, hist32, fresh_chain):
         "sigmaG_final_rel": float(abs(hist32[-1, 4] - hist64[-1, 4]) / hist64[-1, 4]),
         "h2_posterior_mean_fp32": float(np.mean(h2a[burn:])), "h2_posterior_mean_fp64": float(np.mean(h2b[burn:])),
         "h2_posterior_mean_mc_sem": sem(h2b[burn:]),
-        "fp64_chain_device_marker_updates_per_s": m * len(orders) / (k_ms * 1e-3) if k_ms > 0 else None,
+        "fp64_chain_device_marker_updates_per_s": (m * (len(orders) - args.warmup) / (k_ms * 1e-3)
+                                                   if k_ms > 0 else None),
         "seconds": time.time() - t0,
     }
     log(f"errors vs fp64: num max {out['num_max_abs']:.3e} (rel {out['num_max_rel_to_max_num']:.2e}), "
@@ -442,27 +456,21 @@ def run_bayesw(cfg, args, with_cpu):
         h.set_state(eps, np.zeros(m), np.zeros(m, np.int32))
         h.set_scalars(sc["mu"], sc["alpha"], sc["sigma_b"], sc["d"], sc["pi"], sc["v"])
 
-    order = np.arange(m, dtype=np.uint32)
+    # one chain from init: its first `warmup` iterations untimed, the next `steps` timed
     fresh_chain()
+    orders = chain_orders(args.seed, m, args.warmup + steps)
     for it in range(args.warmup):
-        brr.shuffle(args.seed + 1, it, 0, order)
-        h.run_chain(args.seed + 1, 0, it, order)
-    fresh_chain()
-    order = np.arange(m, dtype=np.uint32)
-    orders = np.empty((steps, m), dtype=np.uint32)
-    for it in range(steps):
-        brr.shuffle(args.seed, it, 0, order)
-        orders[it] = order
+        h.run_chain(args.seed, 0, it, orders[it])
     kernel_ms, changes, rounds, hist, phases = [], [], [], [], []
     t_start = time.perf_counter()
-    for it in range(steps):
+    for it in range(args.warmup, args.warmup + steps):
         hist.append(h.run_chain(args.seed, 0, it, orders[it])[0])
         s = h.stats()
         kernel_ms.append(s["kernel_ms"])
         changes.append(s["changes"])
         rounds.append(s["rounds"])
         phases.append(list(s["phase_ms"].values()))
-        if it % 20 == 0 or it == steps - 1:
+        if it % 20 == 0 or it == args.warmup + steps - 1:
             ph = ", ".join(f"{k} {v:.1f}" for k, v in s["phase_ms"].items())
             log(f"bayesw iteration {it}: {s['kernel_ms']:.2f} ms, rounds {s['rounds']}, changes {s['changes']}, "
                 f"mu {hist[-1][0]:.4f} alpha {hist[-1][1]:.4f} nonzero {hist[-1][3]:.0f}, "
@@ -489,6 +497,7 @@ def run_bayesw(cfg, args, with_cpu):
             "layout": ("dense 2-bit" if cfg["layout"] == "dense" else "ragged (held on the device as 2-bit codes)"),
             "maf": list(cfg["maf"]), "causal_fraction": cfg["causal"], "C": list(cfg["C"]),
             "quad_points": cfg["quad_points"], "censored_fraction": cfg["censor"], "timed_iterations": steps,
+            "chain": f"one chain from init: {args.warmup} untimed warm-up iterations, then the timed ones",
             "precision": "fp64", "workgroups": stats["workgroups"], "block_markers": stats["block_markers"],
             "slice": stats["slice"], "ragged_list_bytes": 4 * nnz if nnz else None,
             "parallelism": "single GPU, persistent cooperative kernel",
@@ -539,27 +548,21 @@ def run_batched(cfg, args, with_cpu):
             h.set_scalars(sc["mu"], sc["sigmaE"], sc["epsilon_sum"], sc["sigmaG"], sc["pi"], trait=t)
         h.set_state(eps, np.zeros((m, T)), np.zeros((m, T), np.int32))
 
-    order = np.arange(m, dtype=np.uint32)
-    fresh_chain(args.seed + 1)
-    for it in range(args.warmup):
-        brr.shuffle(args.seed + 1, it, 0, order)
-        h.run_chain(args.seed + 1, 0, it, order)
+    # one set of chains from init: the first `warmup` iterations untimed, the next `steps` timed
     fresh_chain(args.seed)
-    order = np.arange(m, dtype=np.uint32)
-    orders = np.empty((steps, m), dtype=np.uint32)
-    for it in range(steps):
-        brr.shuffle(args.seed, it, 0, order)
-        orders[it] = order
+    orders = chain_orders(args.seed, m, args.warmup + steps)
+    for it in range(args.warmup):
+        h.run_chain(args.seed, 0, it, orders[it])
     kernel_ms, changes, rounds, phases, hist = [], [], [], [], []
     t_start = time.perf_counter()
-    for it in range(steps):
+    for it in range(args.warmup, args.warmup + steps):
         hist.append(h.run_chain(args.seed, 0, it, orders[it])[0])
         s = h.stats()
         kernel_ms.append(s["kernel_ms"])
         changes.append(s["changes"])
         rounds.append(s["rounds"])
         phases.append(list(s["phase_ms"].values()))
-        if it % 20 == 0 or it == steps - 1:
+        if it % 20 == 0 or it == args.warmup + steps - 1:
             ph = ", ".join(f"{k} {v:.1f}" for k, v in s["phase_ms"].items())
             log(f"batched iteration {it}: {s['kernel_ms']:.2f} ms, rounds {s['rounds']}, changes {s['changes']}, "
                 f"decision rounds {s['decision_rounds']} | phases ms: {ph}")
@@ -587,6 +590,7 @@ def run_batched(cfg, args, with_cpu):
             "workload": cfg["workload"], "N": n, "M": m, "traits": T, "layout": "dense 2-bit",
             "maf": list(cfg["maf"]), "causal_fraction_range": list(cfg["causal"]), "h2_range": list(cfg["h2"]),
             "S": S_DEFAULT.tolist(), "timed_iterations": steps,
+            "chain": f"one set of chains from init: {args.warmup} untimed warm-up iterations, then the timed ones",
             "precision": ("fp32 residual tiles (f32 matrix cores, per-tile products summed in fp64), fp64 "
                           "decisions and draws, no refresh" if precision == "fp32" else "fp64"),
             "workgroups": stats["workgroups"], "block_markers": stats["block_markers"], "slice": stats["slice"],
@@ -896,14 +900,16 @@ def main():
         h.set_state(eps, np.zeros(m), np.zeros(m, np.int32))
         h.set_scalars(sc["mu"], sc["sigmaE"], sc["epsilon_sum"], sc["sigmaG"], sc["pi"])
 
-    order = np.arange(m, dtype=np.uint32)
-    # warm-up: a short separate chain (code paths, caches), then a fresh chain is timed
+    # one chain from init: its first `warmup` iterations are untimed (the start of
+    # the burn-in), the next `steps` are timed
     fresh_chain()
+    orders = chain_orders(args.seed, m, args.warmup + steps)
+    hist = []
     for it in range(args.warmup):
-        brr.shuffle(args.seed + 1, it, 0, order)
-        h.run_chain(args.seed + 1, 0, it, order)
+        hist.append(h.run_chain(args.seed, 0, it, orders[it])[0])
     # the launch shape: the autotuner times each candidate (workgroups x markers per
-    # block) on one iteration from the warm-up chain's state and keeps the fastest
+    # block) on one iteration from the chain's state after the warm-up, restores that
+    # state, and keeps the fastest
     tune = None
     if not (args.workgroups or args.block or args.async_window or args.no_autotune):
         t0 = time.time()
@@ -911,24 +917,17 @@ def main():
         st = h.stats()
         log(f"autotune: {len(tune)} shapes in {time.time() - t0:.1f}s -> {st['workgroups']} workgroups, "
             f"{st['block_markers']}-marker blocks")
-    order = np.arange(m, dtype=np.uint32)
-    fresh_chain()
-    orders = np.empty((steps, m), dtype=np.uint32)
-    for it in range(steps):
-        brr.shuffle(args.seed, it, 0, order)
-        orders[it] = order
 
     kernel_ms, changes, rounds, phases = [], [], [], []
-    hist = []
     t_start = time.perf_counter()
-    for it in range(steps):
+    for it in range(args.warmup, args.warmup + steps):
         hist.append(h.run_chain(args.seed, 0, it, orders[it])[0])
         s = h.stats()
         kernel_ms.append(s["kernel_ms"])
         changes.append(s["changes"])
         rounds.append(s["rounds"])
         phases.append(list(s["phase_ms"].values()))
-        if it % 20 == 0 or it == steps - 1:
+        if it % 20 == 0 or it == args.warmup + steps - 1:
             ph = ", ".join(f"{k} {v:.1f}" for k, v in s["phase_ms"].items())
             log(f"iteration {it}: {s['kernel_ms']:.2f} ms, rounds {s['rounds']}, changes {s['changes']} "
                 f"| phases ms: {ph}")
@@ -947,9 +946,12 @@ def main():
     avg_ms = float(kernel_ms.mean())
     bytes_per_launch = np.mean([algorithmic_bytes(cfg, data, stats["storage"], m, c) for c in changes])
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    # h2 over the whole chain (warm-up and timed iterations) after its burn-in
     sigmaE, sigmaG = hist[:, 1], hist[:, 4]
-    burn = min(steps // 2, 50)
-    h2 = float(np.mean(sigmaG[burn:] / (sigmaG[burn:] + sigmaE[burn:]))) if steps > 1 else float("nan")
+    burn = min(len(hist) // 2, 50)
+    h2 = float(np.mean(sigmaG[burn:] / (sigmaG[burn:] + sigmaE[burn:]))) if len(hist) > 1 else float("nan")
+    # the late part of the timed iterations (the chain near its stationary state), device time
+    late = kernel_ms[-max(1, steps // 4):]
 
     line = {
         "metric": METRIC,
@@ -969,6 +971,9 @@ def main():
             "device_storage": "2-bit codes" if stats["storage"] == 2 else "index lists",
             "maf": list(cfg["maf"]), "causal_fraction": cfg["causal"], "h2_simulated": cfg["h2"],
             "S": S_DEFAULT.tolist(), "timed_iterations": steps,
+            "chain": (f"one chain from init: iterations 0..{args.warmup - 1} untimed (warm-up), "
+                      f"{args.warmup}..{args.warmup + steps - 1} timed" if args.warmup else
+                      f"one chain from init: iterations 0..{steps - 1} timed"),
             "precision": (f"fp32 residual and dots (compensated wave reductions), fp64 refresh every "
                           f"{args.refresh} markers, fp64 decisions and draws" if precision == "fp32" else "fp64"),
             "workgroups": stats["workgroups"], "block_markers": stats["block_markers"],
@@ -1003,6 +1008,11 @@ def main():
             "rounds_per_launch": float(np.mean(rounds)), "changes_per_launch": float(np.mean(changes)),
             "phase_ms_per_launch": dict(zip(brr.HipSweep.PHASES, np.mean(phases, axis=0).round(3).tolist())),
         },
+    }
+    line["late_iterations"] = {
+        "iterations": f"the last {len(late)} timed ({args.warmup + steps - len(late)}..{args.warmup + steps - 1})",
+        "kernel_ms_per_iteration": float(np.mean(late)),
+        "device_marker_updates_per_s": float(m / (np.mean(late) * 1e-3)),
     }
     if errors is not None:
         line["errors_vs_fp64"] = errors
