@@ -667,8 +667,11 @@ bayesr_batch_kernel(BatchParams p)
                         c1 = dbl4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
                         for (int s = 0; s < NT; ++s) {
-                            const uint32_t a0 = (s < nt) ? *(const uint32_t*)&B.codes[tq][4 * s] : 0u;
-                            const uint32_t a1 = (s < nt) ? *(const uint32_t*)&B.codes[16 + tq][4 * s] : 0u;
+                            // loaded without a branch (every row holds kMaxNT words), masked after
+                            const uint32_t x0 = *(const uint32_t*)&B.codes[tq][4 * s];
+                            const uint32_t x1 = *(const uint32_t*)&B.codes[16 + tq][4 * s];
+                            const uint32_t a0 = (s < nt) ? x0 : 0u;
+                            const uint32_t a1 = (s < nt) ? x1 : 0u;
                             flt4 t0 = {0.f, 0.f, 0.f, 0.f}, t1 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                             for (int r = 0; r < 4; ++r) {
@@ -722,9 +725,10 @@ bayesr_batch_kernel(BatchParams p)
                             g = dbl4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
                             for (int s = 0; s < NT; ++s) {
-                                const uint32_t ra0 = (s < nt) ? *(const uint32_t*)&B.codes[16 * (int)mi + tq][4 * s] : 0u;
-                                const uint32_t rb0 = (s < nt) ? *(const uint32_t*)&B.codes[16 * (int)mc + tq][4 * s] : 0u;
-                                const uint32_t rowa = ra0, rowb = rb0;
+                                // loaded without a branch (every row holds kMaxNT words), masked after
+                                const uint32_t xa = *(const uint32_t*)&B.codes[16 * (int)mi + tq][4 * s];
+                                const uint32_t xb = *(const uint32_t*)&B.codes[16 * (int)mc + tq][4 * s];
+                                const uint32_t rowa = (s < nt) ? xa : 0u, rowb = (s < nt) ? xb : 0u;
                                 flt4 gt = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                                 for (int r = 0; r < 4; ++r) {
@@ -965,25 +969,32 @@ bayesr_batch_kernel(BatchParams p)
 #pragma unroll
                         for (int i2 = 0; i2 < HS; ++i2) acc[i2] = ET4{0, 0, 0, 0};
                         for (uint32_t m = fmask; m; m &= m - 1u) {
-                            const int km = 4 * __builtin_ctz(m) + kq;
+                            const int km = 4 * __builtin_ctz(m) + kq;   // < kBB: the rows read below exist
                             const bool kv = km < nb;
+                            // straight-line loads: a row past the slice's tiles or past the block
+                            // reads what the buffer holds and is masked to zero after the load
                             uint32_t row[HS];
 #pragma unroll
                             for (int i2 = 0; i2 < HS; ++i2) {
                                 const int s2 = h * HS + i2;
-                                row[i2] = (s2 < NT && s2 < nt && kv) ? *(const uint32_t*)&B.codes[km][4 * s2] : 0u;
+                                const uint32_t wrow = *(const uint32_t*)&B.codes[km][4 * (s2 < NT ? s2 : 0)];
+                                row[i2] = (s2 < nt && kv) ? wrow : 0u;
                             }
-                            const ET u0 = (ET)B.ut[km][0], u1 = (ET)B.ut[km][1], u2 = (ET)B.ut[km][2],
-                                     u3 = (ET)B.ut[km][3];
-                            const ET bv = kv ? (ET)L.cd[km][16 * wave + tq] : (ET)0;
-                            // straight-line: rows past the slice are masked to zero by i < nloc
+                            ET u0 = (ET)B.ut[km][0], u1 = (ET)B.ut[km][1], u2 = (ET)B.ut[km][2],
+                               u3 = (ET)B.ut[km][3];
+                            // FP32 tiles: the four values converted once here (an empty asm keeps the
+                            // compiler from selecting among the FP64 values and converting per tile)
+                            if constexpr (kF32) asm volatile("" : "+v"(u0), "+v"(u1), "+v"(u2), "+v"(u3));
+                            const ET cdv = (ET)L.cd[km][16 * wave + tq];
+                            const ET bv = kv ? cdv : (ET)0;
+                            // rows past the slice are masked to zero by i < nloc (a select, not a branch)
 #pragma unroll
                             for (int i2 = 0; i2 < HS; ++i2) {
                                 const int s2 = h * HS + i2;
                                 if (s2 < NT) {
                                     const uint32_t i = 16u * (uint32_t)s2 + (uint32_t)tq;
-                                    const ET av =
-                                        (i < nloc) ? wt_t((row[i2] >> (2u * (uint32_t)tq)) & 3u, u0, u1, u2, u3) : (ET)0;
+                                    const ET wv = wt_t((row[i2] >> (2u * (uint32_t)tq)) & 3u, u0, u1, u2, u3);
+                                    const ET av = (i < nloc) ? wv : (ET)0;
                                     acc[i2] = mfma16x16x4(av, bv, acc[i2]);
                                 }
                             }
