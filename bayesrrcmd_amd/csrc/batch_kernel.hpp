@@ -661,8 +661,12 @@ bayesr_batch_kernel(BatchParams p)
                     } else {
                         // per residual tile an FP32 product of 16 individuals, the tiles
                         // summed in FP64; entry r of the B operand is individual 4 kq + r
-                        const float f00 = (float)w00, f01 = (float)w01, f02 = (float)w02, f03 = (float)w03;
-                        const float f10 = (float)w10, f11 = (float)w11, f12 = (float)w12, f13 = (float)w13;
+                        float f00 = (float)w00, f01 = (float)w01, f02 = (float)w02, f03 = (float)w03;
+                        float f10 = (float)w10, f11 = (float)w11, f12 = (float)w12, f13 = (float)w13;
+                        // converted once here (an empty asm keeps the compiler from selecting
+                        // among the FP64 values and converting per step)
+                        asm volatile("" : "+v"(f00), "+v"(f01), "+v"(f02), "+v"(f03));
+                        asm volatile("" : "+v"(f10), "+v"(f11), "+v"(f12), "+v"(f13));
                         c0 = dbl4{0.0, 0.0, 0.0, 0.0};
                         c1 = dbl4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -720,8 +724,11 @@ bayesr_batch_kernel(BatchParams p)
                             g = gg[0] + gg[1];
                         } else {
                             // per tile an FP32 product, the tiles summed in FP64
-                            const float fu0 = (float)u0, fu1 = (float)u1, fu2 = (float)u2, fu3 = (float)u3;
-                            const float fb0 = (float)b0, fb1 = (float)b1, fb2 = (float)b2, fb3 = (float)b3;
+                            float fu0 = (float)u0, fu1 = (float)u1, fu2 = (float)u2, fu3 = (float)u3;
+                            float fb0 = (float)b0, fb1 = (float)b1, fb2 = (float)b2, fb3 = (float)b3;
+                            // converted once (as for the dot tiles)
+                            asm volatile("" : "+v"(fu0), "+v"(fu1), "+v"(fu2), "+v"(fu3));
+                            asm volatile("" : "+v"(fb0), "+v"(fb1), "+v"(fb2), "+v"(fb3));
                             g = dbl4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
                             for (int s = 0; s < NT; ++s) {
