@@ -206,6 +206,49 @@ __device__ __forceinline__ double wave_sum(double x)
     return x;
 }
 
+// The value held by lane (lane ^ kBit), moved without the LDS crossbar: DPP moves
+// within rows of 16 lanes (quad permutations for 1 and 2, row rotations for 4
+// and 8) and gfx950's row swaps for 16 and 32. permlane16_swap(x, x) leaves
+// rows (0, 0, 2, 2) in its first result and (1, 1, 3, 3) in its second;
+// permlane32_swap(x, x) halves (0, 0) and (1, 1); row_ror:n gives lane i the
+// value of lane (i - n) mod 16 of its row. Every lane of the wave must be active
+// (a DPP read of an inactive lane returns 0).
+template <int kBit>
+__device__ __forceinline__ uint32_t lane_xor_u32(uint32_t x, int lane)
+{
+    if constexpr (kBit == 1) {
+        return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xB1, 0xF, 0xF, true);    // quad_perm 1,0,3,2
+    } else if constexpr (kBit == 2) {
+        return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x4E, 0xF, 0xF, true);    // quad_perm 2,3,0,1
+    } else if constexpr (kBit == 4) {
+        const uint32_t dn = (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x124, 0xF, 0xF, true);   // row_ror:4
+        const uint32_t up = (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x12C, 0xF, 0xF, true);   // row_ror:12
+        return (lane & 4) ? dn : up;
+    } else if constexpr (kBit == 8) {
+        return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x128, 0xF, 0xF, true);   // row_ror:8
+    } else if constexpr (kBit == 16) {
+        const auto sw = __builtin_amdgcn_permlane16_swap(x, x, false, false);
+        return (lane & 16) ? sw[0] : sw[1];
+    } else {
+        static_assert(kBit == 32, "lane_xor: 1, 2, 4, 8, 16 or 32");
+        const auto sw = __builtin_amdgcn_permlane32_swap(x, x, false, false);
+        return (lane & 32) ? sw[0] : sw[1];
+    }
+}
+template <int kBit>
+__device__ __forceinline__ float lane_xor(float x, int lane)
+{
+    return __uint_as_float(lane_xor_u32<kBit>(__float_as_uint(x), lane));
+}
+template <int kBit>
+__device__ __forceinline__ double lane_xor(double x, int lane)
+{
+    const uint64_t b = double_as_bits(x);
+    const uint64_t lo = lane_xor_u32<kBit>((uint32_t)b, lane);
+    const uint64_t hi = lane_xor_u32<kBit>((uint32_t)(b >> 32), lane);
+    return bits_as_double((hi << 32) | lo);
+}
+
 // Sums 16 per-lane values over the 64 lanes of a wave at once: halving
 // exchanges (xor 32, 16, 8, 4) leave lanes 4i..4i+3 with value i's partial
 // over their quarter, then a butterfly over xor 2, 1 completes it. 17 shuffles
@@ -220,7 +263,7 @@ __device__ __forceinline__ void transpose_stage(double (&v)[kN], int lane)
 #pragma unroll
     for (int i = 0; i < kHalf; ++i) {
         const double send = up ? v[i] : v[i + kHalf];
-        const double recv = __shfl_xor(send, kBit, 64);
+        const double recv = lane_xor<kBit>(send, lane);
         v[i] = (up ? v[i + kHalf] : v[i]) + recv;
     }
 }
@@ -232,8 +275,8 @@ __device__ __forceinline__ double transpose_reduce16(double (&v)[16], int lane)
     transpose_stage<2, 8>(v, lane);
     transpose_stage<1, 4>(v, lane);
     double x = v[0];
-    x += __shfl_xor(x, 2, 64);
-    x += __shfl_xor(x, 1, 64);
+    x += lane_xor<2>(x, lane);
+    x += lane_xor<1>(x, lane);
     return x;   // total of value ((lane >> 2) & 15)
 }
 
@@ -245,9 +288,9 @@ __device__ __forceinline__ double transpose_reduce8(double (&v)[8], int lane)
     transpose_stage<2, 16>(v, lane);
     transpose_stage<1, 8>(v, lane);
     double x = v[0];
-    x += __shfl_xor(x, 4, 64);
-    x += __shfl_xor(x, 2, 64);
-    x += __shfl_xor(x, 1, 64);
+    x += lane_xor<4>(x, lane);
+    x += lane_xor<2>(x, lane);
+    x += lane_xor<1>(x, lane);
     return x;   // total of value ((lane >> 3) & 7)
 }
 
@@ -317,8 +360,8 @@ __device__ __forceinline__ void transpose_stage_c(float (&hi)[16], float (&lo)[1
     for (int i = 0; i < kHalf; ++i) {
         const float sh = up ? hi[i] : hi[i + kHalf];
         const float sl = up ? lo[i] : lo[i + kHalf];
-        const float rh = __shfl_xor(sh, kBit, 64);
-        const float rl = __shfl_xor(sl, kBit, 64);
+        const float rh = lane_xor<kBit>(sh, lane);
+        const float rl = lane_xor<kBit>(sl, lane);
         const float kh = up ? hi[i + kHalf] : hi[i];
         const float kl = up ? lo[i + kHalf] : lo[i];
         float s, e;
@@ -338,15 +381,14 @@ __device__ __forceinline__ double transpose_reduce16_c(float (&v)[16], int lane)
     transpose_stage_c<2, 8>(v, lo, lane);
     transpose_stage_c<1, 4>(v, lo, lane);
     float h = v[0], l = lo[0];
-#pragma unroll
-    for (int off = 2; off > 0; off >>= 1) {
-        const float rh = __shfl_xor(h, off, 64);
-        const float rl = __shfl_xor(l, off, 64);
+    auto step = [&](float rh, float rl) {
         float s, e;
         two_sum(h, rh, s, e);
         h = s;
         l = (l + rl) + e;
-    }
+    };
+    step(lane_xor<2>(h, lane), lane_xor<2>(l, lane));
+    step(lane_xor<1>(h, lane), lane_xor<1>(l, lane));
     return (double)h + (double)l;   // total of value ((lane >> 2) & 15)
 }
 
