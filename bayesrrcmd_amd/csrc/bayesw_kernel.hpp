@@ -568,7 +568,7 @@ __device__ __forceinline__ bool bw_exchange(const BwParams& p, BwShared<R>& L, u
         double acc = 0.0;
         ok = wait_sum_lanes(p.xchg1, [&](uint32_t w2) { return bw_slot1(p, rnd, v, w2); }, p.P, rnd, p.ctl, lane, &acc);
         if (__any(!ok)) { ok = false; break; }
-        acc = wave_sum(acc);
+        acc = wave_sum_full(acc, lane);
         if (lane < kCopies) publish_pair(p.xchg2, bw_slot2(p, rnd, v, (uint32_t)lane), rnd, acc);
     }
     if (!ok && lane == 0) { L.abort = 1; raise_abort(p, 21); }
@@ -974,7 +974,7 @@ bayesw_sweep_kernel(BwParams p)
                     for (int r = 0; r < R; ++r)
 #pragma unroll
                         for (int e = 0; e < 8; ++e) sv += vi[r][e];
-                    sv = wave_sum(sv);
+                    sv = wave_sum_full(sv, lane);
                     if (tid == 0) publish_pair(p.xchg1, bw_slot1(p, rnd, 0, w), rnd, sv);
                     V = V0;
                     if (spec) {
@@ -1004,10 +1004,10 @@ bayesw_sweep_kernel(BwParams p)
                                     if constexpr (kDense) s3 += (c == 1u) ? x : 0.0;
                                 }
                             }
-                            s0 = wave_sum(s0);
-                            s1 = wave_sum(s1);
-                            s2 = wave_sum(s2);
-                            if constexpr (kDense) s3 = wave_sum(s3);
+                            s0 = wave_sum_full(s0, lane);
+                            s1 = wave_sum_full(s1, lane);
+                            s2 = wave_sum_full(s2, lane);
+                            if constexpr (kDense) s3 = wave_sum_full(s3, lane);
                             if (lane == 0) {
                                 publish_pair(p.xchg1, bw_slot1(p, rnd, V0, w), rnd, s0);
                                 publish_pair(p.xchg1, bw_slot1(p, rnd, V0 + 1, w), rnd, s1);
@@ -1034,10 +1034,10 @@ bayesw_sweep_kernel(BwParams p)
                             if constexpr (kDense) s3 += (c == 1u) ? vi[r][e] : 0.0;
                         }
                     }
-                    s0 = wave_sum(s0);
-                    s1 = wave_sum(s1);
-                    s2 = wave_sum(s2);
-                    if constexpr (kDense) s3 = wave_sum(s3);
+                    s0 = wave_sum_full(s0, lane);
+                    s1 = wave_sum_full(s1, lane);
+                    s2 = wave_sum_full(s2, lane);
+                    if constexpr (kDense) s3 = wave_sum_full(s3, lane);
                     if (tid == 0) {
                         publish_pair(p.xchg1, bw_slot1(p, rnd, 0, w), rnd, s0);
                         publish_pair(p.xchg1, bw_slot1(p, rnd, 1, w), rnd, s1);
@@ -1121,7 +1121,7 @@ bayesw_sweep_kernel(BwParams p)
                         const bool stop = active && (breaker || k == h || sel > 0);
                         const unsigned long long sm = __ballot(stop);
                         const int c = sm ? pos + (int)__ffsll((long long)sm) - 1 : nb;
-                        const int cs = __shfl(sel, (c - pos) & 63, 64);
+                        const int cs = lane_read(sel, (c - pos) & 63);
                         // slots before c keep beta = 0 in class 0; slot k's outputs are
                         // written by workgroup k % P
                         const bool keep0 = k < c && active;

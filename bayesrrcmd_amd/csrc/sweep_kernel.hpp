@@ -249,6 +249,33 @@ __device__ __forceinline__ double lane_xor(double x, int lane)
     return bits_as_double((hi << 32) | lo);
 }
 
+// The value of lane `src` (the same index in every lane) read into a scalar
+// register: no LDS round trip, and inactive source lanes are read as well.
+__device__ __forceinline__ int lane_read(int x, int src)
+{
+    return __builtin_amdgcn_readlane(x, src);
+}
+__device__ __forceinline__ double lane_read(double x, int src)
+{
+    const uint64_t b = double_as_bits(x);
+    const uint64_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, src);
+    const uint64_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), src);
+    return bits_as_double((hi << 32) | lo);
+}
+
+// wave_sum's butterfly (the same additions in the same order) with lane_xor's
+// moves: every lane of the wave must be active.
+__device__ __forceinline__ double wave_sum_full(double x, int lane)
+{
+    x += lane_xor<32>(x, lane);
+    x += lane_xor<16>(x, lane);
+    x += lane_xor<8>(x, lane);
+    x += lane_xor<4>(x, lane);
+    x += lane_xor<2>(x, lane);
+    x += lane_xor<1>(x, lane);
+    return x;
+}
+
 // Sums 16 per-lane values over the 64 lanes of a wave at once: halving
 // exchanges (xor 32, 16, 8, 4) leave lanes 4i..4i+3 with value i's partial
 // over their quarter, then a butterfly over xor 2, 1 completes it. 17 shuffles
@@ -571,9 +598,9 @@ __device__ __forceinline__ bool exchange(const SweepParams& p, Shared<R>& L, uin
             ok = wait_sum_lanes(p.xchg1, [&](uint32_t w2) { return slot1(p, rnd, kMaxB + k, w2); }, p.P, rnd, p.ctl, lane,
                                 &gacc);
         if (__any(!ok)) { ok = false; break; }
-        acc = wave_sum(acc);
+        acc = wave_sum_full(acc, lane);
         if (k >= gpos) {
-            gacc = wave_sum(gacc);
+            gacc = wave_sum_full(gacc, lane);
             if (lane < kCopies) publish_pair(p.xchg2, slot2(p, rnd, kMaxB + k, (uint32_t)lane), rnd, gacc);
         }
         if (lane < kCopies) publish_pair(p.xchg2, slot2(p, rnd, k, (uint32_t)lane), rnd, acc);
@@ -1590,7 +1617,7 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, Shared<R>& L, R
                             const int key = (accepted && sel >= 0) ? g * K + sel : -1;
                             unsigned long long todo = __ballot(key >= 0);
                             while (todo) {
-                                const int kb = __shfl(key, (int)__ffsll((long long)todo) - 1, 64);
+                                const int kb = lane_read(key, (int)__ffsll((long long)todo) - 1);
                                 const unsigned long long same = __ballot(key == kb);
                                 if (lane == 0) L.v[kb] += (double)__popcll(same);
                                 todo &= ~same;
@@ -1600,8 +1627,8 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, Shared<R>& L, R
                         unsigned long long sq = __ballot(accepted && sel > 0);
                         while (sq) {
                             const int b = (int)__ffsll((long long)sq) - 1;
-                            const double bb = __shfl(bnew, b, 64);
-                            const int gb = __shfl(g, b, 64);
+                            const double bb = lane_read(bnew, b);
+                            const int gb = lane_read(g, b);
                             if (lane == 0) L.betasqn[gb] += bb * bb;
                             sq &= sq - 1;
                         }
@@ -1645,19 +1672,19 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, Shared<R>& L, R
                                 unsigned long long mset = cm;
                                 while (mset) {
                                     const int b = (int)__ffsll((long long)mset) - 1;
-                                    eS = eS + __shfl(ed, b, 64);
+                                    eS = eS + lane_read(ed, b);
                                     mset &= mset - 1;
                                 }
                             }
                             if (lane == 0) L.cmask = cm << pos;
                         } else if (!dense && c < nb) {
-                            eS = eS + __shfl(ed, c - pos, 64);
+                            eS = eS + lane_read(ed, c - pos);
                         }
                         cfin = c;
                         if (pass == 0) {
                             c0 = c;
                             if (c != h || gpos >= nb) break;
-                            dcorr = __shfl(dBeta, c - pos, 64);
+                            dcorr = lane_read(dBeta, c - pos);
                             start = c + 1;
                         } else {
                             c1 = c;
