@@ -207,8 +207,13 @@ __device__ __forceinline__ void arms_cumulate(ArmsEnv& e, int lane)
         const int q = lane + 64 * h;
         if (q < n) m = fmax(m, e.y[q]);
     }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    // (the whole wave runs the sampler: lane_xor's moves need every lane active)
+    m = fmax(m, lane_xor<32>(m, lane));
+    m = fmax(m, lane_xor<16>(m, lane));
+    m = fmax(m, lane_xor<8>(m, lane));
+    m = fmax(m, lane_xor<4>(m, lane));
+    m = fmax(m, lane_xor<2>(m, lane));
+    m = fmax(m, lane_xor<1>(m, lane));
     const double ymax = m;
     double eyq[2];
 #pragma unroll
